@@ -1,0 +1,243 @@
+#!/usr/bin/env python3
+"""RGB-D panoramas/s of the sphere-sweeping stereo per-frame path on one MI355X.
+
+Workload (default, "c1"): four seeded synthetic 1216x1216 double-sphere
+fisheyes (uint8 RGB, ray-cast on the GPU), matched at 1024x1024, references 0
+and 2, 32 candidates uniform in inverse distance over [1/100, 1/0.55],
+sigma_i 10, sigma_s 25, 1216x1216 stitch input, 2048x1024 RGB + float32
+distance panorama, fp32 throughout.  One step = one estimate_RGBD_panorama
+call on inputs already resident in HBM.
+
+Prints one JSON line (see the roofline / cpu_baseline fields) and a per-stage
+split on stderr.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sphere-stereo_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "c1": dict(matching=(1024, 1024), stitch=(1216, 1216), pano=(2048, 1024), N=32, refs=(0, 2), sigma_i=10.0,
+               sigma_s=25.0, resolution=1216, seed=0),
+    "c0": dict(matching=(512, 512), stitch=(1216, 1216), pano=(1024, 512), N=32, refs=(0, 2), sigma_i=10.0,
+               sigma_s=12.5, resolution=1216, seed=0),
+}
+
+
+def levels(H, W):
+    return int(min(math.log2(W), math.log2(H)) - 1)
+
+
+def algorithmic_bytes(stage, wl, C=4):
+    """Algorithmic HBM bytes per frame of a stage: each stage reads its inputs
+    once and writes its outputs once, in the reference's stage order (fp32
+    volume e = 4 B, guide g = 3 B)."""
+    W, H = wl["matching"]
+    N, R = wl["N"], len(wl["refs"])
+    e, g = 4, 3
+    L = levels(H, W)
+    P = [math.ceil(H / 2 ** s) * math.ceil(W / 2 ** s) for s in range(L)]
+    if stage == "sweep":
+        return R * (N * P[0] * e + P[0] * (12 * C + 1))
+    if stage == "isb_down":
+        return R * sum((P[s - 1] + P[s]) * (g + N * e) for s in range(1, L))
+    if stage == "isb_up":
+        return R * sum((P[s + 1] + 2 * P[s]) * (g + N * e) for s in range(1, L - 1))
+    if stage == "isb_up_argmin":
+        return R * ((P[1] + 2 * P[0]) * (g + N * e) + N * P[0] * e + 4 * P[0])
+    if stage == "dist_down":
+        return R * sum((P[s - 1] + P[s]) * (g + 4) for s in range(1, L))
+    if stage == "dist_up":
+        return R * sum((P[s + 1] + 2 * P[s]) * (g + 4) for s in range(0, L - 1))
+    if stage == "reproject":
+        return R * (12 * P[0])  # read the map, scatter, plus the fill
+    if stage == "inpaint":
+        return R * 32 * 10 * P[0]
+    if stage == "merge":
+        PW, PH = wl["pano"]
+        sc, sr = wl["stitch"]
+        return PW * PH * (12 * R + 7) + R * (8 * P[0] + 3 * sc * sr)
+    if stage == "guide":
+        return R * P[0] * (12 + 4)
+    if stage == "to_u8":
+        sc, sr = wl["stitch"]
+        return R * sc * sr * 3 * 5
+    return 0
+
+
+def setup(wl, device):
+    import torch
+
+    from sphere_stereo import RGBD_Estimator, synth
+    from sphere_stereo.utils import parse_json_calib
+
+    rig = synth.SyntheticRig(seed=wl["seed"], resolution=wl["resolution"])
+    rendered = rig.render(device)
+    calibs = parse_json_calib(rig.basalt_json()["value0"], list(wl["matching"]), device)
+    viewpoint = torch.zeros([3], device=device)
+    for r in wl["refs"]:
+        viewpoint += calibs[r].rt[:3, 3] / len(wl["refs"])
+    to_match, to_stitch, masks = synth.make_frame(rig, wl["matching"], wl["stitch"], wl["refs"], device=device,
+                                                  rendered=rendered)
+    kw = dict(min_dist=0.55, max_dist=100, candidate_count=wl["N"], references_indices=list(wl["refs"]),
+              reprojection_viewpoint=viewpoint, masks=masks, matching_resolution=wl["matching"],
+              rgb_to_stitch_resolution=wl["stitch"], panorama_resolution=wl["pano"], sigma_i=wl["sigma_i"],
+              sigma_s=wl["sigma_s"])
+    est = RGBD_Estimator(calibs, device=device, **kw)
+    return est, to_match, to_stitch, calibs, kw
+
+
+def profile(lib, est, to_match, to_stitch, frames, mask):
+    import ctypes as C
+
+    import torch
+
+    lib.ss_profile_enable(1, mask)
+    for _ in range(frames):
+        est.estimate_RGBD_panorama(to_match, to_stitch)
+    torch.cuda.synchronize()
+    lib.ss_profile_enable(0, 0)
+    ms = (C.c_double * 11)()
+    n = (C.c_int * 11)()
+    lib.ss_profile_collect(ms, n)
+    return {lib.ss_stage_name(i).decode(): (ms[i], n[i]) for i in range(11)}
+
+
+def cpu_baseline(wl, calibs, kw, to_match, to_stitch, budget_s):
+    """The fp32 oracle (a scalar C port of the reference path) on the host:
+    one full frame of the same workload, timed after its set-up."""
+    import torch
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pipeline import OraclePipeline
+
+    cpu_cal = []
+    for c in calibs:
+        cc = type(c)(c.original_resolution, c.principal.cpu(), c.fl.cpu(), c.xi, c.alpha, c.rt.cpu(),
+                     c.matching_scale.cpu())
+        cpu_cal.append(cc)
+    kwc = dict(kw)
+    kwc["reprojection_viewpoint"] = kw["reprojection_viewpoint"].cpu()
+    kwc["masks"] = [m.cpu() for m in kw["masks"]]
+    ora = OraclePipeline(cpu_cal, **kwc)
+    tm = [t.cpu() for t in to_match]
+    ts = [t.cpu() for t in to_stitch]
+    frames, t0 = 0, time.perf_counter()
+    while True:
+        ora.estimate_RGBD_panorama(tm, ts)
+        frames += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or frames >= 3:
+            break
+    return dict(value=frames / el, unit="panoramas/s", cores=1, kind="port",
+                sample="%d full %s frame(s) (%dx%d matching, %d candidates, %d refs, %dx%d panorama) through the "
+                       "fp32 oracle, single thread, set-up excluded; %.1f s" %
+                       (frames, "c1" if wl["matching"][0] == 1024 else "c0", wl["matching"][0], wl["matching"][1],
+                        wl["N"], len(wl["refs"]), wl["pano"][0], wl["pano"][1], el))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would run its own frames "
+              "or references independently); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from sphere_stereo import _lib
+
+    lib = _lib.lib
+    wl = WORKLOADS[args.workload]
+    device = torch.device("cuda:0")
+    print("[bench] setting up %s" % args.workload, file=sys.stderr, flush=True)
+    est, to_match, to_stitch, calibs, kw = setup(wl, device)
+    for _ in range(args.warmup):
+        est.estimate_RGBD_panorama(to_match, to_stitch)
+    torch.cuda.synchronize()
+
+    # untimed pass: per-stage split, to find the dominant kernel
+    split = profile(lib, est, to_match, to_stitch, 2, (1 << 11) - 1)
+    per_frame = {k: v[0] / 2 for k, v in split.items()}
+    dominant = max(per_frame, key=per_frame.get)
+    dom_bit = [lib.ss_stage_name(i).decode() for i in range(11)].index(dominant)
+    for k, (ms, n) in sorted(split.items(), key=lambda kv: -kv[1][0]):
+        if n:
+            b = algorithmic_bytes(k, wl)
+            print("[bench] %-14s %8.3f ms/frame %4d launches/frame  %8.1f GB/s algorithmic" %
+                  (k, ms / 2, n // 2, b / (ms / 2 * 1e-3) / 1e9 if ms else 0), file=sys.stderr)
+
+    # timed region: only the dominant stage is bracketed by events
+    lib.ss_profile_enable(1, 1 << dom_bit)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        est.estimate_RGBD_panorama(to_match, to_stitch)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    lib.ss_profile_enable(0, 0)
+    import ctypes as C
+
+    ms = (C.c_double * 11)()
+    n = (C.c_int * 11)()
+    lib.ss_profile_collect(ms, n)
+    elapsed = t1 - t0
+    value = args.steps / elapsed
+    launches_per_frame = n[dom_bit] / args.steps
+    avg_launch_ms = ms[dom_bit] / max(n[dom_bit], 1)
+    bytes_per_launch = algorithmic_bytes(dominant, wl) / launches_per_frame
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(dominant)
+
+    out = {
+        "metric": "RGB-D panoramas/s (4x1024^2 fisheye, 32 cands) + % HBM peak",
+        "value": round(value, 3),
+        "unit": "panoramas/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ray-cast double-sphere rig, 4 x 1216^2 uint8 fisheyes)",
+        "config": {"workload": args.workload + ": 4 fisheyes 1216^2 -> %dx%d matching, refs %s, %d candidates "
+                                                "[0.55, 100] m, sigma_i %g, sigma_s %g, %dx%d stitch, %dx%d RGB-D "
+                                                "panorama" % (wl["matching"][0], wl["matching"][1],
+                                                              list(wl["refs"]), wl["N"], wl["sigma_i"],
+                                                              wl["sigma_s"], wl["stitch"][0], wl["stitch"][1],
+                                                              wl["pano"][0], wl["pano"][1]),
+                   "global_batch": 1, "parallelism": "none (1 GPU)"},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "avg_launch_ms": round(avg_launch_ms, 5), "algorithmic_bytes_per_launch": int(bytes_per_launch)},
+        "stage_ms_per_frame": {k: round(v, 4) for k, v in per_frame.items() if v > 0},
+    }
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(wl, calibs, kw, to_match, to_stitch, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

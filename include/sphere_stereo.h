@@ -1,0 +1,208 @@
+/*
+ * sphere_stereo.h - C ABI of the MI355X (gfx950) sphere-sweeping stereo library
+ * (libsphere_stereo.so).  It replaces the native and tensor-op stages of the
+ * per-frame path of KAIST-VCLAB/sphere-stereo:
+ *
+ *   python/depth_estimation.py  RGBD_Estimator (sweep, cost, argmin, camera selection)
+ *   python/isb_filter.py + .cu  ISB_Filter.apply, guideDownsample2xKernel, guideUpsample2xKernel
+ *   python/stitcher.py + .cu    Stitcher, reprojectDistanceKernel, inpaintKernel,
+ *                               createInpaintingWeightsKernel, createBlendingLutKernel,
+ *                               mergeRGBDPanoramaKernel
+ *
+ * Conventions
+ *   - Every pointer named d_* is device memory; everything else is host memory.
+ *   - Every call is asynchronous on `stream` (a hipStream_t passed as void*;
+ *     NULL = the null stream) and returns SS_OK or an ss_status code.  The text
+ *     of the last error of the calling thread is returned by ss_last_error().
+ *   - The library allocates no device memory in its entry points; callers pass
+ *     the workspaces whose sizes ss_*_workspace_bytes() report.
+ *   - Images are HWC float32 in the channel order of the caller (BGR, as cv2
+ *     reads them).  The cost volume inside the library is candidate-contiguous:
+ *     [H][W][Np] with Np = ss_padded_candidates(N) >= N, pads never read as
+ *     results.  The reference's planar [N][H][W] layout is accepted by
+ *     ss_isb_filter_planar (the drop-in for ISB_Filter.apply).
+ */
+#ifndef SPHERE_STEREO_H
+#define SPHERE_STEREO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SS_ABI_VERSION 1
+#define SS_MAX_CAMERAS 8
+#define SS_MAX_CANDIDATES 128
+#define SS_MAX_REFERENCES 8
+
+typedef void *ss_stream; /* hipStream_t */
+
+typedef enum {
+    SS_OK = 0,
+    SS_ERR_ARG = 1,         /* invalid argument (sizes, pointers, counts) */
+    SS_ERR_HIP = 2,         /* a HIP runtime call failed */
+    SS_ERR_UNSUPPORTED = 3, /* valid for the reference but outside this build's limits */
+    SS_ERR_WORKSPACE = 4    /* workspace smaller than required */
+} ss_status;
+
+/* Double-sphere camera as the reference's torch code uses it (utils.py:40-94):
+ * fs = fl * matching_scale and cs = principal * matching_scale are the fp32
+ * products the reference forms; xi and alpha stay Python doubles. */
+typedef struct {
+    float fs[2];
+    float cs[2];
+    double xi;
+    double alpha;
+} ss_camera;
+
+/* Stitcher intrinsics: the 24-byte Intrinsics record of stitcher.cu:31-35,
+ * filled as vectorize_calibration (stitcher.py:35-46) fills it. */
+typedef struct {
+    float fx, fy, cx, cy, xi, alpha;
+} ss_intrinsics;
+
+/* Library ----------------------------------------------------------------- */
+
+/* Has no counterpart in the reference (CuPy compiled the kernels per object). */
+int ss_abi_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char *ss_last_error(void);
+/* Padded candidate stride Np used by the candidate-contiguous volume. */
+int ss_padded_candidates(int N);
+/* Pyramid level count int(min(log2 W, log2 H) - 1) of isb_filter.py:62. */
+int ss_isb_levels(int H, int W);
+
+/* Init-time builders -------------------------------------------------------- */
+
+/* RGBD_Estimator.select_camera for one reference (depth_estimation.py:83-135).
+ * rts: C x 12 host floats, the 3x4 rows of inv(rt_k) @ rt_ref; d_masks: C x H x W;
+ * d_sel: H x W int8, -1 where no camera qualifies. */
+int ss_select_camera(const ss_camera *cams, const float *rts, int C, int ref, const float *d_masks, int H, int W,
+                     double min_dist, double max_dist, int8_t *d_sel, ss_stream stream);
+
+/* Per-pixel unit ray of the reference camera (utils.py:57-73, called at
+ * depth_estimation.py:141-143) with the selected camera index stored in the
+ * 4th lane as a float: d_raylut is H x W x 4 float32. */
+int ss_ray_lut(const ss_camera *ref, const int8_t *d_sel, int H, int W, float *d_raylut, ss_stream stream);
+
+/* createInpaintingWeightsKernel (stitcher.cu:212-272): d_lut is H x W x 2 uint8. */
+int ss_inpaint_lut(const ss_intrinsics *intr, const float *t3, int H, int W, double min_dist, double max_dist,
+                   uint8_t *d_lut, ss_stream stream);
+
+/* Normalised (2r+1)^2 box filter of stitcher.py:145-148 / 162-164 over B planes
+ * of H x W; outside reads pad_value.  d_tmp holds (H + 2r) x W floats. */
+int ss_box_filter(const float *d_in, float *d_out, float *d_tmp, int B, int H, int W, int radius, float pad_value,
+                  ss_stream stream);
+
+/* createBlendingLutKernel (stitcher.cu:338-401).  rot: R x 9 host floats
+ * (inverse rotations), trans: R x 3 host floats; d_masks: R x H x W smoothed
+ * masks; outputs d_sampling R x PH x PW x 2 and d_weights R x PH x PW. */
+int ss_blend_lut(const ss_intrinsics *intr, const float *rot, const float *trans, int R, const float *d_masks, int H,
+                 int W, int PH, int PW, double min_dist, double max_dist, float *d_sampling, float *d_weights,
+                 ss_stream stream);
+
+/* d_w[r][p] /= sum_r d_w[r][p] (stitcher.py:164). */
+int ss_normalize_weights(float *d_w, int R, int P, ss_stream stream);
+
+/* Per-frame stages ---------------------------------------------------------- */
+
+/* rgb2yCbCr + uint8 cast (utils.py:144-155, depth_estimation.py:227), packed
+ * as one uint32 per pixel (bytes: ch0, ch1, ch2, 0). */
+int ss_guide(const float *d_img, int H, int W, uint32_t *d_guide, ss_stream stream);
+
+/* Sweep + raw cost (depth_estimation.py:141-175) into the candidate-contiguous
+ * volume d_cost (H x W x Np float32).  d_images: C host-side device pointers
+ * (H x W x 3 float32); rts as ss_select_camera; cands: N host floats
+ * (1/linspace(1/min, 1/max, N)). */
+int ss_sweep(const float *d_raylut, const float *const *d_images, int C, const ss_camera *cams, const float *rts,
+             int ref, const float *cands, int N, int H, int W, float *d_cost, ss_stream stream);
+
+/* Bytes of workspace for ss_isb_filter / ss_isb_filter_planar (pyramid levels
+ * 1..L-1 of guides and volume, plus the level-0 repack for the planar form). */
+size_t ss_isb_workspace_bytes(int H, int W, int N, int planar);
+
+/* ISB_Filter.apply (isb_filter.py:72-136) on the library layout: d_guide
+ * (H x W packed guide) and d_cost (H x W x Np) are filtered in place. */
+int ss_isb_filter(uint32_t *d_guide, float *d_cost, int H, int W, int N, double sigma_i, double sigma_s,
+                  void *d_workspace, size_t workspace_bytes, ss_stream stream);
+
+/* Drop-in ISB_Filter.apply on the reference layout: d_guide H x W x 3 uint8,
+ * d_cost N x H x W float32, both filtered in place. */
+int ss_isb_filter_planar(uint8_t *d_guide, float *d_cost, int H, int W, int N, double sigma_i, double sigma_s,
+                         void *d_workspace, size_t workspace_bytes, ss_stream stream);
+
+/* Argmin, parabola refinement and index->distance (depth_estimation.py:180-198)
+ * on a filtered candidate-contiguous volume.  d0, dN1: first/last candidate;
+ * A = d0/dN1 - 1 in fp32.  d_dist: H x W. */
+int ss_argmin_refine(const float *d_cost, int H, int W, int N, float d0, float dN1, float A, float *d_dist,
+                     ss_stream stream);
+
+/* Bytes of workspace for ss_estimate_distance. */
+size_t ss_estimate_workspace_bytes(int H, int W, int N);
+
+/* estimate_fisheye_distance (depth_estimation.py:137-203) for one reference:
+ * guide, sweep, cost filter with the final up-pass fused with the argmin,
+ * then the distance filter at (sigma_i/2, sigma_s/2).  d_guide3 (H x W x 3
+ * uint8) is the guide argument of the reference; NULL computes it from
+ * d_images[ref] as estimate_RGBD_panorama does (depth_estimation.py:227).
+ * d_dist receives the filtered H x W distance map. */
+int ss_estimate_distance(const float *d_raylut, const float *const *d_images, int C, const ss_camera *cams,
+                         const float *rts, int ref, const float *cands, int N, int H, int W, double sigma_i,
+                         double sigma_s, const uint8_t *d_guide3, float *d_dist, void *d_workspace,
+                         size_t workspace_bytes, ss_stream stream);
+
+/* reprojectDistanceKernel (stitcher.cu:181-204) with stitcher.py:184-192's
+ * fill: d_out is set to 1e8, then every source pixel takes part in an exact
+ * atomic z-minimum at rint(y)*W + rint(x) (indices outside the map dropped). */
+int ss_reproject(const float *d_in, float *d_out, int H, int W, const ss_intrinsics *intr, const float *t3,
+                 ss_stream stream);
+
+/* iterations passes of inpaintKernel (stitcher.cu:283-323) in Jacobi form:
+ * d_map is updated, d_tmp (H x W) is scratch.  thr_empty = max_dist + 0.1;
+ * thr_src = the reference's "MAX_DIST + 0.1f" as a double. */
+int ss_inpaint(float *d_map, float *d_tmp, const uint8_t *d_lut, int H, int W, int iterations, double thr_empty,
+               double thr_src, ss_stream stream);
+
+/* mergeRGBDPanoramaKernel (stitcher.cu:423-474).  d_reproj, d_dists, d_imgs:
+ * R host-side arrays of device pointers (H x W float32 each; Hs x Ws x 3 uint8
+ * each); intr: R records; trans: R x 3 host floats.  Outputs d_dist_pano
+ * PH x PW float32 and d_rgb_pano PH x PW x 3 uint8. */
+int ss_merge(const float *d_sampling, const float *d_weights, const float *const *d_reproj,
+             const float *const *d_dists, const uint8_t *const *d_imgs, int Hs, int Ws, const float *trans,
+             const ss_intrinsics *intr, int R, int H, int W, int PH, int PW, float *d_dist_pano, uint8_t *d_rgb_pano,
+             ss_stream stream);
+
+/* Truncating float32 -> uint8 cast of n values (depth_estimation.py:239-240). */
+int ss_to_u8(const float *d_in, uint8_t *d_out, size_t n, ss_stream stream);
+
+/* Kernel timing ------------------------------------------------------------- */
+/* No counterpart in the reference (it has no timers).  While enabled, every
+ * launch of a stage whose bit is set in stage_mask is bracketed by a pair of
+ * hipEvents recorded on the launch's stream.  ss_profile_enable(0, 0) stops
+ * recording; ss_profile_collect waits for the recorded events and returns,
+ * per stage, the summed milliseconds and the launch count (arrays of
+ * SS_STAGE_COUNT), then clears the records. */
+enum {
+    SS_STAGE_GUIDE = 0,
+    SS_STAGE_SWEEP,
+    SS_STAGE_ISB_DOWN,
+    SS_STAGE_ISB_UP,
+    SS_STAGE_ISB_UP_ARGMIN,
+    SS_STAGE_DIST_DOWN,
+    SS_STAGE_DIST_UP,
+    SS_STAGE_REPROJECT,
+    SS_STAGE_INPAINT,
+    SS_STAGE_MERGE,
+    SS_STAGE_TO_U8,
+    SS_STAGE_COUNT
+};
+int ss_profile_enable(int on, unsigned stage_mask);
+int ss_profile_collect(double *ms, int *launches);
+const char *ss_stage_name(int stage);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPHERE_STEREO_H */

@@ -1,0 +1,570 @@
+// C ABI of libsphere_stereo.so: argument checks, workspace carving and the
+// per-reference / per-frame launch sequences.  Declarations and the reference
+// interfaces each entry point replaces are in include/sphere_stereo.h.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "ss_kernels.h"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int ok() {
+  g_err[0] = '\0';
+  return SS_OK;
+}
+
+#define SS_HIP(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return fail(SS_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+#define SS_REQUIRE(cond, ...)                                 \
+  do {                                                        \
+    if (!(cond)) return fail(SS_ERR_ARG, __VA_ARGS__);        \
+  } while (0)
+
+inline hipStream_t as_stream(ss_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Kernel timing: a pool of events created when timing is enabled, and the
+// (stage, start, end) records of the bracketed launches.
+struct Profiler {
+  std::mutex mu;
+  bool on = false;
+  unsigned mask = 0;
+  std::vector<hipEvent_t> pool;
+  size_t next = 0;
+  struct Rec {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+};
+Profiler g_prof;
+constexpr size_t kEventPool = 1 << 14;
+
+// Brackets the launches made during its lifetime when `stage` is profiled.
+struct StageTimer {
+  int stage;
+  hipStream_t s;
+  hipEvent_t a = nullptr, b = nullptr;
+  StageTimer(int st, hipStream_t str) : stage(st), s(str) {
+    if (!g_prof.on || !(g_prof.mask & (1u << st))) return;
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    if (g_prof.next + 2 > g_prof.pool.size()) return;
+    a = g_prof.pool[g_prof.next++];
+    b = g_prof.pool[g_prof.next++];
+    (void)hipEventRecord(a, s);
+  }
+  ~StageTimer() {
+    if (!a) return;
+    (void)hipEventRecord(b, s);
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    g_prof.recs.push_back({stage, a, b});
+  }
+};
+
+// Carves 256-byte aligned regions out of one workspace; with base == nullptr
+// it only measures.
+struct Carver {
+  char* base;
+  size_t off = 0;
+  explicit Carver(void* b) : base(static_cast<char*>(b)) {}
+  template <typename T>
+  T* take(size_t count) {
+    off = (off + 255) & ~size_t(255);
+    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += count * sizeof(T);
+    return p;
+  }
+};
+
+int levels_of(int H, int W) {
+  if (H < 1 || W < 1) return -1;
+  const double l = fmin(log2((double)W), log2((double)H));
+  return (int)(l - 1);
+}
+
+int padded(int N) {
+  if (N == 1) return 1;
+  const int v = ss::lanes_v(N);
+  return v < 0 ? -1 : 8 * v;
+}
+
+struct Level {
+  int h, w;
+  uint32_t* g;
+  float* c;
+};
+
+// Level sizes ceil(rows / 2^s) x ceil(cols / 2^s) (isb_filter.py:64-70) and
+// their buffers for s >= 1.
+void carve_levels(Carver& cv, Level* lv, int L, int H, int W, int NP) {
+  for (int s = 1; s < L; ++s) {
+    lv[s].h = (int)ceil(H / pow(2.0, s));
+    lv[s].w = (int)ceil(W / pow(2.0, s));
+    lv[s].g = cv.take<uint32_t>((size_t)lv[s].h * lv[s].w);
+    lv[s].c = cv.take<float>((size_t)lv[s].h * lv[s].w * NP);
+  }
+}
+
+// ISB_Filter.apply's two launch loops (isb_filter.py:87-134).  With
+// fuse_last, the level-0 up-pass computes the argmin instead of writing the
+// filtered volume.
+int run_pyramid(Level* lv, int L, int G, int V, double sigma_i, double sigma_s, bool fuse_last,
+                const ss::ArgminParams& am, float* dist, hipStream_t s, int stage_down, int stage_up) {
+  const double var_inv_s = 1 / (2 * sigma_s * sigma_s);
+  const double var_inv_i = 1 / (2 * sigma_i * sigma_i);
+  const float vi = (float)var_inv_i;
+  for (int k = 1; k < L; ++k) {
+    ss::IsbDownArgs a{lv[k - 1].g, lv[k - 1].c, lv[k - 1].h, lv[k - 1].w, lv[k].g, lv[k].c, lv[k].h, lv[k].w, vi};
+    StageTimer t(stage_down, s);
+    SS_HIP(ss::launch_isb_down(a, G, V, s));
+  }
+  for (int k = L - 2; k >= 0; --k) {
+    const double d = pow(2.0, k) - 0.5;
+    const double wd = exp(-(d * d) * var_inv_s);
+    const double wu = 1 - wd;
+    ss::IsbUpArgs a{lv[k + 1].g, lv[k + 1].c, lv[k + 1].h, lv[k + 1].w, lv[k].g, lv[k].c, lv[k].h, lv[k].w,
+                    (float)wu,   (float)wd,   vi,          dist,        am};
+    const bool fuse = fuse_last && k == 0;
+    StageTimer t(fuse ? SS_STAGE_ISB_UP_ARGMIN : stage_up, s);
+    SS_HIP(ss::launch_isb_up(a, G, V, fuse, s));
+  }
+  if (fuse_last && L == 1) {
+    StageTimer t(SS_STAGE_ISB_UP_ARGMIN, s);
+    SS_HIP(ss::launch_argmin(lv[0].c, lv[0].h * lv[0].w, V, am, dist, s));
+  }
+  return SS_OK;
+}
+
+bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+int check_cams(const ss_camera* cams, const float* rts, int C, int ref) {
+  SS_REQUIRE(cams && rts, "null camera or transform array");
+  SS_REQUIRE(C >= 1 && C <= SS_MAX_CAMERAS, "camera count %d outside [1, %d]", C, SS_MAX_CAMERAS);
+  SS_REQUIRE(ref >= 0 && ref < C, "reference index %d outside [0, %d)", ref, C);
+  return SS_OK;
+}
+
+int fill_sweep_args(ss::SweepArgs& a, const float* d_raylut, const float* const* d_images, int C,
+                    const ss_camera* cams, const float* rts, int ref, const float* cands, int N, int H, int W,
+                    float* d_cost) {
+  int rc = check_cams(cams, rts, C, ref);
+  if (rc) return rc;
+  SS_REQUIRE(d_raylut && d_images && cands, "null pointer argument");
+  SS_REQUIRE(N >= 1 && N <= SS_MAX_CANDIDATES, "candidate count %d outside [1, %d]", N, SS_MAX_CANDIDATES);
+  SS_REQUIRE(H >= 1 && W >= 1, "bad matching resolution %dx%d", W, H);
+  memset(&a, 0, sizeof a);
+  a.raylut = reinterpret_cast<const float4*>(d_raylut);
+  for (int k = 0; k < C; ++k) {
+    SS_REQUIRE(d_images[k], "null image pointer for camera %d", k);
+    a.img[k] = d_images[k];
+    a.cam[k] = ss::make_torch_cam(cams[k]);
+    memcpy(a.rt[k], rts + 12 * k, sizeof(float) * 12);
+  }
+  a.ref_img = d_images[ref];
+  memcpy(a.cands, cands, sizeof(float) * N);
+  a.cost = d_cost;
+  a.H = H;
+  a.W = W;
+  a.N = N;
+  a.Wf = (float)W;
+  a.Hf = (float)H;
+  a.invW = 1.0f / (float)W;
+  a.invH = 1.0f / (float)H;
+  a.pow2W = pow2(W);
+  a.pow2H = pow2(H);
+  return SS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ss_abi_version(void) { return SS_ABI_VERSION; }
+
+const char* ss_last_error(void) { return g_err; }
+
+int ss_padded_candidates(int N) { return padded(N); }
+
+int ss_isb_levels(int H, int W) { return levels_of(H, W); }
+
+int ss_select_camera(const ss_camera* cams, const float* rts, int C, int ref, const float* d_masks, int H, int W,
+                     double min_dist, double max_dist, int8_t* d_sel, ss_stream stream) {
+  int rc = check_cams(cams, rts, C, ref);
+  if (rc) return rc;
+  SS_REQUIRE(d_masks && d_sel, "null pointer argument");
+  SS_REQUIRE(H >= 1 && W >= 1, "bad matching resolution %dx%d", W, H);
+  ss::SelectArgs a;
+  memset(&a, 0, sizeof a);
+  a.ref = ss::make_torch_cam(cams[ref]);
+  for (int k = 0; k < C; ++k) {
+    a.cam[k] = ss::make_torch_cam(cams[k]);
+    memcpy(a.rt[k], rts + 12 * k, sizeof(float) * 12);
+  }
+  a.masks = d_masks;
+  a.sel = d_sel;
+  a.C = C;
+  a.refidx = ref;
+  a.H = H;
+  a.W = W;
+  a.mn = (float)min_dist;
+  a.mx = (float)max_dist;
+  a.Wf = (float)W;
+  a.Hf = (float)H;
+  a.invW = 1.0f / (float)W;
+  a.invH = 1.0f / (float)H;
+  a.pow2W = pow2(W);
+  a.pow2H = pow2(H);
+  SS_HIP(ss::launch_select_camera(a, as_stream(stream)));
+  return ok();
+}
+
+int ss_ray_lut(const ss_camera* ref, const int8_t* d_sel, int H, int W, float* d_raylut, ss_stream stream) {
+  SS_REQUIRE(ref && d_sel && d_raylut, "null pointer argument");
+  SS_REQUIRE(H >= 1 && W >= 1, "bad matching resolution %dx%d", W, H);
+  SS_HIP(ss::launch_ray_lut(ss::make_torch_cam(*ref), d_sel, H, W, reinterpret_cast<float4*>(d_raylut),
+                            as_stream(stream)));
+  return ok();
+}
+
+int ss_inpaint_lut(const ss_intrinsics* intr, const float* t3, int H, int W, double min_dist, double max_dist,
+                   uint8_t* d_lut, ss_stream stream) {
+  SS_REQUIRE(intr && t3 && d_lut, "null pointer argument");
+  SS_REQUIRE(H >= 1 && W >= 1, "bad matching resolution %dx%d", W, H);
+  SS_HIP(ss::launch_inpaint_lut(*intr, t3, H, W, (float)min_dist, (float)max_dist, d_lut, as_stream(stream)));
+  return ok();
+}
+
+int ss_box_filter(const float* d_in, float* d_out, float* d_tmp, int B, int H, int W, int radius, float pad_value,
+                  ss_stream stream) {
+  SS_REQUIRE(d_in && d_out && d_tmp, "null pointer argument");
+  SS_REQUIRE(B >= 1 && H >= 1 && W >= 1 && radius >= 0, "bad box filter shape");
+  SS_REQUIRE(d_in != d_out, "box filter cannot run in place");
+  SS_HIP(ss::launch_box_filter(d_in, d_out, d_tmp, B, H, W, radius, pad_value, as_stream(stream)));
+  return ok();
+}
+
+int ss_blend_lut(const ss_intrinsics* intr, const float* rot, const float* trans, int R, const float* d_masks, int H,
+                 int W, int PH, int PW, double min_dist, double max_dist, float* d_sampling, float* d_weights,
+                 ss_stream stream) {
+  SS_REQUIRE(intr && rot && trans && d_masks && d_sampling && d_weights, "null pointer argument");
+  SS_REQUIRE(R >= 1 && R <= SS_MAX_REFERENCES, "reference count %d outside [1, %d]", R, SS_MAX_REFERENCES);
+  SS_REQUIRE(H >= 2 && W >= 2 && PH >= 1 && PW >= 1, "bad blend table shape");
+  ss::BlendArgs a;
+  memset(&a, 0, sizeof a);
+  for (int r = 0; r < R; ++r) {
+    a.intr[r] = intr[r];
+    memcpy(a.rot[r], rot + 9 * r, sizeof(float) * 9);
+    memcpy(a.trans[r], trans + 3 * r, sizeof(float) * 3);
+  }
+  a.masks = d_masks;
+  a.sampling = reinterpret_cast<float2*>(d_sampling);
+  a.weights = d_weights;
+  a.R = R;
+  a.H = H;
+  a.W = W;
+  a.PH = PH;
+  a.PW = PW;
+  a.mn = (float)min_dist;
+  a.mx = (float)max_dist;
+  SS_HIP(ss::launch_blend_lut(a, as_stream(stream)));
+  return ok();
+}
+
+int ss_normalize_weights(float* d_w, int R, int P, ss_stream stream) {
+  SS_REQUIRE(d_w && R >= 1 && P >= 1, "bad argument");
+  SS_HIP(ss::launch_normalize(d_w, R, P, as_stream(stream)));
+  return ok();
+}
+
+int ss_guide(const float* d_img, int H, int W, uint32_t* d_guide, ss_stream stream) {
+  SS_REQUIRE(d_img && d_guide && H >= 1 && W >= 1, "bad argument");
+  StageTimer t(SS_STAGE_GUIDE, as_stream(stream));
+  SS_HIP(ss::launch_guide(d_img, H * W, d_guide, as_stream(stream)));
+  return ok();
+}
+
+int ss_sweep(const float* d_raylut, const float* const* d_images, int C, const ss_camera* cams, const float* rts,
+             int ref, const float* cands, int N, int H, int W, float* d_cost, ss_stream stream) {
+  SS_REQUIRE(d_cost, "null cost volume");
+  ss::SweepArgs a;
+  int rc = fill_sweep_args(a, d_raylut, d_images, C, cams, rts, ref, cands, N, H, W, d_cost);
+  if (rc) return rc;
+  const int V = ss::lanes_v(N);
+  SS_REQUIRE(V > 0, "candidate count %d unsupported", N);
+  StageTimer t(SS_STAGE_SWEEP, as_stream(stream));
+  SS_HIP(ss::launch_sweep(a, V, as_stream(stream)));
+  return ok();
+}
+
+size_t ss_isb_workspace_bytes(int H, int W, int N, int planar) {
+  const int L = levels_of(H, W), NP = padded(N);
+  if (L < 1 || NP < 0) return 0;
+  Carver cv(nullptr);
+  Level lv[32];
+  if (planar) {
+    cv.take<uint32_t>((size_t)H * W);
+    cv.take<float>((size_t)H * W * NP);
+  }
+  carve_levels(cv, lv, L, H, W, NP);
+  return cv.off + 256;
+}
+
+static int isb_filter_impl(uint32_t* d_guide, float* d_cost, int H, int W, int N, double sigma_i, double sigma_s,
+                           Carver& cv, hipStream_t s) {
+  const int L = levels_of(H, W);
+  const int NP = padded(N);
+  Level lv[32];
+  lv[0] = Level{H, W, d_guide, d_cost};
+  carve_levels(cv, lv, L, H, W, NP);
+  const int G = N == 1 ? 1 : 8, V = N == 1 ? 1 : NP / 8;
+  ss::ArgminParams am{};
+  return run_pyramid(lv, L, G, V, sigma_i, sigma_s, false, am, nullptr, s, SS_STAGE_ISB_DOWN, SS_STAGE_ISB_UP);
+}
+
+static int check_isb(int H, int W, int N, void* ws, size_t bytes, int planar) {
+  SS_REQUIRE(H >= 1 && W >= 1, "bad resolution %dx%d", W, H);
+  SS_REQUIRE(levels_of(H, W) >= 1, "resolution %dx%d gives no pyramid level (min side must be >= 4)", W, H);
+  SS_REQUIRE(N >= 1 && padded(N) > 0, "candidate count %d unsupported (1..%d)", N, SS_MAX_CANDIDATES);
+  const size_t need = ss_isb_workspace_bytes(H, W, N, planar);
+  if (!ws || bytes < need)
+    return fail(SS_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need, ws ? bytes : (size_t)0);
+  return SS_OK;
+}
+
+int ss_isb_filter(uint32_t* d_guide, float* d_cost, int H, int W, int N, double sigma_i, double sigma_s,
+                  void* d_workspace, size_t workspace_bytes, ss_stream stream) {
+  SS_REQUIRE(d_guide && d_cost, "null pointer argument");
+  int rc = check_isb(H, W, N, d_workspace, workspace_bytes, 0);
+  if (rc) return rc;
+  Carver cv(d_workspace);
+  rc = isb_filter_impl(d_guide, d_cost, H, W, N, sigma_i, sigma_s, cv, as_stream(stream));
+  return rc ? rc : ok();
+}
+
+int ss_isb_filter_planar(uint8_t* d_guide, float* d_cost, int H, int W, int N, double sigma_i, double sigma_s,
+                         void* d_workspace, size_t workspace_bytes, ss_stream stream) {
+  SS_REQUIRE(d_guide && d_cost, "null pointer argument");
+  int rc = check_isb(H, W, N, d_workspace, workspace_bytes, 1);
+  if (rc) return rc;
+  hipStream_t s = as_stream(stream);
+  Carver cv(d_workspace);
+  const int NP = padded(N);
+  uint32_t* g = cv.take<uint32_t>((size_t)H * W);
+  float* c = cv.take<float>((size_t)H * W * NP);
+  SS_HIP(ss::launch_guide3_to_packed(d_guide, g, H * W, s));
+  SS_HIP(ss::launch_planar_to_cc(d_cost, c, H * W, N, NP, s));
+  rc = isb_filter_impl(g, c, H, W, N, sigma_i, sigma_s, cv, s);
+  if (rc) return rc;
+  SS_HIP(ss::launch_packed_to_guide3(g, d_guide, H * W, s));
+  SS_HIP(ss::launch_cc_to_planar(c, d_cost, H * W, N, NP, s));
+  return ok();
+}
+
+int ss_argmin_refine(const float* d_cost, int H, int W, int N, float d0, float dN1, float A, float* d_dist,
+                     ss_stream stream) {
+  SS_REQUIRE(d_cost && d_dist && H >= 1 && W >= 1, "bad argument");
+  SS_REQUIRE(N >= 2 && N <= SS_MAX_CANDIDATES, "candidate count %d outside [2, %d]", N, SS_MAX_CANDIDATES);
+  ss::ArgminParams am{N, d0, dN1, A};
+  SS_HIP(ss::launch_argmin(d_cost, H * W, ss::lanes_v(N), am, d_dist, as_stream(stream)));
+  return ok();
+}
+
+static size_t estimate_layout(Carver& cv, int H, int W, int N, uint32_t** g0, float** c0, Level* lv, Level* dl) {
+  const int L = levels_of(H, W), NP = padded(N < 2 ? 2 : N);
+  *g0 = cv.take<uint32_t>((size_t)H * W);
+  *c0 = cv.take<float>((size_t)H * W * NP);
+  carve_levels(cv, lv, L, H, W, NP);
+  for (int s = 1; s < L; ++s) {
+    dl[s].h = lv[s].h;
+    dl[s].w = lv[s].w;
+    dl[s].g = lv[s].g;  // the distance filter reuses the guide pyramid buffers
+    dl[s].c = cv.take<float>((size_t)lv[s].h * lv[s].w);
+  }
+  return cv.off + 256;
+}
+
+size_t ss_estimate_workspace_bytes(int H, int W, int N) {
+  if (levels_of(H, W) < 1 || N < 1 || padded(N) < 0) return 0;
+  Carver cv(nullptr);
+  uint32_t* g0;
+  float* c0;
+  Level lv[32], dl[32];
+  return estimate_layout(cv, H, W, N, &g0, &c0, lv, dl);
+}
+
+int ss_estimate_distance(const float* d_raylut, const float* const* d_images, int C, const ss_camera* cams,
+                         const float* rts, int ref, const float* cands, int N, int H, int W, double sigma_i,
+                         double sigma_s, const uint8_t* d_guide3, float* d_dist, void* d_workspace,
+                         size_t workspace_bytes, ss_stream stream) {
+  hipStream_t s = as_stream(stream);
+  SS_REQUIRE(d_dist, "null distance output");
+  SS_REQUIRE(levels_of(H, W) >= 1, "resolution %dx%d gives no pyramid level (min side must be >= 4)", W, H);
+  ss::SweepArgs sa;
+  int rc = fill_sweep_args(sa, d_raylut, d_images, C, cams, rts, ref, cands, N, H, W, nullptr);
+  if (rc) return rc;
+  const size_t need = ss_estimate_workspace_bytes(H, W, N);
+  if (!d_workspace || workspace_bytes < need)
+    return fail(SS_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need, workspace_bytes);
+  const int L = levels_of(H, W);
+  Carver cv(d_workspace);
+  uint32_t* g0;
+  float* c0;
+  Level lv[32], dl[32];
+  estimate_layout(cv, H, W, N, &g0, &c0, lv, dl);
+  const float d0 = cands[0], dN1 = cands[N - 1];
+  {
+    StageTimer t(SS_STAGE_GUIDE, s);
+    if (d_guide3)
+      SS_HIP(ss::launch_guide3_to_packed(d_guide3, g0, H * W, s));
+    else
+      SS_HIP(ss::launch_guide(d_images[ref], H * W, g0, s));
+  }
+  if (N == 1) {
+    // one candidate: max == min everywhere, so the reference returns d_{N-1}
+    // (depth_estimation.py:198) whatever the cost
+    SS_HIP(ss::launch_fill(d_dist, dN1, (size_t)H * W, s));
+  } else {
+    sa.cost = c0;
+    const int V = ss::lanes_v(N);
+    {
+      StageTimer t(SS_STAGE_SWEEP, s);
+      SS_HIP(ss::launch_sweep(sa, V, s));
+    }
+    lv[0] = Level{H, W, g0, c0};
+    ss::ArgminParams am{N, d0, dN1, (float)(d0 / dN1) - 1.0f};
+    rc = run_pyramid(lv, L, 8, V, sigma_i, sigma_s, true, am, d_dist, s, SS_STAGE_ISB_DOWN, SS_STAGE_ISB_UP);
+    if (rc) return rc;
+  }
+  dl[0] = Level{H, W, g0, d_dist};
+  rc = run_pyramid(dl, L, 1, 1, sigma_i / 2, sigma_s / 2, false, ss::ArgminParams{}, nullptr, s, SS_STAGE_DIST_DOWN,
+                   SS_STAGE_DIST_UP);
+  if (rc) return rc;
+  return ok();
+}
+
+int ss_reproject(const float* d_in, float* d_out, int H, int W, const ss_intrinsics* intr, const float* t3,
+                 ss_stream stream) {
+  SS_REQUIRE(d_in && d_out && intr && t3 && H >= 1 && W >= 1, "bad argument");
+  SS_REQUIRE(d_in != d_out, "reprojection cannot run in place");
+  hipStream_t s = as_stream(stream);
+  StageTimer t(SS_STAGE_REPROJECT, s);
+  SS_HIP(ss::launch_fill(d_out, 1e8f, (size_t)H * W, s));
+  SS_HIP(ss::launch_reproject(d_in, d_out, H, W, *intr, t3, s));
+  return ok();
+}
+
+int ss_inpaint(float* d_map, float* d_tmp, const uint8_t* d_lut, int H, int W, int iterations, double thr_empty,
+               double thr_src, ss_stream stream) {
+  SS_REQUIRE(d_map && d_tmp && d_lut && H >= 1 && W >= 1 && iterations >= 0, "bad argument");
+  hipStream_t s = as_stream(stream);
+  float* a = d_map;
+  float* b = d_tmp;
+  for (int i = 0; i < iterations; ++i) {
+    StageTimer tm(SS_STAGE_INPAINT, s);
+    SS_HIP(ss::launch_inpaint_step(a, b, d_lut, H, W, thr_empty, thr_src, s));
+    float* t = a;
+    a = b;
+    b = t;
+  }
+  if (a != d_map) SS_HIP(hipMemcpyAsync(d_map, a, sizeof(float) * H * W, hipMemcpyDeviceToDevice, s));
+  return ok();
+}
+
+int ss_merge(const float* d_sampling, const float* d_weights, const float* const* d_reproj,
+             const float* const* d_dists, const uint8_t* const* d_imgs, int Hs, int Ws, const float* trans,
+             const ss_intrinsics* intr, int R, int H, int W, int PH, int PW, float* d_dist_pano, uint8_t* d_rgb_pano,
+             ss_stream stream) {
+  SS_REQUIRE(d_sampling && d_weights && d_reproj && d_dists && d_imgs && trans && intr && d_dist_pano && d_rgb_pano,
+             "null pointer argument");
+  SS_REQUIRE(R >= 1 && R <= SS_MAX_REFERENCES, "reference count %d outside [1, %d]", R, SS_MAX_REFERENCES);
+  SS_REQUIRE(H >= 2 && W >= 2 && Hs >= 2 && Ws >= 2 && PH >= 1 && PW >= 1, "bad merge shape");
+  ss::MergeArgs a;
+  memset(&a, 0, sizeof a);
+  for (int r = 0; r < R; ++r) {
+    SS_REQUIRE(d_reproj[r] && d_dists[r] && d_imgs[r], "null map or image pointer for reference %d", r);
+    a.reproj[r] = d_reproj[r];
+    a.dists[r] = d_dists[r];
+    a.imgs[r] = d_imgs[r];
+    a.intr[r] = intr[r];
+    memcpy(a.trans[r], trans + 3 * r, sizeof(float) * 3);
+  }
+  a.sampling = reinterpret_cast<const float2*>(d_sampling);
+  a.weights = d_weights;
+  a.R = R;
+  a.H = H;
+  a.W = W;
+  a.Hs = Hs;
+  a.Ws = Ws;
+  a.PH = PH;
+  a.PW = PW;
+  a.dist_out = d_dist_pano;
+  a.rgb_out = d_rgb_pano;
+  StageTimer t(SS_STAGE_MERGE, as_stream(stream));
+  SS_HIP(ss::launch_merge(a, as_stream(stream)));
+  return ok();
+}
+
+int ss_to_u8(const float* d_in, uint8_t* d_out, size_t n, ss_stream stream) {
+  SS_REQUIRE(d_in && d_out, "null pointer argument");
+  if (n == 0) return ok();
+  StageTimer t(SS_STAGE_TO_U8, as_stream(stream));
+  SS_HIP(ss::launch_to_u8(d_in, d_out, n, as_stream(stream)));
+  return ok();
+}
+
+int ss_profile_enable(int on, unsigned stage_mask) {
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  if (on && g_prof.pool.empty()) {
+    g_prof.pool.resize(kEventPool);
+    for (auto& e : g_prof.pool) SS_HIP(hipEventCreate(&e));
+  }
+  g_prof.on = on != 0;
+  g_prof.mask = stage_mask;
+  return ok();
+}
+
+int ss_profile_collect(double* ms, int* launches) {
+  SS_REQUIRE(ms && launches, "null output arrays");
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  for (int i = 0; i < SS_STAGE_COUNT; ++i) {
+    ms[i] = 0.0;
+    launches[i] = 0;
+  }
+  for (const auto& r : g_prof.recs) {
+    SS_HIP(hipEventSynchronize(r.b));
+    float t = 0.0f;
+    SS_HIP(hipEventElapsedTime(&t, r.a, r.b));
+    ms[r.stage] += t;
+    launches[r.stage] += 1;
+  }
+  g_prof.recs.clear();
+  g_prof.next = 0;
+  return ok();
+}
+
+const char* ss_stage_name(int stage) {
+  static const char* names[SS_STAGE_COUNT] = {"guide",      "sweep",     "isb_down", "isb_up",
+                                              "isb_up_argmin", "dist_down", "dist_up",  "reproject",
+                                              "inpaint",    "merge",     "to_u8"};
+  return (stage >= 0 && stage < SS_STAGE_COUNT) ? names[stage] : "";
+}
+
+}  // extern "C"

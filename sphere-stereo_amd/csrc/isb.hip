@@ -1,0 +1,477 @@
+// Multiscale edge-preserving ("ISB") cost-volume filter on gfx950.
+//
+// Behaviour: isb_filter.cu:44-330 and isb_filter.py:72-136 of the reference.
+// Design: the volume is candidate-contiguous ([h][w][NP], NP = G*V).  A group
+// of G lanes owns one coarse pixel; lane j holds candidates [j*V, j*V+V) and
+// moves them with 16-byte loads, so a wavefront streams 8 whole pixels per
+// instruction.  The guide weights of a pixel are computed once, spread over
+// the G lanes (lane j evaluates weight j) and broadcast with wavefront
+// shuffles.  G = 1 is the scalar form used for the N = 1 distance filter.
+// The last up-pass can be fused with the argmin/refinement so that the
+// filtered level-0 volume is never written back to HBM.
+#include "ss_kernels.h"
+
+namespace ss {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// (row, col) of the 25 up-sampling taps in the 3x3 coarse neighbourhood,
+// 4 bits each (row*4 + col), in the order of isb_filter.cu:173-208:
+// stencil 1 = taps 0-8, stencil 2 = 9-14, stencil 3 = 15-20, stencil 4 = 21-24.
+constexpr uint64_t kUpTapsLo = 0x5A68495A98654210ull;
+constexpr uint64_t kUpTapsHi = 0xA965A9216ull;
+
+__host__ __device__ constexpr int up_tap_code(int k) {
+  return k < 16 ? (int)((kUpTapsLo >> (4 * k)) & 15) : (int)((kUpTapsHi >> (4 * (k - 16))) & 15);
+}
+__host__ __device__ constexpr int up_tap_flat(int k) { return (up_tap_code(k) >> 2) * 3 + (up_tap_code(k) & 3); }
+__host__ __device__ constexpr int up_stencil(int k) { return k < 9 ? 0 : (k < 15 ? 1 : (k < 21 ? 2 : 3)); }
+// spatial base weights {1,1,1,1,8,1,1,1,1}, {8,8,1,1,1,1} x2, {1,1,1,1}
+__host__ __device__ constexpr float up_base(int k) {
+  return (k == 4 || k == 9 || k == 10 || k == 15 || k == 16) ? 8.0f : 1.0f;
+}
+constexpr int kStencilBegin[5] = {0, 9, 15, 21, 25};
+
+// Value selects (by-value parameters keep the choice in registers; selecting
+// between lvalues lets the compiler pick between stack addresses instead).
+__device__ __forceinline__ int pick3(int i, int a, int b, int c) { return i == 0 ? a : (i == 1 ? b : c); }
+__device__ __forceinline__ uint32_t pick4(int i, uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+  return i == 0 ? a : (i == 1 ? b : (i == 2 ? c : d));
+}
+
+template <int n>
+__device__ __forceinline__ void load_n(const float* __restrict__ p, float* o) {
+  if constexpr (n % 4 == 0) {
+#pragma unroll
+    for (int i = 0; i < n / 4; ++i) {
+      float4 t = reinterpret_cast<const float4*>(p)[i];
+      o[4 * i] = t.x;
+      o[4 * i + 1] = t.y;
+      o[4 * i + 2] = t.z;
+      o[4 * i + 3] = t.w;
+    }
+  } else if constexpr (n % 2 == 0) {
+#pragma unroll
+    for (int i = 0; i < n / 2; ++i) {
+      float2 t = reinterpret_cast<const float2*>(p)[i];
+      o[2 * i] = t.x;
+      o[2 * i + 1] = t.y;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < n; ++i) o[i] = p[i];
+  }
+}
+
+template <int n>
+__device__ __forceinline__ void store_n(float* __restrict__ p, const float* v) {
+  if constexpr (n % 4 == 0) {
+#pragma unroll
+    for (int i = 0; i < n / 4; ++i)
+      reinterpret_cast<float4*>(p)[i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+  } else if constexpr (n % 2 == 0) {
+#pragma unroll
+    for (int i = 0; i < n / 2; ++i) reinterpret_cast<float2*>(p)[i] = make_float2(v[2 * i], v[2 * i + 1]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < n; ++i) p[i] = v[i];
+  }
+}
+
+// candidates handled per inner step: whole 16-byte vectors when possible
+template <int V>
+constexpr int chunk_of() {
+  return V % 4 == 0 ? 4 : V;
+}
+
+// ---------------------------------------------------------------------------
+// Down-sampling: guideDownsample2xKernel (isb_filter.cu:44-119)
+// ---------------------------------------------------------------------------
+template <int G, int V>
+__global__ void __launch_bounds__(kBlock) k_isb_down(IsbDownArgs a) {
+  constexpr int NP = G * V;
+  constexpr int CH = chunk_of<V>();
+  const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  int pix = (tile * kBlock + (int)threadIdx.x) / G;
+  const int npix = a.hout * a.wout;
+  const bool active = pix < npix;
+  if (G == 1 && !active) return;
+  if (!active) pix = npix - 1;  // keep the lanes for the shuffles, store nothing
+
+  const int x = pix % a.wout, y = pix / a.wout;
+  const int x2 = 2 * x, y2 = 2 * y;
+  const int xp = min(x2 + 1, a.win - 1), yp = min(y2 + 1, a.hin - 1);
+  const int xm = max(x2 - 1, 0), ym = max(y2 - 1, 0);
+  // neighbour order of isb_filter.cu:68-77
+  const int nb[8] = {ym * a.win + xm, y2 * a.win + xm, yp * a.win + xm, yp * a.win + x2,
+                     yp * a.win + xp, y2 * a.win + xp, ym * a.win + xp, ym * a.win + x2};
+  const int ctr = y2 * a.win + x2;
+  const uint32_t gc = a.gin[ctr];
+
+  auto weight = [&](int j) __attribute__((always_inline)) -> float {
+    const int row = pick3((j == 0 || j == 6 || j == 7) ? 0 : ((j == 1 || j == 5) ? 1 : 2), ym, y2, yp);
+    const int col = pick3((j <= 2) ? 0 : ((j == 3 || j == 7) ? 1 : 2), xm, x2, xp);
+    const float base = (j & 1) ? 0.5f : 0.25f;
+    const float d = guide_sad(gc, a.gin[row * a.win + col]);
+    return base * (expf_cw(-d * d * a.var_inv_i) + 0.01f);
+  };
+
+  float w[8];
+  if constexpr (G == 1) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) w[j] = weight(j);
+  } else {
+    const float wl = weight(lane & 7);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) w[j] = __shfl(wl, j, G);
+  }
+  float ws = 1.0f;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) ws = ws + w[j];
+
+  if (lane == 0 && active) {
+    float s0 = gch(gc, 0), s1 = gch(gc, 1), s2 = gch(gc, 2);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t gn = a.gin[nb[j]];
+      s0 = s0 + gch(gn, 0) * w[j];
+      s1 = s1 + gch(gn, 1) * w[j];
+      s2 = s2 + gch(gn, 2) * w[j];
+    }
+    const float inv = 1.0f / ws;
+    a.gout[pix] = pack_guide(s0 * inv, s1 * inv, s2 * inv);
+  }
+
+  const size_t base_c = (size_t)lane * V;
+#pragma unroll
+  for (int c0 = 0; c0 < V; c0 += CH) {
+    float acc[CH], t[CH];
+    load_n<CH>(a.cin + (size_t)ctr * NP + base_c + c0, acc);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      load_n<CH>(a.cin + (size_t)nb[j] * NP + base_c + c0, t);
+#pragma unroll
+      for (int v = 0; v < CH; ++v) acc[v] = acc[v] + t[v] * w[j];
+    }
+#pragma unroll
+    for (int v = 0; v < CH; ++v) acc[v] = acc[v] / ws;
+    if (active) store_n<CH>(a.cout + (size_t)pix * NP + base_c + c0, acc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Argmin + parabola refinement + index->distance for one pixel whose NP
+// candidates are spread over a G-lane group (depth_estimation.py:180-198).
+// ---------------------------------------------------------------------------
+template <int G, int V>
+__device__ __forceinline__ float argmin_refine_group(const float* vals, int lane, const ArgminParams& p) {
+  const int n0 = lane * V;
+  float m = __builtin_huge_valf(), M = -__builtin_huge_valf();
+  int im = 0x7fffffff;
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    if (n0 + v < p.N) {
+      if (vals[v] < m) {
+        m = vals[v];
+        im = n0 + v;
+      }
+      M = fmaxf(M, vals[v]);
+    }
+  }
+  if constexpr (G > 1) {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+      const float om = __shfl_xor(m, off, G);
+      const int oi = __shfl_xor(im, off, G);
+      const float oM = __shfl_xor(M, off, G);
+      if (om < m || (om == m && oi < im)) {
+        m = om;
+        im = oi;
+      }
+      M = fmaxf(M, oM);
+    }
+  }
+  const int il = max(im - 1, 0), ir = min(im + 1, p.N - 1);
+  float L, R;
+  {
+    // pick element (il % V) / (ir % V) from the owning lane
+    float sl = vals[0], sr = vals[0];
+#pragma unroll
+    for (int v = 1; v < V; ++v) {
+      sl = (il % V == v) ? vals[v] : sl;
+      sr = (ir % V == v) ? vals[v] : sr;
+    }
+    if constexpr (G > 1) {
+      L = __shfl(sl, il / V, G);
+      R = __shfl(sr, ir / V, G);
+    } else {
+      L = sl;
+      R = sr;
+    }
+  }
+  float var = (0.5f * (L - R)) / (((L + R) - 2.0f * m) + 1e-8f);
+  var = var < -0.5f ? -0.5f : (var > 0.5f ? 0.5f : var);
+  if (im == p.N - 1 || im == 0) var = 0.0f;
+  float idx = (float)im + var;
+  if (M == m) idx = (float)(p.N - 1);
+  float d = p.d0 / (((p.A * idx) / (float)(p.N - 1)) + 1.0f);
+  if (fabsf(M - m) < 1e-8f) d = p.dN1;
+  return d;
+}
+
+// ---------------------------------------------------------------------------
+// Up-sampling: guideUpsample2xKernel (isb_filter.cu:132-330).  FUSE_ARGMIN
+// replaces the level-0 write-back with the argmin of argmin_refine_group.
+// ---------------------------------------------------------------------------
+template <int G, int V, bool FUSE_ARGMIN>
+__global__ void __launch_bounds__(kBlock) k_isb_up(IsbUpArgs a) {
+  constexpr int NP = G * V;
+  constexpr int CH = chunk_of<V>();
+  const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  int pix = (tile * kBlock + (int)threadIdx.x) / G;
+  const int npix = a.hin * a.win;
+  const bool active = pix < npix;
+  if (G == 1 && !active) return;
+  if (!active) pix = npix - 1;
+
+  const int x = pix % a.win, y = pix / a.win;
+  const int r0 = max(y - 1, 0), r2 = min(y + 1, a.hin - 1);
+  const int c0 = max(x - 1, 0), c2 = min(x + 1, a.win - 1);
+  const int x2 = 2 * x, y2 = 2 * y;
+  const int xpu = min(x2 + 1, a.wio - 1), ypu = min(y2 + 1, a.hio - 1);
+  const bool setY = y2 != ypu, setX = x2 != xpu;
+  const bool doit[4] = {true, setY, setX, setY && setX};
+  const int fine[4] = {y2 * a.wio + x2, ypu * a.wio + x2, y2 * a.wio + xpu, ypu * a.wio + xpu};
+  const int rows[3] = {r0, y, r2}, cols[3] = {c0, x, c2};
+  int tap[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) tap[r * 3 + c] = rows[r] * a.win + cols[c];
+
+  uint32_t gF[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) gF[s] = a.gio[fine[s]];
+
+  auto weight = [&](int k) __attribute__((always_inline)) -> float {
+    const int code = up_tap_code(k);
+    const int rr = code >> 2, cc = code & 3;
+    const int row = pick3(rr, r0, y, r2);
+    const int col = pick3(cc, c0, x, c2);
+    const int s = up_stencil(k);
+    const uint32_t gf = pick4(s, gF[0], gF[1], gF[2], gF[3]);
+    const float d = guide_sad(a.gin[row * a.win + col], gf);
+    return up_base(k) * (a.weight_down * (expf_cw(-d * d * a.var_inv_i) + 0.01f));
+  };
+
+  float w[25];
+  if constexpr (G == 1) {
+#pragma unroll
+    for (int k = 0; k < 25; ++k) w[k] = weight(k);
+  } else {
+    static_assert(G == 8, "group of 8 lanes");
+    const float w0 = weight(lane), w1 = weight(lane + 8), w2 = weight(lane + 16);
+#pragma unroll
+    for (int k = 0; k < 24; ++k) w[k] = __shfl(k < 8 ? w0 : (k < 16 ? w1 : w2), k & 7, G);
+    w[24] = weight(24);
+  }
+
+  float inv[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    float ws = a.weight_up;
+#pragma unroll
+    for (int k = kStencilBegin[s]; k < kStencilBegin[s + 1]; ++k) ws = ws + w[k];
+    inv[s] = 1.0f / ws;
+  }
+
+  if (!FUSE_ARGMIN && lane == 0 && active) {
+    uint32_t gD[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) gD[t] = a.gin[tap[t]];
+    uint32_t gout[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      float s0 = gch(gF[s], 0) * a.weight_up, s1 = gch(gF[s], 1) * a.weight_up, s2 = gch(gF[s], 2) * a.weight_up;
+#pragma unroll
+      for (int k = kStencilBegin[s]; k < kStencilBegin[s + 1]; ++k) {
+        const uint32_t g = gD[up_tap_flat(k)];
+        s0 = s0 + gch(g, 0) * w[k];
+        s1 = s1 + gch(g, 1) * w[k];
+        s2 = s2 + gch(g, 2) * w[k];
+      }
+      gout[s] = pack_guide(s0 * inv[s], s1 * inv[s], s2 * inv[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      if (doit[s]) a.gio[fine[s]] = gout[s];
+  }
+
+  const size_t base_c = (size_t)lane * V;
+  float res[FUSE_ARGMIN ? 4 : 1][FUSE_ARGMIN ? V : 1];
+#pragma unroll
+  for (int cb = 0; cb < V; cb += CH) {
+    float cD[9][CH];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) load_n<CH>(a.cin + (size_t)tap[t] * NP + base_c + cb, cD[t]);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if (!doit[s]) continue;
+      float o[CH];
+      load_n<CH>(a.cio + (size_t)fine[s] * NP + base_c + cb, o);
+#pragma unroll
+      for (int v = 0; v < CH; ++v) {
+        float acc = a.weight_up * o[v];
+#pragma unroll
+        for (int k = kStencilBegin[s]; k < kStencilBegin[s + 1]; ++k) acc = acc + w[k] * cD[up_tap_flat(k)][v];
+        o[v] = acc * inv[s];
+      }
+      if constexpr (FUSE_ARGMIN) {
+#pragma unroll
+        for (int v = 0; v < CH; ++v) res[s][cb + v] = o[v];
+      } else {
+        if (active) store_n<CH>(a.cio + (size_t)fine[s] * NP + base_c + cb, o);
+      }
+    }
+  }
+
+  if constexpr (FUSE_ARGMIN) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      if (!doit[s]) continue;  // uniform within the group: same coarse pixel
+      const float d = argmin_refine_group<G, V>(res[s], lane, a.am);
+      if (active && lane == 0) a.dist[fine[s]] = d;
+    }
+  }
+}
+
+// Argmin over an already filtered level-0 volume (unfused form).
+template <int G, int V>
+__global__ void __launch_bounds__(kBlock) k_argmin(const float* __restrict__ cost, int npix, ArgminParams p,
+                                                   float* __restrict__ dist) {
+  constexpr int NP = G * V;
+  const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
+  int pix = (int)((blockIdx.x * kBlock + threadIdx.x) / G);
+  const bool active = pix < npix;
+  if (G == 1 && !active) return;
+  if (!active) pix = npix - 1;
+  float vals[V];
+  load_n<V>(cost + (size_t)pix * NP + (size_t)lane * V, vals);
+  const float d = argmin_refine_group<G, V>(vals, lane, p);
+  if (active && lane == 0) dist[pix] = d;
+}
+
+// Layout changes for the planar drop-in (ISB_Filter.apply's [N][H][W]).
+__global__ void k_planar_to_cc(const float* __restrict__ src, float* __restrict__ dst, int HW, int N, int NP) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= HW) return;
+  for (int n = 0; n < NP; ++n) dst[(size_t)p * NP + n] = n < N ? src[(size_t)n * HW + p] : 0.0f;
+}
+
+__global__ void k_cc_to_planar(const float* __restrict__ src, float* __restrict__ dst, int HW, int N, int NP) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= HW) return;
+  for (int n = 0; n < N; ++n) dst[(size_t)n * HW + p] = src[(size_t)p * NP + n];
+}
+
+__global__ void k_guide3_to_packed(const uint8_t* __restrict__ g3, uint32_t* __restrict__ g, int HW) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= HW) return;
+  g[p] = (uint32_t)g3[3 * p] | ((uint32_t)g3[3 * p + 1] << 8) | ((uint32_t)g3[3 * p + 2] << 16);
+}
+
+__global__ void k_packed_to_guide3(const uint32_t* __restrict__ g, uint8_t* __restrict__ g3, int HW) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= HW) return;
+  const uint32_t v = g[p];
+  g3[3 * p] = (uint8_t)v;
+  g3[3 * p + 1] = (uint8_t)(v >> 8);
+  g3[3 * p + 2] = (uint8_t)(v >> 16);
+}
+
+inline int blocks_for(long long threads) { return (int)((threads + kBlock - 1) / kBlock); }
+
+template <int G, int V>
+hipError_t launch_down_gv(const IsbDownArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((k_isb_down<G, V>), dim3(blocks_for((long long)a.hout * a.wout * G)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int G, int V>
+hipError_t launch_up_gv(const IsbUpArgs& a, bool fuse, hipStream_t s) {
+  const dim3 grid(blocks_for((long long)a.hin * a.win * G));
+  if (fuse)
+    hipLaunchKernelGGL((k_isb_up<G, V, true>), grid, dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_isb_up<G, V, false>), grid, dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+#define SS_DISPATCH_V(V, CALL)                       \
+  switch (V) {                                       \
+    case 1: return CALL(1);                          \
+    case 2: return CALL(2);                          \
+    case 3: return CALL(3);                          \
+    case 4: return CALL(4);                          \
+    case 6: return CALL(6);                          \
+    case 8: return CALL(8);                          \
+    case 12: return CALL(12);                        \
+    case 16: return CALL(16);                        \
+    default: return hipErrorInvalidValue;            \
+  }
+
+hipError_t launch_isb_down(const IsbDownArgs& a, int G, int V, hipStream_t s) {
+  if (G == 1) return V == 1 ? launch_down_gv<1, 1>(a, s) : hipErrorInvalidValue;
+#define SS_CALL(v) launch_down_gv<8, v>(a, s)
+  SS_DISPATCH_V(V, SS_CALL)
+#undef SS_CALL
+}
+
+hipError_t launch_isb_up(const IsbUpArgs& a, int G, int V, bool fuse_argmin, hipStream_t s) {
+  if (G == 1) return (V == 1 && !fuse_argmin) ? launch_up_gv<1, 1>(a, false, s) : hipErrorInvalidValue;
+#define SS_CALL(v) launch_up_gv<8, v>(a, fuse_argmin, s)
+  SS_DISPATCH_V(V, SS_CALL)
+#undef SS_CALL
+}
+
+namespace {
+template <int V>
+hipError_t launch_argmin_v(const float* cost, int npix, const ArgminParams& p, float* dist, hipStream_t s) {
+  hipLaunchKernelGGL((k_argmin<8, V>), dim3(blocks_for((long long)npix * 8)), dim3(kBlock), 0, s, cost, npix, p, dist);
+  return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_argmin(const float* cost, int npix, int V, const ArgminParams& p, float* dist, hipStream_t s) {
+#define SS_CALL(v) launch_argmin_v<v>(cost, npix, p, dist, s)
+  SS_DISPATCH_V(V, SS_CALL)
+#undef SS_CALL
+}
+
+hipError_t launch_planar_to_cc(const float* src, float* dst, int HW, int N, int NP, hipStream_t s) {
+  hipLaunchKernelGGL(k_planar_to_cc, dim3(blocks_for(HW)), dim3(kBlock), 0, s, src, dst, HW, N, NP);
+  return hipGetLastError();
+}
+
+hipError_t launch_cc_to_planar(const float* src, float* dst, int HW, int N, int NP, hipStream_t s) {
+  hipLaunchKernelGGL(k_cc_to_planar, dim3(blocks_for(HW)), dim3(kBlock), 0, s, src, dst, HW, N, NP);
+  return hipGetLastError();
+}
+
+hipError_t launch_guide3_to_packed(const uint8_t* g3, uint32_t* g, int HW, hipStream_t s) {
+  hipLaunchKernelGGL(k_guide3_to_packed, dim3(blocks_for(HW)), dim3(kBlock), 0, s, g3, g, HW);
+  return hipGetLastError();
+}
+
+hipError_t launch_packed_to_guide3(const uint32_t* g, uint8_t* g3, int HW, hipStream_t s) {
+  hipLaunchKernelGGL(k_packed_to_guide3, dim3(blocks_for(HW)), dim3(kBlock), 0, s, g, g3, HW);
+  return hipGetLastError();
+}
+
+}  // namespace ss

@@ -1,0 +1,149 @@
+// Device-side building blocks shared by the sphere-sweeping stereo kernels.
+//
+// Every floating-point expression here is written in the operation order of
+// the reference (KAIST-VCLAB/sphere-stereo) and the library is compiled with
+// -ffp-contract=off and IEEE division/sqrt, so each operation is one
+// round-to-nearest fp32 operation.  That is what makes argmin indices, guide
+// bytes and z-test winners reproducible bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ss_types.h"
+
+namespace ss {
+
+// ---------------------------------------------------------------------------
+// exp: Cody-Waite reduction (ln2 = C1 + C2) and the degree-6 polynomial of the
+// Cephes library's expf (S. L. Moshier).  Used wherever the reference calls
+// __expf (isb_filter.cu:87,229,236,241,248) or expf (stitcher.cu:389).
+// The scaling by 2^n is one correctly rounded v_ldexp_f32, so subnormal
+// results keep their value (fp32 denormals are not flushed in this build).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float expf_cw(float x) {
+  if (!(x >= -104.0f)) return (x != x) ? x : 0.0f;
+  if (x > 88.7228394f) return __builtin_huge_valf();
+  float n = floorf(x * 1.44269504088896341f + 0.5f);
+  float r = x - n * 0.693359375f;
+  r = r - n * -2.12194440e-4f;
+  float z = r * r;
+  float p = 1.9875691500e-4f;
+  p = p * r + 1.3981999507e-3f;
+  p = p * r + 8.3334519073e-3f;
+  p = p * r + 4.1665795894e-2f;
+  p = p * r + 1.6666665459e-1f;
+  p = p * r + 5.0000001201e-1f;
+  p = p * z + r + 1.0f;
+  return ldexpf(p, (int)n);
+}
+
+// ---------------------------------------------------------------------------
+// Double-sphere camera, torch form (utils.py:57-94): fp32 tensor arithmetic
+// whose Python-double scalars were rounded to fp32 once on the host.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void unproject_t(const TorchCam& c, float u, float v, float& px, float& py,
+                                            float& pz, bool& valid) {
+  float mx = (u - c.cs0) / c.fs0;
+  float my = (v - c.cs1) / c.fs1;
+  float r2 = mx * mx + my * my;
+  float t = 1.0f - c.am2_1 * r2;
+  float tc = t > 0.0f ? t : 0.0f;
+  float mz = (1.0f - c.a2 * r2) / ((c.a * sqrtf(tc) + 1.0f) - c.a);
+  float mz2 = mz * mz;
+  float k = (mz * c.xi + sqrtf(mz2 + c.one_m_xi2 * r2)) / (mz2 + r2);
+  px = k * mx;
+  py = k * my;
+  pz = k * mz - c.xi;
+  valid = t >= 0.0f;
+}
+
+__device__ __forceinline__ void project_t(const TorchCam& c, float qx, float qy, float qz, float& u, float& v,
+                                          bool& valid) {
+  float xy2 = qx * qx + qy * qy;
+  float d1 = sqrtf(xy2 + qz * qz);
+  float cc = c.xi * d1 + qz;
+  float d2 = sqrtf(xy2 + cc * cc);
+  float nrm = c.a * d2 + c.one_m_a * cc;
+  valid = qz > c.mw2 * d1;
+  u = (c.fs0 * qx) / nrm + c.cs0;
+  v = (c.fs1 * qy) / nrm + c.cs1;
+}
+
+// torch.matmul(cat([p, 1]), rt.T) for one 3x4 row-major transform, as the
+// k-ordered multiply-add chain of a GEMM micro-kernel.
+__device__ __forceinline__ void rigid(const float* rt, float px, float py, float pz, float& qx, float& qy,
+                                      float& qz) {
+  qx = __builtin_fmaf(pz, rt[2], __builtin_fmaf(py, rt[1], px * rt[0])) + rt[3];
+  qy = __builtin_fmaf(pz, rt[6], __builtin_fmaf(py, rt[5], px * rt[4])) + rt[7];
+  qz = __builtin_fmaf(pz, rt[10], __builtin_fmaf(py, rt[9], px * rt[8])) + rt[11];
+}
+
+// grid_sample's align_corners=False round trip: normalise (u+0.5)/size*2-1
+// as depth_estimation.py:160-161 does, then torch's ((g+1)*size-1)/2.
+__device__ __forceinline__ float grid_coord(float u, float size) {
+  float g = ((u + 0.5f) / size) * 2.0f - 1.0f;
+  return ((g + 1.0f) * size - 1.0f) / 2.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Double-sphere camera, stitcher form (stitcher.cu:121-168): fp32 throughout,
+// no clamp in unproject; float2/float divides by reciprocal multiply.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void unproject_c(const ss_intrinsics& c, float u, float v, float& px, float& py,
+                                            float& pz) {
+  float mx = (u - c.cx) / c.fx;
+  float my = (v - c.cy) / c.fy;
+  float r2 = mx * mx + my * my;
+  float mz = (1.0f - c.alpha * c.alpha * r2) /
+             (c.alpha * sqrtf(1.0f - (2.0f * c.alpha - 1.0f) * r2) + 1.0f - c.alpha);
+  float mz2 = mz * mz;
+  float k = (mz * c.xi + sqrtf(mz2 + (1.0f - c.xi * c.xi) * r2)) / (mz2 + r2);
+  px = mx * k;
+  py = my * k;
+  pz = mz * k - c.xi;
+}
+
+template <bool WITH_VALID>
+__device__ __forceinline__ void project_c(const ss_intrinsics& c, float px, float py, float pz, float& u, float& v,
+                                          bool& valid) {
+  float d1 = sqrtf((px * px + py * py) + pz * pz);
+  float cc = c.xi * d1 + pz;
+  float d2 = sqrtf((px * px + py * py) + cc * cc);
+  float nrm = c.alpha * d2 + (1.0f - c.alpha) * cc;
+  if (WITH_VALID) {
+    float w1 = (double)c.alpha > 0.5 ? (1.0f - c.alpha) / c.alpha : c.alpha / (1.0f - c.alpha);
+    float w2 = (w1 + c.xi) / sqrtf(2.0f * w1 * c.xi + c.xi * c.xi + 1.0f);
+    valid = valid && (pz > -w2 * d1);
+  }
+  float inv = 1.0f / nrm;
+  u = (c.fx * px) * inv + c.cx;
+  v = (c.fy * py) * inv + c.cy;
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+// ---------------------------------------------------------------------------
+// Workgroups are dealt round-robin to the 8 XCDs; remapping gives each XCD a
+// contiguous run of tiles so that stencil halos are shared inside one L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  int q = n >> 3;
+  if (b >= (q << 3)) return b;
+  return (b & 7) * q + (b >> 3);
+}
+
+// Sum of absolute byte differences over the three guide channels (byte 3 is
+// zero in both words); the exact integer the reference's float absSum gives.
+__device__ __forceinline__ float guide_sad(uint32_t a, uint32_t b) {
+  return (float)__builtin_amdgcn_sad_u8(a, b, 0u);
+}
+
+__device__ __forceinline__ float gch(uint32_t g, int c) { return (float)((g >> (8 * c)) & 0xffu); }
+
+// float -> uint8 by truncation (vec_utils.cuh:111-114); inputs are convex
+// combinations of bytes, inside [0, 256).
+__device__ __forceinline__ uint32_t pack_guide(float a, float b, float c) {
+  return (uint32_t)(uint8_t)a | ((uint32_t)(uint8_t)b << 8) | ((uint32_t)(uint8_t)c << 16);
+}
+
+}  // namespace ss

@@ -1,0 +1,118 @@
+// Kernel argument records and launch functions (host side of the .hip files).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ss_device.h"
+#include "ss_types.h"
+
+namespace ss {
+
+struct IsbDownArgs {
+  const uint32_t* gin;
+  const float* cin;
+  int hin, win;
+  uint32_t* gout;
+  float* cout;
+  int hout, wout;
+  float var_inv_i;
+};
+
+struct ArgminParams {
+  int N;
+  float d0, dN1, A;
+};
+
+struct IsbUpArgs {
+  const uint32_t* gin;
+  const float* cin;
+  int hin, win;
+  uint32_t* gio;
+  float* cio;
+  int hio, wio;
+  float weight_up, weight_down, var_inv_i;
+  float* dist;  // fused argmin output (level-0 distance map)
+  ArgminParams am;
+};
+
+struct SweepArgs {
+  const float4* raylut;
+  const float* ref_img;
+  const float* img[SS_MAX_CAMERAS];
+  TorchCam cam[SS_MAX_CAMERAS];
+  float rt[SS_MAX_CAMERAS][12];
+  float cands[SS_MAX_CANDIDATES];
+  float* cost;
+  int H, W, N;
+  float Wf, Hf, invW, invH;
+  int pow2W, pow2H;
+};
+
+struct SelectArgs {
+  TorchCam ref;
+  TorchCam cam[SS_MAX_CAMERAS];
+  float rt[SS_MAX_CAMERAS][12];
+  const float* masks;
+  int8_t* sel;
+  int C, refidx, H, W;
+  float mn, mx;
+  float Wf, Hf, invW, invH;
+  int pow2W, pow2H;
+};
+
+struct MergeArgs {
+  const float2* sampling;
+  const float* weights;
+  const float* reproj[SS_MAX_REFERENCES];
+  const float* dists[SS_MAX_REFERENCES];
+  const uint8_t* imgs[SS_MAX_REFERENCES];
+  ss_intrinsics intr[SS_MAX_REFERENCES];
+  float trans[SS_MAX_REFERENCES][3];
+  int R, H, W, Hs, Ws, PH, PW;
+  float* dist_out;
+  uint8_t* rgb_out;
+};
+
+struct BlendArgs {
+  ss_intrinsics intr[SS_MAX_REFERENCES];
+  float rot[SS_MAX_REFERENCES][9];
+  float trans[SS_MAX_REFERENCES][3];
+  const float* masks;
+  float2* sampling;
+  float* weights;
+  int R, H, W, PH, PW;
+  float mn, mx;
+};
+
+// isb.hip
+hipError_t launch_isb_down(const IsbDownArgs& a, int G, int V, hipStream_t s);
+hipError_t launch_isb_up(const IsbUpArgs& a, int G, int V, bool fuse_argmin, hipStream_t s);
+hipError_t launch_argmin(const float* cost, int npix, int V, const ArgminParams& p, float* dist, hipStream_t s);
+hipError_t launch_planar_to_cc(const float* src, float* dst, int HW, int N, int NP, hipStream_t s);
+hipError_t launch_cc_to_planar(const float* src, float* dst, int HW, int N, int NP, hipStream_t s);
+hipError_t launch_guide3_to_packed(const uint8_t* g3, uint32_t* g, int HW, hipStream_t s);
+hipError_t launch_packed_to_guide3(const uint32_t* g, uint8_t* g3, int HW, hipStream_t s);
+
+// sweep.hip
+hipError_t launch_guide(const float* img, int HW, uint32_t* guide, hipStream_t s);
+hipError_t launch_ray_lut(const TorchCam& ref, const int8_t* sel, int H, int W, float4* lut, hipStream_t s);
+hipError_t launch_select_camera(const SelectArgs& a, hipStream_t s);
+hipError_t launch_sweep(const SweepArgs& a, int V, hipStream_t s);
+
+// stitch.hip
+hipError_t launch_fill(float* p, float v, size_t n, hipStream_t s);
+hipError_t launch_reproject(const float* din, float* dout, int H, int W, const ss_intrinsics& c, const float* t,
+                            hipStream_t s);
+hipError_t launch_inpaint_step(const float* din, float* dout, const uint8_t* lut, int H, int W, double thr_empty,
+                               double thr_src, hipStream_t s);
+hipError_t launch_inpaint_lut(const ss_intrinsics& c, const float* t, int H, int W, float mn, float mx, uint8_t* lut,
+                              hipStream_t s);
+hipError_t launch_box_filter(const float* in, float* out, float* tmp, int B, int H, int W, int r, float pad,
+                             hipStream_t s);
+hipError_t launch_blend_lut(const BlendArgs& a, hipStream_t s);
+hipError_t launch_normalize(float* w, int R, int P, hipStream_t s);
+hipError_t launch_merge(const MergeArgs& a, hipStream_t s);
+hipError_t launch_to_u8(const float* in, uint8_t* out, size_t n, hipStream_t s);
+
+}  // namespace ss

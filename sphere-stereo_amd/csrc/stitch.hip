@@ -1,0 +1,342 @@
+// Panorama stitching on gfx950: z-tested reprojection, inpainting, blending
+// tables and the RGB-D merge.
+//
+// Behaviour: stitcher.cu:181-474 and stitcher.py:49-220 of the reference, with
+// two intentional changes that remove the reference's data races:
+//   * reprojection takes an exact z-minimum with an integer atomicMin on the
+//     bit pattern of the (non-negative) distance instead of a non-atomic
+//     read-compare-write run twice (stitcher.cu:200-202, stitcher.py:184-192);
+//   * each inpainting pass reads the previous pass's map and writes a second
+//     buffer (Jacobi) instead of updating the map in place while other threads
+//     read it (stitcher.cu:310, 319).
+#include "ss_kernels.h"
+
+namespace ss {
+
+namespace {
+
+constexpr int kBlock = 256;
+inline int blocks_for(long long threads) { return (int)((threads + kBlock - 1) / kBlock); }
+
+// __float2int_rn: round half to even, NaN -> 0, saturating
+__device__ __forceinline__ int cvt_rni(float v) {
+  if (v != v) return 0;
+  const float r = rintf(v);
+  if (r >= 2147483647.0f) return 2147483647;
+  if (r <= -2147483648.0f) return -2147483647 - 1;
+  return (int)r;
+}
+
+// bilinear read with floor and no bounds check (stitcher.cu:89-116)
+__device__ __forceinline__ float interp_f(const float* __restrict__ m, int cols, float u, float v) {
+  const int u1 = (int)floorf(u), v1 = (int)floorf(v), u2 = u1 + 1, v2 = v1 + 1;
+  const float u1f = (float)u1, u2f = (float)u2, v1f = (float)v1, v2f = (float)v2;
+  const float w1 = (u2f - u) * (v2f - v), w2 = (u2f - u) * (v - v1f), w3 = (u - u1f) * (v2f - v),
+              w4 = (u - u1f) * (v - v1f);
+  const float p1 = m[(size_t)v1 * cols + u1], p2 = m[(size_t)v2 * cols + u1], p3 = m[(size_t)v1 * cols + u2],
+              p4 = m[(size_t)v2 * cols + u2];
+  return ((w1 * p1 + w2 * p2) + w3 * p3) + w4 * p4;
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill(float* __restrict__ p, float v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// reprojectDistanceKernel (stitcher.cu:181-204) as an exact atomic z-minimum
+__global__ void __launch_bounds__(kBlock) k_reproject(const float* __restrict__ din, float* __restrict__ dout, int H,
+                                                      int W, ss_intrinsics c, float tx, float ty, float tz) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= H * W) return;
+  float px, py, pz, u, v;
+  bool valid = true;
+  unproject_c(c, (float)(i % W), (float)(i / W), px, py, pz);
+  const float d = din[i];
+  px = px * d - tx;
+  py = py * d - ty;
+  pz = pz * d - tz;
+  project_c<false>(c, px, py, pz, u, v, valid);
+  const float dist = sqrtf((px * px + py * py) + pz * pz);
+  const long long o = (long long)cvt_rni(v) * W + cvt_rni(u);
+  // distances are >= 0 (or NaN, which never wins), so the uint order of the
+  // bit patterns is the float order; 1e8 is the fill value
+  if (o >= 0 && o < (long long)H * W && dist < 1e8f)
+    atomicMin(reinterpret_cast<unsigned int*>(dout) + o, __float_as_uint(dist));
+}
+
+// one Jacobi pass of inpaintKernel (stitcher.cu:283-323)
+__global__ void __launch_bounds__(kBlock) k_inpaint_step(const float* __restrict__ din, float* __restrict__ dout,
+                                                         const uchar2* __restrict__ lut, int H, int W,
+                                                         double thr_empty, double thr_src) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= H * W) return;
+  const float d = din[i];
+  float r = d;
+  const int x = i % W, y = i / W;
+  if ((double)d >= thr_empty && x < W - 1 && y < H - 1 && y >= 1 && x >= 1) {
+    const uchar2 b = lut[i];
+    const int bs[2] = {b.x, b.y};
+    float val = 0.0f, wsum = 0.0f;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int nx = x + (bs[s] & 3) - 1, ny = y + ((bs[s] & 12) >> 2) - 1;
+      const float w = (float)(bs[s] & 240);
+      const float sd = din[ny * W + nx];
+      if ((double)sd <= thr_src && w > 0.0f) {
+        val = val + w * sd;
+        wsum = wsum + w;
+      }
+    }
+    if (wsum > 0.0f) r = val / wsum;
+  }
+  dout[i] = r;
+}
+
+// createInpaintingWeightsKernel (stitcher.cu:212-272)
+__global__ void __launch_bounds__(kBlock) k_inpaint_lut(ss_intrinsics c, float tx, float ty, float tz, int H, int W,
+                                                        float mn, float mx, uchar2* __restrict__ lut) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= H * W) return;
+  float ux, uy, uz, pcx, pcy, pfx, pfy;
+  bool valid = true;
+  unproject_c(c, (float)(i % W), (float)(i / W), ux, uy, uz);
+  project_c<false>(c, ux * mn - tx, uy * mn - ty, uz * mn - tz, pcx, pcy, valid);
+  project_c<false>(c, ux * mx - tx, uy * mx - ty, uz * mx - tz, pfx, pfy, valid);
+  float dx = pfx - pcx, dy = pfy - pcy;
+  const float il = 1.0f / sqrtf(dx * dx + dy * dy);
+  dx = dx * il;
+  dy = dy * il;
+  float wt0 = 0.0f, wt1 = 0.0f;
+  int nx0 = 0, ny0 = 0, nx1 = 0, ny1 = 0;
+#pragma unroll
+  for (int n = -1; n <= 1; ++n)
+#pragma unroll
+    for (int m = -1; m <= 1; ++m) {
+      if (!(n || m)) continue;
+      const float ip = 1.0f / sqrtf((float)n * (float)n + (float)m * (float)m);
+      const float w = ((float)n * ip) * dx + ((float)m * ip) * dy;
+      if (w > wt1) {
+        wt0 = wt1;
+        nx0 = nx1;
+        ny0 = ny1;
+        wt1 = w;
+        nx1 = n;
+        ny1 = m;
+      } else if (w > wt0) {
+        wt0 = w;
+        nx0 = n;
+        ny0 = m;
+      }
+    }
+  const unsigned b0 = ((unsigned)(uint8_t)(wt0 * 255.0f) & 240u) + 4u * (unsigned)(1 + ny0) + (unsigned)(1 + nx0);
+  const unsigned b1 = ((unsigned)(uint8_t)(wt1 * 255.0f) & 240u) + 4u * (unsigned)(1 + ny1) + (unsigned)(1 + nx1);
+  lut[i] = make_uchar2((uint8_t)b0, (uint8_t)b1);
+}
+
+// Separable box filter: row sums over the padded rows, then column sums.
+__global__ void __launch_bounds__(kBlock) k_box_rows(const float* __restrict__ in, float* __restrict__ rows, int H,
+                                                     int W, int r, float pad) {
+  const int Hp = H + 2 * r;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= Hp * W) return;
+  const int yy = i / W, x = i % W, y = yy - r;
+  float s = 0.0f;
+  for (int j = -r; j <= r; ++j) {
+    const int xx = x + j;
+    const float v = (y < 0 || y >= H || xx < 0 || xx >= W) ? pad : in[(size_t)y * W + xx];
+    s = s + v;
+  }
+  rows[i] = s;
+}
+
+__global__ void __launch_bounds__(kBlock) k_box_cols(const float* __restrict__ rows, float* __restrict__ out, int H,
+                                                     int W, int r, float k) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= H * W) return;
+  const int y = i / W, x = i % W;
+  float s = 0.0f;
+  for (int t = 0; t <= 2 * r; ++t) s = s + rows[(size_t)(y + t) * W + x];
+  out[i] = s * k;
+}
+
+// createBlendingLutKernel (stitcher.cu:338-401)
+__global__ void __launch_bounds__(kBlock) k_blend_lut(BlendArgs a) {
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  const int P = a.PH * a.PW;
+  if (idx >= P) return;
+  const double kPi = 3.14159265;  // stitcher.cu:325
+  const int x = idx % a.PW, y = idx / a.PW;
+  const float phi = (float)(((double)((float)y + 0.5f) * kPi) / a.PH - kPi / 2.0);
+  const float theta = (float)(((double)(((float)x + 0.5f) * 2.0f) * kPi) / a.PW + kPi);
+  const float cp = (float)cos((double)phi), sp = (float)sin((double)phi);
+  const float ct = (float)cos((double)theta), st = (float)sin((double)theta);
+  const float ux = cp * st, uy = sp, uz = cp * ct;
+  float bw[SS_MAX_REFERENCES];
+  float bsum = 0.0f;
+  const float Wm = (float)a.W - 1.1f, Hm = (float)a.H - 1.1f;
+#pragma unroll
+  for (int r = 0; r < SS_MAX_REFERENCES; ++r) {
+    if (r >= a.R) break;
+    const float* rm = a.rot[r];
+    const float fx = (rm[0] * ux + rm[1] * uy) + rm[2] * uz;
+    const float fy = (rm[3] * ux + rm[4] * uy) + rm[5] * uz;
+    const float fz = (rm[6] * ux + rm[7] * uy) + rm[8] * uz;
+    bool valid = true;
+    float u, v;
+    project_c<true>(a.intr[r], fx, fy, fz, u, v, valid);
+    u = fminf(fmaxf(u, 0.1f), Wm);
+    v = fminf(fmaxf(v, 0.1f), Hm);
+    a.sampling[(size_t)r * P + idx] = make_float2(u, v);
+    bw[r] = 1e-8f;
+    const float tx = a.trans[r][0], ty = a.trans[r][1], tz = a.trans[r][2];
+    float nu, nv, fu, fv;
+    project_c<true>(a.intr[r], fx * a.mn - tx, fy * a.mn - ty, fz * a.mn - tz, nu, nv, valid);
+    nu = fminf(fmaxf(nu, 0.1f), Wm);
+    nv = (float)(r * a.H) + fminf(fmaxf(nv, 0.1f), Hm);
+    project_c<true>(a.intr[r], fx * a.mx - tx, fy * a.mx - ty, fz * a.mx - tz, fu, fv, valid);
+    fu = fminf(fmaxf(fu, 0.1f), Wm);
+    fv = (float)(r * a.H) + fminf(fmaxf(fv, 0.1f), Hm);
+    if (valid && (double)interp_f(a.masks, a.W, nu, nv) > 0.99 && (double)interp_f(a.masks, a.W, fu, fv) > 0.99) {
+      const float dx = fu - nu, dy = fv - nv;
+      const float ds = sqrtf(dx * dx + dy * dy);
+      bw[r] = expf_cw((float)((double)(-ds * ds) / (1e-4 * a.H * a.W)));
+    }
+    bsum = bsum + bw[r];
+  }
+#pragma unroll
+  for (int r = 0; r < SS_MAX_REFERENCES; ++r) {
+    if (r >= a.R) break;
+    a.weights[(size_t)r * P + idx] = bw[r] / bsum;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_normalize(float* __restrict__ w, int R, int P) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= P) return;
+  float s = 0.0f;
+  for (int r = 0; r < R; ++r) s = s + w[(size_t)r * P + p];
+  for (int r = 0; r < R; ++r) w[(size_t)r * P + p] = w[(size_t)r * P + p] / s;
+}
+
+// mergeRGBDPanoramaKernel (stitcher.cu:423-474)
+__global__ void __launch_bounds__(kBlock) k_merge(MergeArgs a) {
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  const int P = a.PH * a.PW;
+  if (idx >= P) return;
+  const float rx = (float)a.Ws / (float)a.W, ry = (float)a.Hs / (float)a.H;
+  const float Wsm = (float)a.Ws - 1.1f, Hsm = (float)a.Hs - 1.1f;
+  float inv_sum = 0.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+#pragma unroll
+  for (int r = 0; r < SS_MAX_REFERENCES; ++r) {
+    if (r >= a.R) break;
+    const float bw = a.weights[(size_t)r * P + idx];
+    const float2 uv = a.sampling[(size_t)r * P + idx];
+    // the reference offsets v into the stacked maps (an fp32 add), then samples
+    const float vo = uv.y + (float)(r * a.H);
+    const float* rep = a.reproj[r] - (size_t)r * a.H * a.W;
+    const float* dst = a.dists[r] - (size_t)r * a.H * a.W;
+    const float rd = interp_f(rep, a.W, uv.x, vo);
+    inv_sum = inv_sum + (bw * 1.0f) / rd;
+    float dc = interp_f(dst, a.W, uv.x, vo);
+    dc = fminf(rd, dc);
+    float px, py, pz, u, v;
+    bool valid = true;
+    unproject_c(a.intr[r], uv.x, uv.y, px, py, pz);
+    px = px * dc + a.trans[r][0];
+    py = py * dc + a.trans[r][1];
+    pz = pz * dc + a.trans[r][2];
+    project_c<false>(a.intr[r], px, py, pz, u, v, valid);
+    u = u * rx;
+    v = v * ry;
+    u = fminf(fmaxf(u, 0.1f), Wsm);
+    v = fminf(fmaxf(v, 0.1f), Hsm);
+    v = v + (float)(r * a.Hs);
+    // bilinear uint8 read (stitcher.cu:56-83) on the stacked images
+    const uint8_t* img = a.imgs[r] - (size_t)r * a.Hs * a.Ws * 3;
+    const int u1 = (int)floorf(u), v1 = (int)floorf(v), u2 = u1 + 1, v2 = v1 + 1;
+    const float u1f = (float)u1, u2f = (float)u2, v1f = (float)v1, v2f = (float)v2;
+    const float w1 = (u2f - u) * (v2f - v), w2 = (u2f - u) * (v - v1f), w3 = (u - u1f) * (v2f - v),
+                w4 = (u - u1f) * (v - v1f);
+    const uint8_t* p1 = img + ((size_t)v1 * a.Ws + u1) * 3;
+    const uint8_t* p2 = img + ((size_t)v2 * a.Ws + u1) * 3;
+    const uint8_t* p3 = img + ((size_t)v1 * a.Ws + u2) * 3;
+    const uint8_t* p4 = img + ((size_t)v2 * a.Ws + u2) * 3;
+    float s0 = (((float)p1[0] * w1 + (float)p2[0] * w2) + (float)p3[0] * w3) + (float)p4[0] * w4;
+    float s1 = (((float)p1[1] * w1 + (float)p2[1] * w2) + (float)p3[1] * w3) + (float)p4[1] * w4;
+    float s2 = (((float)p1[2] * w1 + (float)p2[2] * w2) + (float)p3[2] * w3) + (float)p4[2] * w4;
+    c0 = c0 + s0 * bw;
+    c1 = c1 + s1 * bw;
+    c2 = c2 + s2 * bw;
+  }
+  a.dist_out[idx] = 1.0f / inv_sum;
+  a.rgb_out[3 * idx] = (uint8_t)c0;
+  a.rgb_out[3 * idx + 1] = (uint8_t)c1;
+  a.rgb_out[3 * idx + 2] = (uint8_t)c2;
+}
+
+__global__ void __launch_bounds__(kBlock) k_to_u8(const float* __restrict__ in, uint8_t* __restrict__ out, size_t n) {
+  const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) out[i] = (uint8_t)in[i];
+}
+
+}  // namespace
+
+hipError_t launch_fill(float* p, float v, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill, dim3(blocks_for((long long)n)), dim3(kBlock), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_reproject(const float* din, float* dout, int H, int W, const ss_intrinsics& c, const float* t,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_reproject, dim3(blocks_for((long long)H * W)), dim3(kBlock), 0, s, din, dout, H, W, c, t[0],
+                     t[1], t[2]);
+  return hipGetLastError();
+}
+
+hipError_t launch_inpaint_step(const float* din, float* dout, const uint8_t* lut, int H, int W, double thr_empty,
+                               double thr_src, hipStream_t s) {
+  hipLaunchKernelGGL(k_inpaint_step, dim3(blocks_for((long long)H * W)), dim3(kBlock), 0, s, din, dout,
+                     reinterpret_cast<const uchar2*>(lut), H, W, thr_empty, thr_src);
+  return hipGetLastError();
+}
+
+hipError_t launch_inpaint_lut(const ss_intrinsics& c, const float* t, int H, int W, float mn, float mx, uint8_t* lut,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_inpaint_lut, dim3(blocks_for((long long)H * W)), dim3(kBlock), 0, s, c, t[0], t[1], t[2], H,
+                     W, mn, mx, reinterpret_cast<uchar2*>(lut));
+  return hipGetLastError();
+}
+
+hipError_t launch_box_filter(const float* in, float* out, float* tmp, int B, int H, int W, int r, float pad,
+                             hipStream_t s) {
+  const float k = 1.0f / (float)((2 * r + 1) * (2 * r + 1));
+  for (int b = 0; b < B; ++b) {
+    hipLaunchKernelGGL(k_box_rows, dim3(blocks_for((long long)(H + 2 * r) * W)), dim3(kBlock), 0, s,
+                       in + (size_t)b * H * W, tmp, H, W, r, pad);
+    hipLaunchKernelGGL(k_box_cols, dim3(blocks_for((long long)H * W)), dim3(kBlock), 0, s, tmp,
+                       out + (size_t)b * H * W, H, W, r, k);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_blend_lut(const BlendArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_blend_lut, dim3(blocks_for((long long)a.PH * a.PW)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_normalize(float* w, int R, int P, hipStream_t s) {
+  hipLaunchKernelGGL(k_normalize, dim3(blocks_for(P)), dim3(kBlock), 0, s, w, R, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_merge(const MergeArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_merge, dim3(blocks_for((long long)a.PH * a.PW)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_u8(const float* in, uint8_t* out, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_u8, dim3(blocks_for((long long)n)), dim3(kBlock), 0, s, in, out, n);
+  return hipGetLastError();
+}
+
+}  // namespace ss

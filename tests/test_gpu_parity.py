@@ -1,0 +1,134 @@
+"""Parity of every HIP stage with the CPU oracle (bit-exact), through the C ABI.
+
+Sizes are small enough for the oracle to finish in seconds; the odd shapes
+exercise the border clamps and the setPlusX/setPlusY guards of the filter.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import first_mismatch, frame, to_np
+from oracle import Oracle
+from pipeline import OraclePipeline
+from sphere_stereo import ISB_Filter, RGBD_Estimator, _lib
+from sphere_stereo.utils import camera_array, distance_candidates, relative_transforms
+
+pytestmark = pytest.mark.gpu
+
+O32 = Oracle("f32")
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def assert_same(a, b, what):
+    msg = first_mismatch(a, b)
+    assert msg is None, "%s: %s" % (what, msg)
+
+
+def test_guide_bit_exact(gpu):
+    g = torch.Generator().manual_seed(3)
+    img = (torch.rand([37, 53, 3], generator=g) * 255).to(gpu)
+    out = torch.empty([37, 53], dtype=torch.int32, device=gpu)
+    _lib.check(_lib.lib.ss_guide(img.data_ptr(), 37, 53, out.data_ptr(), _stream()))
+    packed = to_np(out).view(np.uint32)
+    got = np.stack([(packed >> (8 * c)) & 255 for c in range(3)], -1).astype(np.uint8)
+    assert_same(got, O32.guide(to_np(img)), "guide")
+
+
+@pytest.mark.parametrize("shape,N", [((8, 8), 1), ((9, 13), 3), ((33, 17), 8), ((48, 64), 32), ((31, 45), 48),
+                                     ((16, 20), 64), ((24, 40), 128)])
+def test_isb_filter_planar_bit_exact(gpu, shape, N):
+    H, W = shape
+    g = torch.Generator().manual_seed(H * 131 + W + N)
+    guide = torch.randint(0, 256, [H, W, 3], generator=g, dtype=torch.uint8)
+    # piecewise-smooth guide so that the edge weights vary
+    guide[: H // 2] = guide[: H // 2] // 8 + 100
+    cost = torch.rand([N, H, W], generator=g) * 500
+    want_c, want_g = O32.isb_filter(guide.numpy(), cost.numpy(), 10.0, 25.0)
+    f = ISB_Filter(N, (W, H), gpu)
+    gd, cd = guide.to(gpu), cost.to(gpu)
+    rc, rg = f.apply(gd, cd, 10.0, 25.0)
+    assert rc.data_ptr() == cd.data_ptr() and rg.data_ptr() == gd.data_ptr()
+    assert_same(to_np(gd), want_g, "guide")
+    assert_same(to_np(cd), want_c, "cost")
+
+
+def test_isb_filter_rejects_tiny(gpu):
+    with pytest.raises(ValueError):
+        ISB_Filter(4, (3, 3), gpu)
+
+
+@pytest.fixture(scope="module")
+def small(gpu):
+    fr = frame(seed=1, resolution=160, matching=(80, 72), stitch=(96, 96), refs=(0, 2), device=gpu, baseline=0.1)
+    kw = dict(min_dist=0.55, max_dist=100, candidate_count=32, references_indices=fr["refs"],
+              reprojection_viewpoint=fr["viewpoint"], masks=fr["masks"], matching_resolution=fr["matching"],
+              rgb_to_stitch_resolution=fr["stitch"], panorama_resolution=(128, 64), sigma_i=10, sigma_s=12.5)
+    est = RGBD_Estimator(fr["calibs"], device=gpu, **kw)
+    ora = OraclePipeline(fr["calibs"], **kw)
+    return fr, est, ora
+
+
+def test_select_camera_bit_exact(small):
+    fr, est, ora = small
+    for slot, r in enumerate(fr["refs"]):
+        got = to_np(est.selected_cameras[slot][0])
+        assert_same(got, ora.sel[r], "selected camera ref %d" % r)
+        assert not np.any(got == r), "the reference camera is never selected"
+
+
+def test_sweep_bit_exact(small, gpu):
+    fr, est, ora = small
+    H, W = 72, 80
+    N = 32
+    NP = _lib.lib.ss_padded_candidates(N)
+    for slot, r in enumerate(fr["refs"]):
+        cost = torch.empty([H, W, NP], device=gpu)
+        imgs = [i.contiguous() for i in fr["to_match"]]
+        _lib.check(_lib.lib.ss_sweep(est._raylut[slot].data_ptr(), _lib.ptr_array(imgs), len(imgs), est._cams,
+                                     est._rts[r], r, est._cands, N, H, W, cost.data_ptr(), _stream()))
+        want = O32.sweep(np.stack([to_np(i) for i in imgs]), ora.sel[r], ora.cands, ora.cams, ora.rts[r], r)
+        assert_same(np.transpose(to_np(cost)[..., :N], (2, 0, 1)), want, "cost volume ref %d" % r)
+
+
+def test_estimate_distance_bit_exact(small):
+    fr, est, ora = small
+    for slot, r in enumerate(fr["refs"]):
+        keep = {}
+        want = ora.estimate_fisheye_distance(fr["to_match"], r, keep=keep)
+        calib = fr["calibs"][r]
+        got = est.estimate_fisheye_distance(None, None, calib, None, fr["to_match"])
+        assert_same(to_np(got), want, "distance map ref %d" % r)
+
+
+def test_stitch_tables_bit_exact(small):
+    fr, est, ora = small
+    st = est.fishey_stitcher
+    for i in range(len(fr["refs"])):
+        assert_same(to_np(st.inpainting_weights_list[i]), ora.luts[i], "inpaint LUT %d" % i)
+    assert_same(to_np(st.blending_sampling), ora.sampling, "blend sampling")
+    assert_same(to_np(st.blending_weights), ora.weights, "blend weights")
+
+
+def test_panorama_bit_exact(small):
+    fr, est, ora = small
+    keep = {}
+    want_rgb, want_dist = ora.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"], keep=keep)
+    rgb, dist = est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])
+    for i in range(len(fr["refs"])):
+        assert_same(to_np(est._distance_maps[i]), keep["dmaps"][i], "distance map %d" % i)
+        assert_same(to_np(est.fishey_stitcher.reprojected_distances[i]), keep["reproj"][i], "reprojected %d" % i)
+    assert_same(to_np(dist), want_dist, "distance panorama")
+    assert_same(to_np(rgb), want_rgb, "rgb panorama")
+
+
+def test_panorama_deterministic(small):
+    fr, est, ora = small
+    rgb1, dist1 = [t.clone() for t in est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])]
+    rgb2, dist2 = est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])
+    assert torch.equal(rgb1, rgb2)
+    assert torch.equal(dist1.view(torch.int32), dist2.view(torch.int32))
