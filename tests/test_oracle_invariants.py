@@ -1,0 +1,386 @@
+"""Closed forms, invariants and independent restatements that pin the oracle
+where the reference's CUDA kernels (isb_filter.cu, stitcher.cu) cannot run.
+
+Each check is chosen so that a plausible slip in the oracle (a wrong
+neighbour or base weight, a transposed index, a missing clamp, a dropped term)
+fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.ndimage
+
+from oracle import Oracle
+
+O32, O64 = Oracle("f32"), Oracle("f64")
+
+
+# ---------------------------------------------------------------------------
+# exp (Cody-Waite + Cephes polynomial)
+# ---------------------------------------------------------------------------
+def _ulp32(x):
+    return np.spacing(np.abs(x).astype(np.float32)).astype(np.float64)
+
+
+def test_exp_normal_range_within_2ulp():
+    x = np.concatenate([np.linspace(-87.3, 88.7, 20001), -np.logspace(-8, 1.9, 3001)]).astype(np.float32)
+    got = O32.expf(x).astype(np.float64)
+    ref = np.exp(x.astype(np.float64))
+    err = np.abs(got - ref) / _ulp32(ref)
+    assert err.max() <= 2.0, err.max()
+
+
+def test_exp_subnormal_and_limits():
+    x = np.linspace(-103.9, -87.4, 2001).astype(np.float32)
+    got = O32.expf(x).astype(np.float64)
+    ref = np.exp(x.astype(np.float64))
+    assert np.all(got > 0)
+    assert np.abs(got - ref).max() <= 2 * 2.0 ** -149
+    assert O32.expf([-105.0])[0] == 0.0
+    assert np.isinf(O32.expf([89.0])[0])
+    assert O32.expf([0.0])[0] == 1.0
+
+
+# ---------------------------------------------------------------------------
+# double-sphere model
+# ---------------------------------------------------------------------------
+PINHOLE = [300.0, 290.0, 320.5, 240.25, 0.0, 0.0]  # fs, cs, xi = alpha = 0
+DS = [364.8 * 1024 / 1216, 364.8 * 1024 / 1216, 512.0, 512.0, -0.18, 0.59]
+
+
+@pytest.mark.parametrize("o,tol", [(O64, 1e-9), (O32, 2e-3)])
+def test_pinhole_reduction(o, tol):
+    rng = np.random.RandomState(0)
+    pts = rng.randn(500, 3)
+    pts[:, 2] = np.abs(pts[:, 2]) + 0.5
+    want = np.stack([300.0 * pts[:, 0] / pts[:, 2] + 320.5, 290.0 * pts[:, 1] / pts[:, 2] + 240.25], -1)
+    uv, val = o.project_torch(PINHOLE, pts)
+    assert np.abs(uv - want).max() < tol * 10 and val.all()
+    uvc = o.project_cuda(PINHOLE, pts)
+    assert np.abs(uvc - want).max() < tol * 10
+    uvs = rng.rand(500, 2) * [640, 480]
+    p, _ = o.unproject_torch(PINHOLE, uvs)
+    ray = np.stack([(uvs[:, 0] - 320.5) / 300.0, (uvs[:, 1] - 240.25) / 290.0, np.ones(500)], -1)
+    ray /= np.linalg.norm(ray, axis=-1, keepdims=True)
+    assert np.abs(p - ray).max() < tol
+    assert np.abs(o.unproject_cuda(PINHOLE, uvs) - ray).max() < tol
+
+
+@pytest.mark.parametrize("o,tol", [(O64, 1e-8), (O32, 5e-3)])
+def test_round_trip_and_homogeneity(o, tol):
+    rng = np.random.RandomState(1)
+    uv = 512 + (rng.rand(2000, 2) - 0.5) * 1000
+    p, val = o.unproject_torch(DS, uv)
+    back, _ = o.project_torch(DS, p[val])
+    assert np.abs(back - uv[val]).max() < tol
+    back2, _ = o.project_torch(DS, p[val] * 37.5)
+    assert np.abs(back2 - uv[val]).max() < tol
+    pc = o.unproject_cuda(DS, uv[val])
+    assert np.abs(o.project_cuda(DS, pc * 3.0) - uv[val]).max() < tol
+    assert np.abs(np.linalg.norm(p, axis=-1) - 1).max() < 1e-6
+
+
+def test_projection_validity_boundary():
+    # the DS model is valid for z > -w2 * |p| (utils.py:86-92)
+    al, xi = DS[5], DS[4]
+    w1 = (1 - al) / al
+    w2 = (w1 + xi) / math.sqrt(2 * w1 * xi + xi * xi + 1)
+    ang = math.acos(-w2)
+    th = np.array([ang - 1e-3, ang + 1e-3])
+    pts = np.stack([np.sin(th), np.zeros(2), np.cos(th)], -1)
+    _, val = O64.project_torch(DS, pts)
+    assert val.tolist() == [True, False]
+    _, val = O64.project_cuda(DS, pts, with_valid=True)
+    assert val.tolist() == [True, False]
+
+
+# ---------------------------------------------------------------------------
+# ISB filter
+# ---------------------------------------------------------------------------
+def _levels(W, H):
+    return int(min(math.log2(W), math.log2(H)) - 1)
+
+
+def _clampi(i, n):
+    return np.clip(i, 0, n - 1)
+
+
+def stencil_pyramid_np(cost, sigma_s):
+    """The filter for a guide whose every pixel is 0 (all range weights equal
+    exp(0) + 0.01), written independently as gathers over index grids."""
+    N, H, W = cost.shape
+    L = _levels(W, H)
+    e = 1.01
+    vs = 1 / (2 * sigma_s * sigma_s)
+    lev = [cost.astype(np.float64)]
+    for s in range(1, L):
+        src = lev[-1]
+        h, w = src.shape[1:]
+        ho, wo = math.ceil(H / 2 ** s), math.ceil(W / 2 ** s)
+        y, x = np.meshgrid(np.arange(ho), np.arange(wo), indexing="ij")
+        acc, ws = np.zeros((N, ho, wo)), 0.0
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                b = 1.0 if (dx, dy) == (0, 0) else (0.5 * e if dx == 0 or dy == 0 else 0.25 * e)
+                acc += b * src[:, _clampi(2 * y + dy, h), _clampi(2 * x + dx, w)]
+                ws += b
+        lev.append(acc / ws)
+    for s in range(L - 2, -1, -1):
+        fine, coarse = lev[s].copy(), lev[s + 1]
+        hc, wc = coarse.shape[1:]
+        hf, wf = fine.shape[1:]
+        wd = math.exp(-((2 ** s - 0.5) ** 2) * vs)
+        wu = 1 - wd
+        y, x = np.meshgrid(np.arange(hc), np.arange(wc), indexing="ij")
+
+        def c(dy, dx):
+            return coarse[:, _clampi(y + dy, hc), _clampi(x + dx, wc)]
+
+        taps = {
+            (0, 0): [((dy, dx), 8.0 if (dy, dx) == (0, 0) else 1.0) for dy in (-1, 0, 1) for dx in (-1, 0, 1)],
+            (1, 0): [((0, 0), 8.0), ((1, 0), 8.0), ((0, -1), 1.0), ((1, -1), 1.0), ((0, 1), 1.0), ((1, 1), 1.0)],
+            (0, 1): [((0, 0), 8.0), ((0, 1), 8.0), ((-1, 0), 1.0), ((-1, 1), 1.0), ((1, 0), 1.0), ((1, 1), 1.0)],
+            (1, 1): [((0, 0), 1.0), ((0, 1), 1.0), ((1, 0), 1.0), ((1, 1), 1.0)],
+        }
+        out = fine.copy()
+        for (oy, ox), tl in taps.items():
+            fy, fx = 2 * y + oy, 2 * x + ox
+            ok = (fy < hf) & (fx < wf)
+            acc = wu * fine[:, np.minimum(fy, hf - 1), np.minimum(fx, wf - 1)]
+            ws = wu
+            for (dy, dx), b in tl:
+                acc = acc + b * wd * e * c(dy, dx)
+                ws += b * wd * e
+            out[:, fy[ok], fx[ok]] = (acc / ws)[:, ok]
+        lev[s] = out
+    return lev[0]
+
+
+@pytest.mark.parametrize("H,W", [(8, 8), (9, 13), (16, 33), (40, 24), (5, 64)])
+def test_isb_constant_guide_is_fixed_stencil(H, W):
+    rng = np.random.RandomState(H * W)
+    cost = rng.rand(3, H, W) * 100
+    guide = np.zeros((H, W, 3), np.uint8)
+    got, g = O64.isb_filter(guide, cost, 10.0, 25.0)
+    want = stencil_pyramid_np(cost, 25.0)
+    assert np.abs(got - want).max() < 1e-9 * 100
+    assert not g.any()
+    got32, _ = O32.isb_filter(guide, cost.astype(np.float32), 10.0, 25.0)
+    assert np.abs(got32 - want).max() < 1e-3
+
+
+def test_isb_permutation_is_exact_and_linearity():
+    rng = np.random.RandomState(5)
+    H, W, N = 23, 31, 6
+    guide = rng.randint(0, 255, (H, W, 3)).astype(np.uint8)
+    guide[:, : W // 2] = 40
+    c1, c2 = (rng.rand(N, H, W) * 100).astype(np.float32), (rng.rand(N, H, W) * 100).astype(np.float32)
+    f1, g1 = O32.isb_filter(guide, c1, 10.0, 25.0)
+    perm = rng.permutation(N)
+    fp, gp = O32.isb_filter(guide, c1[perm], 10.0, 25.0)
+    assert np.array_equal(fp, f1[perm]) and np.array_equal(gp, g1)
+    d1, _ = O64.isb_filter(guide, c1, 10.0, 25.0)
+    d2, _ = O64.isb_filter(guide, c2, 10.0, 25.0)
+    d12, _ = O64.isb_filter(guide, 2.0 * c1.astype(np.float64) - 0.5 * c2.astype(np.float64), 10.0, 25.0)
+    assert np.abs(d12 - (2 * d1 - 0.5 * d2)).max() < 1e-10
+    const, _ = O32.isb_filter(guide, np.full((N, H, W), 123.25, np.float32), 10.0, 25.0)
+    assert np.abs(const - 123.25).max() < 123.25 * 1e-5
+
+
+def test_isb_edge_preservation():
+    # two flat guide regions: a cost step aligned with the guide edge survives,
+    # the same step across a flat guide is smoothed away
+    H, W = 32, 32
+    guide = np.zeros((H, W, 3), np.uint8)
+    guide[:, 16:] = 200
+    cost = np.zeros((1, H, W))
+    cost[0, :, 16:] = 100.0
+    edge, _ = O64.isb_filter(guide, cost, 10.0, 25.0)
+    flat, _ = O64.isb_filter(np.zeros_like(guide), cost, 10.0, 25.0)
+    assert edge[0, :, 15].max() < 5.0 and edge[0, :, 16].min() > 95.0
+    assert flat[0, :, 15].min() > 20.0
+
+
+def test_isb_constant_guide_bytes_truncate_by_at_most_one():
+    g = np.full((16, 16, 3), 137, np.uint8)
+    _, out = O32.isb_filter(g, np.zeros((1, 16, 16), np.float32), 10.0, 25.0)
+    assert out.min() >= 137 - 2 and out.max() <= 137
+
+
+def test_levels_formula():
+    for W, H in [(4, 4), (5, 7), (8, 8), (512, 512), (768, 768), (1024, 1024), (1216, 1024), (2048, 1024)]:
+        assert O32.levels(W, H) == _levels(W, H)
+    with pytest.raises(ValueError):
+        O32.isb_filter(np.zeros((3, 3, 3), np.uint8), np.zeros((1, 3, 3), np.float32), 10, 25)
+
+
+# ---------------------------------------------------------------------------
+# argmin + refinement
+# ---------------------------------------------------------------------------
+def _cands(N, lo=0.55, hi=100.0):
+    c = 1 / np.linspace(1 / lo, 1 / hi, N)
+    return c, c[0] / c[-1] - 1
+
+
+def test_argmin_exact_parabola_recovers_subindex():
+    N = 32
+    c, A = _cands(N)
+    i0 = np.linspace(0.6, N - 1.6, 200)
+    n = np.arange(N)[:, None]
+    vol = 3.0 * (n - i0[None]) ** 2 + 2.0
+    d, idx = O64.argmin_refine(vol[:, None, :], c[0], c[-1], A)
+    want = c[0] / (A * i0 / (N - 1) + 1)
+    assert np.abs(d[0] - want).max() < 1e-9 * want.max()
+    assert np.array_equal(idx[0], np.rint(i0).astype(int))
+
+
+def test_argmin_ties_constant_and_endpoints():
+    N = 8
+    c, A = _cands(N)
+    vol = np.full((N, 1, 4), 10.0)
+    vol[3, 0, 0] = vol[5, 0, 0] = 1.0  # tie: first index wins
+    vol[0, 0, 1] = 0.5  # minimum at 0: no refinement
+    vol[N - 1, 0, 2] = 0.5  # minimum at N-1: no refinement
+    d, idx = O64.argmin_refine(vol, c[0], c[-1], A)
+    assert idx[0].tolist()[:3] == [3, 0, N - 1]
+    assert d[0, 1] == c[0]
+    assert abs(d[0, 2] - c[-1]) < 1e-12 * c[-1]
+    assert d[0, 3] == c[-1]  # constant volume (max == min) -> farthest candidate
+
+
+# ---------------------------------------------------------------------------
+# stitcher
+# ---------------------------------------------------------------------------
+INTR = [0.3 * 256, 0.3 * 256, 128.0, 128.0, -0.18, 0.59]
+
+
+def test_reproject_without_translation_is_identity():
+    H = W = 256
+    rng = np.random.RandomState(3)
+    d = (1 + rng.rand(H, W) * 20).astype(np.float32)
+    out = O32.reproject_zmin(d, INTR, [0, 0, 0])
+    inner = np.s_[20:-20, 20:-20]
+    close = np.abs(out[inner] - d[inner]) <= 2e-6 * d[inner]
+    assert close.mean() > 0.999
+
+
+def test_reproject_keeps_the_nearest():
+    # a translation along the optical axis maps several source pixels to one
+    # target; every target holds the minimum of the distances sent to it
+    H = W = 64
+    intr = [0.3 * 64, 0.3 * 64, 32.0, 32.0, -0.18, 0.59]
+    d = np.linspace(2, 30, H * W).reshape(H, W)
+    t = [0.05, -0.02, 0.3]
+    out = O64.reproject_zmin(d, intr, t)
+    u, v = np.meshgrid(np.arange(W), np.arange(H))
+    p = O64.unproject_cuda(intr, np.stack([u, v], -1).reshape(-1, 2)) * d.reshape(-1, 1) - np.array(t)
+    uv = O64.project_cuda(intr, p)
+    dist = np.linalg.norm(p, axis=-1)
+    idx = np.rint(uv[:, 1]).astype(np.int64) * W + np.rint(uv[:, 0]).astype(np.int64)
+    want = np.full(H * W, 1e8)
+    ok = (idx >= 0) & (idx < H * W)
+    np.minimum.at(want, idx[ok], dist[ok])
+    assert np.array_equal(out.reshape(-1), want)
+
+
+def _thr(max_dist=100):
+    return float(max_dist) + 0.1, float(np.float32(max_dist) + np.float32(0.1))
+
+
+def test_inpaint_fills_isolated_hole_with_lut_mean_and_keeps_full_maps():
+    H = W = 64
+    lut = O32.inpaint_lut(INTR[:2] + [32.0, 32.0] + INTR[4:], [0.1, 0.0, 0.0], H, W, 0.55, 100)
+    rng = np.random.RandomState(4)
+    d = (1 + rng.rand(H, W) * 10).astype(np.float32)
+    te, ts = _thr()
+    assert np.array_equal(O32.inpaint_step(d, lut, te, ts), d)
+    d2 = d.copy()
+    y, x = 30, 33
+    d2[y, x] = 1e8
+    out = O32.inpaint_step(d2, lut, te, ts)
+    num = den = 0.0
+    for s in range(2):
+        b = int(lut[y, x, s])
+        w = float(b & 240)
+        val = d2[y + ((b & 12) >> 2) - 1, x + (b & 3) - 1]
+        num, den = num + w * val, den + w
+    assert den > 0 and abs(out[y, x] - num / den) <= 1e-6 * out[y, x]
+    d3 = d.copy()
+    d3[0, 5] = 1e8
+    assert O32.inpaint_step(d3, lut, te, ts)[0, 5] == np.float32(1e8)  # border never filled
+
+
+def test_inpaint_lut_picks_the_two_best_aligned_neighbours():
+    H = W = 48
+    intr = [0.3 * 48, 0.3 * 48, 24.0, 24.0, -0.18, 0.59]
+    t = [0.12, -0.03, 0.05]
+    lut = O64.inpaint_lut(intr, t, H, W, 0.55, 100)
+    u, v = np.meshgrid(np.arange(W), np.arange(H))
+    unit = O64.unproject_cuda(intr, np.stack([u, v], -1).reshape(-1, 2))
+    near = O64.project_cuda(intr, unit * 0.55 - np.array(t))
+    far = O64.project_cuda(intr, unit * 100 - np.array(t))
+    dv = far - near
+    dv /= np.linalg.norm(dv, axis=-1, keepdims=True)
+    dirs = np.array([(n, m) for n in (-1, 0, 1) for m in (-1, 0, 1) if n or m], float)
+    cosv = dv @ (dirs / np.linalg.norm(dirs, axis=-1, keepdims=True)).T
+    order = np.argsort(-cosv, axis=-1)
+    for i in range(H * W):
+        top = cosv[i, order[i]]
+        if top[1] - top[2] < 1e-9 or top[0] - top[1] < 1e-9 or top[1] <= 0:
+            continue
+        b = lut.reshape(-1, 2)[i]
+        got = {((int(bb) & 3) - 1, ((int(bb) & 12) >> 2) - 1) for bb in b}
+        want = {tuple(dirs[order[i, 0]].astype(int)), tuple(dirs[order[i, 1]].astype(int))}
+        assert got == want, i
+        assert int(b[1]) & 240 == (int(top[0] * 255) & 240)
+
+
+@pytest.mark.parametrize("pad", [0.0, 1.0])
+def test_box_filter_matches_uniform_filter(pad):
+    rng = np.random.RandomState(6)
+    x = rng.rand(2, 37, 53)
+    got = O64.box_filter(x, 7, pad)
+    want = np.stack([scipy.ndimage.uniform_filter(x[b], size=15, mode="constant", cval=pad) for b in range(2)])
+    assert np.abs(got - want).max() < 1e-12
+    got32 = O32.box_filter(x.astype(np.float32), 7, pad)
+    assert np.abs(got32 - want).max() < 1e-5
+
+
+def _rig2():
+    intr = np.array([[0.3 * 96, 0.3 * 96, 48.0, 48.0, -0.18, 0.59]] * 2)
+    c, s = math.cos(math.pi / 4), math.sin(math.pi / 4)
+    r0 = np.array([[c, 0, -s], [0, 1, 0], [s, 0, c]])  # inverse of a +45 deg yaw
+    rot = np.stack([r0.reshape(-1), r0.T.reshape(-1)])
+    trans = np.array([[0.0, 0.0, -0.05], [0.0, 0.0, -0.05]])
+    return intr, rot, trans
+
+
+def test_blend_lut_weights_sum_to_one_and_sampling_projects():
+    intr, rot, trans = _rig2()
+    masks = np.ones((2, 96, 96))
+    PH, PW = 24, 48
+    samp, raw = O64.blend_lut(intr, rot, trans, masks, PH, PW, 0.55, 100)
+    assert np.abs(raw.sum(0) - 1).max() < 1e-12
+    w = O64.normalize_weights(O64.box_filter(raw, 3, 0.0))
+    assert np.abs(w.sum(0) - 1).max() < 1e-12
+    y, x = np.meshgrid(np.arange(PH), np.arange(PW), indexing="ij")
+    phi = (y + 0.5) * 3.14159265 / PH - 3.14159265 / 2
+    th = (x + 0.5) * 2 * 3.14159265 / PW + 3.14159265
+    unit = np.stack([np.cos(phi) * np.sin(th), np.sin(phi), np.cos(phi) * np.cos(th)], -1).reshape(-1, 3)
+    for r in range(2):
+        uv = O64.project_cuda(intr[r], unit @ rot[r].reshape(3, 3).T)
+        uv = np.stack([np.clip(uv[:, 0], 0.1, 96 - 1.1), np.clip(uv[:, 1], 0.1, 96 - 1.1)], -1)
+        assert np.abs(samp[r].reshape(-1, 2) - uv).max() < 1e-9
+
+
+def test_merge_of_constant_maps_and_images():
+    intr, rot, trans = _rig2()
+    PH, PW = 24, 48
+    samp, raw = O32.blend_lut(intr, rot, trans, np.ones((2, 96, 96), np.float32), PH, PW, 0.55, 100)
+    w = O32.normalize_weights(O32.box_filter(raw, 3, 0.0))
+    maps = np.full((2, 96, 96), 5.0, np.float32)
+    imgs = np.full((2, 120, 120, 3), 100, np.uint8)
+    dist, rgb = O32.merge(samp, w, maps, maps, imgs, trans, intr)
+    assert np.abs(dist - 5.0).max() < 5e-6
+    assert rgb.min() >= 99 and rgb.max() <= 100
