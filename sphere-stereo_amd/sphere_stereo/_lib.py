@@ -62,6 +62,12 @@ class SphereStereoError(RuntimeError):
 
 
 def _load():
+    # torch first: it brings its own HIP runtime (libamdhip64.so.7), and the
+    # library must bind to that same runtime by soname.  Loading the library
+    # first would map /opt/rocm's runtime as well, and two HIP runtimes in one
+    # process do not share devices, streams or allocations.
+    import torch  # noqa: F401
+
     if not os.path.exists(LIB_PATH):
         raise ImportError("libsphere_stereo.so is not built (%s); run __graft_entry__.build() or "
                           "python -m sphere_stereo._build" % LIB_PATH)
