@@ -72,14 +72,21 @@ def algorithmic_bytes(stage, wl, C=4):
     return 0
 
 
-def setup(wl, device):
+def setup(wl, device, frames_cache=None):
     import torch
 
     from sphere_stereo import RGBD_Estimator, synth
     from sphere_stereo.utils import parse_json_calib
 
     rig = synth.SyntheticRig(seed=wl["seed"], resolution=wl["resolution"])
-    rendered = rig.render(device)
+    if frames_cache and os.path.exists(frames_cache):
+        saved = torch.load(frames_cache, weights_only=True)
+        rendered = (saved["images"].to(device), saved["masks"].to(device))
+    else:
+        rendered = rig.render(device)
+        if frames_cache:
+            torch.save({"images": rendered[0].cpu(), "masks": rendered[1].cpu()}, frames_cache)
+    print("[bench] rig rendered", file=sys.stderr, flush=True)
     calibs = parse_json_calib(rig.basalt_json()["value0"], list(wl["matching"]), device)
     viewpoint = torch.zeros([3], device=device)
     for r in wl["refs"]:
@@ -151,6 +158,7 @@ def main():
     ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    ap.add_argument("--frames-cache", default=None, help="reuse rendered fisheyes saved at this path")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would run its own frames "
@@ -165,7 +173,7 @@ def main():
     wl = WORKLOADS[args.workload]
     device = torch.device("cuda:0")
     print("[bench] setting up %s" % args.workload, file=sys.stderr, flush=True)
-    est, to_match, to_stitch, calibs, kw = setup(wl, device)
+    est, to_match, to_stitch, calibs, kw = setup(wl, device, args.frames_cache)
     for _ in range(args.warmup):
         est.estimate_RGBD_panorama(to_match, to_stitch)
     torch.cuda.synchronize()
