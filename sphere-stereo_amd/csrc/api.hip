@@ -131,15 +131,66 @@ int run_pyramid(Level* lv, int L, int G, int V, double sigma_i, double sigma_s, 
   const double var_inv_s = 1 / (2 * sigma_s * sigma_s);
   const double var_inv_i = 1 / (2 * sigma_i * sigma_i);
   const float vi = (float)var_inv_i;
-  for (int k = 1; k < L; ++k) {
+  // up-pass weights into level k (isb_filter.py:110-112)
+  auto weights = [&](int k, double& wu, double& wd) {
+    const double d = pow(2.0, k) - 0.5;
+    wd = exp(-(d * d) * var_inv_s);
+    wu = 1 - wd;
+  };
+  // The coarse end runs in one workgroup from level s0 on: every level it
+  // produces holds at most kTailPixels pixels (one CU then does each pass in
+  // about a microsecond) and all of them (guide + NP values per pixel) fit in
+  // the LDS budget.
+  const long long kTailPixels = 256;
+  const int NP = G * V;
+  auto lds_need = [&](int from) {
+    size_t b = 0;
+    for (int k = from + 1; k < L; ++k) b += ((size_t)lv[k].h * lv[k].w * (4 + 4 * NP) + 15) & ~size_t(15);
+    return b;
+  };
+  int s0 = L - 1;
+  for (int k = fuse_last ? 1 : 0; k < L - 1; ++k)
+    if ((long long)lv[k + 1].h * lv[k + 1].w <= kTailPixels && lds_need(k) <= (size_t)ss::kTailLdsBytes) {
+      s0 = k;
+      break;
+    }
+  const bool tail = L - 1 - s0 >= 2 && L - s0 <= ss::kMaxTailLevels;
+  const int last_down = tail ? s0 : L - 1;
+  for (int k = 1; k <= last_down; ++k) {
     ss::IsbDownArgs a{lv[k - 1].g, lv[k - 1].c, lv[k - 1].h, lv[k - 1].w, lv[k].g, lv[k].c, lv[k].h, lv[k].w, vi};
     StageTimer t(stage_down, s);
     SS_HIP(ss::launch_isb_down(a, G, V, s));
   }
-  for (int k = L - 2; k >= 0; --k) {
-    const double d = pow(2.0, k) - 0.5;
-    const double wd = exp(-(d * d) * var_inv_s);
-    const double wu = 1 - wd;
+  if (tail) {
+    ss::IsbTailArgs ta;
+    memset(&ta, 0, sizeof ta);
+    ta.n = L - s0;
+    ta.var_inv_i = vi;
+    ta.g0 = lv[s0].g;
+    ta.c0 = lv[s0].c;
+    unsigned off = 0;
+    for (int e = 0; e < ta.n; ++e) {
+      ta.h[e] = lv[s0 + e].h;
+      ta.w[e] = lv[s0 + e].w;
+      if (e > 0) {
+        const unsigned px = (unsigned)(ta.h[e] * ta.w[e]);
+        ta.goff[e] = off;
+        off += (px * 4 + 15) & ~15u;
+        ta.coff[e] = off;
+        off += (px * 4 * NP + 15) & ~15u;
+      }
+      double wu, wd;
+      weights(s0 + e, wu, wd);
+      ta.wu[e] = (float)wu;
+      ta.wd[e] = (float)wd;
+    }
+    ta.lds_bytes = off;
+    StageTimer t(stage_down, s);
+    SS_HIP(ss::launch_isb_tail(ta, G, V, s));
+  }
+  for (int k = (tail ? s0 : L - 1) - 1; k >= 0; --k) {
+    double wu, wd;
+    weights(k, wu, wd);
     ss::IsbUpArgs a{lv[k + 1].g, lv[k + 1].c, lv[k + 1].h, lv[k + 1].w, lv[k].g, lv[k].c, lv[k].h, lv[k].w,
                     (float)wu,   (float)wd,   vi,          dist,        am};
     const bool fuse = fuse_last && k == 0;
@@ -477,9 +528,11 @@ int ss_inpaint(float* d_map, float* d_tmp, const uint8_t* d_lut, int H, int W, i
   hipStream_t s = as_stream(stream);
   float* a = d_map;
   float* b = d_tmp;
-  for (int i = 0; i < iterations; ++i) {
+  const int per = ss::inpaint_steps_per_launch();
+  for (int i = 0; i < iterations; i += per) {
     StageTimer tm(SS_STAGE_INPAINT, s);
-    SS_HIP(ss::launch_inpaint_step(a, b, d_lut, H, W, thr_empty, thr_src, s));
+    SS_HIP(ss::launch_inpaint_multi(a, b, d_lut, H, W, iterations - i < per ? iterations - i : per, thr_empty,
+                                    thr_src, s));
     float* t = a;
     a = b;
     b = t;

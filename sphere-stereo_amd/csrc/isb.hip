@@ -89,17 +89,13 @@ constexpr int chunk_of() {
 // ---------------------------------------------------------------------------
 // Down-sampling: guideDownsample2xKernel (isb_filter.cu:44-119)
 // ---------------------------------------------------------------------------
+// One coarse pixel of the down pass, handled by the G lanes of a group.
+// Inactive groups (past the end of the level) compute on a valid pixel so
+// that the shuffles see every lane, and store nothing.
 template <int G, int V>
-__global__ void __launch_bounds__(kBlock) k_isb_down(IsbDownArgs a) {
+__device__ __forceinline__ void down_pixel(const IsbDownArgs& a, int pix, bool active, int lane) {
   constexpr int NP = G * V;
   constexpr int CH = chunk_of<V>();
-  const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  int pix = (tile * kBlock + (int)threadIdx.x) / G;
-  const int npix = a.hout * a.wout;
-  const bool active = pix < npix;
-  if (G == 1 && !active) return;
-  if (!active) pix = npix - 1;  // keep the lanes for the shuffles, store nothing
 
   const int x = pix % a.wout, y = pix / a.wout;
   const int x2 = 2 * x, y2 = 2 * y;
@@ -160,6 +156,18 @@ __global__ void __launch_bounds__(kBlock) k_isb_down(IsbDownArgs a) {
     for (int v = 0; v < CH; ++v) acc[v] = acc[v] / ws;
     if (active) store_n<CH>(a.cout + (size_t)pix * NP + base_c + c0, acc);
   }
+}
+
+template <int G, int V>
+__global__ void __launch_bounds__(kBlock) k_isb_down(IsbDownArgs a) {
+  const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  int pix = (tile * kBlock + (int)threadIdx.x) / G;
+  const int npix = a.hout * a.wout;
+  const bool active = pix < npix;
+  if (G == 1 && !active) return;
+  if (!active) pix = npix - 1;
+  down_pixel<G, V>(a, pix, active, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -226,17 +234,12 @@ __device__ __forceinline__ float argmin_refine_group(const float* vals, int lane
 // Up-sampling: guideUpsample2xKernel (isb_filter.cu:132-330).  FUSE_ARGMIN
 // replaces the level-0 write-back with the argmin of argmin_refine_group.
 // ---------------------------------------------------------------------------
+// One coarse pixel of the up pass (its four fine pixels), handled by the G
+// lanes of a group; inactive groups as in down_pixel.
 template <int G, int V, bool FUSE_ARGMIN>
-__global__ void __launch_bounds__(kBlock) k_isb_up(IsbUpArgs a) {
+__device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool active, int lane) {
   constexpr int NP = G * V;
   constexpr int CH = chunk_of<V>();
-  const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  int pix = (tile * kBlock + (int)threadIdx.x) / G;
-  const int npix = a.hin * a.win;
-  const bool active = pix < npix;
-  if (G == 1 && !active) return;
-  if (!active) pix = npix - 1;
 
   const int x = pix % a.win, y = pix / a.win;
   const int r0 = max(y - 1, 0), r2 = min(y + 1, a.hin - 1);
@@ -349,6 +352,53 @@ __global__ void __launch_bounds__(kBlock) k_isb_up(IsbUpArgs a) {
   }
 }
 
+template <int G, int V, bool FUSE_ARGMIN>
+__global__ void __launch_bounds__(kBlock) k_isb_up(IsbUpArgs a) {
+  const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  int pix = (tile * kBlock + (int)threadIdx.x) / G;
+  const int npix = a.hin * a.win;
+  const bool active = pix < npix;
+  if (G == 1 && !active) return;
+  if (!active) pix = npix - 1;
+  up_pixel<G, V, FUSE_ARGMIN>(a, pix, active, lane);
+}
+
+// The coarse end of the pyramid in one workgroup: the down passes into
+// levels 1..n-1 of the table, then the up passes back into level 0, each
+// separated by a workgroup barrier.  Levels 1..n-1 live only in LDS (they
+// are not needed once the pass is over); level 0 is in global memory.  The
+// per-pixel functions work on generic pointers, so the same code serves both.
+constexpr int kTailBlock = 1024;
+
+template <int G, int V>
+__global__ void __launch_bounds__(kTailBlock) k_isb_tail(IsbTailArgs t) {
+  extern __shared__ __align__(16) char smem[];
+  const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
+  const int slot = threadIdx.x / G, slots = kTailBlock / G;
+  auto gp = [&](int k) __attribute__((always_inline)) -> uint32_t* {
+    return k == 0 ? t.g0 : reinterpret_cast<uint32_t*>(smem + t.goff[k]);
+  };
+  auto cp = [&](int k) __attribute__((always_inline)) -> float* {
+    return k == 0 ? t.c0 : reinterpret_cast<float*>(smem + t.coff[k]);
+  };
+  for (int k = 1; k < t.n; ++k) {
+    const IsbDownArgs a{gp(k - 1), cp(k - 1), t.h[k - 1], t.w[k - 1], gp(k), cp(k), t.h[k], t.w[k], t.var_inv_i};
+    const int npix = t.h[k] * t.w[k];
+    for (int base = 0; base < npix; base += slots)
+      if (base + slot < npix) down_pixel<G, V>(a, base + slot, true, lane);
+    __syncthreads();
+  }
+  for (int k = t.n - 2; k >= 0; --k) {
+    const IsbUpArgs a{gp(k + 1), cp(k + 1), t.h[k + 1], t.w[k + 1], gp(k), cp(k), t.h[k], t.w[k],
+                      t.wu[k],   t.wd[k],   t.var_inv_i, nullptr,  ArgminParams{}};
+    const int npix = t.h[k + 1] * t.w[k + 1];
+    for (int base = 0; base < npix; base += slots)
+      if (base + slot < npix) up_pixel<G, V, false>(a, base + slot, true, lane);
+    __syncthreads();
+  }
+}
+
 // Argmin over an already filtered level-0 volume (unfused form).
 template <int G, int V>
 __global__ void __launch_bounds__(kBlock) k_argmin(const float* __restrict__ cost, int npix, ArgminParams p,
@@ -402,6 +452,12 @@ hipError_t launch_down_gv(const IsbDownArgs& a, hipStream_t s) {
 }
 
 template <int G, int V>
+hipError_t launch_tail_gv(const IsbTailArgs& t, hipStream_t s) {
+  hipLaunchKernelGGL((k_isb_tail<G, V>), dim3(1), dim3(kTailBlock), t.lds_bytes, s, t);
+  return hipGetLastError();
+}
+
+template <int G, int V>
 hipError_t launch_up_gv(const IsbUpArgs& a, bool fuse, hipStream_t s) {
   const dim3 grid(blocks_for((long long)a.hin * a.win * G));
   if (fuse)
@@ -429,6 +485,13 @@ hipError_t launch_up_gv(const IsbUpArgs& a, bool fuse, hipStream_t s) {
 hipError_t launch_isb_down(const IsbDownArgs& a, int G, int V, hipStream_t s) {
   if (G == 1) return V == 1 ? launch_down_gv<1, 1>(a, s) : hipErrorInvalidValue;
 #define SS_CALL(v) launch_down_gv<8, v>(a, s)
+  SS_DISPATCH_V(V, SS_CALL)
+#undef SS_CALL
+}
+
+hipError_t launch_isb_tail(const IsbTailArgs& t, int G, int V, hipStream_t s) {
+  if (G == 1) return V == 1 ? launch_tail_gv<1, 1>(t, s) : hipErrorInvalidValue;
+#define SS_CALL(v) launch_tail_gv<8, v>(t, s)
   SS_DISPATCH_V(V, SS_CALL)
 #undef SS_CALL
 }

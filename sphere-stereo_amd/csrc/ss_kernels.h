@@ -36,6 +36,21 @@ struct IsbUpArgs {
   ArgminParams am;
 };
 
+// Levels s0..s0+n-1 of a pyramid for the single-workgroup coarse tail:
+// entry 0 in global memory, entries 1..n-1 at byte offsets in LDS.
+constexpr int kMaxTailLevels = 16;
+constexpr int kTailLdsBytes = 60 * 1024;
+struct IsbTailArgs {
+  int n;
+  uint32_t* g0;
+  float* c0;
+  unsigned goff[kMaxTailLevels], coff[kMaxTailLevels];
+  int h[kMaxTailLevels], w[kMaxTailLevels];
+  float wu[kMaxTailLevels], wd[kMaxTailLevels];  // up pass into entry k
+  float var_inv_i;
+  unsigned lds_bytes;
+};
+
 struct SweepArgs {
   const float4* raylut;
   const float* ref_img;
@@ -88,6 +103,7 @@ struct BlendArgs {
 // isb.hip
 hipError_t launch_isb_down(const IsbDownArgs& a, int G, int V, hipStream_t s);
 hipError_t launch_isb_up(const IsbUpArgs& a, int G, int V, bool fuse_argmin, hipStream_t s);
+hipError_t launch_isb_tail(const IsbTailArgs& t, int G, int V, hipStream_t s);
 hipError_t launch_argmin(const float* cost, int npix, int V, const ArgminParams& p, float* dist, hipStream_t s);
 hipError_t launch_planar_to_cc(const float* src, float* dst, int HW, int N, int NP, hipStream_t s);
 hipError_t launch_cc_to_planar(const float* src, float* dst, int HW, int N, int NP, hipStream_t s);
@@ -106,6 +122,9 @@ hipError_t launch_reproject(const float* din, float* dout, int H, int W, const s
                             hipStream_t s);
 hipError_t launch_inpaint_step(const float* din, float* dout, const uint8_t* lut, int H, int W, double thr_empty,
                                double thr_src, hipStream_t s);
+hipError_t launch_inpaint_multi(const float* din, float* dout, const uint8_t* lut, int H, int W, int steps,
+                                double thr_empty, double thr_src, hipStream_t s);
+int inpaint_steps_per_launch();
 hipError_t launch_inpaint_lut(const ss_intrinsics& c, const float* t, int H, int W, float mn, float mx, uint8_t* lut,
                               hipStream_t s);
 hipError_t launch_box_filter(const float* in, float* out, float* tmp, int B, int H, int W, int r, float pad,

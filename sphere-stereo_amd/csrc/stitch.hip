@@ -92,6 +92,99 @@ __global__ void __launch_bounds__(kBlock) k_inpaint_step(const float* __restrict
   dout[i] = r;
 }
 
+// Several Jacobi passes of inpaintKernel in one launch.  A workgroup owns a
+// kTile x kTile block of the map; it stages the block with a halo of kSteps
+// pixels (map values and table bytes) in LDS, twice, and lists the pixels
+// that can be filled (empty and inside both the map and the staged region).
+// Each pass recomputes only the listed pixels, reading one LDS copy and
+// writing the other; unlisted pixels never change, so both copies keep them.
+// A pass moves information by at most one pixel, so the block's own pixels,
+// kSteps inside the staged region, are exact after kSteps passes.
+constexpr int kTile = 48;
+constexpr int kSteps = 8;
+constexpr int kRegion = kTile + 2 * kSteps;  // 64: index math by shifts
+static_assert(kRegion == 64, "region side is 64");
+
+__global__ void __launch_bounds__(kBlock) k_inpaint_multi(const float* __restrict__ din, float* __restrict__ dout,
+                                                          const uchar2* __restrict__ lut, int H, int W, int steps,
+                                                          double thr_empty, double thr_src) {
+  __shared__ float bufa[kRegion * kRegion];
+  __shared__ float bufb[kRegion * kRegion];
+  __shared__ uchar2 tab[kRegion * kRegion];
+  __shared__ uint16_t list[kRegion * kRegion];
+  __shared__ unsigned count;
+  const int ox = blockIdx.x * kTile - kSteps, oy = blockIdx.y * kTile - kSteps;
+  if (threadIdx.x == 0) count = 0;
+  // stage: all loads of a thread are issued before any is used
+  constexpr int kPer = kRegion * kRegion / kBlock;
+  float dv[kPer];
+  uchar2 tv[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int i = threadIdx.x + k * kBlock;
+    const int gx = ox + (i & (kRegion - 1)), gy = oy + (i >> 6);
+    const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
+    const int g = in ? gy * W + gx : 0;
+    dv[k] = in ? din[g] : 0.0f;
+    tv[k] = in ? lut[g] : make_uchar2(0, 0);
+  }
+  __syncthreads();  // count is zero
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int i = threadIdx.x + k * kBlock;
+    const int lx = i & (kRegion - 1), ly = i >> 6;
+    const int gx = ox + lx, gy = oy + ly;
+    bufa[i] = dv[k];
+    bufb[i] = dv[k];
+    tab[i] = tv[k];
+    if ((double)dv[k] >= thr_empty && gx >= 1 && gy >= 1 && gx < W - 1 && gy < H - 1 && lx >= 1 && ly >= 1 &&
+        lx < kRegion - 1 && ly < kRegion - 1)
+      list[atomicAdd(&count, 1u)] = (uint16_t)i;
+  }
+  __syncthreads();
+  const int n = (int)count;
+  float* a = bufa;
+  float* b = bufb;
+  if (n > 0) {
+    for (int s = 0; s < steps; ++s) {
+      for (int j = threadIdx.x; j < n; j += kBlock) {
+        const int i = list[j];
+        const float d = a[i];
+        float r = d;
+        if ((double)d >= thr_empty) {
+          const int lx = i & (kRegion - 1), ly = i >> 6;
+          const uchar2 t = tab[i];
+          const int bs[2] = {t.x, t.y};
+          float val = 0.0f, wsum = 0.0f;
+#pragma unroll
+          for (int k = 0; k < 2; ++k) {
+            // table offsets are 0..2 (createInpaintingWeightsKernel), so the
+            // neighbour of a listed pixel lies inside the staged region
+            const int nx = lx + (bs[k] & 3) - 1, ny = ly + ((bs[k] & 12) >> 2) - 1;
+            const float w = (float)(bs[k] & 240);
+            const float sd = a[min(ny, kRegion - 1) * kRegion + min(nx, kRegion - 1)];
+            if ((double)sd <= thr_src && w > 0.0f) {
+              val = val + w * sd;
+              wsum = wsum + w;
+            }
+          }
+          if (wsum > 0.0f) r = val / wsum;
+        }
+        b[i] = r;
+      }
+      __syncthreads();
+      float* t = a;
+      a = b;
+      b = t;
+    }
+  }
+  for (int i = threadIdx.x; i < kTile * kTile; i += kBlock) {
+    const int lx = kSteps + i % kTile, ly = kSteps + i / kTile;
+    const int gx = ox + lx, gy = oy + ly;
+    if (gx < W && gy < H) dout[gy * W + gx] = a[ly * kRegion + lx];
+  }
+}
+
 // createInpaintingWeightsKernel (stitcher.cu:212-272)
 __global__ void __launch_bounds__(kBlock) k_inpaint_lut(ss_intrinsics c, float tx, float ty, float tz, int H, int W,
                                                         float mn, float mx, uchar2* __restrict__ lut) {
@@ -299,6 +392,16 @@ hipError_t launch_inpaint_step(const float* din, float* dout, const uint8_t* lut
                      reinterpret_cast<const uchar2*>(lut), H, W, thr_empty, thr_src);
   return hipGetLastError();
 }
+
+hipError_t launch_inpaint_multi(const float* din, float* dout, const uint8_t* lut, int H, int W, int steps,
+                                double thr_empty, double thr_src, hipStream_t s) {
+  const dim3 grid((W + kTile - 1) / kTile, (H + kTile - 1) / kTile);
+  hipLaunchKernelGGL(k_inpaint_multi, grid, dim3(kBlock), 0, s, din, dout, reinterpret_cast<const uchar2*>(lut), H, W,
+                     steps, thr_empty, thr_src);
+  return hipGetLastError();
+}
+
+int inpaint_steps_per_launch() { return kSteps; }
 
 hipError_t launch_inpaint_lut(const ss_intrinsics& c, const float* t, int H, int W, float mn, float mx, uint8_t* lut,
                               hipStream_t s) {

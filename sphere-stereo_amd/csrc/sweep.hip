@@ -20,10 +20,12 @@ constexpr int kBlock = 256;
 inline int blocks_for(long long threads) { return (int)((threads + kBlock - 1) / kBlock); }
 
 // grid_coord with the division by a power-of-two size done as an exact
-// multiply by its reciprocal.
-__device__ __forceinline__ float grid_coord2(float u, float size, float inv, int pow2) {
+// multiply by its reciprocal (a compile-time choice, so that the other
+// form is not evaluated as well).
+template <bool POW2>
+__device__ __forceinline__ float grid_coord2(float u, float size, float inv) {
   const float t = u + 0.5f;
-  const float q = pow2 ? t * inv : t / size;
+  const float q = POW2 ? t * inv : t / size;
   const float g = q * 2.0f - 1.0f;
   return ((g + 1.0f) * size - 1.0f) * 0.5f;
 }
@@ -49,10 +51,11 @@ __device__ __forceinline__ void sample_zero(const float* __restrict__ img, int H
   wse = (bx1 && by1) ? wse : 0.0f;
   const int xa = bx0 ? x0 : 0, xb = bx1 ? x0 + 1 : W - 1;
   const int ya = by0 ? y0 : 0, yb = by1 ? y0 + 1 : H - 1;
-  const float* pnw = img + ((size_t)ya * W + xa) * NCH;
-  const float* pne = img + ((size_t)ya * W + xb) * NCH;
-  const float* psw = img + ((size_t)yb * W + xa) * NCH;
-  const float* pse = img + ((size_t)yb * W + xb) * NCH;
+  // 32-bit offsets: an image plane holds fewer than 2^31 / NCH texels
+  const float* pnw = img + (ya * W + xa) * NCH;
+  const float* pne = img + (ya * W + xb) * NCH;
+  const float* psw = img + (yb * W + xa) * NCH;
+  const float* pse = img + (yb * W + xb) * NCH;
 #pragma unroll
   for (int c = 0; c < NCH; ++c) {
     float s = pnw[c] * wnw;
@@ -120,10 +123,12 @@ __global__ void __launch_bounds__(kBlock) k_select_camera(SelectArgs a) {
     const float disp = sqrtf(dx * dx + dy * dy);
     const float* mk = a.masks + (size_t)k * HW;
     float mnear, mfar;
-    sample_zero<1>(mk, a.H, a.W, grid_coord2(un, a.Wf, a.invW, a.pow2W), grid_coord2(vn, a.Hf, a.invH, a.pow2H),
-                   &mnear);
-    sample_zero<1>(mk, a.H, a.W, grid_coord2(uf, a.Wf, a.invW, a.pow2W), grid_coord2(vf, a.Hf, a.invH, a.pow2H),
-                   &mfar);
+    const float gun = a.pow2W ? grid_coord2<true>(un, a.Wf, a.invW) : grid_coord2<false>(un, a.Wf, a.invW);
+    const float gvn = a.pow2H ? grid_coord2<true>(vn, a.Hf, a.invH) : grid_coord2<false>(vn, a.Hf, a.invH);
+    const float guf = a.pow2W ? grid_coord2<true>(uf, a.Wf, a.invW) : grid_coord2<false>(uf, a.Wf, a.invW);
+    const float gvf = a.pow2H ? grid_coord2<true>(vf, a.Hf, a.invH) : grid_coord2<false>(vf, a.Hf, a.invH);
+    sample_zero<1>(mk, a.H, a.W, gun, gvn, &mnear);
+    sample_zero<1>(mk, a.H, a.W, guf, gvf, &mfar);
     if (disp > maxd && ref_valid && okn && okf && mref >= 0.9f && mnear >= 0.9f && mfar >= 0.9f) {
       maxd = disp;
       best = k;
@@ -133,7 +138,7 @@ __global__ void __launch_bounds__(kBlock) k_select_camera(SelectArgs a) {
 }
 
 // Sweep + raw cost (depth_estimation.py:141-175)
-template <int V>
+template <int V, bool POW2W, bool POW2H>
 __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
   constexpr int G = 8, NP = G * V;
   const int lane = threadIdx.x & (G - 1);
@@ -166,8 +171,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
         rigid(rt, d * rl.x, d * rl.y, d * rl.z, qx, qy, qz);
         project_t(cam, qx, qy, qz, u, w, valid);
         float s[3];
-        sample_zero<3>(img, a.H, a.W, grid_coord2(u, a.Wf, a.invW, a.pow2W), grid_coord2(w, a.Hf, a.invH, a.pow2H),
-                       s);
+        sample_zero<3>(img, a.H, a.W, grid_coord2<POW2W>(u, a.Wf, a.invW), grid_coord2<POW2H>(w, a.Hf, a.invH), s);
         c = (fabsf(s[0] - r0) + fabsf(s[1] - r1)) + fabsf(s[2] - r2);
         c = c > 500.0f ? 500.0f : c;
       }
@@ -190,7 +194,15 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
 
 template <int V>
 hipError_t launch_sweep_v(const SweepArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_sweep<V>), dim3(blocks_for((long long)a.H * a.W * 8)), dim3(kBlock), 0, s, a);
+  const dim3 grid(blocks_for((long long)a.H * a.W * 8));
+  if (a.pow2W && a.pow2H)
+    hipLaunchKernelGGL((k_sweep<V, true, true>), grid, dim3(kBlock), 0, s, a);
+  else if (a.pow2W)
+    hipLaunchKernelGGL((k_sweep<V, true, false>), grid, dim3(kBlock), 0, s, a);
+  else if (a.pow2H)
+    hipLaunchKernelGGL((k_sweep<V, false, true>), grid, dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_sweep<V, false, false>), grid, dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 

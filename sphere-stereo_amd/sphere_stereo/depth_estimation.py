@@ -64,8 +64,15 @@ class RGBD_Estimator:
         nbytes = _lib.lib.ss_estimate_workspace_bytes(self.rows, self.cols, self.candidate_count)
         if nbytes == 0:
             raise ValueError("unsupported matching resolution / candidate count")
-        self._workspace = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
-        self._distance_maps = torch.zeros([len(self.references_indices), self.rows, self.cols], device=self.device)
+        R = len(self.references_indices)
+        # one workspace and one stream per reference: the references' chains
+        # are independent until the merge and run concurrently
+        self._workspaces = [torch.empty(nbytes, dtype=torch.uint8, device=self.device) for _ in range(R)]
+        self._workspace = self._workspaces[0]
+        self._streams = [torch.cuda.Stream(device=self.device) for _ in range(R)]
+        self._fork = torch.cuda.Event()
+        self._joins = [torch.cuda.Event() for _ in range(R)]
+        self._distance_maps = torch.zeros([R, self.rows, self.cols], device=self.device)
         self._stitch_u8 = None
         self.select_camera(masks)
 
@@ -99,13 +106,14 @@ class RGBD_Estimator:
                 return slot, r
         raise ValueError("reference_calibration is not one of the estimator's references")
 
-    def _estimate(self, slot, r, images, guide, out):
+    def _estimate(self, slot, r, images, guide, out, workspace=None, stream=None):
         ptrs = _lib.ptr_array(images)
+        ws = self._workspace if workspace is None else workspace
+        st = self._stream() if stream is None else _lib.C.c_void_p(stream.cuda_stream)
         _lib.check(_lib.lib.ss_estimate_distance(
             self._raylut[slot].data_ptr(), ptrs, len(images), self._cams, self._rts[r], r, self._cands,
             self.candidate_count, self.rows, self.cols, float(self.sigma_i), float(self.sigma_s),
-            guide.data_ptr() if guide is not None else None, out.data_ptr(), self._workspace.data_ptr(),
-            self._workspace.numel(), self._stream()))
+            guide.data_ptr() if guide is not None else None, out.data_ptr(), ws.data_ptr(), ws.numel(), st))
         return out
 
     def estimate_fisheye_distance(self, reference_image, guide, reference_calibration, selected_camera, images):
@@ -131,14 +139,26 @@ class RGBD_Estimator:
             (buffers reused by the next call)
         """
         imgs = [_hwc(i) for i in images_to_match]
-        for slot, r in enumerate(self.references_indices):
-            self._estimate(slot, r, imgs, None, self._distance_maps[slot])
         stitch = [i.contiguous() for i in images_to_stitch]
+        R = len(self.references_indices)
+        if len(stitch) != R:
+            raise ValueError("expected %d images to stitch" % R)
         if self._stitch_u8 is None or tuple(self._stitch_u8.shape[1:]) != tuple(stitch[0].shape):
-            self._stitch_u8 = torch.empty([len(stitch)] + list(stitch[0].shape), dtype=torch.uint8,
-                                          device=self.device)
-        stream = self._stream()
-        for k, img in enumerate(stitch):
-            _lib.check(_lib.lib.ss_to_u8(img.data_ptr(), self._stitch_u8[k].data_ptr(), img.numel(), stream))
-        return self.fishey_stitcher.stitch([self._stitch_u8[k] for k in range(len(stitch))],
-                                           [self._distance_maps[k] for k in range(len(stitch))])
+            self._stitch_u8 = torch.empty([R] + list(stitch[0].shape), dtype=torch.uint8, device=self.device)
+        cur = torch.cuda.current_stream(self.device)
+        self._fork.record(cur)
+        for slot, r in enumerate(self.references_indices):
+            s = self._streams[slot]
+            s.wait_event(self._fork)
+            for t in imgs + stitch:
+                t.record_stream(s)
+            self._estimate(slot, r, imgs, None, self._distance_maps[slot], self._workspaces[slot], s)
+            st = _lib.C.c_void_p(s.cuda_stream)
+            _lib.check(_lib.lib.ss_to_u8(stitch[slot].data_ptr(), self._stitch_u8[slot].data_ptr(),
+                                         stitch[slot].numel(), st))
+            self.fishey_stitcher.reproject_and_inpaint(slot, self._distance_maps[slot], stream=s)
+            self._joins[slot].record(s)
+        for ev in self._joins:
+            cur.wait_event(ev)
+        return self.fishey_stitcher.merge([self._stitch_u8[k] for k in range(R)],
+                                          [self._distance_maps[k] for k in range(R)])

@@ -82,7 +82,7 @@ class Stitcher:
 
         # buffers reused by every stitch() call
         self.reprojected_distances = torch.zeros([R, H, W], device=dev)
-        self.inpaint_scratch = torch.zeros([H, W], device=dev)
+        self.inpaint_scratch = torch.zeros([R, H, W], device=dev)  # one per reference: they may run concurrently
         self.RGB_panorama = torch.zeros([self.prows, self.pcols, 3], dtype=torch.uint8, device=dev)
         self.distance_panorama = torch.zeros([self.prows, self.pcols], device=dev)
 
@@ -113,31 +113,42 @@ class Stitcher:
                                                  stream))
         torch.cuda.synchronize(dev)  # tmp and raw are released on return
 
+    def reproject_and_inpaint(self, r, distance_map, stream=None):
+        """Reference r's part of stitch(): z-tested reprojection to the
+        panorama viewpoint and hole filling (stitcher.py:178-200), enqueued on
+        ``stream`` (default: torch's current stream)."""
+        if distance_map.dtype != torch.float32 or distance_map.numel() != self.rows * self.cols or \
+                not distance_map.is_contiguous():
+            raise ValueError("distance maps must be contiguous float32 [%d, %d]" % (self.rows, self.cols))
+        st = _lib.C.c_void_p(stream.cuda_stream) if stream is not None else _lib.stream_of(self.distance_panorama)
+        rep = self.reprojected_distances[r]
+        _lib.check(_lib.lib.ss_reproject(distance_map.data_ptr(), rep.data_ptr(), self.rows, self.cols,
+                                         self.intrinsics[r], _lib.float_array(self.translations_list[r]), st))
+        _lib.check(_lib.lib.ss_inpaint(rep.data_ptr(), self.inpaint_scratch[r].data_ptr(),
+                                       self.inpainting_weights_list[r].data_ptr(), self.rows, self.cols,
+                                       self.inpainting_iterations, self.thr_empty, self.thr_src, st))
+
+    def merge(self, images, distance_maps, stream=None):
+        """The blend of stitch() (stitcher.py:205-218) over the reprojected maps
+        left by reproject_and_inpaint for every reference."""
+        for img in images:
+            if img.dtype != torch.uint8 or tuple(img.shape) != (self.srows, self.scols, 3) or not img.is_contiguous():
+                raise ValueError("images must be contiguous uint8 [%d, %d, 3]" % (self.srows, self.scols))
+        st = _lib.C.c_void_p(stream.cuda_stream) if stream is not None else _lib.stream_of(self.distance_panorama)
+        reproj = _lib.ptr_array([self.reprojected_distances[r] for r in range(self.R)])
+        _lib.check(_lib.lib.ss_merge(
+            self.blending_sampling.data_ptr(), self.blending_weights.data_ptr(), reproj,
+            _lib.ptr_array(distance_maps), _lib.ptr_array(images), self.srows, self.scols, self.translations_host,
+            self.intrinsics, self.R, self.rows, self.cols, self.prows, self.pcols, self.distance_panorama.data_ptr(),
+            self.RGB_panorama.data_ptr(), st))
+        return self.RGB_panorama, self.distance_panorama
+
     def stitch(self, images, distance_maps):
         """images: R x uint8 [srows, scols, 3]; distance_maps: R x float32
         [rows, cols].  Returns the (rgb, distance) panoramas, buffers reused
         by the next call (stitcher.py:166-220)."""
         if len(images) != self.R or len(distance_maps) != self.R:
             raise ValueError("expected %d images and distance maps" % self.R)
-        for img in images:
-            if img.dtype != torch.uint8 or tuple(img.shape) != (self.srows, self.scols, 3) or not img.is_contiguous():
-                raise ValueError("images must be contiguous uint8 [%d, %d, 3]" % (self.srows, self.scols))
-        for d in distance_maps:
-            if d.dtype != torch.float32 or d.numel() != self.rows * self.cols or not d.is_contiguous():
-                raise ValueError("distance maps must be contiguous float32 [%d, %d]" % (self.rows, self.cols))
-        stream = _lib.stream_of(self.distance_panorama)
         for r in range(self.R):
-            rep = self.reprojected_distances[r]
-            _lib.check(_lib.lib.ss_reproject(distance_maps[r].data_ptr(), rep.data_ptr(), self.rows, self.cols,
-                                             self.intrinsics[r], _lib.float_array(self.translations_list[r]),
-                                             stream))
-            _lib.check(_lib.lib.ss_inpaint(rep.data_ptr(), self.inpaint_scratch.data_ptr(),
-                                           self.inpainting_weights_list[r].data_ptr(), self.rows, self.cols,
-                                           self.inpainting_iterations, self.thr_empty, self.thr_src, stream))
-        reproj = _lib.ptr_array([self.reprojected_distances[r] for r in range(self.R)])
-        _lib.check(_lib.lib.ss_merge(
-            self.blending_sampling.data_ptr(), self.blending_weights.data_ptr(), reproj,
-            _lib.ptr_array(distance_maps), _lib.ptr_array(images), self.srows, self.scols, self.translations_host,
-            self.intrinsics, self.R, self.rows, self.cols, self.prows, self.pcols, self.distance_panorama.data_ptr(),
-            self.RGB_panorama.data_ptr(), stream))
-        return self.RGB_panorama, self.distance_panorama
+            self.reproject_and_inpaint(r, distance_maps[r])
+        return self.merge(images, distance_maps)

@@ -57,6 +57,46 @@ def test_isb_filter_planar_bit_exact(gpu, shape, N):
     assert_same(to_np(cd), want_c, "cost")
 
 
+@pytest.mark.parametrize("iters", [1, 5, 8, 13, 32])
+def test_inpaint_bit_exact(gpu, iters):
+    H, W = 97, 150
+    intr = [0.3 * 150, 0.3 * 150, 75.0, 48.0, -0.18, 0.59]
+    t = [0.08, -0.01, 0.03]
+    rec = _lib.ss_intrinsics(*[float(np.float32(v)) for v in intr])
+    lut = torch.empty([H, W, 2], dtype=torch.uint8, device=gpu)
+    _lib.check(_lib.lib.ss_inpaint_lut(rec, _lib.float_array(t), H, W, 0.55, 100, lut.data_ptr(), _stream()))
+    want_lut = O32.inpaint_lut([float(np.float32(v)) for v in intr], [float(np.float32(v)) for v in t], H, W,
+                               0.55, 100)
+    assert_same(to_np(lut), want_lut, "inpaint LUT")
+    rng = np.random.RandomState(iters)
+    d = (1 + rng.rand(H, W) * 50).astype(np.float32)
+    for _ in range(12):  # holes of up to ~40 px, some touching the border
+        y, x, h, w = rng.randint(0, H), rng.randint(0, W), rng.randint(1, 40), rng.randint(1, 25)
+        d[y:y + h, x:x + w] = 1e8
+    d[rng.rand(H, W) < 0.02] = 1e8
+    te, ts = 100.1, float(np.float32(100) + np.float32(0.1))
+    want = O32.inpaint(d, want_lut, iters, te, ts)
+    m = torch.from_numpy(d).to(gpu)
+    tmp = torch.empty_like(m)
+    _lib.check(_lib.lib.ss_inpaint(m.data_ptr(), tmp.data_ptr(), lut.data_ptr(), H, W, iters, te, ts, _stream()))
+    assert_same(to_np(m), want, "inpainted map (%d passes)" % iters)
+
+
+def test_reproject_bit_exact(gpu):
+    H, W = 97, 150
+    intr = [float(np.float32(v)) for v in [0.3 * 150, 0.3 * 150, 75.0, 48.0, -0.18, 0.59]]
+    t = [float(np.float32(v)) for v in [0.2, -0.05, 0.1]]
+    rng = np.random.RandomState(1)
+    d = (0.6 + rng.rand(H, W) * 30).astype(np.float32)
+    d[:, 70:75] = 0.7  # a near strip that occludes its neighbourhood
+    want = O32.reproject_zmin(d, intr, t)
+    out = torch.empty([H, W], device=gpu)
+    _lib.check(_lib.lib.ss_reproject(torch.from_numpy(d).to(gpu).data_ptr(), out.data_ptr(), H, W,
+                                     _lib.ss_intrinsics(*intr), _lib.float_array(t), _stream()))
+    assert_same(to_np(out), want, "reprojected map")
+    assert (want < 1e8).mean() > 0.5 and (want >= 1e8).any()
+
+
 def test_isb_filter_rejects_tiny(gpu):
     with pytest.raises(ValueError):
         ISB_Filter(4, (3, 3), gpu)
