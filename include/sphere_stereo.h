@@ -177,6 +177,51 @@ int ss_merge(const float *d_sampling, const float *d_weights, const float *const
 /* Truncating float32 -> uint8 cast of n values (depth_estimation.py:239-240). */
 int ss_to_u8(const float *d_in, uint8_t *d_out, size_t n, ss_stream stream);
 
+/* Per-frame executor ---------------------------------------------------------- */
+
+/* Everything estimate_RGBD_panorama (depth_estimation.py:205-243) needs besides
+ * the frame's images: the rig, the per-reference tables built at set-up, the
+ * device buffers the frame writes, and the panorama outputs. */
+typedef struct {
+    int C, R, N, H, W;                /* cameras, references, candidates, matching rows and cols */
+    int Hs, Ws, PH, PW;               /* stitch image rows/cols, panorama rows/cols */
+    int inpaint_iterations;
+    int refs[SS_MAX_REFERENCES];      /* camera index of each reference */
+    ss_camera cams[SS_MAX_CAMERAS];
+    float rts[SS_MAX_REFERENCES][SS_MAX_CAMERAS * 12]; /* per reference, as ss_sweep */
+    float cands[SS_MAX_CANDIDATES];
+    double sigma_i, sigma_s;
+    double thr_empty, thr_src;        /* as ss_inpaint */
+    const float *d_raylut[SS_MAX_REFERENCES];
+    void *d_workspace[SS_MAX_REFERENCES]; /* ss_estimate_workspace_bytes each */
+    size_t workspace_bytes;
+    float *d_dist[SS_MAX_REFERENCES]; /* per-reference distance maps, H x W */
+    ss_intrinsics intr[SS_MAX_REFERENCES];
+    float trans[SS_MAX_REFERENCES][3];
+    const uint8_t *d_inpaint_lut[SS_MAX_REFERENCES];
+    float *d_reproj[SS_MAX_REFERENCES];
+    float *d_scratch[SS_MAX_REFERENCES]; /* H x W each */
+    uint8_t *d_stitch_u8[SS_MAX_REFERENCES]; /* Hs x Ws x 3 each */
+    const float *d_sampling, *d_weights;
+    float *d_dist_pano;
+    uint8_t *d_rgb_pano;
+} ss_frame_desc;
+
+/* No counterpart in the reference (its per-frame loop is Python).  Creates an
+ * executor for desc (copied); it owns one stream per reference and one for
+ * graph replay, all at default priority. */
+int ss_frame_create(const ss_frame_desc *desc, void **frame);
+/* Enqueues one frame after the work already on `stream` and makes `stream`
+ * wait for it: each reference's estimate_fisheye_distance, uint8 cast of its
+ * stitch image, reprojection and inpainting run on that reference's stream;
+ * the merge runs on `stream`.  d_match: C float32 H x W x 3 images; d_stitch:
+ * R float32 Hs x Ws x 3 images.  With use_graph, the frame is captured into a
+ * HIP graph on the first call (and again whenever the image pointers change)
+ * and replayed afterwards. */
+int ss_frame_run(void *frame, const float *const *d_match, const float *const *d_stitch, int use_graph,
+                 ss_stream stream);
+int ss_frame_destroy(void *frame);
+
 /* Kernel timing ------------------------------------------------------------- */
 /* No counterpart in the reference (it has no timers).  While enabled, every
  * launch of a stage whose bit is set in stage_mask is bracketed by a pair of

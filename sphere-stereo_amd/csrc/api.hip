@@ -246,6 +246,11 @@ int fill_sweep_args(ss::SweepArgs& a, const float* d_raylut, const float* const*
 
 }  // namespace
 
+namespace ss {
+int report_error(int code, const char* what) { return fail(code, "%s", what); }
+bool profiler_active() { return g_prof.on; }
+}  // namespace ss
+
 extern "C" {
 
 int ss_abi_version(void) { return SS_ABI_VERSION; }

@@ -1,0 +1,178 @@
+// Per-frame executor: estimate_RGBD_panorama (depth_estimation.py:205-243 and
+// stitcher.py:166-220) as one call.  Each reference's chain runs on a stream
+// of its own (they are independent until the merge), so one reference's
+// latency-bound coarse pyramid levels overlap the other's sweep.  The frame
+// can be captured once into a HIP graph and replayed, which removes the
+// per-launch host cost of its ~70 kernels.
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+
+#include "ss_kernels.h"
+
+namespace {
+
+struct Frame {
+  ss_frame_desc d;
+  hipStream_t side[SS_MAX_REFERENCES] = {};
+  hipStream_t main = nullptr;
+  hipEvent_t fork = nullptr, pre = nullptr, done = nullptr;
+  hipEvent_t join[SS_MAX_REFERENCES] = {};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const float* cap_match[SS_MAX_CAMERAS] = {};
+  const float* cap_stitch[SS_MAX_REFERENCES] = {};
+};
+
+#define FR_HIP(expr)                                                            \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      char m_[256];                                                             \
+      snprintf(m_, sizeof m_, "%s: %s", #expr, hipGetErrorString(e_));          \
+      return ss::report_error(SS_ERR_HIP, m_);                                  \
+    }                                                                           \
+  } while (0)
+
+#define FR_CALL(expr)         \
+  do {                        \
+    int rc_ = (expr);         \
+    if (rc_ != SS_OK) return rc_; \
+  } while (0)
+
+// The frame's work after the events on `s`; `s` waits for all of it.
+int enqueue(Frame* f, const float* const* match, const float* const* stitch, hipStream_t s) {
+  const ss_frame_desc& d = f->d;
+  FR_HIP(hipEventRecord(f->fork, s));
+  const float* reproj[SS_MAX_REFERENCES];
+  const float* dists[SS_MAX_REFERENCES];
+  const uint8_t* imgs[SS_MAX_REFERENCES];
+  for (int r = 0; r < d.R; ++r) {
+    hipStream_t sr = f->side[r];
+    FR_HIP(hipStreamWaitEvent(sr, f->fork, 0));
+    FR_CALL(ss_estimate_distance(d.d_raylut[r], match, d.C, d.cams, d.rts[r], d.refs[r], d.cands, d.N, d.H, d.W,
+                                 d.sigma_i, d.sigma_s, nullptr, d.d_dist[r], d.d_workspace[r], d.workspace_bytes,
+                                 sr));
+    FR_CALL(ss_to_u8(stitch[r], d.d_stitch_u8[r], (size_t)d.Hs * d.Ws * 3, sr));
+    FR_CALL(ss_reproject(d.d_dist[r], d.d_reproj[r], d.H, d.W, &d.intr[r], d.trans[r], sr));
+    FR_CALL(ss_inpaint(d.d_reproj[r], d.d_scratch[r], d.d_inpaint_lut[r], d.H, d.W, d.inpaint_iterations,
+                       d.thr_empty, d.thr_src, sr));
+    FR_HIP(hipEventRecord(f->join[r], sr));
+    FR_HIP(hipStreamWaitEvent(s, f->join[r], 0));
+    reproj[r] = d.d_reproj[r];
+    dists[r] = d.d_dist[r];
+    imgs[r] = d.d_stitch_u8[r];
+  }
+  return ss_merge(d.d_sampling, d.d_weights, reproj, dists, imgs, d.Hs, d.Ws, &d.trans[0][0], d.intr, d.R, d.H, d.W,
+                  d.PH, d.PW, d.d_dist_pano, d.d_rgb_pano, s);
+}
+
+int drop_graph(Frame* f) {
+  if (f->exec) FR_HIP(hipGraphExecDestroy(f->exec));
+  if (f->graph) FR_HIP(hipGraphDestroy(f->graph));
+  f->exec = nullptr;
+  f->graph = nullptr;
+  return SS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ss_frame_create(const ss_frame_desc* desc, void** frame) {
+  if (!desc || !frame) return ss::report_error(SS_ERR_ARG, "null descriptor or output handle");
+  const ss_frame_desc& d = *desc;
+  if (d.C < 1 || d.C > SS_MAX_CAMERAS || d.R < 1 || d.R > SS_MAX_REFERENCES || d.N < 1 || d.N > SS_MAX_CANDIDATES ||
+      d.H < 4 || d.W < 4 || d.Hs < 2 || d.Ws < 2 || d.PH < 1 || d.PW < 1 || d.inpaint_iterations < 0)
+    return ss::report_error(SS_ERR_ARG, "frame descriptor sizes out of range");
+  for (int r = 0; r < d.R; ++r)
+    if (d.refs[r] < 0 || d.refs[r] >= d.C || !d.d_raylut[r] || !d.d_workspace[r] || !d.d_dist[r] ||
+        !d.d_inpaint_lut[r] || !d.d_reproj[r] || !d.d_scratch[r] || !d.d_stitch_u8[r])
+      return ss::report_error(SS_ERR_ARG, "frame descriptor: bad reference index or null buffer");
+  if (!d.d_sampling || !d.d_weights || !d.d_dist_pano || !d.d_rgb_pano)
+    return ss::report_error(SS_ERR_ARG, "frame descriptor: null stitching table or output");
+  Frame* f = new (std::nothrow) Frame();
+  if (!f) return ss::report_error(SS_ERR_ARG, "out of host memory");
+  f->d = d;
+  int rc = SS_OK;
+  auto check = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && rc == SS_OK) {
+      char m[256];
+      snprintf(m, sizeof m, "%s: %s", what, hipGetErrorString(e));
+      rc = ss::report_error(SS_ERR_HIP, m);
+    }
+  };
+  for (int r = 0; r < d.R; ++r) {
+    check(hipStreamCreateWithFlags(&f->side[r], hipStreamNonBlocking), "hipStreamCreateWithFlags");
+    check(hipEventCreateWithFlags(&f->join[r], hipEventDisableTiming), "hipEventCreateWithFlags");
+  }
+  check(hipStreamCreateWithFlags(&f->main, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+  check(hipEventCreateWithFlags(&f->fork, hipEventDisableTiming), "hipEventCreateWithFlags");
+  check(hipEventCreateWithFlags(&f->pre, hipEventDisableTiming), "hipEventCreateWithFlags");
+  check(hipEventCreateWithFlags(&f->done, hipEventDisableTiming), "hipEventCreateWithFlags");
+  if (rc != SS_OK) {
+    ss_frame_destroy(f);
+    return rc;
+  }
+  *frame = f;
+  return SS_OK;
+}
+
+int ss_frame_run(void* frame, const float* const* d_match, const float* const* d_stitch, int use_graph,
+                 ss_stream stream) {
+  Frame* f = static_cast<Frame*>(frame);
+  if (!f || !d_match || !d_stitch) return ss::report_error(SS_ERR_ARG, "null frame or image array");
+  for (int k = 0; k < f->d.C; ++k)
+    if (!d_match[k]) return ss::report_error(SS_ERR_ARG, "null matching image");
+  for (int r = 0; r < f->d.R; ++r)
+    if (!d_stitch[r]) return ss::report_error(SS_ERR_ARG, "null stitch image");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (!use_graph) return enqueue(f, d_match, d_stitch, s);
+  if (ss::profiler_active()) return ss::report_error(SS_ERR_UNSUPPORTED, "graph replay while kernel timing is on");
+
+  bool same = f->exec != nullptr;
+  for (int k = 0; k < f->d.C && same; ++k) same = f->cap_match[k] == d_match[k];
+  for (int r = 0; r < f->d.R && same; ++r) same = f->cap_stitch[r] == d_stitch[r];
+  // the replay stream starts after the caller's work
+  FR_HIP(hipEventRecord(f->pre, s));
+  FR_HIP(hipStreamWaitEvent(f->main, f->pre, 0));
+  if (!same) {
+    FR_CALL(drop_graph(f));
+    FR_HIP(hipStreamBeginCapture(f->main, hipStreamCaptureModeRelaxed));
+    const int rc = enqueue(f, d_match, d_stitch, f->main);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(f->main, &g);
+    if (rc != SS_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    FR_HIP(e);
+    f->graph = g;
+    FR_HIP(hipGraphInstantiate(&f->exec, f->graph, nullptr, nullptr, 0));
+    for (int k = 0; k < f->d.C; ++k) f->cap_match[k] = d_match[k];
+    for (int r = 0; r < f->d.R; ++r) f->cap_stitch[r] = d_stitch[r];
+  }
+  FR_HIP(hipGraphLaunch(f->exec, f->main));
+  FR_HIP(hipEventRecord(f->done, f->main));
+  FR_HIP(hipStreamWaitEvent(s, f->done, 0));
+  return SS_OK;
+}
+
+int ss_frame_destroy(void* frame) {
+  Frame* f = static_cast<Frame*>(frame);
+  if (!f) return SS_OK;
+  drop_graph(f);
+  for (int r = 0; r < SS_MAX_REFERENCES; ++r) {
+    if (f->side[r]) (void)hipStreamDestroy(f->side[r]);
+    if (f->join[r]) (void)hipEventDestroy(f->join[r]);
+  }
+  if (f->main) (void)hipStreamDestroy(f->main);
+  if (f->fork) (void)hipEventDestroy(f->fork);
+  if (f->pre) (void)hipEventDestroy(f->pre);
+  if (f->done) (void)hipEventDestroy(f->done);
+  delete f;
+  return SS_OK;
+}
+
+}  // extern "C"

@@ -100,6 +100,10 @@ struct BlendArgs {
   float mn, mx;
 };
 
+// api.hip: error text of the calling thread; whether kernel timing is on
+int report_error(int code, const char* what);
+bool profiler_active();
+
 // isb.hip
 hipError_t launch_isb_down(const IsbDownArgs& a, int G, int V, hipStream_t s);
 hipError_t launch_isb_up(const IsbUpArgs& a, int G, int V, bool fuse_argmin, hipStream_t s);

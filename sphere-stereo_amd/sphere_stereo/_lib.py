@@ -23,6 +23,22 @@ class ss_intrinsics(C.Structure):
                 ("alpha", C.c_float)]
 
 
+class ss_frame_desc(C.Structure):
+    _fields_ = [("C", C.c_int), ("R", C.c_int), ("N", C.c_int), ("H", C.c_int), ("W", C.c_int),
+                ("Hs", C.c_int), ("Ws", C.c_int), ("PH", C.c_int), ("PW", C.c_int), ("inpaint_iterations", C.c_int),
+                ("refs", C.c_int * SS_MAX_REFERENCES), ("cams", ss_camera * SS_MAX_CAMERAS),
+                ("rts", (C.c_float * (SS_MAX_CAMERAS * 12)) * SS_MAX_REFERENCES),
+                ("cands", C.c_float * SS_MAX_CANDIDATES), ("sigma_i", C.c_double), ("sigma_s", C.c_double),
+                ("thr_empty", C.c_double), ("thr_src", C.c_double),
+                ("d_raylut", C.c_void_p * SS_MAX_REFERENCES), ("d_workspace", C.c_void_p * SS_MAX_REFERENCES),
+                ("workspace_bytes", C.c_size_t), ("d_dist", C.c_void_p * SS_MAX_REFERENCES),
+                ("intr", ss_intrinsics * SS_MAX_REFERENCES), ("trans", (C.c_float * 3) * SS_MAX_REFERENCES),
+                ("d_inpaint_lut", C.c_void_p * SS_MAX_REFERENCES), ("d_reproj", C.c_void_p * SS_MAX_REFERENCES),
+                ("d_scratch", C.c_void_p * SS_MAX_REFERENCES), ("d_stitch_u8", C.c_void_p * SS_MAX_REFERENCES),
+                ("d_sampling", C.c_void_p), ("d_weights", C.c_void_p), ("d_dist_pano", C.c_void_p),
+                ("d_rgb_pano", C.c_void_p)]
+
+
 _i, _d, _p, _z, _fl = C.c_int, C.c_double, C.c_void_p, C.c_size_t, C.c_float
 _CAM = C.POINTER(ss_camera)
 _INTR = C.POINTER(ss_intrinsics)
@@ -51,6 +67,9 @@ SIGNATURES = {
     "ss_inpaint": (_i, [_p, _p, _p, _i, _i, _i, _d, _d, _p]),
     "ss_merge": (_i, [_p, _p, _p, _p, _p, _i, _i, _p, _INTR, _i, _i, _i, _i, _i, _p, _p, _p]),
     "ss_to_u8": (_i, [_p, _p, _z, _p]),
+    "ss_frame_create": (_i, [C.POINTER(ss_frame_desc), C.POINTER(C.c_void_p)]),
+    "ss_frame_run": (_i, [_p, _p, _p, _i, _p]),
+    "ss_frame_destroy": (_i, [_p]),
     "ss_profile_enable": (_i, [_i, C.c_uint]),
     "ss_profile_collect": (_i, [_p, _p]),
     "ss_stage_name": (C.c_char_p, [_i]),

@@ -1,12 +1,15 @@
 """RGBD_Estimator with the reference's interface (python/depth_estimation.py).
 
 The estimator keeps the reference's constructor, ``select_camera``,
-``estimate_fisheye_distance`` and ``estimate_RGBD_panorama``.  Each reference
-camera's chain - guide, sweep, cost filter with the final up-pass fused with
-the argmin, distance filter - is one ss_estimate_distance call that enqueues
-the gfx950 kernels on torch's current stream; the panorama is produced by the
-Stitcher.  Init-time work (camera selection, unit-ray tables, stitching
-tables) also runs on the GPU.
+``estimate_fisheye_distance`` and ``estimate_RGBD_panorama``.  A frame is one
+ss_frame_run call into the native executor: each reference camera's chain
+(guide, sweep, cost filter with the final up-pass fused with the argmin,
+distance filter, uint8 cast of its stitch image, reprojection, inpainting)
+runs on a stream of its own, and the merge runs on torch's current stream,
+which waits for the frame.  With ``use_graph = True`` the frame is captured
+once into a HIP graph and replayed on later calls with the same input
+tensors.  Init-time work (camera selection, unit-ray tables, stitching tables)
+also runs on the GPU.
 """
 import torch
 
@@ -65,16 +68,56 @@ class RGBD_Estimator:
         if nbytes == 0:
             raise ValueError("unsupported matching resolution / candidate count")
         R = len(self.references_indices)
-        # one workspace and one stream per reference: the references' chains
-        # are independent until the merge and run concurrently
+        # one workspace per reference: the references' chains run concurrently
         self._workspaces = [torch.empty(nbytes, dtype=torch.uint8, device=self.device) for _ in range(R)]
         self._workspace = self._workspaces[0]
-        self._streams = [torch.cuda.Stream(device=self.device) for _ in range(R)]
-        self._fork = torch.cuda.Event()
-        self._joins = [torch.cuda.Event() for _ in range(R)]
         self._distance_maps = torch.zeros([R, self.rows, self.cols], device=self.device)
-        self._stitch_u8 = None
+        self._stitch_res = (int(rgb_to_stitch_resolution[1]), int(rgb_to_stitch_resolution[0]))
+        self._stitch_u8 = torch.zeros([R, self._stitch_res[0], self._stitch_res[1], 3], dtype=torch.uint8,
+                                      device=self.device)
+        self.use_graph = False
+        self._frame = None
         self.select_camera(masks)
+        self._frame = self._make_frame()
+
+    def _make_frame(self):
+        st = self.fishey_stitcher
+        d = _lib.ss_frame_desc()
+        d.C, d.R, d.N, d.H, d.W = len(self.calibrations), len(self.references_indices), self.candidate_count, \
+            self.rows, self.cols
+        d.Hs, d.Ws, d.PH, d.PW = st.srows, st.scols, st.prows, st.pcols
+        d.inpaint_iterations = st.inpainting_iterations
+        for k in range(d.C):
+            d.cams[k] = self._cams[k]
+        for n, c in enumerate(self._cands_list):
+            d.cands[n] = c
+        d.sigma_i, d.sigma_s = float(self.sigma_i), float(self.sigma_s)
+        d.thr_empty, d.thr_src = st.thr_empty, st.thr_src
+        d.workspace_bytes = self._workspaces[0].numel()
+        d.d_sampling, d.d_weights = st.blending_sampling.data_ptr(), st.blending_weights.data_ptr()
+        d.d_dist_pano, d.d_rgb_pano = st.distance_panorama.data_ptr(), st.RGB_panorama.data_ptr()
+        for slot, r in enumerate(self.references_indices):
+            d.refs[slot] = r
+            for i in range(12 * d.C):
+                d.rts[slot][i] = self._rts[r][i]
+            d.d_raylut[slot] = self._raylut[slot].data_ptr()
+            d.d_workspace[slot] = self._workspaces[slot].data_ptr()
+            d.d_dist[slot] = self._distance_maps[slot].data_ptr()
+            d.intr[slot] = st.intrinsics[slot]
+            for i in range(3):
+                d.trans[slot][i] = float(st.translations_list[slot][i])
+            d.d_inpaint_lut[slot] = st.inpainting_weights_list[slot].data_ptr()
+            d.d_reproj[slot] = st.reprojected_distances[slot].data_ptr()
+            d.d_scratch[slot] = st.inpaint_scratch[slot].data_ptr()
+            d.d_stitch_u8[slot] = self._stitch_u8[slot].data_ptr()
+        handle = _lib.C.c_void_p()
+        _lib.check(_lib.lib.ss_frame_create(_lib.C.byref(d), _lib.C.byref(handle)))
+        return handle
+
+    def __del__(self):
+        if getattr(self, "_frame", None):
+            _lib.lib.ss_frame_destroy(self._frame)
+            self._frame = None
 
     def _stream(self):
         return _lib.stream_of(self._workspace)
@@ -140,25 +183,18 @@ class RGBD_Estimator:
         """
         imgs = [_hwc(i) for i in images_to_match]
         stitch = [i.contiguous() for i in images_to_stitch]
-        R = len(self.references_indices)
-        if len(stitch) != R:
-            raise ValueError("expected %d images to stitch" % R)
-        if self._stitch_u8 is None or tuple(self._stitch_u8.shape[1:]) != tuple(stitch[0].shape):
-            self._stitch_u8 = torch.empty([R] + list(stitch[0].shape), dtype=torch.uint8, device=self.device)
-        cur = torch.cuda.current_stream(self.device)
-        self._fork.record(cur)
-        for slot, r in enumerate(self.references_indices):
-            s = self._streams[slot]
-            s.wait_event(self._fork)
-            for t in imgs + stitch:
-                t.record_stream(s)
-            self._estimate(slot, r, imgs, None, self._distance_maps[slot], self._workspaces[slot], s)
-            st = _lib.C.c_void_p(s.cuda_stream)
-            _lib.check(_lib.lib.ss_to_u8(stitch[slot].data_ptr(), self._stitch_u8[slot].data_ptr(),
-                                         stitch[slot].numel(), st))
-            self.fishey_stitcher.reproject_and_inpaint(slot, self._distance_maps[slot], stream=s)
-            self._joins[slot].record(s)
-        for ev in self._joins:
-            cur.wait_event(ev)
-        return self.fishey_stitcher.merge([self._stitch_u8[k] for k in range(R)],
-                                          [self._distance_maps[k] for k in range(R)])
+        if len(imgs) != len(self.calibrations) or len(stitch) != len(self.references_indices):
+            raise ValueError("expected %d images to match and %d to stitch" % (len(self.calibrations),
+                                                                              len(self.references_indices)))
+        for t in imgs:
+            if t.dtype != torch.float32 or tuple(t.shape) != (self.rows, self.cols, 3) or t.device != self.device:
+                raise ValueError("images to match must be float32 [%d, %d, 3] on %s" % (self.rows, self.cols,
+                                                                                       self.device))
+        for t in stitch:
+            if t.dtype != torch.float32 or tuple(t.shape) != self._stitch_res + (3,) or t.device != self.device:
+                raise ValueError("images to stitch must be float32 [%d, %d, 3] on %s" % (self._stitch_res + (
+                    self.device,)))
+        self._inputs = (imgs, stitch)  # kept alive until the next call (the frame may still read them)
+        _lib.check(_lib.lib.ss_frame_run(self._frame, _lib.ptr_array(imgs), _lib.ptr_array(stitch),
+                                         int(bool(self.use_graph)), self._stream()))
+        return self.fishey_stitcher.RGB_panorama, self.fishey_stitcher.distance_panorama

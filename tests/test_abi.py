@@ -97,3 +97,22 @@ def test_isb_filter_rejects_tiny_resolution_on_cpu():
 
     with pytest.raises(ValueError):
         ISB_Filter(4, (3, 3), "cpu")
+
+
+def test_frame_desc_layout_matches_header(tmp_path):
+    import subprocess
+
+    src = tmp_path / "layout.c"
+    fields = [f[0] for f in _lib.ss_frame_desc._fields_]
+    body = "\n".join('printf("%s %%zu\\n", offsetof(ss_frame_desc, %s));' % (f, f) for f in fields)
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "sphere_stereo.h"\nint main(void){\n'
+                   'printf("size %%zu\\n", sizeof(ss_frame_desc));\nprintf("camera %%zu\\n", sizeof(ss_camera));\n'
+                   '%s\nreturn 0;}\n' % body)
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    out = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                       check=True).stdout.splitlines())
+    assert int(out["size"]) == C.sizeof(_lib.ss_frame_desc)
+    assert int(out["camera"]) == C.sizeof(_lib.ss_camera)
+    for f in fields:
+        assert int(out[f]) == getattr(_lib.ss_frame_desc, f).offset, f

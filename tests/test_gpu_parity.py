@@ -172,3 +172,27 @@ def test_panorama_deterministic(small):
     rgb2, dist2 = est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])
     assert torch.equal(rgb1, rgb2)
     assert torch.equal(dist1.view(torch.int32), dist2.view(torch.int32))
+
+
+def test_graph_replay_matches_eager(small):
+    fr, est, ora = small
+    rgb0, dist0 = [t.clone() for t in est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])]
+    imgs = [t.contiguous() for t in fr["to_match"]]
+    stitch = [t.contiguous() for t in fr["to_stitch"]]
+    est.use_graph = True
+    try:
+        for rep in range(4):
+            rgb, dist = est.estimate_RGBD_panorama(imgs, stitch)
+            torch.cuda.synchronize()
+            assert torch.equal(rgb, rgb0), "replay %d" % rep
+            assert torch.equal(dist.view(torch.int32), dist0.view(torch.int32)), "replay %d" % rep
+            # eager work and a synchronisation between replays
+            junk = torch.randn(1 << 20, device=rgb.device).sum().item()
+            assert junk == junk
+        # new input tensors: the frame is captured again
+        imgs2 = [t.clone() for t in imgs]
+        rgb, dist = est.estimate_RGBD_panorama(imgs2, stitch)
+        torch.cuda.synchronize()
+        assert torch.equal(rgb, rgb0) and torch.equal(dist.view(torch.int32), dist0.view(torch.int32))
+    finally:
+        est.use_graph = False
