@@ -22,6 +22,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "sphere-stereo_amd"))
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "RGB-D panoramas/s (4×1024² fisheye, 32 cands) + % HBM peak; FP16 vs FP32"
 
 WORKLOADS = {
     "c1": dict(matching=(1024, 1024), stitch=(1216, 1216), pano=(2048, 1024), N=32, refs=(0, 2), sigma_i=10.0,
@@ -223,6 +224,30 @@ def main():
     g_elapsed = time.perf_counter() - g0
     est.use_graph = False
 
+    # precision forms: the same frames with the cost volume stored in fp16 /
+    # bf16 (fp32 arithmetic), timed the same way
+    from sphere_stereo import RGBD_Estimator
+
+    forms = {}
+    for dt in ("float16", "bfloat16"):
+        est_h = RGBD_Estimator(calibs, device=device, volume_dtype=dt, **kw)
+        for _ in range(args.warmup):
+            est_h.estimate_RGBD_panorama(to_match, to_stitch)
+        torch.cuda.synchronize()
+        split_h = profile(lib, est_h, to_match, to_stitch, 2, (1 << 11) - 1)
+        print("[bench] %s stages (ms/frame): %s" % (dt, ", ".join("%s %.3f" % (k, v[0] / 2) for k, v in sorted(
+            split_h.items(), key=lambda kv: -kv[1][0]) if v[1])), file=sys.stderr, flush=True)
+        h0 = time.perf_counter()
+        for _ in range(args.steps):
+            est_h.estimate_RGBD_panorama(to_match, to_stitch)
+        torch.cuda.synchronize()
+        h_el = time.perf_counter() - h0
+        forms[dt] = {"value": round(args.steps / h_el, 3), "unit": "panoramas/s",
+                     "ms_per_step": round(1e3 * h_el / args.steps, 4),
+                     "vs_fp32": round((args.steps / h_el) / value, 4)}
+        print("[bench] %s volume: %.1f panoramas/s" % (dt, args.steps / h_el), file=sys.stderr, flush=True)
+        del est_h
+
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
     if os.path.exists(tpath):
@@ -230,7 +255,7 @@ def main():
             traffic = json.load(f).get(dominant)
 
     out = {
-        "metric": "RGB-D panoramas/s (4x1024^2 fisheye, 32 cands) + % HBM peak",
+        "metric": METRIC,
         "value": round(value, 3),
         "unit": "panoramas/s",
         "n_gpus": 1,
@@ -255,6 +280,7 @@ def main():
         "stage_ms_per_frame": {k: round(v, 4) for k, v in per_frame.items() if v > 0},
         "graph_replay": {"value": round(args.steps / g_elapsed, 3), "unit": "panoramas/s",
                          "ms_per_step": round(1e3 * g_elapsed / args.steps, 4)},
+        "precision_forms": forms,
     }
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)

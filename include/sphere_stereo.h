@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define SS_ABI_VERSION 1
+#define SS_ABI_VERSION 2
 #define SS_MAX_CAMERAS 8
 #define SS_MAX_CANDIDATES 128
 #define SS_MAX_REFERENCES 8
@@ -46,6 +46,15 @@ typedef enum {
     SS_ERR_UNSUPPORTED = 3, /* valid for the reference but outside this build's limits */
     SS_ERR_WORKSPACE = 4    /* workspace smaller than required */
 } ss_status;
+
+/* Element type of the stored cost volume.  All arithmetic is fp32; with F16 /
+ * BF16 only the values written to the volume (sweep output, pyramid levels)
+ * are rounded to nearest even.  F32 is the reference's type. */
+typedef enum {
+    SS_VOLUME_F32 = 0,
+    SS_VOLUME_F16 = 1,
+    SS_VOLUME_BF16 = 2
+} ss_volume_type;
 
 /* Double-sphere camera as the reference's torch code uses it (utils.py:40-94):
  * fs = fl * matching_scale and cs = principal * matching_scale are the fp32
@@ -113,19 +122,22 @@ int ss_normalize_weights(float *d_w, int R, int P, ss_stream stream);
 int ss_guide(const float *d_img, int H, int W, uint32_t *d_guide, ss_stream stream);
 
 /* Sweep + raw cost (depth_estimation.py:141-175) into the candidate-contiguous
- * volume d_cost (H x W x Np float32).  d_images: C host-side device pointers
- * (H x W x 3 float32); rts as ss_select_camera; cands: N host floats
- * (1/linspace(1/min, 1/max, N)). */
+ * volume d_cost (H x W x Np elements of vtype).  d_images: C host-side device
+ * pointers (H x W x 3 float32); rts as ss_select_camera; cands: N host floats
+ * (1/linspace(1/min, 1/max, N)).  d_guide (nullable) also receives the packed
+ * guide of d_images[ref], as ss_guide computes it, from the same pass. */
 int ss_sweep(const float *d_raylut, const float *const *d_images, int C, const ss_camera *cams, const float *rts,
-             int ref, const float *cands, int N, int H, int W, float *d_cost, ss_stream stream);
+             int ref, const float *cands, int N, int H, int W, int vtype, void *d_cost, uint32_t *d_guide,
+             ss_stream stream);
 
-/* Bytes of workspace for ss_isb_filter / ss_isb_filter_planar (pyramid levels
- * 1..L-1 of guides and volume, plus the level-0 repack for the planar form). */
-size_t ss_isb_workspace_bytes(int H, int W, int N, int planar);
+/* Bytes of workspace for ss_isb_filter (vtype volume) / ss_isb_filter_planar
+ * (planar = 1, vtype must be SS_VOLUME_F32): pyramid levels 1..L-1 of guides
+ * and volume, plus the level-0 repack for the planar form. */
+size_t ss_isb_workspace_bytes(int H, int W, int N, int vtype, int planar);
 
 /* ISB_Filter.apply (isb_filter.py:72-136) on the library layout: d_guide
- * (H x W packed guide) and d_cost (H x W x Np) are filtered in place. */
-int ss_isb_filter(uint32_t *d_guide, float *d_cost, int H, int W, int N, double sigma_i, double sigma_s,
+ * (H x W packed guide) and d_cost (H x W x Np of vtype) are filtered in place. */
+int ss_isb_filter(uint32_t *d_guide, void *d_cost, int H, int W, int N, int vtype, double sigma_i, double sigma_s,
                   void *d_workspace, size_t workspace_bytes, ss_stream stream);
 
 /* Drop-in ISB_Filter.apply on the reference layout: d_guide H x W x 3 uint8,
@@ -136,21 +148,22 @@ int ss_isb_filter_planar(uint8_t *d_guide, float *d_cost, int H, int W, int N, d
 /* Argmin, parabola refinement and index->distance (depth_estimation.py:180-198)
  * on a filtered candidate-contiguous volume.  d0, dN1: first/last candidate;
  * A = d0/dN1 - 1 in fp32.  d_dist: H x W. */
-int ss_argmin_refine(const float *d_cost, int H, int W, int N, float d0, float dN1, float A, float *d_dist,
-                     ss_stream stream);
+int ss_argmin_refine(const void *d_cost, int H, int W, int N, int vtype, float d0, float dN1, float A,
+                     float *d_dist, ss_stream stream);
 
-/* Bytes of workspace for ss_estimate_distance. */
-size_t ss_estimate_workspace_bytes(int H, int W, int N);
+/* Bytes of workspace for ss_estimate_distance with a vtype volume. */
+size_t ss_estimate_workspace_bytes(int H, int W, int N, int vtype);
 
 /* estimate_fisheye_distance (depth_estimation.py:137-203) for one reference:
  * guide, sweep, cost filter with the final up-pass fused with the argmin,
  * then the distance filter at (sigma_i/2, sigma_s/2).  d_guide3 (H x W x 3
  * uint8) is the guide argument of the reference; NULL computes it from
  * d_images[ref] as estimate_RGBD_panorama does (depth_estimation.py:227).
+ * The cost volume is stored as vtype; the distance map is always fp32.
  * d_dist receives the filtered H x W distance map. */
 int ss_estimate_distance(const float *d_raylut, const float *const *d_images, int C, const ss_camera *cams,
                          const float *rts, int ref, const float *cands, int N, int H, int W, double sigma_i,
-                         double sigma_s, const uint8_t *d_guide3, float *d_dist, void *d_workspace,
+                         double sigma_s, int vtype, const uint8_t *d_guide3, float *d_dist, void *d_workspace,
                          size_t workspace_bytes, ss_stream stream);
 
 /* reprojectDistanceKernel (stitcher.cu:181-204) with stitcher.py:184-192's
@@ -186,6 +199,7 @@ typedef struct {
     int C, R, N, H, W;                /* cameras, references, candidates, matching rows and cols */
     int Hs, Ws, PH, PW;               /* stitch image rows/cols, panorama rows/cols */
     int inpaint_iterations;
+    int volume_type;                  /* ss_volume_type of the cost volumes */
     int refs[SS_MAX_REFERENCES];      /* camera index of each reference */
     ss_camera cams[SS_MAX_CAMERAS];
     float rts[SS_MAX_REFERENCES][SS_MAX_CAMERAS * 12]; /* per reference, as ss_sweep */
@@ -193,7 +207,7 @@ typedef struct {
     double sigma_i, sigma_s;
     double thr_empty, thr_src;        /* as ss_inpaint */
     const float *d_raylut[SS_MAX_REFERENCES];
-    void *d_workspace[SS_MAX_REFERENCES]; /* ss_estimate_workspace_bytes each */
+    void *d_workspace[SS_MAX_REFERENCES]; /* ss_estimate_workspace_bytes(H, W, N, volume_type) each */
     size_t workspace_bytes;
     float *d_dist[SS_MAX_REFERENCES]; /* per-reference distance maps, H x W */
     ss_intrinsics intr[SS_MAX_REFERENCES];
@@ -208,13 +222,17 @@ typedef struct {
 } ss_frame_desc;
 
 /* No counterpart in the reference (its per-frame loop is Python).  Creates an
- * executor for desc (copied); it owns one stream per reference and one for
- * graph replay, all at default priority. */
+ * executor for desc (copied) on the current device.  The reference chains run
+ * on a pool of three streams (plus one for graph capture and replay) that
+ * every executor of the device shares; the pool is created on first use, at
+ * default priority, and kept until the process ends.  Calls of ss_frame_run
+ * on one device are serialised while they enqueue work. */
 int ss_frame_create(const ss_frame_desc *desc, void **frame);
 /* Enqueues one frame after the work already on `stream` and makes `stream`
- * wait for it: each reference's estimate_fisheye_distance, uint8 cast of its
- * stitch image, reprojection and inpainting run on that reference's stream;
- * the merge runs on `stream`.  d_match: C float32 H x W x 3 images; d_stitch:
+ * wait for it: one sweep launch covers every reference (and writes their
+ * guides); then each reference's cost filter, argmin, distance filter, uint8
+ * cast of its stitch image, reprojection and inpainting run on that
+ * reference's stream; the merge runs on `stream`.  d_match: C float32 H x W x 3 images; d_stitch:
  * R float32 Hs x Ws x 3 images.  With use_graph, the frame is captured into a
  * HIP graph on the first call (and again whenever the image pointers change)
  * and replayed afterwards. */

@@ -108,7 +108,9 @@ class Oracle:
                                       _ptr(masks), _d(min_dist), _d(max_dist), _ptr(sel))
         return sel
 
-    def sweep(self, imgs, sel, cands, cams6, rts, ref):
+    def sweep(self, imgs, sel, cands, cams6, rts, ref, vtype=0):
+        """Cost volume [N][H][W]; vtype 1 / 2 rounds every stored value to
+        fp16 / bf16 (the precision form; ignored by the fp64 oracle)."""
         imgs = np.ascontiguousarray(imgs, np.float32)
         Cn, H, W, _ = imgs.shape
         sel = np.ascontiguousarray(sel, np.int32)
@@ -116,7 +118,7 @@ class Oracle:
         N = cands.shape[0]
         cost = np.zeros((N, H, W), self.real)
         self._fn("ora_sweep")(_f(Cn), _f(H), _f(W), _f(ref), _ptr(imgs), _ptr(sel), _ptr(cands), _f(N),
-                              _ptr(self._d64(cams6)), _ptr(self._d64(rts)), _ptr(cost))
+                              _ptr(self._d64(cams6)), _ptr(self._d64(rts)), _ptr(cost), _f(vtype))
         return cost
 
     def guide(self, img):
@@ -126,17 +128,17 @@ class Oracle:
         self._fn("ora_guide")(_ptr(img), _f(H * W), _ptr(g))
         return g
 
-    def isb_down(self, gin, cin, hout, wout, var_inv_i):
+    def isb_down(self, gin, cin, hout, wout, var_inv_i, vtype=0):
         gin = np.ascontiguousarray(gin, np.uint8)
         cin = np.ascontiguousarray(cin, self.real)
         N, hin, win = cin.shape
         gout = np.zeros((hout, wout, 3), np.uint8)
         cout = np.zeros((N, hout, wout), self.real)
         self._fn("ora_isb_down")(_ptr(gin), _ptr(cin), _f(hin), _f(win), _ptr(gout), _ptr(cout), _f(hout),
-                                 _f(wout), _f(N), _d(var_inv_i))
+                                 _f(wout), _f(N), _d(var_inv_i), _f(vtype))
         return gout, cout
 
-    def isb_up(self, gin, cin, gio, cio, weight_up, weight_down, var_inv_i):
+    def isb_up(self, gin, cin, gio, cio, weight_up, weight_down, var_inv_i, vtype=0):
         """Returns updated copies of (gio, cio)."""
         gin = np.ascontiguousarray(gin, np.uint8)
         cin = np.ascontiguousarray(cin, self.real)
@@ -145,17 +147,20 @@ class Oracle:
         N, hin, win = cin.shape
         _, hio, wio = cio.shape
         self._fn("ora_isb_up")(_ptr(gin), _ptr(cin), _f(hin), _f(win), _ptr(gio), _ptr(cio), _f(hio), _f(wio),
-                               _f(N), _d(weight_up), _d(weight_down), _d(var_inv_i))
+                               _f(N), _d(weight_up), _d(weight_down), _d(var_inv_i), _f(vtype))
         return gio, cio
 
-    def isb_filter(self, guide, cost, sigma_i, sigma_s):
-        """ISB_Filter.apply on copies: returns (cost, guide) filtered."""
+    def isb_filter(self, guide, cost, sigma_i, sigma_s, vtype=0, round_level0=True):
+        """ISB_Filter.apply on copies: returns (cost, guide) filtered.  vtype:
+        storage rounding of every level written; round_level0=False leaves
+        the final level-0 values unrounded (the fused argmin reads them)."""
         guide = np.array(guide, np.uint8, copy=True, order="C")
         cost = np.array(cost, self.real, copy=True, order="C")
         if cost.ndim == 2:
             cost = cost[None]
         N, H, W = cost.shape
-        rc = self._fn("ora_isb_filter")(_ptr(guide), _ptr(cost), _f(H), _f(W), _f(N), _d(sigma_i), _d(sigma_s))
+        rc = self._fn("ora_isb_filter")(_ptr(guide), _ptr(cost), _f(H), _f(W), _f(N), _d(sigma_i), _d(sigma_s),
+                                        _f(vtype), _f(int(bool(round_level0))))
         if rc < 0:
             raise ValueError("resolution %dx%d gives no pyramid level" % (W, H))
         return cost, guide
@@ -167,6 +172,12 @@ class Oracle:
         idx = np.zeros((H, W), np.int32)
         self._fn("ora_argmin_refine")(_ptr(cost), _f(N), _f(H * W), _d(d0), _d(dN1), _d(A), _ptr(dist), _ptr(idx))
         return dist, idx
+
+    def round_volume(self, x, vtype):
+        """Values rounded as the precision form stores them (1 fp16, 2 bf16)."""
+        x = np.array(x, self.real, copy=True, order="C")
+        self._fn("ora_round_volume")(_ptr(x), C.c_long(x.size), _f(vtype))
+        return x
 
     # ---- stitcher ------------------------------------------------------------
     def reproject_zmin(self, din, intr6, t3):
@@ -245,11 +256,12 @@ _SIGNATURES = {
     "ora_unproject_cuda": [_p, _p, _f, _p],
     "ora_project_cuda": [_p, _p, _f, _p, _p],
     "ora_select_camera": [_f, _f, _f, _f, _p, _p, _p, _d, _d, _p],
-    "ora_sweep": [_f, _f, _f, _f, _p, _p, _p, _f, _p, _p, _p],
+    "ora_sweep": [_f, _f, _f, _f, _p, _p, _p, _f, _p, _p, _p, _f],
     "ora_guide": [_p, _f, _p],
-    "ora_isb_down": [_p, _p, _f, _f, _p, _p, _f, _f, _f, _d],
-    "ora_isb_up": [_p, _p, _f, _f, _p, _p, _f, _f, _f, _d, _d, _d],
-    "ora_isb_filter": [_p, _p, _f, _f, _f, _d, _d],
+    "ora_isb_down": [_p, _p, _f, _f, _p, _p, _f, _f, _f, _d, _f],
+    "ora_isb_up": [_p, _p, _f, _f, _p, _p, _f, _f, _f, _d, _d, _d, _f],
+    "ora_isb_filter": [_p, _p, _f, _f, _f, _d, _d, _f, _f],
+    "ora_round_volume": [_p, C.c_long, _f],
     "ora_argmin_refine": [_p, _f, _f, _d, _d, _d, _p, _p],
     "ora_reproject_zmin": [_p, _p, _f, _f, _p, _p],
     "ora_inpaint_lut": [_p, _f, _f, _p, _p, _d, _d],

@@ -36,8 +36,11 @@ def thresholds(max_dist):
 class OraclePipeline:
     def __init__(self, calibrations, min_dist, max_dist, candidate_count, references_indices, reprojection_viewpoint,
                  masks, matching_resolution, rgb_to_stitch_resolution, panorama_resolution, sigma_i, sigma_s,
-                 precision="f32", smoothing_radius=15, inpainting_iterations=32):
+                 precision="f32", smoothing_radius=15, inpainting_iterations=32, volume_dtype="float32"):
         self.o = Oracle(precision)
+        # storage rounding of the cost volume (precision form); the fp64
+        # oracle ignores it
+        self.vtype = {"float32": 0, "float16": 1, "bfloat16": 2}[volume_dtype]
         dt = torch.float32 if precision == "f32" else torch.float64
         self.dt = dt
         self.calibs = calibrations
@@ -93,11 +96,12 @@ class OraclePipeline:
         """depth_estimation.py:137-203 for camera r."""
         imgs = np.stack([_np(i) for i in images_to_match])
         guide = self.o.guide(imgs[r])
-        cost = self.o.sweep(imgs, self.sel[r], self.cands, self.cams, self.rts[r], r)
+        cost = self.o.sweep(imgs, self.sel[r], self.cands, self.cams, self.rts[r], r, self.vtype)
         if keep is not None:
             keep["cost_raw"] = cost.copy()
             keep["guide"] = guide.copy()
-        cost, _ = self.o.isb_filter(guide, cost, self.sigma_i, self.sigma_s)
+        # the last up pass feeds the argmin without a stored (rounded) level 0
+        cost, _ = self.o.isb_filter(guide, cost, self.sigma_i, self.sigma_s, self.vtype, round_level0=False)
         dist, idx = self.o.argmin_refine(cost, self.cands[0], self.cands[-1], self.A)
         if keep is not None:
             keep["cost_filtered"] = cost

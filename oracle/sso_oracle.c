@@ -103,6 +103,54 @@ static real ora_exp(real x)
 real SFX(ora_expf)(real x) { return ora_exp(x); }
 
 /* ------------------------------------------------------------------------ */
+/* Storage rounding of the cost volume (the precision form): vt = 0 keeps    */
+/* fp32, vt = 1 rounds to the nearest binary16 value, vt = 2 to the nearest  */
+/* bfloat16 value, ties to even, NaN kept.  Applied where a volume value is  */
+/* stored (sweep output, pyramid levels); all arithmetic stays fp32.  The    */
+/* fp64 build ignores vt: it is the unrounded yardstick.                     */
+/* ------------------------------------------------------------------------ */
+#if !ORA_F64
+static float round_f16(float x)
+{
+    if (x != x || x == 0.0f || isinf(x)) return x;
+    int e;
+    frexpf(x, &e); /* |x| = m * 2^e, m in [0.5, 1) */
+    int q = e - 11; /* exponent of one unit in the last place of an 11-bit significand */
+    if (q < -24) q = -24; /* binary16 subnormals are multiples of 2^-24 */
+    float y = ldexpf(rintf(ldexpf(x, -q)), q); /* exact scalings; rintf rounds to nearest even */
+    return fabsf(y) > 65504.0f ? copysignf(INFINITY, x) : y;
+}
+
+static float round_bf16(float x)
+{
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u)
+        u |= 0x00400000u;
+    else
+        u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    memcpy(&x, &u, 4);
+    return x;
+}
+#endif
+
+static real vstore(real x, int vt)
+{
+#if ORA_F64
+    (void)vt;
+    return x;
+#else
+    return vt == 1 ? round_f16(x) : (vt == 2 ? round_bf16(x) : x);
+#endif
+}
+
+void SFX(ora_round_volume)(real *x, long n, int vt)
+{
+    for (long i = 0; i < n; i++) x[i] = vstore(x[i], vt);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Double-sphere model as utils.py:57-94 writes it with torch tensors:       */
 /* fp32 tensor ops whose Python-double scalar operands (alpha**2, 2*alpha-1, */
 /* 1-xi**2, 1-alpha, -w2) are rounded to fp32 once.  Camera record:         */
@@ -291,7 +339,7 @@ void SFX(ora_select_camera)(int C, int H, int W, int ref, const double *cams, co
  * distance candidates 1/linspace(1/min, 1/max, N); cost: N x H x W,
  * cost = min(500, |s0-r0| + |s1-r1| + |s2-r2|), s = 0 where sel < 0. */
 void SFX(ora_sweep)(int C, int H, int W, int ref, const float *imgs, const int32_t *sel,
-                    const double *cands, int N, const double *cams, const double *rts, real *cost)
+                    const double *cands, int N, const double *cams, const double *rts, real *cost, int vt)
 {
     TCam rc, *cc = (TCam *)malloc(sizeof(TCam) * C);
     real *rt = (real *)malloc(sizeof(real) * 12 * C);
@@ -320,7 +368,7 @@ void SFX(ora_sweep)(int C, int H, int W, int ref, const float *imgs, const int32
                                 grid_coord(uv[1], H), s);
                 }
                 real c = (RFABS(s[0] - r0) + RFABS(s[1] - r1)) + RFABS(s[2] - r2);
-                cost[(size_t)n * HW + pix] = c > K(500.0) ? K(500.0) : c;
+                cost[(size_t)n * HW + pix] = vstore(c > K(500.0) ? K(500.0) : c, vt);
             }
         }
     free(cc);
@@ -356,7 +404,7 @@ static real gdiff(const uint8_t *a, const uint8_t *b)
 
 /* guideDownsample2xKernel (isb_filter.cu:44-119) */
 void SFX(ora_isb_down)(const uint8_t *gin, const real *cin, int hin, int win, uint8_t *gout, real *cout,
-                       int hout, int wout, int N, double var_inv_i)
+                       int hout, int wout, int N, double var_inv_i, int vt)
 {
     static const real base[8] = {K(0.25), K(0.5), K(0.25), K(0.5), K(0.25), K(0.5), K(0.25), K(0.5)};
     const real vi = (real)var_inv_i;
@@ -388,7 +436,7 @@ void SFX(ora_isb_down)(const uint8_t *gin, const real *cin, int hin, int win, ui
                 const real *cz = cin + (size_t)z * pin;
                 real acc = cz[ctr];
                 for (int i = 0; i < 8; i++) acc = acc + cz[nb[i]] * w[i];
-                cout[(size_t)z * pout + o] = acc / ws;
+                cout[(size_t)z * pout + o] = vstore(acc / ws, vt);
             }
         }
 }
@@ -396,7 +444,7 @@ void SFX(ora_isb_down)(const uint8_t *gin, const real *cin, int hin, int win, ui
 /* guideUpsample2xKernel (isb_filter.cu:132-330): coarse (gin, cin) merged into
  * fine (gio, cio) in place. */
 void SFX(ora_isb_up)(const uint8_t *gin, const real *cin, int hin, int win, uint8_t *gio, real *cio, int hio,
-                     int wio, int N, double weight_up, double weight_down, double var_inv_i)
+                     int wio, int N, double weight_up, double weight_down, double var_inv_i, int vt)
 {
     const real wu = (real)weight_up, wd = (real)weight_down, vi = (real)var_inv_i;
     const size_t pin = (size_t)hin * win, pio = (size_t)hio * wio;
@@ -450,7 +498,7 @@ void SFX(ora_isb_up)(const uint8_t *gin, const real *cin, int hin, int win, uint
                     if (!doit[s]) continue;
                     real acc = wu * fz[fine[s]];
                     for (int i = 0; i < ntap[s]; i++) acc = acc + w[s][i] * cz[tapidx[s][i]];
-                    fz[fine[s]] = acc * inv[s];
+                    fz[fine[s]] = vstore(acc * inv[s], vt);
                 }
             }
         }
@@ -467,8 +515,11 @@ int SFX(ora_isb_levels)(int W, int H) { return ilog_levels(W, H); }
 
 /* ISB_Filter.apply (isb_filter.py:72-136): guide (H x W x 3) and cost
  * (N x H x W) are filtered in place. Returns the level count, or -1 when the
- * reference itself would fail (fewer than one level). */
-int SFX(ora_isb_filter)(uint8_t *guide, real *cost, int H, int W, int N, double sigma_i, double sigma_s)
+ * reference itself would fail (fewer than one level).  vt: storage rounding of
+ * every level written (see vstore); round_level0 = 0 leaves the final level-0
+ * values unrounded, as when the last up pass feeds the argmin directly. */
+int SFX(ora_isb_filter)(uint8_t *guide, real *cost, int H, int W, int N, double sigma_i, double sigma_s, int vt,
+                        int round_level0)
 {
     int L = ilog_levels(W, H);
     if (L < 1) return -1;
@@ -488,12 +539,13 @@ int SFX(ora_isb_filter)(uint8_t *guide, real *cost, int H, int W, int N, double 
     double var_inv_s = 1 / (2 * sigma_s * sigma_s);
     double var_inv_i = 1 / (2 * sigma_i * sigma_i);
     for (int s = 1; s < L; s++)
-        SFX(ora_isb_down)(g[s - 1], c[s - 1], hs[s - 1], ws[s - 1], g[s], c[s], hs[s], ws[s], N, var_inv_i);
+        SFX(ora_isb_down)(g[s - 1], c[s - 1], hs[s - 1], ws[s - 1], g[s], c[s], hs[s], ws[s], N, var_inv_i, vt);
     for (int s = L - 2; s >= 0; s--) {
         double dist = pow(2.0, s) - 0.5;
         double wdn = exp(-(dist * dist) * var_inv_s);
         double wup = 1 - wdn;
-        SFX(ora_isb_up)(g[s + 1], c[s + 1], hs[s + 1], ws[s + 1], g[s], c[s], hs[s], ws[s], N, wup, wdn, var_inv_i);
+        SFX(ora_isb_up)(g[s + 1], c[s + 1], hs[s + 1], ws[s + 1], g[s], c[s], hs[s], ws[s], N, wup, wdn, var_inv_i,
+                        (s > 0 || round_level0) ? vt : 0);
     }
     for (int s = 1; s < L; s++) {
         free(g[s]);

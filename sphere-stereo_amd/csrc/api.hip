@@ -109,24 +109,27 @@ int padded(int N) {
 struct Level {
   int h, w;
   uint32_t* g;
-  float* c;
+  void* c;
 };
 
+size_t elem_bytes(int vt) { return vt == SS_VOLUME_F32 ? 4 : 2; }
+bool valid_vtype(int vt) { return vt == SS_VOLUME_F32 || vt == SS_VOLUME_F16 || vt == SS_VOLUME_BF16; }
+
 // Level sizes ceil(rows / 2^s) x ceil(cols / 2^s) (isb_filter.py:64-70) and
-// their buffers for s >= 1.
-void carve_levels(Carver& cv, Level* lv, int L, int H, int W, int NP) {
+// their buffers for s >= 1 (volume elements of eb bytes).
+void carve_levels(Carver& cv, Level* lv, int L, int H, int W, int NP, size_t eb) {
   for (int s = 1; s < L; ++s) {
     lv[s].h = (int)ceil(H / pow(2.0, s));
     lv[s].w = (int)ceil(W / pow(2.0, s));
     lv[s].g = cv.take<uint32_t>((size_t)lv[s].h * lv[s].w);
-    lv[s].c = cv.take<float>((size_t)lv[s].h * lv[s].w * NP);
+    lv[s].c = cv.take<char>((size_t)lv[s].h * lv[s].w * NP * eb);
   }
 }
 
 // ISB_Filter.apply's two launch loops (isb_filter.py:87-134).  With
 // fuse_last, the level-0 up-pass computes the argmin instead of writing the
 // filtered volume.
-int run_pyramid(Level* lv, int L, int G, int V, double sigma_i, double sigma_s, bool fuse_last,
+int run_pyramid(Level* lv, int L, int G, int V, int vt, double sigma_i, double sigma_s, bool fuse_last,
                 const ss::ArgminParams& am, float* dist, hipStream_t s, int stage_down, int stage_up) {
   const double var_inv_s = 1 / (2 * sigma_s * sigma_s);
   const double var_inv_i = 1 / (2 * sigma_i * sigma_i);
@@ -143,9 +146,13 @@ int run_pyramid(Level* lv, int L, int G, int V, double sigma_i, double sigma_s, 
   // the LDS budget.
   const long long kTailPixels = 256;
   const int NP = G * V;
+  const size_t eb = elem_bytes(vt);
   auto lds_need = [&](int from) {
     size_t b = 0;
-    for (int k = from + 1; k < L; ++k) b += ((size_t)lv[k].h * lv[k].w * (4 + 4 * NP) + 15) & ~size_t(15);
+    for (int k = from + 1; k < L; ++k) {
+      const size_t px = (size_t)lv[k].h * lv[k].w;
+      b += ((px * 4 + 15) & ~size_t(15)) + ((px * NP * eb + 15) & ~size_t(15));
+    }
     return b;
   };
   int s0 = L - 1;
@@ -159,7 +166,7 @@ int run_pyramid(Level* lv, int L, int G, int V, double sigma_i, double sigma_s, 
   for (int k = 1; k <= last_down; ++k) {
     ss::IsbDownArgs a{lv[k - 1].g, lv[k - 1].c, lv[k - 1].h, lv[k - 1].w, lv[k].g, lv[k].c, lv[k].h, lv[k].w, vi};
     StageTimer t(stage_down, s);
-    SS_HIP(ss::launch_isb_down(a, G, V, s));
+    SS_HIP(ss::launch_isb_down(a, G, V, vt, s));
   }
   if (tail) {
     ss::IsbTailArgs ta;
@@ -177,7 +184,7 @@ int run_pyramid(Level* lv, int L, int G, int V, double sigma_i, double sigma_s, 
         ta.goff[e] = off;
         off += (px * 4 + 15) & ~15u;
         ta.coff[e] = off;
-        off += (px * 4 * NP + 15) & ~15u;
+        off += (px * (unsigned)(NP * eb) + 15) & ~15u;
       }
       double wu, wd;
       weights(s0 + e, wu, wd);
@@ -186,7 +193,7 @@ int run_pyramid(Level* lv, int L, int G, int V, double sigma_i, double sigma_s, 
     }
     ta.lds_bytes = off;
     StageTimer t(stage_down, s);
-    SS_HIP(ss::launch_isb_tail(ta, G, V, s));
+    SS_HIP(ss::launch_isb_tail(ta, G, V, vt, s));
   }
   for (int k = (tail ? s0 : L - 1) - 1; k >= 0; --k) {
     double wu, wd;
@@ -195,11 +202,11 @@ int run_pyramid(Level* lv, int L, int G, int V, double sigma_i, double sigma_s, 
                     (float)wu,   (float)wd,   vi,          dist,        am};
     const bool fuse = fuse_last && k == 0;
     StageTimer t(fuse ? SS_STAGE_ISB_UP_ARGMIN : stage_up, s);
-    SS_HIP(ss::launch_isb_up(a, G, V, fuse, s));
+    SS_HIP(ss::launch_isb_up(a, G, V, vt, fuse, s));
   }
   if (fuse_last && L == 1) {
     StageTimer t(SS_STAGE_ISB_UP_ARGMIN, s);
-    SS_HIP(ss::launch_argmin(lv[0].c, lv[0].h * lv[0].w, V, am, dist, s));
+    SS_HIP(ss::launch_argmin(lv[0].c, lv[0].h * lv[0].w, V, vt, am, dist, s));
   }
   return SS_OK;
 }
@@ -213,34 +220,60 @@ int check_cams(const ss_camera* cams, const float* rts, int C, int ref) {
   return SS_OK;
 }
 
-int fill_sweep_args(ss::SweepArgs& a, const float* d_raylut, const float* const* d_images, int C,
-                    const ss_camera* cams, const float* rts, int ref, const float* cands, int N, int H, int W,
-                    float* d_cost) {
-  int rc = check_cams(cams, rts, C, ref);
-  if (rc) return rc;
-  SS_REQUIRE(d_raylut && d_images && cands, "null pointer argument");
+int check_sweep(const float* const* d_images, int C, const ss_camera* cams, const float* cands, int N, int H,
+                int W, int vt) {
+  SS_REQUIRE(d_images && cams && cands, "null pointer argument");
+  SS_REQUIRE(C >= 1 && C <= SS_MAX_CAMERAS, "camera count %d outside [1, %d]", C, SS_MAX_CAMERAS);
   SS_REQUIRE(N >= 1 && N <= SS_MAX_CANDIDATES, "candidate count %d outside [1, %d]", N, SS_MAX_CANDIDATES);
   SS_REQUIRE(H >= 1 && W >= 1, "bad matching resolution %dx%d", W, H);
+  SS_REQUIRE(valid_vtype(vt), "unknown volume type %d", vt);
+  for (int k = 0; k < C; ++k) SS_REQUIRE(d_images[k], "null image pointer for camera %d", k);
+  return SS_OK;
+}
+
+// One sweep launch over references refs[0..R) (R * C <= kSweepMaxPairs):
+// rts[r] holds reference r's C transforms, costs[r] / guides[r] its outputs
+// (guides[r] may be null).  Requires N >= 2 (the volume stride is 8 * V).
+int launch_sweep_refs(const float* const* d_raylut, const float* const* d_images, int C, const ss_camera* cams,
+                      const float* const* rts, const int* refs, int R, const float* cands, int N, int H, int W,
+                      int vt, void* const* costs, uint32_t* const* guides, hipStream_t s) {
+  ss::SweepArgs a;
   memset(&a, 0, sizeof a);
-  a.raylut = reinterpret_cast<const float4*>(d_raylut);
   for (int k = 0; k < C; ++k) {
-    SS_REQUIRE(d_images[k], "null image pointer for camera %d", k);
     a.img[k] = d_images[k];
     a.cam[k] = ss::make_torch_cam(cams[k]);
-    memcpy(a.rt[k], rts + 12 * k, sizeof(float) * 12);
   }
-  a.ref_img = d_images[ref];
+  for (int r = 0; r < R; ++r) {
+    SS_REQUIRE(refs[r] >= 0 && refs[r] < C, "reference index %d outside [0, %d)", refs[r], C);
+    SS_REQUIRE(d_raylut[r] && costs[r] && rts[r], "null ray table, cost volume or transforms");
+    a.ref[r].raylut = reinterpret_cast<const float4*>(d_raylut[r]);
+    a.ref[r].img = d_images[refs[r]];
+    a.ref[r].cost = costs[r];
+    a.ref[r].guide = guides ? guides[r] : nullptr;
+    a.ref[r].rt0 = r * C;
+    for (int k = 0; k < C; ++k) memcpy(a.rt[r * C + k], rts[r] + 12 * k, sizeof(float) * 12);
+  }
   memcpy(a.cands, cands, sizeof(float) * N);
-  a.cost = d_cost;
+  const int V = ss::lanes_v(N);
+  SS_REQUIRE(V > 0, "candidate count %d unsupported", N);
+  a.R = R;
   a.H = H;
   a.W = W;
   a.N = N;
+  a.NP = 8 * V;
+  a.tiles_x = (W + 63) / 64;
+  a.blocks_per_ref = a.tiles_x * ((H + 3) / 4);
   a.Wf = (float)W;
   a.Hf = (float)H;
   a.invW = 1.0f / (float)W;
   a.invH = 1.0f / (float)H;
-  a.pow2W = pow2(W);
-  a.pow2H = pow2(H);
+  a.inv2W = 2.0f / (float)W;
+  a.inv2H = 2.0f / (float)H;
+  a.halfW = (float)W * 0.5f;
+  a.halfH = (float)H * 0.5f;
+  a.pow2 = pow2(W) && pow2(H);
+  StageTimer t(SS_STAGE_SWEEP, s);
+  SS_HIP(ss::launch_sweep(a, vt, s));
   return SS_OK;
 }
 
@@ -358,67 +391,71 @@ int ss_guide(const float* d_img, int H, int W, uint32_t* d_guide, ss_stream stre
 }
 
 int ss_sweep(const float* d_raylut, const float* const* d_images, int C, const ss_camera* cams, const float* rts,
-             int ref, const float* cands, int N, int H, int W, float* d_cost, ss_stream stream) {
-  SS_REQUIRE(d_cost, "null cost volume");
-  ss::SweepArgs a;
-  int rc = fill_sweep_args(a, d_raylut, d_images, C, cams, rts, ref, cands, N, H, W, d_cost);
+             int ref, const float* cands, int N, int H, int W, int vtype, void* d_cost, uint32_t* d_guide,
+             ss_stream stream) {
+  SS_REQUIRE(d_cost && d_raylut && rts, "null pointer argument");
+  int rc = check_sweep(d_images, C, cams, cands, N, H, W, vtype);
   if (rc) return rc;
-  const int V = ss::lanes_v(N);
-  SS_REQUIRE(V > 0, "candidate count %d unsupported", N);
-  StageTimer t(SS_STAGE_SWEEP, as_stream(stream));
-  SS_HIP(ss::launch_sweep(a, V, as_stream(stream)));
-  return ok();
+  SS_REQUIRE(ref >= 0 && ref < C, "reference index %d outside [0, %d)", ref, C);
+  SS_REQUIRE(N >= 2, "the sweep needs at least 2 candidates (1 candidate gives d_{N-1} everywhere)");
+  rc = launch_sweep_refs(&d_raylut, d_images, C, cams, &rts, &ref, 1, cands, N, H, W, vtype, &d_cost, &d_guide,
+                         as_stream(stream));
+  return rc ? rc : ok();
 }
 
-size_t ss_isb_workspace_bytes(int H, int W, int N, int planar) {
+size_t ss_isb_workspace_bytes(int H, int W, int N, int vtype, int planar) {
   const int L = levels_of(H, W), NP = padded(N);
-  if (L < 1 || NP < 0) return 0;
+  if (L < 1 || NP < 0 || !valid_vtype(vtype)) return 0;
+  if (planar && vtype != SS_VOLUME_F32) return 0;
+  const size_t eb = elem_bytes(vtype);
   Carver cv(nullptr);
   Level lv[32];
   if (planar) {
     cv.take<uint32_t>((size_t)H * W);
     cv.take<float>((size_t)H * W * NP);
   }
-  carve_levels(cv, lv, L, H, W, NP);
+  carve_levels(cv, lv, L, H, W, NP, eb);
   return cv.off + 256;
 }
 
-static int isb_filter_impl(uint32_t* d_guide, float* d_cost, int H, int W, int N, double sigma_i, double sigma_s,
-                           Carver& cv, hipStream_t s) {
+static int isb_filter_impl(uint32_t* d_guide, void* d_cost, int H, int W, int N, int vt, double sigma_i,
+                           double sigma_s, Carver& cv, hipStream_t s) {
   const int L = levels_of(H, W);
   const int NP = padded(N);
   Level lv[32];
   lv[0] = Level{H, W, d_guide, d_cost};
-  carve_levels(cv, lv, L, H, W, NP);
+  carve_levels(cv, lv, L, H, W, NP, elem_bytes(vt));
   const int G = N == 1 ? 1 : 8, V = N == 1 ? 1 : NP / 8;
   ss::ArgminParams am{};
-  return run_pyramid(lv, L, G, V, sigma_i, sigma_s, false, am, nullptr, s, SS_STAGE_ISB_DOWN, SS_STAGE_ISB_UP);
+  return run_pyramid(lv, L, G, V, vt, sigma_i, sigma_s, false, am, nullptr, s, SS_STAGE_ISB_DOWN, SS_STAGE_ISB_UP);
 }
 
-static int check_isb(int H, int W, int N, void* ws, size_t bytes, int planar) {
+static int check_isb(int H, int W, int N, int vt, void* ws, size_t bytes, int planar) {
   SS_REQUIRE(H >= 1 && W >= 1, "bad resolution %dx%d", W, H);
   SS_REQUIRE(levels_of(H, W) >= 1, "resolution %dx%d gives no pyramid level (min side must be >= 4)", W, H);
   SS_REQUIRE(N >= 1 && padded(N) > 0, "candidate count %d unsupported (1..%d)", N, SS_MAX_CANDIDATES);
-  const size_t need = ss_isb_workspace_bytes(H, W, N, planar);
+  SS_REQUIRE(valid_vtype(vt), "unknown volume type %d", vt);
+  SS_REQUIRE(N > 1 || vt == SS_VOLUME_F32, "a one-candidate volume is filtered in fp32 only");
+  const size_t need = ss_isb_workspace_bytes(H, W, N, vt, planar);
   if (!ws || bytes < need)
     return fail(SS_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need, ws ? bytes : (size_t)0);
   return SS_OK;
 }
 
-int ss_isb_filter(uint32_t* d_guide, float* d_cost, int H, int W, int N, double sigma_i, double sigma_s,
+int ss_isb_filter(uint32_t* d_guide, void* d_cost, int H, int W, int N, int vtype, double sigma_i, double sigma_s,
                   void* d_workspace, size_t workspace_bytes, ss_stream stream) {
   SS_REQUIRE(d_guide && d_cost, "null pointer argument");
-  int rc = check_isb(H, W, N, d_workspace, workspace_bytes, 0);
+  int rc = check_isb(H, W, N, vtype, d_workspace, workspace_bytes, 0);
   if (rc) return rc;
   Carver cv(d_workspace);
-  rc = isb_filter_impl(d_guide, d_cost, H, W, N, sigma_i, sigma_s, cv, as_stream(stream));
+  rc = isb_filter_impl(d_guide, d_cost, H, W, N, vtype, sigma_i, sigma_s, cv, as_stream(stream));
   return rc ? rc : ok();
 }
 
 int ss_isb_filter_planar(uint8_t* d_guide, float* d_cost, int H, int W, int N, double sigma_i, double sigma_s,
                          void* d_workspace, size_t workspace_bytes, ss_stream stream) {
   SS_REQUIRE(d_guide && d_cost, "null pointer argument");
-  int rc = check_isb(H, W, N, d_workspace, workspace_bytes, 1);
+  int rc = check_isb(H, W, N, SS_VOLUME_F32, d_workspace, workspace_bytes, 1);
   if (rc) return rc;
   hipStream_t s = as_stream(stream);
   Carver cv(d_workspace);
@@ -427,27 +464,32 @@ int ss_isb_filter_planar(uint8_t* d_guide, float* d_cost, int H, int W, int N, d
   float* c = cv.take<float>((size_t)H * W * NP);
   SS_HIP(ss::launch_guide3_to_packed(d_guide, g, H * W, s));
   SS_HIP(ss::launch_planar_to_cc(d_cost, c, H * W, N, NP, s));
-  rc = isb_filter_impl(g, c, H, W, N, sigma_i, sigma_s, cv, s);
+  rc = isb_filter_impl(g, c, H, W, N, SS_VOLUME_F32, sigma_i, sigma_s, cv, s);
   if (rc) return rc;
   SS_HIP(ss::launch_packed_to_guide3(g, d_guide, H * W, s));
   SS_HIP(ss::launch_cc_to_planar(c, d_cost, H * W, N, NP, s));
   return ok();
 }
 
-int ss_argmin_refine(const float* d_cost, int H, int W, int N, float d0, float dN1, float A, float* d_dist,
-                     ss_stream stream) {
+int ss_argmin_refine(const void* d_cost, int H, int W, int N, int vtype, float d0, float dN1, float A,
+                     float* d_dist, ss_stream stream) {
   SS_REQUIRE(d_cost && d_dist && H >= 1 && W >= 1, "bad argument");
   SS_REQUIRE(N >= 2 && N <= SS_MAX_CANDIDATES, "candidate count %d outside [2, %d]", N, SS_MAX_CANDIDATES);
+  SS_REQUIRE(valid_vtype(vtype), "unknown volume type %d", vtype);
   ss::ArgminParams am{N, d0, dN1, A};
-  SS_HIP(ss::launch_argmin(d_cost, H * W, ss::lanes_v(N), am, d_dist, as_stream(stream)));
+  SS_HIP(ss::launch_argmin(d_cost, H * W, ss::lanes_v(N), vtype, am, d_dist, as_stream(stream)));
   return ok();
 }
 
-static size_t estimate_layout(Carver& cv, int H, int W, int N, uint32_t** g0, float** c0, Level* lv, Level* dl) {
+// Workspace of ss_estimate_distance: level-0 guide and volume, the cost
+// pyramid, and the distance pyramid (which shares the guide levels).
+static size_t estimate_layout(Carver& cv, int H, int W, int N, int vt, uint32_t** g0, void** c0, Level* lv,
+                              Level* dl) {
   const int L = levels_of(H, W), NP = padded(N < 2 ? 2 : N);
+  const size_t eb = elem_bytes(vt);
   *g0 = cv.take<uint32_t>((size_t)H * W);
-  *c0 = cv.take<float>((size_t)H * W * NP);
-  carve_levels(cv, lv, L, H, W, NP);
+  *c0 = cv.take<char>((size_t)H * W * NP * eb);
+  carve_levels(cv, lv, L, H, W, NP, eb);
   for (int s = 1; s < L; ++s) {
     dl[s].h = lv[s].h;
     dl[s].w = lv[s].w;
@@ -457,63 +499,116 @@ static size_t estimate_layout(Carver& cv, int H, int W, int N, uint32_t** g0, fl
   return cv.off + 256;
 }
 
-size_t ss_estimate_workspace_bytes(int H, int W, int N) {
-  if (levels_of(H, W) < 1 || N < 1 || padded(N) < 0) return 0;
+size_t ss_estimate_workspace_bytes(int H, int W, int N, int vtype) {
+  if (levels_of(H, W) < 1 || N < 1 || padded(N) < 0 || !valid_vtype(vtype)) return 0;
   Carver cv(nullptr);
   uint32_t* g0;
-  float* c0;
+  void* c0;
   Level lv[32], dl[32];
-  return estimate_layout(cv, H, W, N, &g0, &c0, lv, dl);
+  return estimate_layout(cv, H, W, N, vtype, &g0, &c0, lv, dl);
 }
 
-int ss_estimate_distance(const float* d_raylut, const float* const* d_images, int C, const ss_camera* cams,
-                         const float* rts, int ref, const float* cands, int N, int H, int W, double sigma_i,
-                         double sigma_s, const uint8_t* d_guide3, float* d_dist, void* d_workspace,
-                         size_t workspace_bytes, ss_stream stream) {
-  hipStream_t s = as_stream(stream);
-  SS_REQUIRE(d_dist, "null distance output");
-  SS_REQUIRE(levels_of(H, W) >= 1, "resolution %dx%d gives no pyramid level (min side must be >= 4)", W, H);
-  ss::SweepArgs sa;
-  int rc = fill_sweep_args(sa, d_raylut, d_images, C, cams, rts, ref, cands, N, H, W, nullptr);
-  if (rc) return rc;
-  const size_t need = ss_estimate_workspace_bytes(H, W, N);
-  if (!d_workspace || workspace_bytes < need)
-    return fail(SS_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need, workspace_bytes);
-  const int L = levels_of(H, W);
-  Carver cv(d_workspace);
-  uint32_t* g0;
-  float* c0;
+}  // extern "C"
+
+namespace ss {
+
+size_t volume_elem_bytes(int vt) { return elem_bytes(vt); }
+
+void estimate_level0(void* ws, int H, int W, int N, int vt, uint32_t** g0, void** c0) {
+  Carver cv(ws);
   Level lv[32], dl[32];
-  estimate_layout(cv, H, W, N, &g0, &c0, lv, dl);
+  estimate_layout(cv, H, W, N, vt, g0, c0, lv, dl);
+}
+
+int estimate_after_sweep(int H, int W, int N, int vt, const float* cands, double sigma_i, double sigma_s,
+                         float* d_dist, void* ws, hipStream_t s) {
+  const int L = levels_of(H, W);
+  Carver cv(ws);
+  uint32_t* g0;
+  void* c0;
+  Level lv[32], dl[32];
+  estimate_layout(cv, H, W, N, vt, &g0, &c0, lv, dl);
   const float d0 = cands[0], dN1 = cands[N - 1];
-  {
-    StageTimer t(SS_STAGE_GUIDE, s);
-    if (d_guide3)
-      SS_HIP(ss::launch_guide3_to_packed(d_guide3, g0, H * W, s));
-    else
-      SS_HIP(ss::launch_guide(d_images[ref], H * W, g0, s));
-  }
   if (N == 1) {
     // one candidate: max == min everywhere, so the reference returns d_{N-1}
     // (depth_estimation.py:198) whatever the cost
     SS_HIP(ss::launch_fill(d_dist, dN1, (size_t)H * W, s));
   } else {
-    sa.cost = c0;
-    const int V = ss::lanes_v(N);
-    {
-      StageTimer t(SS_STAGE_SWEEP, s);
-      SS_HIP(ss::launch_sweep(sa, V, s));
-    }
     lv[0] = Level{H, W, g0, c0};
     ss::ArgminParams am{N, d0, dN1, (float)(d0 / dN1) - 1.0f};
-    rc = run_pyramid(lv, L, 8, V, sigma_i, sigma_s, true, am, d_dist, s, SS_STAGE_ISB_DOWN, SS_STAGE_ISB_UP);
+    const int rc = run_pyramid(lv, L, 8, ss::lanes_v(N), vt, sigma_i, sigma_s, true, am, d_dist, s,
+                               SS_STAGE_ISB_DOWN, SS_STAGE_ISB_UP);
     if (rc) return rc;
   }
   dl[0] = Level{H, W, g0, d_dist};
-  rc = run_pyramid(dl, L, 1, 1, sigma_i / 2, sigma_s / 2, false, ss::ArgminParams{}, nullptr, s, SS_STAGE_DIST_DOWN,
-                   SS_STAGE_DIST_UP);
+  return run_pyramid(dl, L, 1, 1, SS_VOLUME_F32, sigma_i / 2, sigma_s / 2, false, ss::ArgminParams{}, nullptr, s,
+                     SS_STAGE_DIST_DOWN, SS_STAGE_DIST_UP);
+}
+
+// Sweeps (and guides) of R references into their estimate workspaces, in as
+// few launches as the transform table allows.  With N == 1 there is no sweep
+// to run and only the guides are computed.
+int sweep_references(const float* const* d_raylut, const float* const* d_images, int C, const ss_camera* cams,
+                     const float* const* rts, const int* refs, int R, const float* cands, int N, int H, int W,
+                     int vt, void* const* ws, hipStream_t s) {
+  void* costs[SS_MAX_REFERENCES];
+  uint32_t* guides[SS_MAX_REFERENCES];
+  for (int r = 0; r < R; ++r) estimate_level0(ws[r], H, W, N, vt, &guides[r], &costs[r]);
+  if (N == 1) {
+    for (int r = 0; r < R; ++r) {
+      StageTimer t(SS_STAGE_GUIDE, s);
+      SS_HIP(ss::launch_guide(d_images[refs[r]], H * W, guides[r], s));
+    }
+    return SS_OK;
+  }
+  const int per = kSweepMaxPairs / C;
+  for (int r0 = 0; r0 < R; r0 += per) {
+    const int n = R - r0 < per ? R - r0 : per;
+    const int rc = launch_sweep_refs(d_raylut + r0, d_images, C, cams, rts + r0, refs + r0, n, cands, N, H, W, vt,
+                                     costs + r0, guides + r0, s);
+    if (rc) return rc;
+  }
+  return SS_OK;
+}
+
+}  // namespace ss
+
+extern "C" {
+
+int ss_estimate_distance(const float* d_raylut, const float* const* d_images, int C, const ss_camera* cams,
+                         const float* rts, int ref, const float* cands, int N, int H, int W, double sigma_i,
+                         double sigma_s, int vtype, const uint8_t* d_guide3, float* d_dist, void* d_workspace,
+                         size_t workspace_bytes, ss_stream stream) {
+  hipStream_t s = as_stream(stream);
+  SS_REQUIRE(d_dist && d_raylut && rts, "null pointer argument");
+  int rc = check_sweep(d_images, C, cams, cands, N, H, W, vtype);
   if (rc) return rc;
-  return ok();
+  SS_REQUIRE(ref >= 0 && ref < C, "reference index %d outside [0, %d)", ref, C);
+  SS_REQUIRE(levels_of(H, W) >= 1, "resolution %dx%d gives no pyramid level (min side must be >= 4)", W, H);
+  const size_t need = ss_estimate_workspace_bytes(H, W, N, vtype);
+  if (!d_workspace || workspace_bytes < need)
+    return fail(SS_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need, workspace_bytes);
+  if (d_guide3) {
+    // the caller's guide; the sweep then writes only the volume
+    uint32_t* g0;
+    void* c0;
+    ss::estimate_level0(d_workspace, H, W, N, vtype, &g0, &c0);
+    {
+      StageTimer t(SS_STAGE_GUIDE, s);
+      SS_HIP(ss::launch_guide3_to_packed(d_guide3, g0, H * W, s));
+    }
+    if (N > 1) {
+      uint32_t* none = nullptr;
+      rc = launch_sweep_refs(&d_raylut, d_images, C, cams, &rts, &ref, 1, cands, N, H, W, vtype, &c0, &none, s);
+      if (rc) return rc;
+    }
+  } else {
+    void* const ws[1] = {d_workspace};
+    rc = ss::sweep_references(&d_raylut, d_images, C, cams, &rts, &ref, 1, cands, N, H, W, vtype, ws, s);
+    if (rc) return rc;
+  }
+  rc = ss::estimate_after_sweep(H, W, N, vtype, cands, sigma_i, sigma_s, d_dist, d_workspace, s);
+  return rc ? rc : ok();
 }
 
 int ss_reproject(const float* d_in, float* d_out, int H, int W, const ss_intrinsics* intr, const float* t3,

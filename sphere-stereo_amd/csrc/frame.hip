@@ -1,22 +1,51 @@
 // Per-frame executor: estimate_RGBD_panorama (depth_estimation.py:205-243 and
-// stitcher.py:166-220) as one call.  Each reference's chain runs on a stream
-// of its own (they are independent until the merge), so one reference's
-// latency-bound coarse pyramid levels overlap the other's sweep.  The frame
-// can be captured once into a HIP graph and replayed, which removes the
-// per-launch host cost of its ~70 kernels.
+// stitcher.py:166-220) as one call.  The sweep of every reference is one
+// launch (it also writes the guides); after it, each reference's chain runs
+// on a stream of its own (they are independent until the merge), so one
+// reference's latency-bound coarse pyramid levels overlap the other's
+// full-size passes.  The frame can be captured once into a HIP graph and
+// replayed, which removes the per-launch host cost of its ~70 kernels.
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 
 #include "ss_kernels.h"
 
 namespace {
 
+// Streams shared by every executor on a device, created on first use and
+// kept until the process ends.  A device has a few hardware queues (4 by
+// default) and each new stream is bound to one of them; streams made per
+// executor end up sharing queues, and two reference chains on one queue run
+// one after the other.  One pool per device keeps a frame's concurrent
+// chains on distinct queues however many executors exist.  Reference r runs
+// on lane r % kLanes.  The mutex serialises the enqueueing (host work only)
+// of frames that share the pool, and with it the use of the capture stream.
+constexpr int kLanes = 3;
+constexpr int kMaxDevices = 64;
+struct StreamPool {
+  std::once_flag once;
+  hipError_t err = hipSuccess;
+  hipStream_t lane[kLanes] = {};
+  hipStream_t capture = nullptr;
+  std::mutex mu;
+};
+StreamPool g_pool[kMaxDevices];
+
+hipError_t pool_init(StreamPool& p) {
+  std::call_once(p.once, [&p] {
+    for (int i = 0; i < kLanes && p.err == hipSuccess; ++i)
+      p.err = hipStreamCreateWithFlags(&p.lane[i], hipStreamNonBlocking);
+    if (p.err == hipSuccess) p.err = hipStreamCreateWithFlags(&p.capture, hipStreamNonBlocking);
+  });
+  return p.err;
+}
+
 struct Frame {
   ss_frame_desc d;
-  hipStream_t side[SS_MAX_REFERENCES] = {};
-  hipStream_t main = nullptr;
+  StreamPool* pool = nullptr;
   hipEvent_t fork = nullptr, pre = nullptr, done = nullptr;
   hipEvent_t join[SS_MAX_REFERENCES] = {};
   hipGraph_t graph = nullptr;
@@ -44,16 +73,19 @@ struct Frame {
 // The frame's work after the events on `s`; `s` waits for all of it.
 int enqueue(Frame* f, const float* const* match, const float* const* stitch, hipStream_t s) {
   const ss_frame_desc& d = f->d;
+  const float* rts[SS_MAX_REFERENCES];
+  for (int r = 0; r < d.R; ++r) rts[r] = d.rts[r];
+  FR_CALL(ss::sweep_references(d.d_raylut, match, d.C, d.cams, rts, d.refs, d.R, d.cands, d.N, d.H, d.W,
+                               d.volume_type, d.d_workspace, s));
   FR_HIP(hipEventRecord(f->fork, s));
   const float* reproj[SS_MAX_REFERENCES];
   const float* dists[SS_MAX_REFERENCES];
   const uint8_t* imgs[SS_MAX_REFERENCES];
   for (int r = 0; r < d.R; ++r) {
-    hipStream_t sr = f->side[r];
+    hipStream_t sr = f->pool->lane[r % kLanes];
     FR_HIP(hipStreamWaitEvent(sr, f->fork, 0));
-    FR_CALL(ss_estimate_distance(d.d_raylut[r], match, d.C, d.cams, d.rts[r], d.refs[r], d.cands, d.N, d.H, d.W,
-                                 d.sigma_i, d.sigma_s, nullptr, d.d_dist[r], d.d_workspace[r], d.workspace_bytes,
-                                 sr));
+    FR_CALL(ss::estimate_after_sweep(d.H, d.W, d.N, d.volume_type, d.cands, d.sigma_i, d.sigma_s, d.d_dist[r],
+                                     d.d_workspace[r], sr));
     FR_CALL(ss_to_u8(stitch[r], d.d_stitch_u8[r], (size_t)d.Hs * d.Ws * 3, sr));
     FR_CALL(ss_reproject(d.d_dist[r], d.d_reproj[r], d.H, d.W, &d.intr[r], d.trans[r], sr));
     FR_CALL(ss_inpaint(d.d_reproj[r], d.d_scratch[r], d.d_inpaint_lut[r], d.H, d.W, d.inpaint_iterations,
@@ -86,15 +118,24 @@ int ss_frame_create(const ss_frame_desc* desc, void** frame) {
   if (d.C < 1 || d.C > SS_MAX_CAMERAS || d.R < 1 || d.R > SS_MAX_REFERENCES || d.N < 1 || d.N > SS_MAX_CANDIDATES ||
       d.H < 4 || d.W < 4 || d.Hs < 2 || d.Ws < 2 || d.PH < 1 || d.PW < 1 || d.inpaint_iterations < 0)
     return ss::report_error(SS_ERR_ARG, "frame descriptor sizes out of range");
+  if (d.volume_type != SS_VOLUME_F32 && d.volume_type != SS_VOLUME_F16 && d.volume_type != SS_VOLUME_BF16)
+    return ss::report_error(SS_ERR_ARG, "frame descriptor: unknown volume type");
+  if (d.workspace_bytes < ss_estimate_workspace_bytes(d.H, d.W, d.N, d.volume_type))
+    return ss::report_error(SS_ERR_WORKSPACE, "frame descriptor: workspace_bytes below ss_estimate_workspace_bytes");
   for (int r = 0; r < d.R; ++r)
     if (d.refs[r] < 0 || d.refs[r] >= d.C || !d.d_raylut[r] || !d.d_workspace[r] || !d.d_dist[r] ||
         !d.d_inpaint_lut[r] || !d.d_reproj[r] || !d.d_scratch[r] || !d.d_stitch_u8[r])
       return ss::report_error(SS_ERR_ARG, "frame descriptor: bad reference index or null buffer");
   if (!d.d_sampling || !d.d_weights || !d.d_dist_pano || !d.d_rgb_pano)
     return ss::report_error(SS_ERR_ARG, "frame descriptor: null stitching table or output");
+  int dev = 0;
+  FR_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return ss::report_error(SS_ERR_UNSUPPORTED, "device index beyond the pool");
+  FR_HIP(pool_init(g_pool[dev]));
   Frame* f = new (std::nothrow) Frame();
   if (!f) return ss::report_error(SS_ERR_ARG, "out of host memory");
   f->d = d;
+  f->pool = &g_pool[dev];
   int rc = SS_OK;
   auto check = [&](hipError_t e, const char* what) {
     if (e != hipSuccess && rc == SS_OK) {
@@ -103,11 +144,8 @@ int ss_frame_create(const ss_frame_desc* desc, void** frame) {
       rc = ss::report_error(SS_ERR_HIP, m);
     }
   };
-  for (int r = 0; r < d.R; ++r) {
-    check(hipStreamCreateWithFlags(&f->side[r], hipStreamNonBlocking), "hipStreamCreateWithFlags");
+  for (int r = 0; r < d.R; ++r)
     check(hipEventCreateWithFlags(&f->join[r], hipEventDisableTiming), "hipEventCreateWithFlags");
-  }
-  check(hipStreamCreateWithFlags(&f->main, hipStreamNonBlocking), "hipStreamCreateWithFlags");
   check(hipEventCreateWithFlags(&f->fork, hipEventDisableTiming), "hipEventCreateWithFlags");
   check(hipEventCreateWithFlags(&f->pre, hipEventDisableTiming), "hipEventCreateWithFlags");
   check(hipEventCreateWithFlags(&f->done, hipEventDisableTiming), "hipEventCreateWithFlags");
@@ -128,21 +166,23 @@ int ss_frame_run(void* frame, const float* const* d_match, const float* const* d
   for (int r = 0; r < f->d.R; ++r)
     if (!d_stitch[r]) return ss::report_error(SS_ERR_ARG, "null stitch image");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  std::lock_guard<std::mutex> lk(f->pool->mu);
   if (!use_graph) return enqueue(f, d_match, d_stitch, s);
   if (ss::profiler_active()) return ss::report_error(SS_ERR_UNSUPPORTED, "graph replay while kernel timing is on");
 
+  hipStream_t main = f->pool->capture;
   bool same = f->exec != nullptr;
   for (int k = 0; k < f->d.C && same; ++k) same = f->cap_match[k] == d_match[k];
   for (int r = 0; r < f->d.R && same; ++r) same = f->cap_stitch[r] == d_stitch[r];
   // the replay stream starts after the caller's work
   FR_HIP(hipEventRecord(f->pre, s));
-  FR_HIP(hipStreamWaitEvent(f->main, f->pre, 0));
+  FR_HIP(hipStreamWaitEvent(main, f->pre, 0));
   if (!same) {
     FR_CALL(drop_graph(f));
-    FR_HIP(hipStreamBeginCapture(f->main, hipStreamCaptureModeRelaxed));
-    const int rc = enqueue(f, d_match, d_stitch, f->main);
+    FR_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeRelaxed));
+    const int rc = enqueue(f, d_match, d_stitch, main);
     hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(f->main, &g);
+    const hipError_t e = hipStreamEndCapture(main, &g);
     if (rc != SS_OK) {
       if (g) (void)hipGraphDestroy(g);
       return rc;
@@ -153,8 +193,8 @@ int ss_frame_run(void* frame, const float* const* d_match, const float* const* d
     for (int k = 0; k < f->d.C; ++k) f->cap_match[k] = d_match[k];
     for (int r = 0; r < f->d.R; ++r) f->cap_stitch[r] = d_stitch[r];
   }
-  FR_HIP(hipGraphLaunch(f->exec, f->main));
-  FR_HIP(hipEventRecord(f->done, f->main));
+  FR_HIP(hipGraphLaunch(f->exec, main));
+  FR_HIP(hipEventRecord(f->done, main));
   FR_HIP(hipStreamWaitEvent(s, f->done, 0));
   return SS_OK;
 }
@@ -163,11 +203,8 @@ int ss_frame_destroy(void* frame) {
   Frame* f = static_cast<Frame*>(frame);
   if (!f) return SS_OK;
   drop_graph(f);
-  for (int r = 0; r < SS_MAX_REFERENCES; ++r) {
-    if (f->side[r]) (void)hipStreamDestroy(f->side[r]);
+  for (int r = 0; r < SS_MAX_REFERENCES; ++r)
     if (f->join[r]) (void)hipEventDestroy(f->join[r]);
-  }
-  if (f->main) (void)hipStreamDestroy(f->main);
   if (f->fork) (void)hipEventDestroy(f->fork);
   if (f->pre) (void)hipEventDestroy(f->pre);
   if (f->done) (void)hipEventDestroy(f->done);

@@ -41,61 +41,19 @@ __device__ __forceinline__ uint32_t pick4(int i, uint32_t a, uint32_t b, uint32_
   return i == 0 ? a : (i == 1 ? b : (i == 2 ? c : d));
 }
 
-template <int n>
-__device__ __forceinline__ void load_n(const float* __restrict__ p, float* o) {
-  if constexpr (n % 4 == 0) {
-#pragma unroll
-    for (int i = 0; i < n / 4; ++i) {
-      float4 t = reinterpret_cast<const float4*>(p)[i];
-      o[4 * i] = t.x;
-      o[4 * i + 1] = t.y;
-      o[4 * i + 2] = t.z;
-      o[4 * i + 3] = t.w;
-    }
-  } else if constexpr (n % 2 == 0) {
-#pragma unroll
-    for (int i = 0; i < n / 2; ++i) {
-      float2 t = reinterpret_cast<const float2*>(p)[i];
-      o[2 * i] = t.x;
-      o[2 * i + 1] = t.y;
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < n; ++i) o[i] = p[i];
-  }
-}
-
-template <int n>
-__device__ __forceinline__ void store_n(float* __restrict__ p, const float* v) {
-  if constexpr (n % 4 == 0) {
-#pragma unroll
-    for (int i = 0; i < n / 4; ++i)
-      reinterpret_cast<float4*>(p)[i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
-  } else if constexpr (n % 2 == 0) {
-#pragma unroll
-    for (int i = 0; i < n / 2; ++i) reinterpret_cast<float2*>(p)[i] = make_float2(v[2 * i], v[2 * i + 1]);
-  } else {
-#pragma unroll
-    for (int i = 0; i < n; ++i) p[i] = v[i];
-  }
-}
-
-// candidates handled per inner step: whole 16-byte vectors when possible
-template <int V>
-constexpr int chunk_of() {
-  return V % 4 == 0 ? 4 : V;
-}
-
 // ---------------------------------------------------------------------------
 // Down-sampling: guideDownsample2xKernel (isb_filter.cu:44-119)
 // ---------------------------------------------------------------------------
 // One coarse pixel of the down pass, handled by the G lanes of a group.
 // Inactive groups (past the end of the level) compute on a valid pixel so
-// that the shuffles see every lane, and store nothing.
-template <int G, int V>
+// that the shuffles see every lane, and store nothing.  T is the volume's
+// storage type; sums and weights are fp32.
+template <int G, int V, typename T>
 __device__ __forceinline__ void down_pixel(const IsbDownArgs& a, int pix, bool active, int lane) {
   constexpr int NP = G * V;
-  constexpr int CH = chunk_of<V>();
+  constexpr int CH = chunk_of<V, T>();
+  const T* cin = static_cast<const T*>(a.cin);
+  T* cout = static_cast<T*>(a.cout);
 
   const int x = pix % a.wout, y = pix / a.wout;
   const int x2 = 2 * x, y2 = 2 * y;
@@ -145,20 +103,20 @@ __device__ __forceinline__ void down_pixel(const IsbDownArgs& a, int pix, bool a
 #pragma unroll
   for (int c0 = 0; c0 < V; c0 += CH) {
     float acc[CH], t[CH];
-    load_n<CH>(a.cin + (size_t)ctr * NP + base_c + c0, acc);
+    load_n<CH>(cin + (size_t)ctr * NP + base_c + c0, acc);
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      load_n<CH>(a.cin + (size_t)nb[j] * NP + base_c + c0, t);
+      load_n<CH>(cin + (size_t)nb[j] * NP + base_c + c0, t);
 #pragma unroll
       for (int v = 0; v < CH; ++v) acc[v] = acc[v] + t[v] * w[j];
     }
 #pragma unroll
     for (int v = 0; v < CH; ++v) acc[v] = acc[v] / ws;
-    if (active) store_n<CH>(a.cout + (size_t)pix * NP + base_c + c0, acc);
+    if (active) store_n<CH>(cout + (size_t)pix * NP + base_c + c0, acc);
   }
 }
 
-template <int G, int V>
+template <int G, int V, typename T>
 __global__ void __launch_bounds__(kBlock) k_isb_down(IsbDownArgs a) {
   const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -167,7 +125,7 @@ __global__ void __launch_bounds__(kBlock) k_isb_down(IsbDownArgs a) {
   const bool active = pix < npix;
   if (G == 1 && !active) return;
   if (!active) pix = npix - 1;
-  down_pixel<G, V>(a, pix, active, lane);
+  down_pixel<G, V, T>(a, pix, active, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -236,10 +194,12 @@ __device__ __forceinline__ float argmin_refine_group(const float* vals, int lane
 // ---------------------------------------------------------------------------
 // One coarse pixel of the up pass (its four fine pixels), handled by the G
 // lanes of a group; inactive groups as in down_pixel.
-template <int G, int V, bool FUSE_ARGMIN>
+template <int G, int V, typename T, bool FUSE_ARGMIN>
 __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool active, int lane) {
   constexpr int NP = G * V;
-  constexpr int CH = chunk_of<V>();
+  constexpr int CH = chunk_of<V, T>();
+  const T* cin = static_cast<const T*>(a.cin);
+  T* cio = static_cast<T*>(a.cio);
 
   const int x = pix % a.win, y = pix / a.win;
   const int r0 = max(y - 1, 0), r2 = min(y + 1, a.hin - 1);
@@ -320,12 +280,12 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
   for (int cb = 0; cb < V; cb += CH) {
     float cD[9][CH];
 #pragma unroll
-    for (int t = 0; t < 9; ++t) load_n<CH>(a.cin + (size_t)tap[t] * NP + base_c + cb, cD[t]);
+    for (int t = 0; t < 9; ++t) load_n<CH>(cin + (size_t)tap[t] * NP + base_c + cb, cD[t]);
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       if (!doit[s]) continue;
       float o[CH];
-      load_n<CH>(a.cio + (size_t)fine[s] * NP + base_c + cb, o);
+      load_n<CH>(cio + (size_t)fine[s] * NP + base_c + cb, o);
 #pragma unroll
       for (int v = 0; v < CH; ++v) {
         float acc = a.weight_up * o[v];
@@ -337,7 +297,7 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
 #pragma unroll
         for (int v = 0; v < CH; ++v) res[s][cb + v] = o[v];
       } else {
-        if (active) store_n<CH>(a.cio + (size_t)fine[s] * NP + base_c + cb, o);
+        if (active) store_n<CH>(cio + (size_t)fine[s] * NP + base_c + cb, o);
       }
     }
   }
@@ -352,7 +312,7 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
   }
 }
 
-template <int G, int V, bool FUSE_ARGMIN>
+template <int G, int V, typename T, bool FUSE_ARGMIN>
 __global__ void __launch_bounds__(kBlock) k_isb_up(IsbUpArgs a) {
   const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -361,7 +321,7 @@ __global__ void __launch_bounds__(kBlock) k_isb_up(IsbUpArgs a) {
   const bool active = pix < npix;
   if (G == 1 && !active) return;
   if (!active) pix = npix - 1;
-  up_pixel<G, V, FUSE_ARGMIN>(a, pix, active, lane);
+  up_pixel<G, V, T, FUSE_ARGMIN>(a, pix, active, lane);
 }
 
 // The coarse end of the pyramid in one workgroup: the down passes into
@@ -371,7 +331,7 @@ __global__ void __launch_bounds__(kBlock) k_isb_up(IsbUpArgs a) {
 // per-pixel functions work on generic pointers, so the same code serves both.
 constexpr int kTailBlock = 1024;
 
-template <int G, int V>
+template <int G, int V, typename T>
 __global__ void __launch_bounds__(kTailBlock) k_isb_tail(IsbTailArgs t) {
   extern __shared__ __align__(16) char smem[];
   const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
@@ -379,14 +339,14 @@ __global__ void __launch_bounds__(kTailBlock) k_isb_tail(IsbTailArgs t) {
   auto gp = [&](int k) __attribute__((always_inline)) -> uint32_t* {
     return k == 0 ? t.g0 : reinterpret_cast<uint32_t*>(smem + t.goff[k]);
   };
-  auto cp = [&](int k) __attribute__((always_inline)) -> float* {
-    return k == 0 ? t.c0 : reinterpret_cast<float*>(smem + t.coff[k]);
+  auto cp = [&](int k) __attribute__((always_inline)) -> void* {
+    return k == 0 ? t.c0 : static_cast<void*>(smem + t.coff[k]);
   };
   for (int k = 1; k < t.n; ++k) {
     const IsbDownArgs a{gp(k - 1), cp(k - 1), t.h[k - 1], t.w[k - 1], gp(k), cp(k), t.h[k], t.w[k], t.var_inv_i};
     const int npix = t.h[k] * t.w[k];
     for (int base = 0; base < npix; base += slots)
-      if (base + slot < npix) down_pixel<G, V>(a, base + slot, true, lane);
+      if (base + slot < npix) down_pixel<G, V, T>(a, base + slot, true, lane);
     __syncthreads();
   }
   for (int k = t.n - 2; k >= 0; --k) {
@@ -394,14 +354,14 @@ __global__ void __launch_bounds__(kTailBlock) k_isb_tail(IsbTailArgs t) {
                       t.wu[k],   t.wd[k],   t.var_inv_i, nullptr,  ArgminParams{}};
     const int npix = t.h[k + 1] * t.w[k + 1];
     for (int base = 0; base < npix; base += slots)
-      if (base + slot < npix) up_pixel<G, V, false>(a, base + slot, true, lane);
+      if (base + slot < npix) up_pixel<G, V, T, false>(a, base + slot, true, lane);
     __syncthreads();
   }
 }
 
 // Argmin over an already filtered level-0 volume (unfused form).
-template <int G, int V>
-__global__ void __launch_bounds__(kBlock) k_argmin(const float* __restrict__ cost, int npix, ArgminParams p,
+template <int G, int V, typename T>
+__global__ void __launch_bounds__(kBlock) k_argmin(const T* __restrict__ cost, int npix, ArgminParams p,
                                                    float* __restrict__ dist) {
   constexpr int NP = G * V;
   const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
@@ -445,29 +405,35 @@ __global__ void k_packed_to_guide3(const uint32_t* __restrict__ g, uint8_t* __re
 
 inline int blocks_for(long long threads) { return (int)((threads + kBlock - 1) / kBlock); }
 
-template <int G, int V>
+template <int G, int V, typename T>
 hipError_t launch_down_gv(const IsbDownArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_isb_down<G, V>), dim3(blocks_for((long long)a.hout * a.wout * G)), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((k_isb_down<G, V, T>), dim3(blocks_for((long long)a.hout * a.wout * G)), dim3(kBlock), 0, s,
+                     a);
   return hipGetLastError();
 }
 
-template <int G, int V>
+template <int G, int V, typename T>
 hipError_t launch_tail_gv(const IsbTailArgs& t, hipStream_t s) {
-  hipLaunchKernelGGL((k_isb_tail<G, V>), dim3(1), dim3(kTailBlock), t.lds_bytes, s, t);
+  hipLaunchKernelGGL((k_isb_tail<G, V, T>), dim3(1), dim3(kTailBlock), t.lds_bytes, s, t);
   return hipGetLastError();
 }
 
-template <int G, int V>
+template <int G, int V, typename T>
 hipError_t launch_up_gv(const IsbUpArgs& a, bool fuse, hipStream_t s) {
   const dim3 grid(blocks_for((long long)a.hin * a.win * G));
   if (fuse)
-    hipLaunchKernelGGL((k_isb_up<G, V, true>), grid, dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_isb_up<G, V, T, true>), grid, dim3(kBlock), 0, s, a);
   else
-    hipLaunchKernelGGL((k_isb_up<G, V, false>), grid, dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_isb_up<G, V, T, false>), grid, dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
-}  // namespace
+template <int V, typename T>
+hipError_t launch_argmin_vt(const void* cost, int npix, const ArgminParams& p, float* dist, hipStream_t s) {
+  hipLaunchKernelGGL((k_argmin<8, V, T>), dim3(blocks_for((long long)npix * 8)), dim3(kBlock), 0, s,
+                     static_cast<const T*>(cost), npix, p, dist);
+  return hipGetLastError();
+}
 
 #define SS_DISPATCH_V(V, CALL)                       \
   switch (V) {                                       \
@@ -482,38 +448,64 @@ hipError_t launch_up_gv(const IsbUpArgs& a, bool fuse, hipStream_t s) {
     default: return hipErrorInvalidValue;            \
   }
 
-hipError_t launch_isb_down(const IsbDownArgs& a, int G, int V, hipStream_t s) {
-  if (G == 1) return V == 1 ? launch_down_gv<1, 1>(a, s) : hipErrorInvalidValue;
-#define SS_CALL(v) launch_down_gv<8, v>(a, s)
+template <typename T>
+hipError_t down_t(const IsbDownArgs& a, int V, hipStream_t s) {
+#define SS_CALL(v) launch_down_gv<8, v, T>(a, s)
   SS_DISPATCH_V(V, SS_CALL)
 #undef SS_CALL
 }
 
-hipError_t launch_isb_tail(const IsbTailArgs& t, int G, int V, hipStream_t s) {
-  if (G == 1) return V == 1 ? launch_tail_gv<1, 1>(t, s) : hipErrorInvalidValue;
-#define SS_CALL(v) launch_tail_gv<8, v>(t, s)
+template <typename T>
+hipError_t tail_t(const IsbTailArgs& t, int V, hipStream_t s) {
+#define SS_CALL(v) launch_tail_gv<8, v, T>(t, s)
   SS_DISPATCH_V(V, SS_CALL)
 #undef SS_CALL
 }
 
-hipError_t launch_isb_up(const IsbUpArgs& a, int G, int V, bool fuse_argmin, hipStream_t s) {
-  if (G == 1) return (V == 1 && !fuse_argmin) ? launch_up_gv<1, 1>(a, false, s) : hipErrorInvalidValue;
-#define SS_CALL(v) launch_up_gv<8, v>(a, fuse_argmin, s)
+template <typename T>
+hipError_t up_t(const IsbUpArgs& a, int V, bool fuse, hipStream_t s) {
+#define SS_CALL(v) launch_up_gv<8, v, T>(a, fuse, s)
   SS_DISPATCH_V(V, SS_CALL)
 #undef SS_CALL
 }
 
-namespace {
-template <int V>
-hipError_t launch_argmin_v(const float* cost, int npix, const ArgminParams& p, float* dist, hipStream_t s) {
-  hipLaunchKernelGGL((k_argmin<8, V>), dim3(blocks_for((long long)npix * 8)), dim3(kBlock), 0, s, cost, npix, p, dist);
-  return hipGetLastError();
+template <typename T>
+hipError_t argmin_t(const void* cost, int npix, int V, const ArgminParams& p, float* dist, hipStream_t s) {
+#define SS_CALL(v) launch_argmin_vt<v, T>(cost, npix, p, dist, s)
+  SS_DISPATCH_V(V, SS_CALL)
+#undef SS_CALL
 }
+
 }  // namespace
 
-hipError_t launch_argmin(const float* cost, int npix, int V, const ArgminParams& p, float* dist, hipStream_t s) {
-#define SS_CALL(v) launch_argmin_v<v>(cost, npix, p, dist, s)
-  SS_DISPATCH_V(V, SS_CALL)
+// G = 1 (the distance filter) exists for fp32 volumes only.
+hipError_t launch_isb_down(const IsbDownArgs& a, int G, int V, int vt, hipStream_t s) {
+  if (G == 1) return (V == 1 && vt == SS_VOLUME_F32) ? launch_down_gv<1, 1, float>(a, s) : hipErrorInvalidValue;
+#define SS_CALL(T) down_t<T>(a, V, s)
+  SS_DISPATCH_T(vt, SS_CALL)
+#undef SS_CALL
+}
+
+hipError_t launch_isb_tail(const IsbTailArgs& t, int G, int V, int vt, hipStream_t s) {
+  if (G == 1) return (V == 1 && vt == SS_VOLUME_F32) ? launch_tail_gv<1, 1, float>(t, s) : hipErrorInvalidValue;
+#define SS_CALL(T) tail_t<T>(t, V, s)
+  SS_DISPATCH_T(vt, SS_CALL)
+#undef SS_CALL
+}
+
+hipError_t launch_isb_up(const IsbUpArgs& a, int G, int V, int vt, bool fuse_argmin, hipStream_t s) {
+  if (G == 1)
+    return (V == 1 && vt == SS_VOLUME_F32 && !fuse_argmin) ? launch_up_gv<1, 1, float>(a, false, s)
+                                                           : hipErrorInvalidValue;
+#define SS_CALL(T) up_t<T>(a, V, fuse_argmin, s)
+  SS_DISPATCH_T(vt, SS_CALL)
+#undef SS_CALL
+}
+
+hipError_t launch_argmin(const void* cost, int npix, int V, int vt, const ArgminParams& p, float* dist,
+                         hipStream_t s) {
+#define SS_CALL(T) argmin_t<T>(cost, npix, V, p, dist, s)
+  SS_DISPATCH_T(vt, SS_CALL)
 #undef SS_CALL
 }
 

@@ -6,15 +6,17 @@
 
 #include "ss_device.h"
 #include "ss_types.h"
+#include "ss_volume.h"
 
 namespace ss {
 
+// Volumes are passed untyped; the kernels cast them to their storage type.
 struct IsbDownArgs {
   const uint32_t* gin;
-  const float* cin;
+  const void* cin;
   int hin, win;
   uint32_t* gout;
-  float* cout;
+  void* cout;
   int hout, wout;
   float var_inv_i;
 };
@@ -26,10 +28,10 @@ struct ArgminParams {
 
 struct IsbUpArgs {
   const uint32_t* gin;
-  const float* cin;
+  const void* cin;
   int hin, win;
   uint32_t* gio;
-  float* cio;
+  void* cio;
   int hio, wio;
   float weight_up, weight_down, var_inv_i;
   float* dist;  // fused argmin output (level-0 distance map)
@@ -43,7 +45,7 @@ constexpr int kTailLdsBytes = 60 * 1024;
 struct IsbTailArgs {
   int n;
   uint32_t* g0;
-  float* c0;
+  void* c0;
   unsigned goff[kMaxTailLevels], coff[kMaxTailLevels];
   int h[kMaxTailLevels], w[kMaxTailLevels];
   float wu[kMaxTailLevels], wd[kMaxTailLevels];  // up pass into entry k
@@ -51,17 +53,28 @@ struct IsbTailArgs {
   unsigned lds_bytes;
 };
 
-struct SweepArgs {
+// One sweep launch covers up to SS_MAX_REFERENCES references of a rig whose
+// (reference, camera) transforms fit kSweepMaxPairs; ref[r].rt0 is the index
+// of reference r's first transform in rt.
+constexpr int kSweepMaxPairs = 32;
+struct SweepRef {
   const float4* raylut;
-  const float* ref_img;
+  const float* img;  // the reference's own image
+  void* cost;
+  uint32_t* guide;   // nullable: packed guide written from the same pass
+  int rt0;
+};
+struct SweepArgs {
+  SweepRef ref[SS_MAX_REFERENCES];
   const float* img[SS_MAX_CAMERAS];
   TorchCam cam[SS_MAX_CAMERAS];
-  float rt[SS_MAX_CAMERAS][12];
+  float rt[kSweepMaxPairs][12];
   float cands[SS_MAX_CANDIDATES];
-  float* cost;
-  int H, W, N;
-  float Wf, Hf, invW, invH;
-  int pow2W, pow2H;
+  int R, H, W, N, NP;            // NP: candidate stride of the volume (a multiple of 8)
+  int tiles_x, blocks_per_ref;   // 64 x 4 pixel tiles per row of tiles / per reference
+  // grid_sample's coordinate round trip: size, 1/size, 2/size, size/2
+  float Wf, Hf, invW, invH, inv2W, inv2H, halfW, halfH;
+  int pow2;  // both sizes are powers of two
 };
 
 struct SelectArgs {
@@ -104,11 +117,12 @@ struct BlendArgs {
 int report_error(int code, const char* what);
 bool profiler_active();
 
-// isb.hip
-hipError_t launch_isb_down(const IsbDownArgs& a, int G, int V, hipStream_t s);
-hipError_t launch_isb_up(const IsbUpArgs& a, int G, int V, bool fuse_argmin, hipStream_t s);
-hipError_t launch_isb_tail(const IsbTailArgs& t, int G, int V, hipStream_t s);
-hipError_t launch_argmin(const float* cost, int npix, int V, const ArgminParams& p, float* dist, hipStream_t s);
+// isb.hip (vt: ss_volume_type of the volume; G = 1 paths are fp32 only)
+hipError_t launch_isb_down(const IsbDownArgs& a, int G, int V, int vt, hipStream_t s);
+hipError_t launch_isb_up(const IsbUpArgs& a, int G, int V, int vt, bool fuse_argmin, hipStream_t s);
+hipError_t launch_isb_tail(const IsbTailArgs& t, int G, int V, int vt, hipStream_t s);
+hipError_t launch_argmin(const void* cost, int npix, int V, int vt, const ArgminParams& p, float* dist,
+                         hipStream_t s);
 hipError_t launch_planar_to_cc(const float* src, float* dst, int HW, int N, int NP, hipStream_t s);
 hipError_t launch_cc_to_planar(const float* src, float* dst, int HW, int N, int NP, hipStream_t s);
 hipError_t launch_guide3_to_packed(const uint8_t* g3, uint32_t* g, int HW, hipStream_t s);
@@ -118,7 +132,19 @@ hipError_t launch_packed_to_guide3(const uint32_t* g, uint8_t* g3, int HW, hipSt
 hipError_t launch_guide(const float* img, int HW, uint32_t* guide, hipStream_t s);
 hipError_t launch_ray_lut(const TorchCam& ref, const int8_t* sel, int H, int W, float4* lut, hipStream_t s);
 hipError_t launch_select_camera(const SelectArgs& a, hipStream_t s);
-hipError_t launch_sweep(const SweepArgs& a, int V, hipStream_t s);
+hipError_t launch_sweep(const SweepArgs& a, int vt, hipStream_t s);
+
+// api.hip: pieces of ss_estimate_distance the frame executor composes
+size_t volume_elem_bytes(int vt);
+// level-0 guide and volume of an estimate workspace
+void estimate_level0(void* ws, int H, int W, int N, int vt, uint32_t** g0, void** c0);
+// everything after the sweep: cost filter + argmin, then the distance filter
+int estimate_after_sweep(int H, int W, int N, int vt, const float* cands, double sigma_i, double sigma_s,
+                         float* d_dist, void* ws, hipStream_t s);
+// one sweep launch over R references (each with its own workspace)
+int sweep_references(const float* const* d_raylut, const float* const* d_images, int C, const ss_camera* cams,
+                     const float* const* rts, const int* refs, int R, const float* cands, int N, int H, int W,
+                     int vt, void* const* ws, hipStream_t s);
 
 // stitch.hip
 hipError_t launch_fill(float* p, float v, size_t n, hipStream_t s);

@@ -4,11 +4,10 @@
 // reference.  The reference evaluates the sweep as a chain of full-volume
 // torch ops per camera (~10 GB of HBM traffic per camera per reference at
 // 1024^2, 32 candidates); here one pass writes the candidate-contiguous cost
-// volume directly.  A group of 8 lanes owns a reference pixel; each lane
-// sweeps V consecutive candidates and stores them as one 16-byte vector, so
-// the volume is written with full-line coalescing.  Only the selected camera
-// of each pixel is sampled (the reference samples every camera and keeps the
-// selected one), which leaves the result unchanged.
+// volume directly, one lane per reference pixel sweeping all candidates.
+// Only the selected camera of each pixel is sampled (the reference samples
+// every camera and keeps the selected one), which leaves the result
+// unchanged.
 #include "ss_kernels.h"
 
 namespace ss {
@@ -137,31 +136,106 @@ __global__ void __launch_bounds__(kBlock) k_select_camera(SelectArgs a) {
   a.sel[p] = (int8_t)best;
 }
 
-// Sweep + raw cost (depth_estimation.py:141-175)
-template <int V, bool POW2W, bool POW2H>
+// grid_coord2 in four operations for a power-of-two size, with inv2 = 2/size
+// and half = size/2 (both exact).  Each step equals the corresponding steps
+// of the long form exactly: t*(2/size) is the exact product 2*(t/size), so
+// fma(t, inv2, -1) = RN(2*(t/size) - 1); (g+1)*size is exact, so
+// fma(s, half, -0.5) = RN((s*size - 1)/2).  (Where t/size would be a
+// subnormal, both forms give g = -1.  Where s*size would overflow, both
+// coordinates fall outside the image and sample zero.)
+__device__ __forceinline__ float grid_coord_pow2(float u, float inv2, float half) {
+  const float t = u + 0.5f;
+  const float g = __builtin_fmaf(t, inv2, -1.0f);
+  const float s = g + 1.0f;
+  return __builtin_fmaf(s, half, -0.5f);
+}
+
+// sample_zero<3> with a fast path for the common case where all four taps
+// lie inside the image: then no tap is clamped or masked, the taps are two
+// pairs of adjacent texels one row apart, and the arithmetic (weights, tap
+// order, products and sums) is exactly that of sample_zero.
+__device__ __forceinline__ void sample_rgb(const float* __restrict__ img, int H, int W, float ix, float iy,
+                                           float* out) {
+  const float x0f = floorf(ix), y0f = floorf(iy);
+  if (x0f >= 0.0f && x0f < (float)(W - 1) && y0f >= 0.0f && y0f < (float)(H - 1)) {
+    const int x0 = (int)x0f, y0 = (int)y0f;
+    const float x1f = x0f + 1.0f, y1f = y0f + 1.0f;
+    const float wnw = (x1f - ix) * (y1f - iy), wne = (ix - x0f) * (y1f - iy);
+    const float wsw = (x1f - ix) * (iy - y0f), wse = (ix - x0f) * (iy - y0f);
+    const float* pn = img + (unsigned)(y0 * W + x0) * 3u;
+    const float* ps = pn + (unsigned)W * 3u;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float s = pn[c] * wnw;
+      s = s + pn[3 + c] * wne;
+      s = s + ps[c] * wsw;
+      s = s + ps[3 + c] * wse;
+      out[c] = s;
+    }
+    return;
+  }
+  sample_zero<3>(img, H, W, ix, iy, out);
+}
+
+// Sweep + raw cost (depth_estimation.py:141-175) for every reference of the
+// launch: blocks [r*blocks_per_ref, (r+1)*blocks_per_ref) sweep reference r.
+// One lane per reference pixel, over a 64 x 4 pixel tile per workgroup (one
+// wavefront per row segment).  Every lane of a wavefront samples the same
+// candidate at the same time, so the 64 lanes' bilinear taps fall on
+// neighbouring texels of the sampled image (coalesced gathers), and each
+// lane stores its pixel's candidates 4 at a time into the candidate-
+// contiguous volume.  The lane also writes its pixel's guide
+// (utils.py:144-155), which needs the same reference colour.
+constexpr int kTileW = 64, kTileH = kBlock / kTileW;
+
+template <typename T, bool POW2>
 __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
-  constexpr int G = 8, NP = G * V;
-  const int lane = threadIdx.x & (G - 1);
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int pix = (tile * kBlock + (int)threadIdx.x) / G;
-  if (pix >= a.H * a.W) return;
-  const float4 rl = a.raylut[pix];
+  const int r = tile / a.blocks_per_ref;
+  const int t = tile - r * a.blocks_per_ref;
+  const int ty = t / a.tiles_x, tx = t - ty * a.tiles_x;
+  const int x = tx * kTileW + (int)(threadIdx.x & (kTileW - 1));
+  const int y = ty * kTileH + (int)(threadIdx.x / kTileW);
+  if (x >= a.W || y >= a.H) return;
+  const SweepRef& rf = a.ref[r];
+  const int pix = y * a.W + x;
+  const float4 rl = rf.raylut[pix];
   const int k = (int)rl.w;
-  const float r0 = a.ref_img[3 * pix], r1 = a.ref_img[3 * pix + 1], r2 = a.ref_img[3 * pix + 2];
-  const int n0 = lane * V;
-  float out[V];
+  const float r0 = rf.img[3 * pix], r1 = rf.img[3 * pix + 1], r2 = rf.img[3 * pix + 2];
+  if (rf.guide) {
+    float gy = ((16.0f + 0.1826f * r0) + 0.6142f * r1) + 0.062f * r2;
+    float cb = ((128.0f - 0.1006f * r0) - 0.3386f * r1) + 0.4392f * r2;
+    float cr = ((128.0f + 0.4392f * r0) - 0.3989f * r1) - 0.0403f * r2;
+    gy = gy < 16.0f ? 16.0f : (gy > 235.0f ? 235.0f : gy);
+    cb = cb < 16.0f ? 16.0f : (cb > 240.0f ? 240.0f : cb);
+    cr = cr < 16.0f ? 16.0f : (cr > 240.0f ? 240.0f : cr);
+    rf.guide[pix] = pack_guide(gy, cb, cr);
+  }
+  T* dst = static_cast<T*>(rf.cost) + (size_t)pix * a.NP;
   if (k < 0) {
     // nothing sampled: the volume holds zeros there (depth_estimation.py:150-152)
     float c = (r0 + r1) + r2;
     c = c > 500.0f ? 500.0f : c;
+    for (int n0 = 0; n0 < a.NP; n0 += 4) {
+      float out[4];
 #pragma unroll
-    for (int v = 0; v < V; ++v) out[v] = (n0 + v < a.N) ? c : 0.0f;
-  } else {
-    const TorchCam& cam = a.cam[k];
-    const float* rt = a.rt[k];
-    const float* img = a.img[k];
+      for (int v = 0; v < 4; ++v) out[v] = (n0 + v < a.N) ? c : 0.0f;
+      store_n<4>(dst + n0, out);
+    }
+    return;
+  }
+  // register copies: read through pointers, the camera and transform would
+  // be reloaded after every store to the volume (the compiler cannot rule
+  // out aliasing)
+  const TorchCam cam = a.cam[k];
+  float rt[12];
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
+  for (int i = 0; i < 12; ++i) rt[i] = a.rt[rf.rt0 + k][i];
+  const float* img = a.img[k];
+  for (int n0 = 0; n0 < a.NP; n0 += 4) {
+    float out[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
       const int n = n0 + v;
       float c = 0.0f;
       if (n < a.N) {
@@ -171,38 +245,26 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
         rigid(rt, d * rl.x, d * rl.y, d * rl.z, qx, qy, qz);
         project_t(cam, qx, qy, qz, u, w, valid);
         float s[3];
-        sample_zero<3>(img, a.H, a.W, grid_coord2<POW2W>(u, a.Wf, a.invW), grid_coord2<POW2H>(w, a.Hf, a.invH), s);
+        if constexpr (POW2)
+          sample_rgb(img, a.H, a.W, grid_coord_pow2(u, a.inv2W, a.halfW), grid_coord_pow2(w, a.inv2H, a.halfH), s);
+        else
+          sample_rgb(img, a.H, a.W, grid_coord2<false>(u, a.Wf, a.invW), grid_coord2<false>(w, a.Hf, a.invH), s);
         c = (fabsf(s[0] - r0) + fabsf(s[1] - r1)) + fabsf(s[2] - r2);
         c = c > 500.0f ? 500.0f : c;
       }
       out[v] = c;
     }
-  }
-  float* dst = a.cost + (size_t)pix * NP + n0;
-  if constexpr (V % 4 == 0) {
-#pragma unroll
-    for (int i = 0; i < V / 4; ++i)
-      reinterpret_cast<float4*>(dst)[i] = make_float4(out[4 * i], out[4 * i + 1], out[4 * i + 2], out[4 * i + 3]);
-  } else if constexpr (V % 2 == 0) {
-#pragma unroll
-    for (int i = 0; i < V / 2; ++i) reinterpret_cast<float2*>(dst)[i] = make_float2(out[2 * i], out[2 * i + 1]);
-  } else {
-#pragma unroll
-    for (int v = 0; v < V; ++v) dst[v] = out[v];
+    store_n<4>(dst + n0, out);
   }
 }
 
-template <int V>
-hipError_t launch_sweep_v(const SweepArgs& a, hipStream_t s) {
-  const dim3 grid(blocks_for((long long)a.H * a.W * 8));
-  if (a.pow2W && a.pow2H)
-    hipLaunchKernelGGL((k_sweep<V, true, true>), grid, dim3(kBlock), 0, s, a);
-  else if (a.pow2W)
-    hipLaunchKernelGGL((k_sweep<V, true, false>), grid, dim3(kBlock), 0, s, a);
-  else if (a.pow2H)
-    hipLaunchKernelGGL((k_sweep<V, false, true>), grid, dim3(kBlock), 0, s, a);
+template <typename T>
+hipError_t launch_sweep_t(const SweepArgs& a, hipStream_t s) {
+  const dim3 grid(a.blocks_per_ref * a.R);
+  if (a.pow2)
+    hipLaunchKernelGGL((k_sweep<T, true>), grid, dim3(kBlock), 0, s, a);
   else
-    hipLaunchKernelGGL((k_sweep<V, false, false>), grid, dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_sweep<T, false>), grid, dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
@@ -223,18 +285,10 @@ hipError_t launch_select_camera(const SelectArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_sweep(const SweepArgs& a, int V, hipStream_t s) {
-  switch (V) {
-    case 1: return launch_sweep_v<1>(a, s);
-    case 2: return launch_sweep_v<2>(a, s);
-    case 3: return launch_sweep_v<3>(a, s);
-    case 4: return launch_sweep_v<4>(a, s);
-    case 6: return launch_sweep_v<6>(a, s);
-    case 8: return launch_sweep_v<8>(a, s);
-    case 12: return launch_sweep_v<12>(a, s);
-    case 16: return launch_sweep_v<16>(a, s);
-    default: return hipErrorInvalidValue;
-  }
+hipError_t launch_sweep(const SweepArgs& a, int vt, hipStream_t s) {
+#define SS_CALL(T) launch_sweep_t<T>(a, s)
+  SS_DISPATCH_T(vt, SS_CALL)
+#undef SS_CALL
 }
 
 }  // namespace ss

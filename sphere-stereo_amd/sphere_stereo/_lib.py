@@ -13,6 +13,10 @@ SS_MAX_CAMERAS = 8
 SS_MAX_CANDIDATES = 128
 SS_MAX_REFERENCES = 8
 
+# ss_volume_type: element type of the stored cost volume
+SS_VOLUME_F32, SS_VOLUME_F16, SS_VOLUME_BF16 = 0, 1, 2
+VOLUME_TYPES = {"float32": SS_VOLUME_F32, "float16": SS_VOLUME_F16, "bfloat16": SS_VOLUME_BF16}
+
 
 class ss_camera(C.Structure):
     _fields_ = [("fs", C.c_float * 2), ("cs", C.c_float * 2), ("xi", C.c_double), ("alpha", C.c_double)]
@@ -26,7 +30,7 @@ class ss_intrinsics(C.Structure):
 class ss_frame_desc(C.Structure):
     _fields_ = [("C", C.c_int), ("R", C.c_int), ("N", C.c_int), ("H", C.c_int), ("W", C.c_int),
                 ("Hs", C.c_int), ("Ws", C.c_int), ("PH", C.c_int), ("PW", C.c_int), ("inpaint_iterations", C.c_int),
-                ("refs", C.c_int * SS_MAX_REFERENCES), ("cams", ss_camera * SS_MAX_CAMERAS),
+                ("volume_type", C.c_int), ("refs", C.c_int * SS_MAX_REFERENCES), ("cams", ss_camera * SS_MAX_CAMERAS),
                 ("rts", (C.c_float * (SS_MAX_CAMERAS * 12)) * SS_MAX_REFERENCES),
                 ("cands", C.c_float * SS_MAX_CANDIDATES), ("sigma_i", C.c_double), ("sigma_s", C.c_double),
                 ("thr_empty", C.c_double), ("thr_src", C.c_double),
@@ -56,13 +60,13 @@ SIGNATURES = {
     "ss_blend_lut": (_i, [_INTR, _p, _p, _i, _p, _i, _i, _i, _i, _d, _d, _p, _p, _p]),
     "ss_normalize_weights": (_i, [_p, _i, _i, _p]),
     "ss_guide": (_i, [_p, _i, _i, _p, _p]),
-    "ss_sweep": (_i, [_p, _p, _i, _CAM, _p, _i, _p, _i, _i, _i, _p, _p]),
-    "ss_isb_workspace_bytes": (_z, [_i, _i, _i, _i]),
-    "ss_isb_filter": (_i, [_p, _p, _i, _i, _i, _d, _d, _p, _z, _p]),
+    "ss_sweep": (_i, [_p, _p, _i, _CAM, _p, _i, _p, _i, _i, _i, _i, _p, _p, _p]),
+    "ss_isb_workspace_bytes": (_z, [_i, _i, _i, _i, _i]),
+    "ss_isb_filter": (_i, [_p, _p, _i, _i, _i, _i, _d, _d, _p, _z, _p]),
     "ss_isb_filter_planar": (_i, [_p, _p, _i, _i, _i, _d, _d, _p, _z, _p]),
-    "ss_argmin_refine": (_i, [_p, _i, _i, _i, _fl, _fl, _fl, _p, _p]),
-    "ss_estimate_workspace_bytes": (_z, [_i, _i, _i]),
-    "ss_estimate_distance": (_i, [_p, _p, _i, _CAM, _p, _i, _p, _i, _i, _i, _d, _d, _p, _p, _p, _z, _p]),
+    "ss_argmin_refine": (_i, [_p, _i, _i, _i, _i, _fl, _fl, _fl, _p, _p]),
+    "ss_estimate_workspace_bytes": (_z, [_i, _i, _i, _i]),
+    "ss_estimate_distance": (_i, [_p, _p, _i, _CAM, _p, _i, _p, _i, _i, _i, _d, _d, _i, _p, _p, _p, _z, _p]),
     "ss_reproject": (_i, [_p, _p, _i, _i, _INTR, _p, _p]),
     "ss_inpaint": (_i, [_p, _p, _p, _i, _i, _i, _d, _d, _p]),
     "ss_merge": (_i, [_p, _p, _p, _p, _p, _i, _i, _p, _INTR, _i, _i, _i, _i, _i, _p, _p, _p]),

@@ -2,14 +2,18 @@
 
 The estimator keeps the reference's constructor, ``select_camera``,
 ``estimate_fisheye_distance`` and ``estimate_RGBD_panorama``.  A frame is one
-ss_frame_run call into the native executor: each reference camera's chain
-(guide, sweep, cost filter with the final up-pass fused with the argmin,
-distance filter, uint8 cast of its stitch image, reprojection, inpainting)
-runs on a stream of its own, and the merge runs on torch's current stream,
-which waits for the frame.  With ``use_graph = True`` the frame is captured
-once into a HIP graph and replayed on later calls with the same input
-tensors.  Init-time work (camera selection, unit-ray tables, stitching tables)
-also runs on the GPU.
+ss_frame_run call into the native executor: one sweep launch covers every
+reference (and writes their guides), then each reference camera's chain (cost
+filter with the final up-pass fused with the argmin, distance filter, uint8
+cast of its stitch image, reprojection, inpainting) runs on a stream of its
+own, and the merge runs on torch's current stream, which waits for the frame.
+With ``use_graph = True`` the frame is captured once into a HIP graph and
+replayed on later calls with the same input tensors.  Init-time work (camera
+selection, unit-ray tables, stitching tables) also runs on the GPU.
+
+``volume_dtype`` (keyword, default "float32" = the reference) selects the
+element type the cost volume is stored in: "float16" or "bfloat16" halve its
+HBM traffic; sums, filter weights and the refinement stay fp32.
 """
 import torch
 
@@ -30,10 +34,16 @@ def _hwc(image):
 class RGBD_Estimator:
     def __init__(self, calibrations, min_dist, max_dist, candidate_count, references_indices, reprojection_viewpoint,
                  masks, matching_resolution, rgb_to_stitch_resolution, panorama_resolution, sigma_i, sigma_s,
-                 device):
+                 device, volume_dtype="float32"):
         """RGB-D estimation set-up (depth_estimation.py:39-81): camera selection
         for adaptive matching, filters and stitcher.  Arguments as the
-        reference: resolutions are (cols, rows), masks are [1, rows, cols]."""
+        reference: resolutions are (cols, rows), masks are [1, rows, cols].
+        volume_dtype: storage type of the cost volume ("float32", "float16",
+        "bfloat16")."""
+        if volume_dtype not in _lib.VOLUME_TYPES:
+            raise ValueError("volume_dtype must be one of %s" % sorted(_lib.VOLUME_TYPES))
+        self.volume_dtype = volume_dtype
+        self._vtype = _lib.VOLUME_TYPES[volume_dtype]
         self.calibrations = calibrations
         self.min_dist = min_dist
         self.max_dist = max_dist
@@ -64,7 +74,7 @@ class RGBD_Estimator:
                      for r in range(C)}
         self._cands_list = distance_candidates(min_dist, max_dist, self.candidate_count)
         self._cands = _lib.float_array(self._cands_list)
-        nbytes = _lib.lib.ss_estimate_workspace_bytes(self.rows, self.cols, self.candidate_count)
+        nbytes = _lib.lib.ss_estimate_workspace_bytes(self.rows, self.cols, self.candidate_count, self._vtype)
         if nbytes == 0:
             raise ValueError("unsupported matching resolution / candidate count")
         R = len(self.references_indices)
@@ -87,6 +97,7 @@ class RGBD_Estimator:
             self.rows, self.cols
         d.Hs, d.Ws, d.PH, d.PW = st.srows, st.scols, st.prows, st.pcols
         d.inpaint_iterations = st.inpainting_iterations
+        d.volume_type = self._vtype
         for k in range(d.C):
             d.cams[k] = self._cams[k]
         for n, c in enumerate(self._cands_list):
@@ -155,7 +166,7 @@ class RGBD_Estimator:
         st = self._stream() if stream is None else _lib.C.c_void_p(stream.cuda_stream)
         _lib.check(_lib.lib.ss_estimate_distance(
             self._raylut[slot].data_ptr(), ptrs, len(images), self._cams, self._rts[r], r, self._cands,
-            self.candidate_count, self.rows, self.cols, float(self.sigma_i), float(self.sigma_s),
+            self.candidate_count, self.rows, self.cols, float(self.sigma_i), float(self.sigma_s), self._vtype,
             guide.data_ptr() if guide is not None else None, out.data_ptr(), ws.data_ptr(), ws.numel(), st))
         return out
 

@@ -29,7 +29,7 @@ class ISB_Filter:
             raise ValueError("resolution %dx%d gives no pyramid level" % (self.cols, self.rows))
         if _lib.lib.ss_padded_candidates(self.candidate_count) < 0:
             raise ValueError("candidate_count %d unsupported (1..%d)" % (candidate_count, _lib.SS_MAX_CANDIDATES))
-        nbytes = _lib.lib.ss_isb_workspace_bytes(self.rows, self.cols, self.candidate_count, 1)
+        nbytes = _lib.lib.ss_isb_workspace_bytes(self.rows, self.cols, self.candidate_count, _lib.SS_VOLUME_F32, 1)
         self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
 
     def apply(self, guide, cost, sigma_i, sigma_s):

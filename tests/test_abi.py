@@ -29,13 +29,20 @@ def test_every_declared_symbol_is_exported_and_bound():
 
 def test_queries():
     lib = _lib.lib
-    assert lib.ss_abi_version() == 1
+    assert lib.ss_abi_version() == 2
     assert [lib.ss_padded_candidates(n) for n in (1, 2, 8, 9, 32, 33, 48, 64, 100, 128, 129)] == \
         [1, 8, 8, 16, 32, 48, 48, 64, 128, 128, -1]
     assert lib.ss_isb_levels(1024, 1024) == 9 and lib.ss_isb_levels(768, 768) == 8
     assert lib.ss_isb_levels(3, 3) == 0 and lib.ss_isb_levels(4, 1000) == 1
-    assert lib.ss_isb_workspace_bytes(3, 3, 4, 0) == 0
-    assert lib.ss_estimate_workspace_bytes(1024, 1024, 32) > 1024 * 1024 * 32 * 4
+    assert lib.ss_isb_workspace_bytes(3, 3, 4, 0, 0) == 0
+    f32 = lib.ss_estimate_workspace_bytes(1024, 1024, 32, _lib.SS_VOLUME_F32)
+    f16 = lib.ss_estimate_workspace_bytes(1024, 1024, 32, _lib.SS_VOLUME_F16)
+    assert f32 > 1024 * 1024 * 32 * 4 and 1024 * 1024 * 32 * 2 < f16 < 1024 * 1024 * 32 * 3
+    assert lib.ss_estimate_workspace_bytes(1024, 1024, 32, _lib.SS_VOLUME_BF16) == f16
+    assert lib.ss_estimate_workspace_bytes(1024, 1024, 32, 7) == 0
+    # the planar drop-in is fp32 only
+    assert lib.ss_isb_workspace_bytes(64, 64, 8, _lib.SS_VOLUME_F16, 1) == 0
+    assert lib.ss_isb_workspace_bytes(64, 64, 8, _lib.SS_VOLUME_F16, 0) < lib.ss_isb_workspace_bytes(64, 64, 8, 0, 0)
     assert lib.ss_stage_name(1) == b"sweep" and lib.ss_stage_name(99) == b""
 
 
@@ -45,18 +52,25 @@ def test_argument_errors_before_device_work():
     rts = _lib.float_array([0.0] * 48)
     cands = _lib.float_array([1.0] * 8)
     imgs = (C.c_void_p * 4)(1, 2, 3, 4)
-    rc = lib.ss_sweep(C.c_void_p(1), imgs, 0, cams, rts, 0, cands, 8, 16, 16, C.c_void_p(1), None)
+    one = C.c_void_p(1)
+    rc = lib.ss_sweep(one, imgs, 0, cams, rts, 0, cands, 8, 16, 16, 0, one, None, None)
     assert rc == 1 and b"camera count" in lib.ss_last_error()
-    rc = lib.ss_sweep(C.c_void_p(1), imgs, 4, cams, rts, 5, cands, 8, 16, 16, C.c_void_p(1), None)
+    rc = lib.ss_sweep(one, imgs, 4, cams, rts, 5, cands, 8, 16, 16, 0, one, None, None)
     assert rc == 1 and b"reference index" in lib.ss_last_error()
-    rc = lib.ss_sweep(C.c_void_p(1), imgs, 4, cams, rts, 0, cands, 200, 16, 16, C.c_void_p(1), None)
+    rc = lib.ss_sweep(one, imgs, 4, cams, rts, 0, cands, 200, 16, 16, 0, one, None, None)
     assert rc == 1 and b"candidate count" in lib.ss_last_error()
-    rc = lib.ss_estimate_distance(C.c_void_p(1), imgs, 4, cams, rts, 0, cands, 8, 16, 16, 10.0, 25.0, None,
-                                  C.c_void_p(1), C.c_void_p(1), 16, None)
+    rc = lib.ss_sweep(one, imgs, 4, cams, rts, 0, cands, 8, 16, 16, 5, one, None, None)
+    assert rc == 1 and b"volume type" in lib.ss_last_error()
+    rc = lib.ss_sweep(one, imgs, 4, cams, rts, 0, cands, 1, 16, 16, 0, one, None, None)
+    assert rc == 1 and b"at least 2" in lib.ss_last_error()
+    rc = lib.ss_estimate_distance(one, imgs, 4, cams, rts, 0, cands, 8, 16, 16, 10.0, 25.0, 0, None, one, one, 16,
+                                  None)
     assert rc == 4 and b"workspace" in lib.ss_last_error()
-    rc = lib.ss_estimate_distance(C.c_void_p(1), imgs, 4, cams, rts, 0, cands, 8, 3, 3, 10.0, 25.0, None,
-                                  C.c_void_p(1), C.c_void_p(1), 1 << 20, None)
+    rc = lib.ss_estimate_distance(one, imgs, 4, cams, rts, 0, cands, 8, 3, 3, 10.0, 25.0, 0, None, one, one,
+                                  1 << 20, None)
     assert rc == 1 and b"pyramid" in lib.ss_last_error()
+    rc = lib.ss_isb_filter(one, one, 16, 16, 1, _lib.SS_VOLUME_F16, 10.0, 25.0, one, 1 << 20, None)
+    assert rc == 1 and b"fp32 only" in lib.ss_last_error()
     assert lib.ss_box_filter(C.c_void_p(8), C.c_void_p(8), C.c_void_p(16), 1, 4, 4, 1, 0.0, None) == 1
     assert lib.ss_isb_filter_planar(C.c_void_p(8), C.c_void_p(16), 16, 16, 4, 10.0, 25.0, None, 0, None) == 4
     assert lib.ss_merge(None, None, None, None, None, 8, 8, None, None, 1, 8, 8, 4, 4, None, None, None) == 1

@@ -121,18 +121,70 @@ def test_select_camera_bit_exact(small):
         assert not np.any(got == r), "the reference camera is never selected"
 
 
-def test_sweep_bit_exact(small, gpu):
+VOLUME_DTYPES = {0: torch.float32, 1: torch.float16, 2: torch.bfloat16}
+
+
+@pytest.mark.parametrize("vtype", [0, 1, 2])
+def test_sweep_bit_exact(small, gpu, vtype):
     fr, est, ora = small
     H, W = 72, 80
     N = 32
     NP = _lib.lib.ss_padded_candidates(N)
     for slot, r in enumerate(fr["refs"]):
-        cost = torch.empty([H, W, NP], device=gpu)
+        cost = torch.empty([H, W, NP], dtype=VOLUME_DTYPES[vtype], device=gpu)
+        guide = torch.empty([H, W], dtype=torch.int32, device=gpu)
         imgs = [i.contiguous() for i in fr["to_match"]]
         _lib.check(_lib.lib.ss_sweep(est._raylut[slot].data_ptr(), _lib.ptr_array(imgs), len(imgs), est._cams,
-                                     est._rts[r], r, est._cands, N, H, W, cost.data_ptr(), _stream()))
-        want = O32.sweep(np.stack([to_np(i) for i in imgs]), ora.sel[r], ora.cands, ora.cams, ora.rts[r], r)
-        assert_same(np.transpose(to_np(cost)[..., :N], (2, 0, 1)), want, "cost volume ref %d" % r)
+                                     est._rts[r], r, est._cands, N, H, W, vtype, cost.data_ptr(), guide.data_ptr(),
+                                     _stream()))
+        want = O32.sweep(np.stack([to_np(i) for i in imgs]), ora.sel[r], ora.cands, ora.cams, ora.rts[r], r, vtype)
+        assert_same(np.transpose(to_np(cost.float())[..., :N], (2, 0, 1)), want, "cost volume ref %d" % r)
+        packed = to_np(guide).view(np.uint32)
+        got_g = np.stack([(packed >> (8 * c)) & 255 for c in range(3)], -1).astype(np.uint8)
+        assert_same(got_g, O32.guide(to_np(imgs[r])), "guide written by the sweep, ref %d" % r)
+
+
+@pytest.mark.parametrize("vtype", [1, 2])
+@pytest.mark.parametrize("shape,N", [((33, 17), 8), ((48, 64), 32), ((31, 45), 48), ((24, 40), 128)])
+def test_isb_filter_precision_form_bit_exact(gpu, vtype, shape, N):
+    H, W = shape
+    NP = _lib.lib.ss_padded_candidates(N)
+    g = torch.Generator().manual_seed(H * 7 + W + N + vtype)
+    guide = torch.randint(0, 256, [H, W, 3], generator=g, dtype=torch.uint8)
+    guide[: H // 2] = guide[: H // 2] // 8 + 100
+    cost = torch.rand([N, H, W], generator=g) * 500
+    cost16 = O32.round_volume(cost.numpy(), vtype)
+    want_c, want_g = O32.isb_filter(guide.numpy(), cost16, 10.0, 25.0, vtype=vtype, round_level0=True)
+    vol = torch.zeros([H, W, NP], dtype=VOLUME_DTYPES[vtype])
+    vol[..., :N] = torch.from_numpy(cost16).permute(1, 2, 0).to(VOLUME_DTYPES[vtype])
+    vol = vol.to(gpu)
+    g32 = guide.to(torch.int32)
+    packed = (g32[..., 0] | (g32[..., 1] << 8) | (g32[..., 2] << 16)).contiguous().to(gpu)
+    nbytes = _lib.lib.ss_isb_workspace_bytes(H, W, N, vtype, 0)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=gpu)
+    _lib.check(_lib.lib.ss_isb_filter(packed.data_ptr(), vol.data_ptr(), H, W, N, vtype, 10.0, 25.0, ws.data_ptr(),
+                                      nbytes, _stream()))
+    got = np.transpose(to_np(vol.float())[..., :N], (2, 0, 1))
+    assert_same(got, want_c, "filtered %s volume" % VOLUME_DTYPES[vtype])
+    pk = to_np(packed).view(np.uint32)
+    assert_same(np.stack([(pk >> (8 * c)) & 255 for c in range(3)], -1).astype(np.uint8), want_g, "guide")
+
+
+@pytest.mark.parametrize("volume_dtype", ["float16", "bfloat16"])
+def test_panorama_precision_form_bit_exact(small, gpu, volume_dtype):
+    fr = small[0]
+    kw = dict(min_dist=0.55, max_dist=100, candidate_count=32, references_indices=fr["refs"],
+              reprojection_viewpoint=fr["viewpoint"], masks=fr["masks"], matching_resolution=fr["matching"],
+              rgb_to_stitch_resolution=fr["stitch"], panorama_resolution=(128, 64), sigma_i=10, sigma_s=12.5)
+    est = RGBD_Estimator(fr["calibs"], device=gpu, volume_dtype=volume_dtype, **kw)
+    ora = OraclePipeline(fr["calibs"], volume_dtype=volume_dtype, **kw)
+    keep = {}
+    want_rgb, want_dist = ora.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"], keep=keep)
+    rgb, dist = est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])
+    for i in range(len(fr["refs"])):
+        assert_same(to_np(est._distance_maps[i]), keep["dmaps"][i], "%s distance map %d" % (volume_dtype, i))
+    assert_same(to_np(dist), want_dist, "%s distance panorama" % volume_dtype)
+    assert_same(to_np(rgb), want_rgb, "%s rgb panorama" % volume_dtype)
 
 
 def test_estimate_distance_bit_exact(small):

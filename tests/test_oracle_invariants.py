@@ -208,6 +208,61 @@ def test_isb_constant_guide_bytes_truncate_by_at_most_one():
     assert out.min() >= 137 - 2 and out.max() <= 137
 
 
+def test_volume_rounding_matches_numpy_float16_and_torch_bfloat16():
+    import torch
+
+    rng = np.random.RandomState(11)
+    x = np.concatenate([
+        (rng.rand(20000) - 0.5) * 1200,  # the cost range
+        rng.rand(5000) * 1e-4,  # binary16 subnormals (below 6.1e-5)
+        np.float32(2.0) ** rng.randint(-30, 20, 2000) * (1 + rng.randint(0, 2048, 2000) / 2048.0 + 1 / 4096.0),  # ties
+        [0.0, -0.0, 65504.0, 65519.99, 65520.0, 1e6, -1e6, np.inf, -np.inf, 5.9604645e-08, 2.9802322e-08, 3e-8],
+    ]).astype(np.float32)
+    f16 = O32.round_volume(x, 1)
+    with np.errstate(over="ignore"):
+        assert np.array_equal(f16, x.astype(np.float16).astype(np.float32))
+    bf = O32.round_volume(x, 2)
+    assert np.array_equal(bf, torch.from_numpy(x).to(torch.bfloat16).float().numpy())
+    assert np.isnan(O32.round_volume(np.array([np.nan], np.float32), 1)).all()
+    assert np.isnan(O32.round_volume(np.array([np.nan], np.float32), 2)).all()
+    assert np.array_equal(O32.round_volume(x, 0), x)
+
+
+def test_precision_form_rounds_exactly_the_stored_values():
+    rng = np.random.RandomState(3)
+    H, W, N = 21, 30, 5
+    guide = rng.randint(0, 255, (H, W, 3)).astype(np.uint8)
+    cost = (rng.rand(N, H, W) * 500).astype(np.float32)
+    for vt, dt in ((1, np.float16), (2, None)):
+        c16 = O32.round_volume(cost, vt)
+        stored, _ = O32.isb_filter(guide, c16, 10.0, 25.0, vtype=vt, round_level0=True)
+        fused, _ = O32.isb_filter(guide, c16, 10.0, 25.0, vtype=vt, round_level0=False)
+        # level 0's last pass is the only difference between the two forms
+        assert np.array_equal(O32.round_volume(fused, vt), stored)
+        if dt is not None:
+            assert np.array_equal(stored.astype(dt).astype(np.float32), stored)
+        # the same pyramid composed from the single passes
+        L = O32.levels(W, H)
+        gs, cs = [guide], [c16]
+        for s in range(1, L):
+            h, w = -(-H // 2 ** s), -(-W // 2 ** s)
+            g, c = O32.isb_down(gs[-1], cs[-1], h, w, 1 / (2 * 10.0 ** 2), vtype=vt)
+            gs.append(g)
+            cs.append(c)
+        for s in range(L - 2, -1, -1):
+            wd = math.exp(-((2.0 ** s - 0.5) ** 2) / (2 * 25.0 ** 2))
+            gs[s], cs[s] = O32.isb_up(gs[s + 1], cs[s + 1], gs[s], cs[s], 1 - wd, wd, 1 / (2 * 10.0 ** 2), vtype=vt)
+        assert np.array_equal(cs[0], stored)
+        # error against the unrounded fp64 method stays at the storage precision
+        ref, _ = O64.isb_filter(guide, cost.astype(np.float64), 10.0, 25.0)
+        tol = 500 * (2.0 ** -10 if vt == 1 else 2.0 ** -7)
+        assert np.abs(stored - ref).max() < 2 * tol
+    # a constant volume exactly representable in fp16 stays exactly constant in
+    # every level: the stencils are normalised and each pass rounds back to it
+    const, _ = O32.isb_filter(guide, np.full((N, H, W), 123.25, np.float32), 10.0, 25.0, vtype=1)
+    assert np.all(const == 123.25)
+
+
 def test_levels_formula():
     for W, H in [(4, 4), (5, 7), (8, 8), (512, 512), (768, 768), (1024, 1024), (1216, 1024), (2048, 1024)]:
         assert O32.levels(W, H) == _levels(W, H)
