@@ -5,8 +5,10 @@ Workload (default, "c1"): four seeded synthetic 1216x1216 double-sphere
 fisheyes (uint8 RGB, ray-cast on the GPU), matched at 1024x1024, references 0
 and 2, 32 candidates uniform in inverse distance over [1/100, 1/0.55],
 sigma_i 10, sigma_s 25, 1216x1216 stitch input, 2048x1024 RGB + float32
-distance panorama, fp32 throughout.  One step = one estimate_RGBD_panorama
-call on inputs already resident in HBM.
+distance panorama, fp32 (the reference's type).  One step = one
+estimate_RGBD_panorama call on inputs already resident in HBM.  The same
+frames are then timed with the cost volume stored in fp16 and bf16
+("precision_forms") and with HIP-graph replay ("graph_replay").
 
 Prints one JSON line (see the roofline / cpu_baseline fields) and a per-stage
 split on stderr.

@@ -126,8 +126,10 @@ class RGBD_Estimator:
         return handle
 
     def __del__(self):
-        if getattr(self, "_frame", None):
-            _lib.lib.ss_frame_destroy(self._frame)
+        # at interpreter shutdown the module globals may already be cleared
+        lib = getattr(_lib, "lib", None)
+        if getattr(self, "_frame", None) and lib is not None:
+            lib.ss_frame_destroy(self._frame)
             self._frame = None
 
     def _stream(self):
