@@ -130,7 +130,8 @@ void carve_levels(Carver& cv, Level* lv, int L, int H, int W, int NP, size_t eb)
 // fuse_last, the level-0 up-pass computes the argmin instead of writing the
 // filtered volume.
 int run_pyramid(Level* lv, int L, int G, int V, int vt, double sigma_i, double sigma_s, bool fuse_last,
-                const ss::ArgminParams& am, float* dist, hipStream_t s, int stage_down, int stage_up) {
+                const ss::ArgminParams& am, float* dist, hipStream_t s, int stage_down, int stage_up,
+                float* fill = nullptr) {
   const double var_inv_s = 1 / (2 * sigma_s * sigma_s);
   const double var_inv_i = 1 / (2 * sigma_i * sigma_i);
   const float vi = (float)var_inv_i;
@@ -199,7 +200,7 @@ int run_pyramid(Level* lv, int L, int G, int V, int vt, double sigma_i, double s
     double wu, wd;
     weights(k, wu, wd);
     ss::IsbUpArgs a{lv[k + 1].g, lv[k + 1].c, lv[k + 1].h, lv[k + 1].w, lv[k].g, lv[k].c, lv[k].h, lv[k].w,
-                    (float)wu,   (float)wd,   vi,          dist,        am};
+                    (float)wu,   (float)wd,   vi,          dist,        am,   fuse_last && k == 0 ? fill : nullptr};
     const bool fuse = fuse_last && k == 0;
     StageTimer t(fuse ? SS_STAGE_ISB_UP_ARGMIN : stage_up, s);
     SS_HIP(ss::launch_isb_up(a, G, V, vt, fuse, s));
@@ -521,7 +522,7 @@ void estimate_level0(void* ws, int H, int W, int N, int vt, uint32_t** g0, void*
 }
 
 int estimate_after_sweep(int H, int W, int N, int vt, const float* cands, double sigma_i, double sigma_s,
-                         float* d_dist, void* ws, hipStream_t s) {
+                         float* d_dist, void* ws, hipStream_t s, float* fill) {
   const int L = levels_of(H, W);
   Carver cv(ws);
   uint32_t* g0;
@@ -533,12 +534,14 @@ int estimate_after_sweep(int H, int W, int N, int vt, const float* cands, double
     // one candidate: max == min everywhere, so the reference returns d_{N-1}
     // (depth_estimation.py:198) whatever the cost
     SS_HIP(ss::launch_fill(d_dist, dN1, (size_t)H * W, s));
+    if (fill) SS_HIP(ss::launch_fill(fill, 1e8f, (size_t)H * W, s));
   } else {
     lv[0] = Level{H, W, g0, c0};
     ss::ArgminParams am{N, d0, dN1, (float)(d0 / dN1) - 1.0f};
     const int rc = run_pyramid(lv, L, 8, ss::lanes_v(N), vt, sigma_i, sigma_s, true, am, d_dist, s,
-                               SS_STAGE_ISB_DOWN, SS_STAGE_ISB_UP);
+                               SS_STAGE_ISB_DOWN, SS_STAGE_ISB_UP, L > 1 ? fill : nullptr);
     if (rc) return rc;
+    if (fill && L == 1) SS_HIP(ss::launch_fill(fill, 1e8f, (size_t)H * W, s));
   }
   dl[0] = Level{H, W, g0, d_dist};
   return run_pyramid(dl, L, 1, 1, SS_VOLUME_F32, sigma_i / 2, sigma_s / 2, false, ss::ArgminParams{}, nullptr, s,
@@ -568,6 +571,14 @@ int sweep_references(const float* const* d_raylut, const float* const* d_images,
                                      costs + r0, guides + r0, s);
     if (rc) return rc;
   }
+  return SS_OK;
+}
+
+int reproject_into(const float* d_in, float* d_out, int H, int W, const ss_intrinsics* intr, const float* t3,
+                   bool fill, hipStream_t s) {
+  StageTimer t(SS_STAGE_REPROJECT, s);
+  if (fill) SS_HIP(ss::launch_fill(d_out, 1e8f, (size_t)H * W, s));
+  SS_HIP(ss::launch_reproject(d_in, d_out, H, W, *intr, t3, s));
   return SS_OK;
 }
 
@@ -607,7 +618,7 @@ int ss_estimate_distance(const float* d_raylut, const float* const* d_images, in
     rc = ss::sweep_references(&d_raylut, d_images, C, cams, &rts, &ref, 1, cands, N, H, W, vtype, ws, s);
     if (rc) return rc;
   }
-  rc = ss::estimate_after_sweep(H, W, N, vtype, cands, sigma_i, sigma_s, d_dist, d_workspace, s);
+  rc = ss::estimate_after_sweep(H, W, N, vtype, cands, sigma_i, sigma_s, d_dist, d_workspace, s, nullptr);
   return rc ? rc : ok();
 }
 
@@ -615,11 +626,8 @@ int ss_reproject(const float* d_in, float* d_out, int H, int W, const ss_intrins
                  ss_stream stream) {
   SS_REQUIRE(d_in && d_out && intr && t3 && H >= 1 && W >= 1, "bad argument");
   SS_REQUIRE(d_in != d_out, "reprojection cannot run in place");
-  hipStream_t s = as_stream(stream);
-  StageTimer t(SS_STAGE_REPROJECT, s);
-  SS_HIP(ss::launch_fill(d_out, 1e8f, (size_t)H * W, s));
-  SS_HIP(ss::launch_reproject(d_in, d_out, H, W, *intr, t3, s));
-  return ok();
+  const int rc = ss::reproject_into(d_in, d_out, H, W, intr, t3, true, as_stream(stream));
+  return rc ? rc : ok();
 }
 
 int ss_inpaint(float* d_map, float* d_tmp, const uint8_t* d_lut, int H, int W, int iterations, double thr_empty,

@@ -1,8 +1,8 @@
 // Per-frame executor: estimate_RGBD_panorama (depth_estimation.py:205-243 and
-// stitcher.py:166-220) as one call.  The sweep of every reference is one
-// launch (it also writes the guides); after it, each reference's chain runs
-// on a stream of its own (they are independent until the merge), so one
-// reference's latency-bound coarse pyramid levels overlap the other's
+// stitcher.py:166-220) as one call.  One sweep launch covers every reference
+// (and writes their guides); after it, each reference's chain runs on a
+// stream of its own (the chains are independent until the merge), so one
+// reference's latency-bound coarse pyramid passes overlap the other's
 // full-size passes.  The frame can be captured once into a HIP graph and
 // replayed, which removes the per-launch host cost of its ~70 kernels.
 #include <stdio.h>
@@ -46,7 +46,7 @@ hipError_t pool_init(StreamPool& p) {
 struct Frame {
   ss_frame_desc d;
   StreamPool* pool = nullptr;
-  hipEvent_t fork = nullptr, pre = nullptr, done = nullptr;
+  hipEvent_t pre = nullptr, done = nullptr, swept = nullptr;
   hipEvent_t join[SS_MAX_REFERENCES] = {};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -73,29 +73,35 @@ struct Frame {
 // The frame's work after the events on `s`; `s` waits for all of it.
 int enqueue(Frame* f, const float* const* match, const float* const* stitch, hipStream_t s) {
   const ss_frame_desc& d = f->d;
+  const float* reproj[SS_MAX_REFERENCES];
+  const float* dists[SS_MAX_REFERENCES];
+  const uint8_t* imgs[SS_MAX_REFERENCES];
+  // One sweep launch for every reference.  (Staggering per-reference sweeps
+  // so that one reference's chain overlaps the next sweep measured no faster:
+  // the ALU-bound sweep occupies the CUs and the overlapped passes crawl.)
   const float* rts[SS_MAX_REFERENCES];
   for (int r = 0; r < d.R; ++r) rts[r] = d.rts[r];
   FR_CALL(ss::sweep_references(d.d_raylut, match, d.C, d.cams, rts, d.refs, d.R, d.cands, d.N, d.H, d.W,
                                d.volume_type, d.d_workspace, s));
-  FR_HIP(hipEventRecord(f->fork, s));
-  const float* reproj[SS_MAX_REFERENCES];
-  const float* dists[SS_MAX_REFERENCES];
-  const uint8_t* imgs[SS_MAX_REFERENCES];
+  FR_HIP(hipEventRecord(f->swept, s));
   for (int r = 0; r < d.R; ++r) {
     hipStream_t sr = f->pool->lane[r % kLanes];
-    FR_HIP(hipStreamWaitEvent(sr, f->fork, 0));
+    FR_HIP(hipStreamWaitEvent(sr, f->swept, 0));
+    // the fused argmin pass also fills the reprojection target with 1e8
     FR_CALL(ss::estimate_after_sweep(d.H, d.W, d.N, d.volume_type, d.cands, d.sigma_i, d.sigma_s, d.d_dist[r],
-                                     d.d_workspace[r], sr));
+                                     d.d_workspace[r], sr, d.d_reproj[r]));
+    // (issued before the sweeps end, this memory-bound cast would crawl
+    // beside the ALU-bound sweep and hold the chain back)
     FR_CALL(ss_to_u8(stitch[r], d.d_stitch_u8[r], (size_t)d.Hs * d.Ws * 3, sr));
-    FR_CALL(ss_reproject(d.d_dist[r], d.d_reproj[r], d.H, d.W, &d.intr[r], d.trans[r], sr));
+    FR_CALL(ss::reproject_into(d.d_dist[r], d.d_reproj[r], d.H, d.W, &d.intr[r], d.trans[r], false, sr));
     FR_CALL(ss_inpaint(d.d_reproj[r], d.d_scratch[r], d.d_inpaint_lut[r], d.H, d.W, d.inpaint_iterations,
                        d.thr_empty, d.thr_src, sr));
     FR_HIP(hipEventRecord(f->join[r], sr));
-    FR_HIP(hipStreamWaitEvent(s, f->join[r], 0));
     reproj[r] = d.d_reproj[r];
     dists[r] = d.d_dist[r];
     imgs[r] = d.d_stitch_u8[r];
   }
+  for (int r = 0; r < d.R; ++r) FR_HIP(hipStreamWaitEvent(s, f->join[r], 0));
   return ss_merge(d.d_sampling, d.d_weights, reproj, dists, imgs, d.Hs, d.Ws, &d.trans[0][0], d.intr, d.R, d.H, d.W,
                   d.PH, d.PW, d.d_dist_pano, d.d_rgb_pano, s);
 }
@@ -146,7 +152,7 @@ int ss_frame_create(const ss_frame_desc* desc, void** frame) {
   };
   for (int r = 0; r < d.R; ++r)
     check(hipEventCreateWithFlags(&f->join[r], hipEventDisableTiming), "hipEventCreateWithFlags");
-  check(hipEventCreateWithFlags(&f->fork, hipEventDisableTiming), "hipEventCreateWithFlags");
+  check(hipEventCreateWithFlags(&f->swept, hipEventDisableTiming), "hipEventCreateWithFlags");
   check(hipEventCreateWithFlags(&f->pre, hipEventDisableTiming), "hipEventCreateWithFlags");
   check(hipEventCreateWithFlags(&f->done, hipEventDisableTiming), "hipEventCreateWithFlags");
   if (rc != SS_OK) {
@@ -205,7 +211,7 @@ int ss_frame_destroy(void* frame) {
   drop_graph(f);
   for (int r = 0; r < SS_MAX_REFERENCES; ++r)
     if (f->join[r]) (void)hipEventDestroy(f->join[r]);
-  if (f->fork) (void)hipEventDestroy(f->fork);
+  if (f->swept) (void)hipEventDestroy(f->swept);
   if (f->pre) (void)hipEventDestroy(f->pre);
   if (f->done) (void)hipEventDestroy(f->done);
   delete f;

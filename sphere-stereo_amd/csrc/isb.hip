@@ -307,7 +307,10 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
     for (int s = 0; s < 4; ++s) {
       if (!doit[s]) continue;  // uniform within the group: same coarse pixel
       const float d = argmin_refine_group<G, V>(res[s], lane, a.am);
-      if (active && lane == 0) a.dist[fine[s]] = d;
+      if (active && lane == 0) {
+        a.dist[fine[s]] = d;
+        if (a.fill) a.fill[fine[s]] = 1e8f;
+      }
     }
   }
 }
@@ -351,7 +354,7 @@ __global__ void __launch_bounds__(kTailBlock) k_isb_tail(IsbTailArgs t) {
   }
   for (int k = t.n - 2; k >= 0; --k) {
     const IsbUpArgs a{gp(k + 1), cp(k + 1), t.h[k + 1], t.w[k + 1], gp(k), cp(k), t.h[k], t.w[k],
-                      t.wu[k],   t.wd[k],   t.var_inv_i, nullptr,  ArgminParams{}};
+                      t.wu[k],   t.wd[k],   t.var_inv_i, nullptr,  ArgminParams{}, nullptr};
     const int npix = t.h[k + 1] * t.w[k + 1];
     for (int base = 0; base < npix; base += slots)
       if (base + slot < npix) up_pixel<G, V, T, false>(a, base + slot, true, lane);

@@ -36,6 +36,7 @@ struct IsbUpArgs {
   float weight_up, weight_down, var_inv_i;
   float* dist;  // fused argmin output (level-0 distance map)
   ArgminParams am;
+  float* fill;  // nullable: with the fused argmin, set to 1e8 at every level-0 pixel
 };
 
 // Levels s0..s0+n-1 of a pyramid for the single-workgroup coarse tail:
@@ -138,9 +139,13 @@ hipError_t launch_sweep(const SweepArgs& a, int vt, hipStream_t s);
 size_t volume_elem_bytes(int vt);
 // level-0 guide and volume of an estimate workspace
 void estimate_level0(void* ws, int H, int W, int N, int vt, uint32_t** g0, void** c0);
-// everything after the sweep: cost filter + argmin, then the distance filter
+// everything after the sweep: cost filter + argmin, then the distance filter;
+// fill (nullable, H x W) is set to 1e8 on the way (the reprojection target)
 int estimate_after_sweep(int H, int W, int N, int vt, const float* cands, double sigma_i, double sigma_s,
-                         float* d_dist, void* ws, hipStream_t s);
+                         float* d_dist, void* ws, hipStream_t s, float* fill);
+// the z-tested reprojection, with or without the 1e8 fill of d_out first
+int reproject_into(const float* d_in, float* d_out, int H, int W, const ss_intrinsics* intr, const float* t3,
+                   bool fill, hipStream_t s);
 // one sweep launch over R references (each with its own workspace)
 int sweep_references(const float* const* d_raylut, const float* const* d_images, int C, const ss_camera* cams,
                      const float* const* rts, const int* refs, int R, const float* cands, int N, int H, int W,

@@ -100,8 +100,8 @@ __global__ void __launch_bounds__(kBlock) k_inpaint_step(const float* __restrict
 // writing the other; unlisted pixels never change, so both copies keep them.
 // A pass moves information by at most one pixel, so the block's own pixels,
 // kSteps inside the staged region, are exact after kSteps passes.
-constexpr int kTile = 48;
-constexpr int kSteps = 8;
+constexpr int kTile = 32;
+constexpr int kSteps = 16;
 constexpr int kRegion = kTile + 2 * kSteps;  // 64: index math by shifts
 static_assert(kRegion == 64, "region side is 64");
 

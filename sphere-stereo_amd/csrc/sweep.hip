@@ -232,27 +232,79 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
 #pragma unroll
   for (int i = 0; i < 12; ++i) rt[i] = a.rt[rf.rt0 + k][i];
   const float* img = a.img[k];
+  const float wlim = (float)(a.W - 1), hlim = (float)(a.H - 1);
   for (int n0 = 0; n0 < a.NP; n0 += 4) {
-    float out[4];
+    // sample coordinates of the chunk's 4 candidates (pads sample a dummy
+    // interior point and are stored as 0)
+    float ix[4], iy[4];
+    bool inner = true;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int n = n0 + v;
-      float c = 0.0f;
-      if (n < a.N) {
-        const float d = a.cands[n];
+      ix[v] = 0.5f;
+      iy[v] = 0.5f;
+      if (n0 + v < a.N) {
+        const float d = a.cands[n0 + v];
         float qx, qy, qz, u, w;
         bool valid;
         rigid(rt, d * rl.x, d * rl.y, d * rl.z, qx, qy, qz);
         project_t(cam, qx, qy, qz, u, w, valid);
-        float s[3];
-        if constexpr (POW2)
-          sample_rgb(img, a.H, a.W, grid_coord_pow2(u, a.inv2W, a.halfW), grid_coord_pow2(w, a.inv2H, a.halfH), s);
-        else
-          sample_rgb(img, a.H, a.W, grid_coord2<false>(u, a.Wf, a.invW), grid_coord2<false>(w, a.Hf, a.invH), s);
-        c = (fabsf(s[0] - r0) + fabsf(s[1] - r1)) + fabsf(s[2] - r2);
-        c = c > 500.0f ? 500.0f : c;
+        if constexpr (POW2) {
+          ix[v] = grid_coord_pow2(u, a.inv2W, a.halfW);
+          iy[v] = grid_coord_pow2(w, a.inv2H, a.halfH);
+        } else {
+          ix[v] = grid_coord2<false>(u, a.Wf, a.invW);
+          iy[v] = grid_coord2<false>(w, a.Hf, a.invH);
+        }
       }
-      out[v] = c;
+      const float x0f = floorf(ix[v]), y0f = floorf(iy[v]);
+      inner = inner && x0f >= 0.0f && x0f < wlim && y0f >= 0.0f && y0f < hlim;
+    }
+    float out[4];
+    if (inner) {
+      // all 16 taps inside the image: issue every load, then combine with
+      // the arithmetic of sample_zero (no tap clamped or masked)
+      float tn[4][6], ts[4][6];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float* pn = img + (unsigned)((int)floorf(iy[v]) * a.W + (int)floorf(ix[v])) * 3u;
+        const float* ps = pn + (unsigned)a.W * 3u;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+          tn[v][i] = pn[i];
+          ts[v][i] = ps[i];
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float x0f = floorf(ix[v]), y0f = floorf(iy[v]);
+        const float x1f = x0f + 1.0f, y1f = y0f + 1.0f;
+        const float wnw = (x1f - ix[v]) * (y1f - iy[v]), wne = (ix[v] - x0f) * (y1f - iy[v]);
+        const float wsw = (x1f - ix[v]) * (iy[v] - y0f), wse = (ix[v] - x0f) * (iy[v] - y0f);
+        float s[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          float t = tn[v][c] * wnw;
+          t = t + tn[v][3 + c] * wne;
+          t = t + ts[v][c] * wsw;
+          t = t + ts[v][3 + c] * wse;
+          s[c] = t;
+        }
+        float c = (fabsf(s[0] - r0) + fabsf(s[1] - r1)) + fabsf(s[2] - r2);
+        c = c > 500.0f ? 500.0f : c;
+        out[v] = (n0 + v < a.N) ? c : 0.0f;
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        float c = 0.0f;
+        if (n0 + v < a.N) {
+          float s[3];
+          sample_zero<3>(img, a.H, a.W, ix[v], iy[v], s);
+          c = (fabsf(s[0] - r0) + fabsf(s[1] - r1)) + fabsf(s[2] - r2);
+          c = c > 500.0f ? 500.0f : c;
+        }
+        out[v] = c;
+      }
     }
     store_n<4>(dst + n0, out);
   }

@@ -57,7 +57,7 @@ def test_isb_filter_planar_bit_exact(gpu, shape, N):
     assert_same(to_np(cd), want_c, "cost")
 
 
-@pytest.mark.parametrize("iters", [1, 5, 8, 13, 32])
+@pytest.mark.parametrize("iters", [1, 5, 8, 13, 16, 17, 32])
 def test_inpaint_bit_exact(gpu, iters):
     H, W = 97, 150
     intr = [0.3 * 150, 0.3 * 150, 75.0, 48.0, -0.18, 0.59]
