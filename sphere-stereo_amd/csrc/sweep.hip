@@ -150,59 +150,42 @@ __device__ __forceinline__ float grid_coord_pow2(float u, float inv2, float half
   return __builtin_fmaf(s, half, -0.5f);
 }
 
-// sample_zero<3> with a fast path for the common case where all four taps
-// lie inside the image: then no tap is clamped or masked, the taps are two
-// pairs of adjacent texels one row apart, and the arithmetic (weights, tap
-// order, products and sums) is exactly that of sample_zero.
-__device__ __forceinline__ void sample_rgb(const float* __restrict__ img, int H, int W, float ix, float iy,
-                                           float* out) {
-  const float x0f = floorf(ix), y0f = floorf(iy);
-  if (x0f >= 0.0f && x0f < (float)(W - 1) && y0f >= 0.0f && y0f < (float)(H - 1)) {
-    const int x0 = (int)x0f, y0 = (int)y0f;
-    const float x1f = x0f + 1.0f, y1f = y0f + 1.0f;
-    const float wnw = (x1f - ix) * (y1f - iy), wne = (ix - x0f) * (y1f - iy);
-    const float wsw = (x1f - ix) * (iy - y0f), wse = (ix - x0f) * (iy - y0f);
-    const float* pn = img + (unsigned)(y0 * W + x0) * 3u;
-    const float* ps = pn + (unsigned)W * 3u;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      float s = pn[c] * wnw;
-      s = s + pn[3 + c] * wne;
-      s = s + ps[c] * wsw;
-      s = s + ps[3 + c] * wse;
-      out[c] = s;
-    }
-    return;
-  }
-  sample_zero<3>(img, H, W, ix, iy, out);
-}
-
 // Sweep + raw cost (depth_estimation.py:141-175) for every reference of the
 // launch: blocks [r*blocks_per_ref, (r+1)*blocks_per_ref) sweep reference r.
 // One lane per reference pixel, over a 64 x 4 pixel tile per workgroup (one
 // wavefront per row segment).  Every lane of a wavefront samples the same
 // candidate at the same time, so the 64 lanes' bilinear taps fall on
-// neighbouring texels of the sampled image (coalesced gathers), and each
-// lane stores its pixel's candidates 4 at a time into the candidate-
-// contiguous volume.  The lane also writes its pixel's guide
-// (utils.py:144-155), which needs the same reference colour.
+// neighbouring texels of the sampled image (coalesced gathers).  Costs go
+// to an LDS block of 64 pixels x 32 candidates per wavefront, which the
+// wavefront then writes out as whole lines (8 lanes per pixel's 32
+// candidates): stored straight from the lanes, 4 candidates at a time, each
+// store would touch 64 lines partially, and lines evicted half-written
+// multiplied the HBM write traffic about three times.  The lane also writes
+// its pixel's guide (utils.py:144-155), which needs the same reference colour.
 constexpr int kTileW = 64, kTileH = kBlock / kTileW;
+constexpr int kGroup = 32;               // candidates staged per flush
+constexpr int kStride = kGroup + 1;      // LDS words per pixel (conflict-free)
 
 template <typename T, bool POW2>
 __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
+  __shared__ float stage[kTileH][kTileW * kStride];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int r = tile / a.blocks_per_ref;
   const int t = tile - r * a.blocks_per_ref;
   const int ty = t / a.tiles_x, tx = t - ty * a.tiles_x;
-  const int x = tx * kTileW + (int)(threadIdx.x & (kTileW - 1));
-  const int y = ty * kTileH + (int)(threadIdx.x / kTileW);
-  if (x >= a.W || y >= a.H) return;
+  const int lane = (int)(threadIdx.x & (kTileW - 1)), wv = (int)(threadIdx.x / kTileW);
+  const int x0 = tx * kTileW, y = ty * kTileH + wv;
+  if (y >= a.H) return;  // a whole wavefront
+  // lanes past the right edge compute on a valid pixel and store nothing, so
+  // that the whole wavefront takes part in the flush
+  const bool active = x0 + lane < a.W;
+  const int x = active ? x0 + lane : a.W - 1;
   const SweepRef& rf = a.ref[r];
   const int pix = y * a.W + x;
   const float4 rl = rf.raylut[pix];
   const int k = (int)rl.w;
   const float r0 = rf.img[3 * pix], r1 = rf.img[3 * pix + 1], r2 = rf.img[3 * pix + 2];
-  if (rf.guide) {
+  if (rf.guide && active) {
     float gy = ((16.0f + 0.1826f * r0) + 0.6142f * r1) + 0.062f * r2;
     float cb = ((128.0f - 0.1006f * r0) - 0.3386f * r1) + 0.4392f * r2;
     float cr = ((128.0f + 0.4392f * r0) - 0.3989f * r1) - 0.0403f * r2;
@@ -211,102 +194,125 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
     cr = cr < 16.0f ? 16.0f : (cr > 240.0f ? 240.0f : cr);
     rf.guide[pix] = pack_guide(gy, cb, cr);
   }
-  T* dst = static_cast<T*>(rf.cost) + (size_t)pix * a.NP;
-  if (k < 0) {
-    // nothing sampled: the volume holds zeros there (depth_estimation.py:150-152)
-    float c = (r0 + r1) + r2;
-    c = c > 500.0f ? 500.0f : c;
-    for (int n0 = 0; n0 < a.NP; n0 += 4) {
-      float out[4];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) out[v] = (n0 + v < a.N) ? c : 0.0f;
-      store_n<4>(dst + n0, out);
-    }
-    return;
-  }
   // register copies: read through pointers, the camera and transform would
-  // be reloaded after every store to the volume (the compiler cannot rule
-  // out aliasing)
-  const TorchCam cam = a.cam[k];
+  // be reloaded after every store (the compiler cannot rule out aliasing)
+  const int kk = k < 0 ? 0 : k;
+  const TorchCam cam = a.cam[kk];
   float rt[12];
 #pragma unroll
-  for (int i = 0; i < 12; ++i) rt[i] = a.rt[rf.rt0 + k][i];
-  const float* img = a.img[k];
+  for (int i = 0; i < 12; ++i) rt[i] = a.rt[rf.rt0 + kk][i];
+  const float* img = a.img[kk];
+  // nothing sampled: the volume holds zeros there (depth_estimation.py:150-152)
+  float cnone = (r0 + r1) + r2;
+  cnone = cnone > 500.0f ? 500.0f : cnone;
   const float wlim = (float)(a.W - 1), hlim = (float)(a.H - 1);
-  for (int n0 = 0; n0 < a.NP; n0 += 4) {
-    // sample coordinates of the chunk's 4 candidates (pads sample a dummy
-    // interior point and are stored as 0)
-    float ix[4], iy[4];
-    bool inner = true;
+  float* st = stage[wv];
+  T* const row = static_cast<T*>(rf.cost) + (size_t)(y * a.W + x0) * a.NP;
+
+  for (int g0 = 0; g0 < a.NP; g0 += kGroup) {
+    const int cnt = a.NP - g0 < kGroup ? a.NP - g0 : kGroup;  // a multiple of 8
+#pragma unroll 1
+    for (int j = 0; j < cnt; j += 4) {
+      const int n0 = g0 + j;
+      float out[4];
+      if (k < 0) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      ix[v] = 0.5f;
-      iy[v] = 0.5f;
-      if (n0 + v < a.N) {
-        const float d = a.cands[n0 + v];
-        float qx, qy, qz, u, w;
-        bool valid;
-        rigid(rt, d * rl.x, d * rl.y, d * rl.z, qx, qy, qz);
-        project_t(cam, qx, qy, qz, u, w, valid);
-        if constexpr (POW2) {
-          ix[v] = grid_coord_pow2(u, a.inv2W, a.halfW);
-          iy[v] = grid_coord_pow2(w, a.inv2H, a.halfH);
+        for (int v = 0; v < 4; ++v) out[v] = (n0 + v < a.N) ? cnone : 0.0f;
+      } else {
+        // sample coordinates of the chunk's 4 candidates (pads sample a
+        // dummy interior point and are stored as 0)
+        float ix[4], iy[4];
+        bool inner = true;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          ix[v] = 0.5f;
+          iy[v] = 0.5f;
+          if (n0 + v < a.N) {
+            const float d = a.cands[n0 + v];
+            float qx, qy, qz, u, w;
+            bool valid;
+            rigid(rt, d * rl.x, d * rl.y, d * rl.z, qx, qy, qz);
+            project_t(cam, qx, qy, qz, u, w, valid);
+            if constexpr (POW2) {
+              ix[v] = grid_coord_pow2(u, a.inv2W, a.halfW);
+              iy[v] = grid_coord_pow2(w, a.inv2H, a.halfH);
+            } else {
+              ix[v] = grid_coord2<false>(u, a.Wf, a.invW);
+              iy[v] = grid_coord2<false>(w, a.Hf, a.invH);
+            }
+          }
+          const float x0f = floorf(ix[v]), y0f = floorf(iy[v]);
+          inner = inner && x0f >= 0.0f && x0f < wlim && y0f >= 0.0f && y0f < hlim;
+        }
+        if (inner) {
+          // all 16 taps inside the image: issue every load, then combine
+          // with the arithmetic of sample_zero (no tap clamped or masked)
+          float tn[4][6], ts[4][6];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const float* pn = img + (unsigned)((int)floorf(iy[v]) * a.W + (int)floorf(ix[v])) * 3u;
+            const float* ps = pn + (unsigned)a.W * 3u;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+              tn[v][i] = pn[i];
+              ts[v][i] = ps[i];
+            }
+          }
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const float x0f = floorf(ix[v]), y0f = floorf(iy[v]);
+            const float x1f = x0f + 1.0f, y1f = y0f + 1.0f;
+            const float wnw = (x1f - ix[v]) * (y1f - iy[v]), wne = (ix[v] - x0f) * (y1f - iy[v]);
+            const float wsw = (x1f - ix[v]) * (iy[v] - y0f), wse = (ix[v] - x0f) * (iy[v] - y0f);
+            float sv[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              float tt = tn[v][c] * wnw;
+              tt = tt + tn[v][3 + c] * wne;
+              tt = tt + ts[v][c] * wsw;
+              tt = tt + ts[v][3 + c] * wse;
+              sv[c] = tt;
+            }
+            float c = (fabsf(sv[0] - r0) + fabsf(sv[1] - r1)) + fabsf(sv[2] - r2);
+            c = c > 500.0f ? 500.0f : c;
+            out[v] = (n0 + v < a.N) ? c : 0.0f;
+          }
         } else {
-          ix[v] = grid_coord2<false>(u, a.Wf, a.invW);
-          iy[v] = grid_coord2<false>(w, a.Hf, a.invH);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            float c = 0.0f;
+            if (n0 + v < a.N) {
+              float sv[3];
+              sample_zero<3>(img, a.H, a.W, ix[v], iy[v], sv);
+              c = (fabsf(sv[0] - r0) + fabsf(sv[1] - r1)) + fabsf(sv[2] - r2);
+              c = c > 500.0f ? 500.0f : c;
+            }
+            out[v] = c;
+          }
         }
       }
-      const float x0f = floorf(ix[v]), y0f = floorf(iy[v]);
-      inner = inner && x0f >= 0.0f && x0f < wlim && y0f >= 0.0f && y0f < hlim;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) st[lane * kStride + j + v] = out[v];
     }
-    float out[4];
-    if (inner) {
-      // all 16 taps inside the image: issue every load, then combine with
-      // the arithmetic of sample_zero (no tap clamped or masked)
-      float tn[4][6], ts[4][6];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const float* pn = img + (unsigned)((int)floorf(iy[v]) * a.W + (int)floorf(ix[v])) * 3u;
-        const float* ps = pn + (unsigned)a.W * 3u;
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-          tn[v][i] = pn[i];
-          ts[v][i] = ps[i];
-        }
-      }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const float x0f = floorf(ix[v]), y0f = floorf(iy[v]);
-        const float x1f = x0f + 1.0f, y1f = y0f + 1.0f;
-        const float wnw = (x1f - ix[v]) * (y1f - iy[v]), wne = (ix[v] - x0f) * (y1f - iy[v]);
-        const float wsw = (x1f - ix[v]) * (iy[v] - y0f), wse = (ix[v] - x0f) * (iy[v] - y0f);
-        float s[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          float t = tn[v][c] * wnw;
-          t = t + tn[v][3 + c] * wne;
-          t = t + ts[v][c] * wsw;
-          t = t + ts[v][3 + c] * wse;
-          s[c] = t;
-        }
-        float c = (fabsf(s[0] - r0) + fabsf(s[1] - r1)) + fabsf(s[2] - r2);
-        c = c > 500.0f ? 500.0f : c;
-        out[v] = (n0 + v < a.N) ? c : 0.0f;
-      }
-    } else {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        float c = 0.0f;
-        if (n0 + v < a.N) {
-          float s[3];
-          sample_zero<3>(img, a.H, a.W, ix[v], iy[v], s);
-          c = (fabsf(s[0] - r0) + fabsf(s[1] - r1)) + fabsf(s[2] - r2);
-          c = c > 500.0f ? 500.0f : c;
-        }
-        out[v] = c;
+    // the wavefront's block: all its lanes' LDS writes precede the reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // flush: quad q = 4 consecutive candidates of one pixel; lanes take
+    // consecutive quads, so a store instruction covers whole lines
+    const int qpp = cnt / 4;  // quads per pixel
+    for (int q = lane; q < kTileW * qpp; q += kTileW) {
+      const int p = q / qpp, c4 = q - p * qpp;
+      if (x0 + p < a.W) {
+        const float* src = st + p * kStride + 4 * c4;
+        const float vals[4] = {src[0], src[1], src[2], src[3]};
+        store_n<4>(row + (size_t)p * a.NP + g0 + 4 * c4, vals);
       }
     }
-    store_n<4>(dst + n0, out);
+    // the next group overwrites the block
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
