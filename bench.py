@@ -31,6 +31,9 @@ WORKLOADS = {
                sigma_s=25.0, resolution=1216, seed=0),
     "c0": dict(matching=(512, 512), stitch=(1216, 1216), pano=(1024, 512), N=32, refs=(0, 2), sigma_i=10.0,
                sigma_s=12.5, resolution=1216, seed=0),
+    # the precision form's configuration: 64 candidates, every camera a reference
+    "c2": dict(matching=(1024, 1024), stitch=(1216, 1216), pano=(2048, 1024), N=64, refs=(0, 1, 2, 3),
+               sigma_i=10.0, sigma_s=25.0, resolution=1216, seed=0),
 }
 
 
@@ -75,7 +78,7 @@ def algorithmic_bytes(stage, wl, C=4):
     return 0
 
 
-def setup(wl, device, frames_cache=None):
+def setup(wl, device, frames_cache=None, build_estimator=True):
     import torch
 
     from sphere_stereo import RGBD_Estimator, synth
@@ -100,7 +103,7 @@ def setup(wl, device, frames_cache=None):
               reprojection_viewpoint=viewpoint, masks=masks, matching_resolution=wl["matching"],
               rgb_to_stitch_resolution=wl["stitch"], panorama_resolution=wl["pano"], sigma_i=wl["sigma_i"],
               sigma_s=wl["sigma_s"])
-    est = RGBD_Estimator(calibs, device=device, **kw)
+    est = RGBD_Estimator(calibs, device=device, **kw) if build_estimator else None
     return est, to_match, to_stitch, calibs, kw
 
 
@@ -153,6 +156,68 @@ def cpu_baseline(wl, calibs, kw, to_match, to_stitch, budget_s):
                         wl["N"], len(wl["refs"]), wl["pano"][0], wl["pano"][1], el))
 
 
+def precision_report(args):
+    """Inverse-distance error of the fp32 / fp16 / bf16 volume forms against the
+    fp64 oracle of the same method on one frame of the workload (MAE, RMSE,
+    bad-pixel ratios at 0.1 and 0.4 as utils.py:233-241 computes them), plus
+    each form's panoramas/s.  Pixels where the fp64 panorama is not finite (the
+    reference's own 0/0 blend weights) are left out and counted."""
+    import torch
+
+    from sphere_stereo import RGBD_Estimator
+    from sphere_stereo.evaluation import error_stats, inverse_distance
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pipeline import OraclePipeline
+
+    wl = WORKLOADS[args.workload]
+    device = torch.device("cuda:0")
+    print("[precision] setting up %s" % args.workload, file=sys.stderr, flush=True)
+    _, to_match, to_stitch, calibs, kw = setup(wl, device, args.frames_cache, build_estimator=False)
+    forms = {}
+    for dt in ("float32", "float16", "bfloat16"):
+        est = RGBD_Estimator(calibs, device=device, volume_dtype=dt, **kw)
+        for _ in range(args.warmup):
+            est.estimate_RGBD_panorama(to_match, to_stitch)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            est.estimate_RGBD_panorama(to_match, to_stitch)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        _, dist = est.estimate_RGBD_panorama(to_match, to_stitch)
+        forms[dt] = {"inv": inverse_distance(dist.clone()), "panoramas_per_s": round(args.steps / el, 2)}
+        print("[precision] %s: %.1f panoramas/s" % (dt, args.steps / el), file=sys.stderr, flush=True)
+        del est
+    # the fp64 oracle on the host, same inputs
+    cpu_cal = [type(c)(c.original_resolution, c.principal.cpu(), c.fl.cpu(), c.xi, c.alpha, c.rt.cpu(),
+                       c.matching_scale.cpu()) for c in calibs]
+    kwc = dict(kw)
+    kwc["reprojection_viewpoint"] = kw["reprojection_viewpoint"].cpu()
+    kwc["masks"] = [m.cpu() for m in kw["masks"]]
+    t0 = time.perf_counter()
+    ora = OraclePipeline(cpu_cal, precision="f64", **kwc)
+    print("[precision] fp64 oracle set up (%.1f s)" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
+    _, dist64 = ora.estimate_RGBD_panorama([t.cpu() for t in to_match], [t.cpu() for t in to_stitch])
+    t_ora = time.perf_counter() - t0
+    print("[precision] fp64 oracle frame done (%.1f s)" % t_ora, file=sys.stderr, flush=True)
+    inv64 = torch.from_numpy((1.0 / dist64).astype("float32")).to(device)
+    out = {}
+    for dt, f in forms.items():
+        st = error_stats(f["inv"], inv64, (0.1, 0.4), finite_only=True)
+        out[dt] = {"mae": st["mae"], "rmse": st["rmse"], "bad_px_ratio_0.1": st["bad_px_ratios"][0],
+                   "bad_px_ratio_0.4": st["bad_px_ratios"][1], "pixels": st["count"], "skipped": st["skipped"],
+                   "panoramas_per_s": f["panoramas_per_s"]}
+    line = {"precision_report": args.workload, "metric": "inverse-distance error vs the fp64 oracle (1/m)",
+            "config": {"matching": list(wl["matching"]), "candidates": wl["N"], "refs": list(wl["refs"]),
+                       "panorama": list(wl["pano"]), "sigma_i": wl["sigma_i"], "sigma_s": wl["sigma_s"]},
+            "forms": out, "fp64_oracle_seconds": round(t_ora, 1),
+            "mae_fp16_minus_fp32": out["float16"]["mae"] - out["float32"]["mae"],
+            "mae_bf16_minus_fp32": out["bfloat16"]["mae"] - out["float32"]["mae"],
+            "fp16_speedup": round(out["float16"]["panoramas_per_s"] / out["float32"]["panoramas_per_s"], 4)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,7 +227,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--frames-cache", default=None, help="reuse rendered fisheyes saved at this path")
+    ap.add_argument("--precision-report", action="store_true",
+                    help="instead of the throughput line: inverse-distance error of the fp32/fp16/bf16 volume forms "
+                         "against the fp64 oracle (use --workload c2)")
     args = ap.parse_args()
+    if args.precision_report:
+        precision_report(args)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would run its own frames "
               "or references independently); run with --gpus 1", file=sys.stderr)

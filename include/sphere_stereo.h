@@ -190,6 +190,25 @@ int ss_merge(const float *d_sampling, const float *d_weights, const float *const
 /* Truncating float32 -> uint8 cast of n values (depth_estimation.py:239-240). */
 int ss_to_u8(const float *d_in, uint8_t *d_out, size_t n, ss_stream stream);
 
+/* Output conversion and evaluation -------------------------------------------- */
+
+/* 1 / distance (main.py:114): the inverse-distance map the reference saves and
+ * evaluates, one IEEE fp32 division per element. */
+int ss_inverse_distance(const float *d_dist, float *d_inv, size_t n, ss_stream stream);
+
+/* The error statistics of evaluate_rgbd_panorama (utils.py:233-241) between
+ * d_a (evaluated) and d_b (reference), n elements each.  d_out (device, 7
+ * doubles) receives: sum |a-b|, sum (a-b)^2 (each error formed in fp32 as the
+ * reference does, summed in fp64 in a fixed order), the number of pixels
+ * counted, then for each of the nthr (<= 4) thresholds the number of pixels
+ * with |a-b| > threshold.  MAE = [0]/[2], RMSE = sqrt([1]/[2]), bad-pixel
+ * ratio = 100 * [3+k]/[2].  With finite_only, pixels where either map is not
+ * finite are skipped (the reference counts every pixel, so a NaN propagates
+ * into its sums). */
+size_t ss_error_stats_workspace_bytes(size_t n);
+int ss_error_stats(const float *d_a, const float *d_b, size_t n, const float *thresholds, int nthr, int finite_only,
+                   double *d_out, void *d_workspace, size_t workspace_bytes, ss_stream stream);
+
 /* Per-frame executor ---------------------------------------------------------- */
 
 /* Everything estimate_RGBD_panorama (depth_estimation.py:205-243) needs besides

@@ -105,9 +105,10 @@ real SFX(ora_expf)(real x) { return ora_exp(x); }
 /* ------------------------------------------------------------------------ */
 /* Storage rounding of the cost volume (the precision form): vt = 0 keeps    */
 /* fp32, vt = 1 rounds to the nearest binary16 value, vt = 2 to the nearest  */
-/* bfloat16 value, ties to even, NaN kept.  Applied where a volume value is  */
-/* stored (sweep output, pyramid levels); all arithmetic stays fp32.  The    */
-/* fp64 build ignores vt: it is the unrounded yardstick.                     */
+/* bfloat16 value, ties to even, NaN kept.  Applied where a level-0 volume   */
+/* value is stored (the sweep output, an unfused last up pass); the coarser  */
+/* pyramid levels and all arithmetic stay fp32.  The fp64 build ignores vt:  */
+/* it is the unrounded yardstick.                                            */
 /* ------------------------------------------------------------------------ */
 #if !ORA_F64
 static float round_f16(float x)
@@ -516,8 +517,9 @@ int SFX(ora_isb_levels)(int W, int H) { return ilog_levels(W, H); }
 /* ISB_Filter.apply (isb_filter.py:72-136): guide (H x W x 3) and cost
  * (N x H x W) are filtered in place. Returns the level count, or -1 when the
  * reference itself would fail (fewer than one level).  vt: storage rounding of
- * every level written (see vstore); round_level0 = 0 leaves the final level-0
- * values unrounded, as when the last up pass feeds the argmin directly. */
+ * the level-0 values the last up pass writes (see vstore; the coarser levels
+ * stay fp32); round_level0 = 0 leaves them unrounded, as when the last up pass
+ * feeds the argmin directly. */
 int SFX(ora_isb_filter)(uint8_t *guide, real *cost, int H, int W, int N, double sigma_i, double sigma_s, int vt,
                         int round_level0)
 {
@@ -539,13 +541,13 @@ int SFX(ora_isb_filter)(uint8_t *guide, real *cost, int H, int W, int N, double 
     double var_inv_s = 1 / (2 * sigma_s * sigma_s);
     double var_inv_i = 1 / (2 * sigma_i * sigma_i);
     for (int s = 1; s < L; s++)
-        SFX(ora_isb_down)(g[s - 1], c[s - 1], hs[s - 1], ws[s - 1], g[s], c[s], hs[s], ws[s], N, var_inv_i, vt);
+        SFX(ora_isb_down)(g[s - 1], c[s - 1], hs[s - 1], ws[s - 1], g[s], c[s], hs[s], ws[s], N, var_inv_i, 0);
     for (int s = L - 2; s >= 0; s--) {
         double dist = pow(2.0, s) - 0.5;
         double wdn = exp(-(dist * dist) * var_inv_s);
         double wup = 1 - wdn;
         SFX(ora_isb_up)(g[s + 1], c[s + 1], hs[s + 1], ws[s + 1], g[s], c[s], hs[s], ws[s], N, wup, wdn, var_inv_i,
-                        (s > 0 || round_level0) ? vt : 0);
+                        (s == 0 && round_level0) ? vt : 0);
     }
     for (int s = 1; s < L; s++) {
         free(g[s]);

@@ -116,19 +116,19 @@ size_t elem_bytes(int vt) { return vt == SS_VOLUME_F32 ? 4 : 2; }
 bool valid_vtype(int vt) { return vt == SS_VOLUME_F32 || vt == SS_VOLUME_F16 || vt == SS_VOLUME_BF16; }
 
 // Level sizes ceil(rows / 2^s) x ceil(cols / 2^s) (isb_filter.py:64-70) and
-// their buffers for s >= 1 (volume elements of eb bytes).
-void carve_levels(Carver& cv, Level* lv, int L, int H, int W, int NP, size_t eb) {
+// their buffers for s >= 1 (fp32 volumes whatever the level-0 type).
+void carve_levels(Carver& cv, Level* lv, int L, int H, int W, int NP) {
   for (int s = 1; s < L; ++s) {
     lv[s].h = (int)ceil(H / pow(2.0, s));
     lv[s].w = (int)ceil(W / pow(2.0, s));
     lv[s].g = cv.take<uint32_t>((size_t)lv[s].h * lv[s].w);
-    lv[s].c = cv.take<char>((size_t)lv[s].h * lv[s].w * NP * eb);
+    lv[s].c = cv.take<float>((size_t)lv[s].h * lv[s].w * NP);
   }
 }
 
 // ISB_Filter.apply's two launch loops (isb_filter.py:87-134).  With
 // fuse_last, the level-0 up-pass computes the argmin instead of writing the
-// filtered volume.
+// filtered volume.  vt is the element type of level 0; levels >= 1 are fp32.
 int run_pyramid(Level* lv, int L, int G, int V, int vt, double sigma_i, double sigma_s, bool fuse_last,
                 const ss::ArgminParams& am, float* dist, hipStream_t s, int stage_down, int stage_up,
                 float* fill = nullptr) {
@@ -147,17 +147,18 @@ int run_pyramid(Level* lv, int L, int G, int V, int vt, double sigma_i, double s
   // the LDS budget.
   const long long kTailPixels = 256;
   const int NP = G * V;
-  const size_t eb = elem_bytes(vt);
   auto lds_need = [&](int from) {
     size_t b = 0;
     for (int k = from + 1; k < L; ++k) {
       const size_t px = (size_t)lv[k].h * lv[k].w;
-      b += ((px * 4 + 15) & ~size_t(15)) + ((px * NP * eb + 15) & ~size_t(15));
+      b += ((px * 4 + 15) & ~size_t(15)) + ((px * NP * 4 + 15) & ~size_t(15));
     }
     return b;
   };
+  // the tail starts at level 1 or deeper (its levels are all fp32, and a
+  // fused level-0 pass stays a regular launch)
   int s0 = L - 1;
-  for (int k = fuse_last ? 1 : 0; k < L - 1; ++k)
+  for (int k = 1; k < L - 1; ++k)
     if ((long long)lv[k + 1].h * lv[k + 1].w <= kTailPixels && lds_need(k) <= (size_t)ss::kTailLdsBytes) {
       s0 = k;
       break;
@@ -167,7 +168,7 @@ int run_pyramid(Level* lv, int L, int G, int V, int vt, double sigma_i, double s
   for (int k = 1; k <= last_down; ++k) {
     ss::IsbDownArgs a{lv[k - 1].g, lv[k - 1].c, lv[k - 1].h, lv[k - 1].w, lv[k].g, lv[k].c, lv[k].h, lv[k].w, vi};
     StageTimer t(stage_down, s);
-    SS_HIP(ss::launch_isb_down(a, G, V, vt, s));
+    SS_HIP(ss::launch_isb_down(a, G, V, k == 1 ? vt : SS_VOLUME_F32, s));
   }
   if (tail) {
     ss::IsbTailArgs ta;
@@ -185,7 +186,7 @@ int run_pyramid(Level* lv, int L, int G, int V, int vt, double sigma_i, double s
         ta.goff[e] = off;
         off += (px * 4 + 15) & ~15u;
         ta.coff[e] = off;
-        off += (px * (unsigned)(NP * eb) + 15) & ~15u;
+        off += (px * (unsigned)(NP * 4) + 15) & ~15u;
       }
       double wu, wd;
       weights(s0 + e, wu, wd);
@@ -194,7 +195,7 @@ int run_pyramid(Level* lv, int L, int G, int V, int vt, double sigma_i, double s
     }
     ta.lds_bytes = off;
     StageTimer t(stage_down, s);
-    SS_HIP(ss::launch_isb_tail(ta, G, V, vt, s));
+    SS_HIP(ss::launch_isb_tail(ta, G, V, s));
   }
   for (int k = (tail ? s0 : L - 1) - 1; k >= 0; --k) {
     double wu, wd;
@@ -203,7 +204,7 @@ int run_pyramid(Level* lv, int L, int G, int V, int vt, double sigma_i, double s
                     (float)wu,   (float)wd,   vi,          dist,        am,   fuse_last && k == 0 ? fill : nullptr};
     const bool fuse = fuse_last && k == 0;
     StageTimer t(fuse ? SS_STAGE_ISB_UP_ARGMIN : stage_up, s);
-    SS_HIP(ss::launch_isb_up(a, G, V, vt, fuse, s));
+    SS_HIP(ss::launch_isb_up(a, G, V, k == 0 ? vt : SS_VOLUME_F32, fuse, s));
   }
   if (fuse_last && L == 1) {
     StageTimer t(SS_STAGE_ISB_UP_ARGMIN, s);
@@ -408,14 +409,13 @@ size_t ss_isb_workspace_bytes(int H, int W, int N, int vtype, int planar) {
   const int L = levels_of(H, W), NP = padded(N);
   if (L < 1 || NP < 0 || !valid_vtype(vtype)) return 0;
   if (planar && vtype != SS_VOLUME_F32) return 0;
-  const size_t eb = elem_bytes(vtype);
-  Carver cv(nullptr);
+  Carver cv(nullptr);  // the pyramid levels are fp32 whatever the level-0 type
   Level lv[32];
   if (planar) {
     cv.take<uint32_t>((size_t)H * W);
     cv.take<float>((size_t)H * W * NP);
   }
-  carve_levels(cv, lv, L, H, W, NP, eb);
+  carve_levels(cv, lv, L, H, W, NP);
   return cv.off + 256;
 }
 
@@ -425,7 +425,7 @@ static int isb_filter_impl(uint32_t* d_guide, void* d_cost, int H, int W, int N,
   const int NP = padded(N);
   Level lv[32];
   lv[0] = Level{H, W, d_guide, d_cost};
-  carve_levels(cv, lv, L, H, W, NP, elem_bytes(vt));
+  carve_levels(cv, lv, L, H, W, NP);
   const int G = N == 1 ? 1 : 8, V = N == 1 ? 1 : NP / 8;
   ss::ArgminParams am{};
   return run_pyramid(lv, L, G, V, vt, sigma_i, sigma_s, false, am, nullptr, s, SS_STAGE_ISB_DOWN, SS_STAGE_ISB_UP);
@@ -490,7 +490,7 @@ static size_t estimate_layout(Carver& cv, int H, int W, int N, int vt, uint32_t*
   const size_t eb = elem_bytes(vt);
   *g0 = cv.take<uint32_t>((size_t)H * W);
   *c0 = cv.take<char>((size_t)H * W * NP * eb);
-  carve_levels(cv, lv, L, H, W, NP, eb);
+  carve_levels(cv, lv, L, H, W, NP);
   for (int s = 1; s < L; ++s) {
     dl[s].h = lv[s].h;
     dl[s].w = lv[s].w;
@@ -688,6 +688,29 @@ int ss_to_u8(const float* d_in, uint8_t* d_out, size_t n, ss_stream stream) {
   if (n == 0) return ok();
   StageTimer t(SS_STAGE_TO_U8, as_stream(stream));
   SS_HIP(ss::launch_to_u8(d_in, d_out, n, as_stream(stream)));
+  return ok();
+}
+
+int ss_inverse_distance(const float* d_dist, float* d_inv, size_t n, ss_stream stream) {
+  SS_REQUIRE(d_dist && d_inv, "null pointer argument");
+  if (n == 0) return ok();
+  SS_HIP(ss::launch_inverse(d_dist, d_inv, n, as_stream(stream)));
+  return ok();
+}
+
+size_t ss_error_stats_workspace_bytes(size_t n) { return ss::error_stats_partial_bytes(n); }
+
+int ss_error_stats(const float* d_a, const float* d_b, size_t n, const float* thresholds, int nthr, int finite_only,
+                   double* d_out, void* d_workspace, size_t workspace_bytes, ss_stream stream) {
+  SS_REQUIRE(d_a && d_b && d_out, "null pointer argument");
+  SS_REQUIRE(n >= 1, "empty maps");
+  SS_REQUIRE(nthr >= 0 && nthr <= ss::error_stats_max_thresholds() && (nthr == 0 || thresholds),
+             "threshold count %d outside [0, %d]", nthr, ss::error_stats_max_thresholds());
+  const size_t need = ss::error_stats_partial_bytes(n);
+  if (!d_workspace || workspace_bytes < need)
+    return fail(SS_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need, workspace_bytes);
+  SS_HIP(ss::launch_error_stats(d_a, d_b, n, thresholds, nthr, finite_only, static_cast<double*>(d_workspace), d_out,
+                                as_stream(stream)));
   return ok();
 }
 

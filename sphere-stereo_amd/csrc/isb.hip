@@ -9,6 +9,13 @@
 // shuffles.  G = 1 is the scalar form used for the N = 1 distance filter.
 // The last up-pass can be fused with the argmin/refinement so that the
 // filtered level-0 volume is never written back to HBM.
+//
+// Element types: the level-0 volume (the sweep output, most of the traffic)
+// may be fp32, fp16 or bf16 (T below); the coarser pyramid levels are always
+// fp32.  With sigma_s = 25 the level-0 up pass takes ~99.98 % of its weight
+// from level 1, so rounding the coarse levels to 16 bits moved the argmin of
+// several per cent of the pixels, while rounding level 0 changed next to
+// nothing.
 #include "ss_kernels.h"
 
 namespace ss {
@@ -46,14 +53,15 @@ __device__ __forceinline__ uint32_t pick4(int i, uint32_t a, uint32_t b, uint32_
 // ---------------------------------------------------------------------------
 // One coarse pixel of the down pass, handled by the G lanes of a group.
 // Inactive groups (past the end of the level) compute on a valid pixel so
-// that the shuffles see every lane, and store nothing.  T is the volume's
-// storage type; sums and weights are fp32.
+// that the shuffles see every lane, and store nothing.  T is the storage
+// type of the input level (level 0 may be 16-bit); the output level is fp32,
+// as are all sums and weights.
 template <int G, int V, typename T>
 __device__ __forceinline__ void down_pixel(const IsbDownArgs& a, int pix, bool active, int lane) {
   constexpr int NP = G * V;
   constexpr int CH = chunk_of<V, T>();
   const T* cin = static_cast<const T*>(a.cin);
-  T* cout = static_cast<T*>(a.cout);
+  float* cout = static_cast<float*>(a.cout);
 
   const int x = pix % a.wout, y = pix / a.wout;
   const int x2 = 2 * x, y2 = 2 * y;
@@ -198,8 +206,8 @@ template <int G, int V, typename T, bool FUSE_ARGMIN>
 __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool active, int lane) {
   constexpr int NP = G * V;
   constexpr int CH = chunk_of<V, T>();
-  const T* cin = static_cast<const T*>(a.cin);
-  T* cio = static_cast<T*>(a.cio);
+  const float* cin = static_cast<const float*>(a.cin);  // coarse level: fp32
+  T* cio = static_cast<T*>(a.cio);                      // fine level: T
 
   const int x = pix % a.win, y = pix / a.win;
   const int r0 = max(y - 1, 0), r2 = min(y + 1, a.hin - 1);
@@ -332,9 +340,10 @@ __global__ void __launch_bounds__(kBlock) k_isb_up(IsbUpArgs a) {
 // separated by a workgroup barrier.  Levels 1..n-1 live only in LDS (they
 // are not needed once the pass is over); level 0 is in global memory.  The
 // per-pixel functions work on generic pointers, so the same code serves both.
+// The tail starts at pyramid level 1 or deeper, so all its levels are fp32.
 constexpr int kTailBlock = 1024;
 
-template <int G, int V, typename T>
+template <int G, int V>
 __global__ void __launch_bounds__(kTailBlock) k_isb_tail(IsbTailArgs t) {
   extern __shared__ __align__(16) char smem[];
   const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
@@ -349,7 +358,7 @@ __global__ void __launch_bounds__(kTailBlock) k_isb_tail(IsbTailArgs t) {
     const IsbDownArgs a{gp(k - 1), cp(k - 1), t.h[k - 1], t.w[k - 1], gp(k), cp(k), t.h[k], t.w[k], t.var_inv_i};
     const int npix = t.h[k] * t.w[k];
     for (int base = 0; base < npix; base += slots)
-      if (base + slot < npix) down_pixel<G, V, T>(a, base + slot, true, lane);
+      if (base + slot < npix) down_pixel<G, V, float>(a, base + slot, true, lane);
     __syncthreads();
   }
   for (int k = t.n - 2; k >= 0; --k) {
@@ -357,7 +366,7 @@ __global__ void __launch_bounds__(kTailBlock) k_isb_tail(IsbTailArgs t) {
                       t.wu[k],   t.wd[k],   t.var_inv_i, nullptr,  ArgminParams{}, nullptr};
     const int npix = t.h[k + 1] * t.w[k + 1];
     for (int base = 0; base < npix; base += slots)
-      if (base + slot < npix) up_pixel<G, V, T, false>(a, base + slot, true, lane);
+      if (base + slot < npix) up_pixel<G, V, float, false>(a, base + slot, true, lane);
     __syncthreads();
   }
 }
@@ -415,9 +424,9 @@ hipError_t launch_down_gv(const IsbDownArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int G, int V, typename T>
+template <int G, int V>
 hipError_t launch_tail_gv(const IsbTailArgs& t, hipStream_t s) {
-  hipLaunchKernelGGL((k_isb_tail<G, V, T>), dim3(1), dim3(kTailBlock), t.lds_bytes, s, t);
+  hipLaunchKernelGGL((k_isb_tail<G, V>), dim3(1), dim3(kTailBlock), t.lds_bytes, s, t);
   return hipGetLastError();
 }
 
@@ -459,13 +468,6 @@ hipError_t down_t(const IsbDownArgs& a, int V, hipStream_t s) {
 }
 
 template <typename T>
-hipError_t tail_t(const IsbTailArgs& t, int V, hipStream_t s) {
-#define SS_CALL(v) launch_tail_gv<8, v, T>(t, s)
-  SS_DISPATCH_V(V, SS_CALL)
-#undef SS_CALL
-}
-
-template <typename T>
 hipError_t up_t(const IsbUpArgs& a, int V, bool fuse, hipStream_t s) {
 #define SS_CALL(v) launch_up_gv<8, v, T>(a, fuse, s)
   SS_DISPATCH_V(V, SS_CALL)
@@ -481,7 +483,9 @@ hipError_t argmin_t(const void* cost, int npix, int V, const ArgminParams& p, fl
 
 }  // namespace
 
-// G = 1 (the distance filter) exists for fp32 volumes only.
+// vt: the element type of the level-0 side of the pass (the input of a down
+// pass, the fine level of an up pass); the other side is fp32.  G = 1 (the
+// distance filter) exists for fp32 only.
 hipError_t launch_isb_down(const IsbDownArgs& a, int G, int V, int vt, hipStream_t s) {
   if (G == 1) return (V == 1 && vt == SS_VOLUME_F32) ? launch_down_gv<1, 1, float>(a, s) : hipErrorInvalidValue;
 #define SS_CALL(T) down_t<T>(a, V, s)
@@ -489,10 +493,10 @@ hipError_t launch_isb_down(const IsbDownArgs& a, int G, int V, int vt, hipStream
 #undef SS_CALL
 }
 
-hipError_t launch_isb_tail(const IsbTailArgs& t, int G, int V, int vt, hipStream_t s) {
-  if (G == 1) return (V == 1 && vt == SS_VOLUME_F32) ? launch_tail_gv<1, 1, float>(t, s) : hipErrorInvalidValue;
-#define SS_CALL(T) tail_t<T>(t, V, s)
-  SS_DISPATCH_T(vt, SS_CALL)
+hipError_t launch_isb_tail(const IsbTailArgs& t, int G, int V, hipStream_t s) {
+  if (G == 1) return V == 1 ? launch_tail_gv<1, 1>(t, s) : hipErrorInvalidValue;
+#define SS_CALL(v) launch_tail_gv<8, v>(t, s)
+  SS_DISPATCH_V(V, SS_CALL)
 #undef SS_CALL
 }
 

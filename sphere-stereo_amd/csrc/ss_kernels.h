@@ -118,10 +118,12 @@ struct BlendArgs {
 int report_error(int code, const char* what);
 bool profiler_active();
 
-// isb.hip (vt: ss_volume_type of the volume; G = 1 paths are fp32 only)
+// isb.hip (vt: ss_volume_type of the level-0 side of a pass, i.e. the input
+// of a down pass or the fine level of an up pass; pyramid levels >= 1 are
+// fp32, and so is everything with G = 1)
 hipError_t launch_isb_down(const IsbDownArgs& a, int G, int V, int vt, hipStream_t s);
 hipError_t launch_isb_up(const IsbUpArgs& a, int G, int V, int vt, bool fuse_argmin, hipStream_t s);
-hipError_t launch_isb_tail(const IsbTailArgs& t, int G, int V, int vt, hipStream_t s);
+hipError_t launch_isb_tail(const IsbTailArgs& t, int G, int V, hipStream_t s);
 hipError_t launch_argmin(const void* cost, int npix, int V, int vt, const ArgminParams& p, float* dist,
                          hipStream_t s);
 hipError_t launch_planar_to_cc(const float* src, float* dst, int HW, int N, int NP, hipStream_t s);
@@ -168,5 +170,13 @@ hipError_t launch_blend_lut(const BlendArgs& a, hipStream_t s);
 hipError_t launch_normalize(float* w, int R, int P, hipStream_t s);
 hipError_t launch_merge(const MergeArgs& a, hipStream_t s);
 hipError_t launch_to_u8(const float* in, uint8_t* out, size_t n, hipStream_t s);
+
+// eval.hip
+hipError_t launch_inverse(const float* d, float* inv, size_t n, hipStream_t s);
+int error_stats_blocks(size_t n);
+size_t error_stats_partial_bytes(size_t n);
+int error_stats_max_thresholds();
+hipError_t launch_error_stats(const float* a, const float* b, size_t n, const float* thr, int nthr, int finite_only,
+                              double* partial, double* out, hipStream_t s);
 
 }  // namespace ss

@@ -100,10 +100,12 @@ __global__ void __launch_bounds__(kBlock) k_inpaint_step(const float* __restrict
 // writing the other; unlisted pixels never change, so both copies keep them.
 // A pass moves information by at most one pixel, so the block's own pixels,
 // kSteps inside the staged region, are exact after kSteps passes.
-constexpr int kTile = 32;
+// 48 + 2 * 16 = 80: 77 KB of LDS, so the 22 x 22 workgroups of a 1024^2
+// map are all resident at once (2 per CU) and each runs 16 passes.
+constexpr int kTile = 48;
 constexpr int kSteps = 16;
-constexpr int kRegion = kTile + 2 * kSteps;  // 64: index math by shifts
-static_assert(kRegion == 64, "region side is 64");
+constexpr int kRegion = kTile + 2 * kSteps;
+static_assert(kRegion * kRegion % 256 == 0 && kRegion * kRegion <= 65536, "region fits the staging loop and list");
 
 __global__ void __launch_bounds__(kBlock) k_inpaint_multi(const float* __restrict__ din, float* __restrict__ dout,
                                                           const uchar2* __restrict__ lut, int H, int W, int steps,
@@ -122,7 +124,7 @@ __global__ void __launch_bounds__(kBlock) k_inpaint_multi(const float* __restric
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
     const int i = threadIdx.x + k * kBlock;
-    const int gx = ox + (i & (kRegion - 1)), gy = oy + (i >> 6);
+    const int gx = ox + i % kRegion, gy = oy + i / kRegion;
     const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
     const int g = in ? gy * W + gx : 0;
     dv[k] = in ? din[g] : 0.0f;
@@ -132,7 +134,7 @@ __global__ void __launch_bounds__(kBlock) k_inpaint_multi(const float* __restric
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
     const int i = threadIdx.x + k * kBlock;
-    const int lx = i & (kRegion - 1), ly = i >> 6;
+    const int lx = i % kRegion, ly = i / kRegion;
     const int gx = ox + lx, gy = oy + ly;
     bufa[i] = dv[k];
     bufb[i] = dv[k];
@@ -152,7 +154,7 @@ __global__ void __launch_bounds__(kBlock) k_inpaint_multi(const float* __restric
         const float d = a[i];
         float r = d;
         if ((double)d >= thr_empty) {
-          const int lx = i & (kRegion - 1), ly = i >> 6;
+          const int lx = i % kRegion, ly = i / kRegion;
           const uchar2 t = tab[i];
           const int bs[2] = {t.x, t.y};
           float val = 0.0f, wsum = 0.0f;

@@ -1,8 +1,9 @@
 """Builds libsphere_stereo.so for gfx950 with hipcc (no CMake, no JIT cache).
 
 The library is written next to this file so that it travels with the source
-tree.  Run as ``python -m sphere_stereo._build`` from ``sphere-stereo_amd/`` or
-through ``__graft_entry__.build()``.
+tree.  Run as ``python sphere-stereo_amd/sphere_stereo/_build.py`` or through
+``__graft_entry__.build()``; both load this file by path, because importing
+the package loads the library (and fails on a stale one).
 """
 import os
 import subprocess
@@ -11,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 LIB = os.path.join(HERE, "libsphere_stereo.so")
-SOURCES = ["api.hip", "frame.hip", "isb.hip", "sweep.hip", "stitch.hip"]
+SOURCES = ["api.hip", "frame.hip", "isb.hip", "sweep.hip", "stitch.hip", "eval.hip"]
 
 # -ffp-contract=off and correctly rounded fp32 division/sqrt keep every
 # operation a single IEEE round-to-nearest step (bit-exact with the oracle);

@@ -71,6 +71,9 @@ SIGNATURES = {
     "ss_inpaint": (_i, [_p, _p, _p, _i, _i, _i, _d, _d, _p]),
     "ss_merge": (_i, [_p, _p, _p, _p, _p, _i, _i, _p, _INTR, _i, _i, _i, _i, _i, _p, _p, _p]),
     "ss_to_u8": (_i, [_p, _p, _z, _p]),
+    "ss_inverse_distance": (_i, [_p, _p, _z, _p]),
+    "ss_error_stats_workspace_bytes": (_z, [_z]),
+    "ss_error_stats": (_i, [_p, _p, _z, _p, _i, _i, _p, _p, _z, _p]),
     "ss_frame_create": (_i, [C.POINTER(ss_frame_desc), C.POINTER(C.c_void_p)]),
     "ss_frame_run": (_i, [_p, _p, _p, _i, _p]),
     "ss_frame_destroy": (_i, [_p]),
@@ -93,7 +96,7 @@ def _load():
 
     if not os.path.exists(LIB_PATH):
         raise ImportError("libsphere_stereo.so is not built (%s); run __graft_entry__.build() or "
-                          "python -m sphere_stereo._build" % LIB_PATH)
+                          "python sphere-stereo_amd/sphere_stereo/_build.py" % LIB_PATH)
     lib = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -226,6 +226,31 @@ def test_panorama_deterministic(small):
     assert torch.equal(dist1.view(torch.int32), dist2.view(torch.int32))
 
 
+def test_inverse_distance_and_error_stats(gpu):
+    from sphere_stereo.evaluation import error_stats, inverse_distance
+
+    rng = np.random.RandomState(4)
+    d = (0.5 + rng.rand(333, 517) * 99.5).astype(np.float32)
+    inv = to_np(inverse_distance(torch.from_numpy(d).to(gpu)))
+    assert_same(inv, (np.float32(1) / d).astype(np.float32), "inverse distance")
+    a = rng.rand(300000).astype(np.float32)
+    b = (a + (rng.rand(300000).astype(np.float32) - 0.5) * 0.9).astype(np.float32)
+    b[::997] = np.nan
+    a[5::1009] = np.inf
+    for finite_only in (True, False):
+        got = error_stats(torch.from_numpy(a).to(gpu), torch.from_numpy(b).to(gpu), (0.1, 0.4, 0.25), finite_only)
+        keep = np.isfinite(a) & np.isfinite(b) if finite_only else np.ones(a.shape, bool)
+        err = np.abs(a[keep] - b[keep])  # fp32, as utils.py:233 forms it
+        assert got["count"] == keep.sum() and got["skipped"] == (~keep).sum()
+        if finite_only:
+            assert abs(got["mae"] - err.astype(np.float64).mean()) < 1e-12
+            assert abs(got["rmse"] - np.sqrt((err * err).astype(np.float64).mean())) < 1e-12
+            for t, ratio in zip((0.1, 0.4, 0.25), got["bad_px_ratios"]):
+                assert ratio == 100.0 * np.sum(err > np.float32(t)) / err.size
+        else:
+            assert np.isnan(got["mae"]) and np.isnan(got["rmse"])
+
+
 def test_graph_replay_matches_eager(small):
     fr, est, ora = small
     rgb0, dist0 = [t.clone() for t in est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])]

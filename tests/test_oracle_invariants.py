@@ -241,18 +241,23 @@ def test_precision_form_rounds_exactly_the_stored_values():
         assert np.array_equal(O32.round_volume(fused, vt), stored)
         if dt is not None:
             assert np.array_equal(stored.astype(dt).astype(np.float32), stored)
-        # the same pyramid composed from the single passes
+        # the same pyramid composed from the single passes: only the level-0
+        # store rounds; the coarser levels stay fp32
         L = O32.levels(W, H)
         gs, cs = [guide], [c16]
         for s in range(1, L):
             h, w = -(-H // 2 ** s), -(-W // 2 ** s)
-            g, c = O32.isb_down(gs[-1], cs[-1], h, w, 1 / (2 * 10.0 ** 2), vtype=vt)
+            g, c = O32.isb_down(gs[-1], cs[-1], h, w, 1 / (2 * 10.0 ** 2))
             gs.append(g)
             cs.append(c)
         for s in range(L - 2, -1, -1):
             wd = math.exp(-((2.0 ** s - 0.5) ** 2) / (2 * 25.0 ** 2))
-            gs[s], cs[s] = O32.isb_up(gs[s + 1], cs[s + 1], gs[s], cs[s], 1 - wd, wd, 1 / (2 * 10.0 ** 2), vtype=vt)
+            gs[s], cs[s] = O32.isb_up(gs[s + 1], cs[s + 1], gs[s], cs[s], 1 - wd, wd, 1 / (2 * 10.0 ** 2),
+                                      vtype=vt if s == 0 else 0)
         assert np.array_equal(cs[0], stored)
+        # ... and with unrounded input the fp32 filter itself
+        f32, _ = O32.isb_filter(guide, c16, 10.0, 25.0)
+        assert np.array_equal(O32.round_volume(f32, vt), stored)
         # error against the unrounded fp64 method stays at the storage precision
         ref, _ = O64.isb_filter(guide, cost.astype(np.float64), 10.0, 25.0)
         tol = 500 * (2.0 ** -10 if vt == 1 else 2.0 ** -7)
