@@ -140,8 +140,15 @@ __global__ void __launch_bounds__(kBlock) k_isb_down(IsbDownArgs a) {
 // Argmin + parabola refinement + index->distance for one pixel whose NP
 // candidates are spread over a G-lane group (depth_estimation.py:180-198).
 // ---------------------------------------------------------------------------
+// The group part: first minimum (index, value), maximum, and the values at
+// the clamped neighbours of the minimum, in every lane of the group.
+struct ArgminGroup {
+  float m, M, L, R;
+  int im;
+};
+
 template <int G, int V>
-__device__ __forceinline__ float argmin_refine_group(const float* vals, int lane, const ArgminParams& p) {
+__device__ __forceinline__ ArgminGroup argmin_reduce_group(const float* vals, int lane, const ArgminParams& p) {
   const int n0 = lane * V;
   float m = __builtin_huge_valf(), M = -__builtin_huge_valf();
   int im = 0x7fffffff;
@@ -186,14 +193,24 @@ __device__ __forceinline__ float argmin_refine_group(const float* vals, int lane
       R = sr;
     }
   }
-  float var = (0.5f * (L - R)) / (((L + R) - 2.0f * m) + 1e-8f);
+  return ArgminGroup{m, M, L, R, im};
+}
+
+// The per-pixel part: parabola refinement and index -> distance.
+__device__ __forceinline__ float argmin_finish(const ArgminGroup& g, const ArgminParams& p) {
+  float var = (0.5f * (g.L - g.R)) / (((g.L + g.R) - 2.0f * g.m) + 1e-8f);
   var = var < -0.5f ? -0.5f : (var > 0.5f ? 0.5f : var);
-  if (im == p.N - 1 || im == 0) var = 0.0f;
-  float idx = (float)im + var;
-  if (M == m) idx = (float)(p.N - 1);
+  if (g.im == p.N - 1 || g.im == 0) var = 0.0f;
+  float idx = (float)g.im + var;
+  if (g.M == g.m) idx = (float)(p.N - 1);
   float d = p.d0 / (((p.A * idx) / (float)(p.N - 1)) + 1.0f);
-  if (fabsf(M - m) < 1e-8f) d = p.dN1;
+  if (fabsf(g.M - g.m) < 1e-8f) d = p.dN1;
   return d;
+}
+
+template <int G, int V>
+__device__ __forceinline__ float argmin_refine_group(const float* vals, int lane, const ArgminParams& p) {
+  return argmin_finish(argmin_reduce_group<G, V>(vals, lane, p), p);
 }
 
 // ---------------------------------------------------------------------------
@@ -311,14 +328,18 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
   }
 
   if constexpr (FUSE_ARGMIN) {
+    // the group reductions need every lane; the refinement of fine pixel s
+    // (three divisions) then runs once, in lane s, instead of in all 8
+    ArgminGroup mine = {};
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       if (!doit[s]) continue;  // uniform within the group: same coarse pixel
-      const float d = argmin_refine_group<G, V>(res[s], lane, a.am);
-      if (active && lane == 0) {
-        a.dist[fine[s]] = d;
-        if (a.fill) a.fill[fine[s]] = 1e8f;
-      }
+      const ArgminGroup g = argmin_reduce_group<G, V>(res[s], lane, a.am);
+      if (lane == s) mine = g;
+    }
+    if (active && lane < 4 && doit[lane]) {
+      a.dist[fine[lane]] = argmin_finish(mine, a.am);
+      if (a.fill) a.fill[fine[lane]] = 1e8f;
     }
   }
 }
