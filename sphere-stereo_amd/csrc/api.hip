@@ -126,31 +126,35 @@ void carve_levels(Carver& cv, Level* lv, int L, int H, int W, int NP) {
   }
 }
 
-// ISB_Filter.apply's two launch loops (isb_filter.py:87-134).  With
-// fuse_last, the level-0 up-pass computes the argmin instead of writing the
-// filtered volume.  vt is the element type of level 0; levels >= 1 are fp32.
-int run_pyramid(Level* lv, int L, int G, int V, int vt, double sigma_i, double sigma_s, bool fuse_last,
-                const ss::ArgminParams& am, float* dist, hipStream_t s, int stage_down, int stage_up,
-                float* fill = nullptr) {
+// ISB_Filter.apply's two launch loops (isb_filter.py:87-134) for nb pyramids
+// of the same shape in the same launches (lv[b] are the levels of entry b).
+// With fuse_last, the level-0 up-pass computes the argmin into dist[b]
+// instead of writing the filtered volume.  vt is the element type of level
+// 0; levels >= 1 are fp32.
+int run_pyramid(Level* const* lv, int nb, int L, int G, int V, int vt, double sigma_i, double sigma_s,
+                bool fuse_last, const ss::ArgminParams& am, float* const* dist, hipStream_t s, int stage_down,
+                int stage_up, float* const* fill = nullptr) {
+  if (nb < 1 || nb > ss::kMaxBatch) return fail(SS_ERR_ARG, "batch of %d pyramids outside [1, %d]", nb, ss::kMaxBatch);
   const double var_inv_s = 1 / (2 * sigma_s * sigma_s);
   const double var_inv_i = 1 / (2 * sigma_i * sigma_i);
   const float vi = (float)var_inv_i;
+  const Level* l0 = lv[0];  // the shapes (identical for every entry)
   // up-pass weights into level k (isb_filter.py:110-112)
   auto weights = [&](int k, double& wu, double& wd) {
     const double d = pow(2.0, k) - 0.5;
     wd = exp(-(d * d) * var_inv_s);
     wu = 1 - wd;
   };
-  // The coarse end runs in one workgroup from level s0 on: every level it
-  // produces holds at most kTailPixels pixels (one CU then does each pass in
-  // about a microsecond) and all of them (guide + NP values per pixel) fit in
-  // the LDS budget.
+  // The coarse end runs in one workgroup (per entry) from level s0 on: every
+  // level it produces holds at most kTailPixels pixels (one CU then does each
+  // pass in about a microsecond) and all of them (guide + NP values per
+  // pixel) fit in the LDS budget.
   const long long kTailPixels = 256;
   const int NP = G * V;
   auto lds_need = [&](int from) {
     size_t b = 0;
     for (int k = from + 1; k < L; ++k) {
-      const size_t px = (size_t)lv[k].h * lv[k].w;
+      const size_t px = (size_t)l0[k].h * l0[k].w;
       b += ((px * 4 + 15) & ~size_t(15)) + ((px * NP * 4 + 15) & ~size_t(15));
     }
     return b;
@@ -159,28 +163,40 @@ int run_pyramid(Level* lv, int L, int G, int V, int vt, double sigma_i, double s
   // fused level-0 pass stays a regular launch)
   int s0 = L - 1;
   for (int k = 1; k < L - 1; ++k)
-    if ((long long)lv[k + 1].h * lv[k + 1].w <= kTailPixels && lds_need(k) <= (size_t)ss::kTailLdsBytes) {
+    if ((long long)l0[k + 1].h * l0[k + 1].w <= kTailPixels && lds_need(k) <= (size_t)ss::kTailLdsBytes) {
       s0 = k;
       break;
     }
   const bool tail = L - 1 - s0 >= 2 && L - s0 <= ss::kMaxTailLevels;
   const int last_down = tail ? s0 : L - 1;
+  ss::IsbDownArgs da[ss::kMaxBatch];
+  ss::IsbUpArgs ua[ss::kMaxBatch];
+  // The first down pass takes the entries last to first: the sweep wrote
+  // them first to last, so the volumes still in the last-level cache (256 MB
+  // of Infinity Cache, less than two fp32 1024^2 x 32 volumes) are read
+  // before they are evicted; the fused level-0 up pass then takes them first
+  // to last, again most recently read first.
   for (int k = 1; k <= last_down; ++k) {
-    ss::IsbDownArgs a{lv[k - 1].g, lv[k - 1].c, lv[k - 1].h, lv[k - 1].w, lv[k].g, lv[k].c, lv[k].h, lv[k].w, vi};
+    for (int b = 0; b < nb; ++b) {
+      const Level* v = lv[k == 1 ? nb - 1 - b : b];
+      da[b] = ss::IsbDownArgs{v[k - 1].g, v[k - 1].c, v[k - 1].h, v[k - 1].w, v[k].g, v[k].c, v[k].h, v[k].w, vi};
+    }
     StageTimer t(stage_down, s);
-    SS_HIP(ss::launch_isb_down(a, G, V, k == 1 ? vt : SS_VOLUME_F32, s));
+    SS_HIP(ss::launch_isb_down(da, nb, G, V, k == 1 ? vt : SS_VOLUME_F32, s));
   }
   if (tail) {
     ss::IsbTailArgs ta;
     memset(&ta, 0, sizeof ta);
     ta.n = L - s0;
     ta.var_inv_i = vi;
-    ta.g0 = lv[s0].g;
-    ta.c0 = lv[s0].c;
+    for (int b = 0; b < nb; ++b) {
+      ta.g0[b] = lv[b][s0].g;
+      ta.c0[b] = lv[b][s0].c;
+    }
     unsigned off = 0;
     for (int e = 0; e < ta.n; ++e) {
-      ta.h[e] = lv[s0 + e].h;
-      ta.w[e] = lv[s0 + e].w;
+      ta.h[e] = l0[s0 + e].h;
+      ta.w[e] = l0[s0 + e].w;
       if (e > 0) {
         const unsigned px = (unsigned)(ta.h[e] * ta.w[e]);
         ta.goff[e] = off;
@@ -195,20 +211,26 @@ int run_pyramid(Level* lv, int L, int G, int V, int vt, double sigma_i, double s
     }
     ta.lds_bytes = off;
     StageTimer t(stage_down, s);
-    SS_HIP(ss::launch_isb_tail(ta, G, V, s));
+    SS_HIP(ss::launch_isb_tail(ta, nb, G, V, s));
   }
   for (int k = (tail ? s0 : L - 1) - 1; k >= 0; --k) {
     double wu, wd;
     weights(k, wu, wd);
-    ss::IsbUpArgs a{lv[k + 1].g, lv[k + 1].c, lv[k + 1].h, lv[k + 1].w, lv[k].g, lv[k].c, lv[k].h, lv[k].w,
-                    (float)wu,   (float)wd,   vi,          dist,        am,   fuse_last && k == 0 ? fill : nullptr};
     const bool fuse = fuse_last && k == 0;
+    for (int b = 0; b < nb; ++b) {
+      const Level* v = lv[b];
+      ua[b] = ss::IsbUpArgs{v[k + 1].g, v[k + 1].c, v[k + 1].h, v[k + 1].w, v[k].g, v[k].c, v[k].h, v[k].w,
+                            (float)wu,  (float)wd,  vi,         fuse ? dist[b] : nullptr, am,
+                            fuse && fill ? fill[b] : nullptr};
+    }
     StageTimer t(fuse ? SS_STAGE_ISB_UP_ARGMIN : stage_up, s);
-    SS_HIP(ss::launch_isb_up(a, G, V, k == 0 ? vt : SS_VOLUME_F32, fuse, s));
+    SS_HIP(ss::launch_isb_up(ua, nb, G, V, k == 0 ? vt : SS_VOLUME_F32, fuse, s));
   }
   if (fuse_last && L == 1) {
-    StageTimer t(SS_STAGE_ISB_UP_ARGMIN, s);
-    SS_HIP(ss::launch_argmin(lv[0].c, lv[0].h * lv[0].w, V, vt, am, dist, s));
+    for (int b = 0; b < nb; ++b) {
+      StageTimer t(SS_STAGE_ISB_UP_ARGMIN, s);
+      SS_HIP(ss::launch_argmin(lv[b][0].c, l0[0].h * l0[0].w, V, vt, am, dist[b], s));
+    }
   }
   return SS_OK;
 }
@@ -428,7 +450,9 @@ static int isb_filter_impl(uint32_t* d_guide, void* d_cost, int H, int W, int N,
   carve_levels(cv, lv, L, H, W, NP);
   const int G = N == 1 ? 1 : 8, V = N == 1 ? 1 : NP / 8;
   ss::ArgminParams am{};
-  return run_pyramid(lv, L, G, V, vt, sigma_i, sigma_s, false, am, nullptr, s, SS_STAGE_ISB_DOWN, SS_STAGE_ISB_UP);
+  Level* const one[1] = {lv};
+  return run_pyramid(one, 1, L, G, V, vt, sigma_i, sigma_s, false, am, nullptr, s, SS_STAGE_ISB_DOWN,
+                     SS_STAGE_ISB_UP);
 }
 
 static int check_isb(int H, int W, int N, int vt, void* ws, size_t bytes, int planar) {
@@ -521,30 +545,40 @@ void estimate_level0(void* ws, int H, int W, int N, int vt, uint32_t** g0, void*
   estimate_layout(cv, H, W, N, vt, g0, c0, lv, dl);
 }
 
-int estimate_after_sweep(int H, int W, int N, int vt, const float* cands, double sigma_i, double sigma_s,
-                         float* d_dist, void* ws, hipStream_t s, float* fill) {
+int estimate_after_sweep(int nb, int H, int W, int N, int vt, const float* cands, double sigma_i, double sigma_s,
+                         float* const* d_dist, void* const* ws, hipStream_t s, float* const* fill) {
+  if (nb < 1 || nb > kMaxBatch) return fail(SS_ERR_ARG, "batch of %d references outside [1, %d]", nb, kMaxBatch);
   const int L = levels_of(H, W);
-  Carver cv(ws);
-  uint32_t* g0;
-  void* c0;
-  Level lv[32], dl[32];
-  estimate_layout(cv, H, W, N, vt, &g0, &c0, lv, dl);
+  Level lv[kMaxBatch][32], dl[kMaxBatch][32];
+  Level* lvp[kMaxBatch];
+  Level* dlp[kMaxBatch];
+  uint32_t* g0[kMaxBatch];
+  void* c0[kMaxBatch];
+  for (int b = 0; b < nb; ++b) {
+    Carver cv(ws[b]);
+    estimate_layout(cv, H, W, N, vt, &g0[b], &c0[b], lv[b], dl[b]);
+    lv[b][0] = Level{H, W, g0[b], c0[b]};
+    dl[b][0] = Level{H, W, g0[b], d_dist[b]};
+    lvp[b] = lv[b];
+    dlp[b] = dl[b];
+  }
   const float d0 = cands[0], dN1 = cands[N - 1];
   if (N == 1) {
     // one candidate: max == min everywhere, so the reference returns d_{N-1}
     // (depth_estimation.py:198) whatever the cost
-    SS_HIP(ss::launch_fill(d_dist, dN1, (size_t)H * W, s));
-    if (fill) SS_HIP(ss::launch_fill(fill, 1e8f, (size_t)H * W, s));
+    for (int b = 0; b < nb; ++b) {
+      SS_HIP(ss::launch_fill(d_dist[b], dN1, (size_t)H * W, s));
+      if (fill) SS_HIP(ss::launch_fill(fill[b], 1e8f, (size_t)H * W, s));
+    }
   } else {
-    lv[0] = Level{H, W, g0, c0};
     ss::ArgminParams am{N, d0, dN1, (float)(d0 / dN1) - 1.0f};
-    const int rc = run_pyramid(lv, L, 8, ss::lanes_v(N), vt, sigma_i, sigma_s, true, am, d_dist, s,
+    const int rc = run_pyramid(lvp, nb, L, 8, ss::lanes_v(N), vt, sigma_i, sigma_s, true, am, d_dist, s,
                                SS_STAGE_ISB_DOWN, SS_STAGE_ISB_UP, L > 1 ? fill : nullptr);
     if (rc) return rc;
-    if (fill && L == 1) SS_HIP(ss::launch_fill(fill, 1e8f, (size_t)H * W, s));
+    if (fill && L == 1)
+      for (int b = 0; b < nb; ++b) SS_HIP(ss::launch_fill(fill[b], 1e8f, (size_t)H * W, s));
   }
-  dl[0] = Level{H, W, g0, d_dist};
-  return run_pyramid(dl, L, 1, 1, SS_VOLUME_F32, sigma_i / 2, sigma_s / 2, false, ss::ArgminParams{}, nullptr, s,
+  return run_pyramid(dlp, nb, L, 1, 1, SS_VOLUME_F32, sigma_i / 2, sigma_s / 2, false, ss::ArgminParams{}, nullptr, s,
                      SS_STAGE_DIST_DOWN, SS_STAGE_DIST_UP);
 }
 
@@ -574,11 +608,50 @@ int sweep_references(const float* const* d_raylut, const float* const* d_images,
   return SS_OK;
 }
 
-int reproject_into(const float* d_in, float* d_out, int H, int W, const ss_intrinsics* intr, const float* t3,
-                   bool fill, hipStream_t s) {
+int reproject_into(int nb, const float* const* d_in, float* const* d_out, int H, int W, const ss_intrinsics* intr,
+                   const float* const* t3, bool fill, hipStream_t s) {
+  if (nb < 1 || nb > kMaxBatch) return fail(SS_ERR_ARG, "batch of %d maps outside [1, %d]", nb, kMaxBatch);
   StageTimer t(SS_STAGE_REPROJECT, s);
-  if (fill) SS_HIP(ss::launch_fill(d_out, 1e8f, (size_t)H * W, s));
-  SS_HIP(ss::launch_reproject(d_in, d_out, H, W, *intr, t3, s));
+  ReprojectArgs a[kMaxBatch];
+  for (int b = 0; b < nb; ++b) {
+    if (fill) SS_HIP(ss::launch_fill(d_out[b], 1e8f, (size_t)H * W, s));
+    a[b].din = d_in[b];
+    a[b].dout = d_out[b];
+    a[b].c = intr[b];
+    memcpy(a[b].t, t3[b], sizeof a[b].t);
+  }
+  SS_HIP(ss::launch_reproject(a, nb, H, W, s));
+  return SS_OK;
+}
+
+int to_u8_maps(const ToU8Args* a, int nb, size_t n, hipStream_t s) {
+  if (nb < 1 || nb > kMaxBatch) return fail(SS_ERR_ARG, "batch of %d images outside [1, %d]", nb, kMaxBatch);
+  if (n == 0) return SS_OK;
+  StageTimer t(SS_STAGE_TO_U8, s);
+  SS_HIP(ss::launch_to_u8(a, nb, n, s));
+  return SS_OK;
+}
+
+int inpaint_maps(int nb, float* const* map, float* const* tmp, const uint8_t* const* lut, int H, int W,
+                 int iterations, double thr_empty, double thr_src, hipStream_t s) {
+  if (nb < 1 || nb > kMaxBatch) return fail(SS_ERR_ARG, "batch of %d maps outside [1, %d]", nb, kMaxBatch);
+  InpaintArgs a[kMaxBatch];
+  for (int b = 0; b < nb; ++b) a[b] = InpaintArgs{map[b], tmp[b], lut[b]};
+  const int per = ss::inpaint_steps_per_launch();
+  bool in_map = true;  // where the current values are
+  for (int i = 0; i < iterations; i += per) {
+    StageTimer tm(SS_STAGE_INPAINT, s);
+    SS_HIP(ss::launch_inpaint_multi(a, nb, H, W, iterations - i < per ? iterations - i : per, thr_empty, thr_src, s));
+    for (int b = 0; b < nb; ++b) {
+      const float* t = a[b].din;
+      a[b].din = a[b].dout;
+      a[b].dout = const_cast<float*>(t);
+    }
+    in_map = !in_map;
+  }
+  if (!in_map)
+    for (int b = 0; b < nb; ++b)
+      SS_HIP(hipMemcpyAsync(map[b], tmp[b], sizeof(float) * H * W, hipMemcpyDeviceToDevice, s));
   return SS_OK;
 }
 
@@ -618,7 +691,9 @@ int ss_estimate_distance(const float* d_raylut, const float* const* d_images, in
     rc = ss::sweep_references(&d_raylut, d_images, C, cams, &rts, &ref, 1, cands, N, H, W, vtype, ws, s);
     if (rc) return rc;
   }
-  rc = ss::estimate_after_sweep(H, W, N, vtype, cands, sigma_i, sigma_s, d_dist, d_workspace, s, nullptr);
+  float* const dists[1] = {d_dist};
+  void* const wss[1] = {d_workspace};
+  rc = ss::estimate_after_sweep(1, H, W, N, vtype, cands, sigma_i, sigma_s, dists, wss, s, nullptr);
   return rc ? rc : ok();
 }
 
@@ -626,27 +701,16 @@ int ss_reproject(const float* d_in, float* d_out, int H, int W, const ss_intrins
                  ss_stream stream) {
   SS_REQUIRE(d_in && d_out && intr && t3 && H >= 1 && W >= 1, "bad argument");
   SS_REQUIRE(d_in != d_out, "reprojection cannot run in place");
-  const int rc = ss::reproject_into(d_in, d_out, H, W, intr, t3, true, as_stream(stream));
+  float* const outs[1] = {d_out};
+  const int rc = ss::reproject_into(1, &d_in, outs, H, W, intr, &t3, true, as_stream(stream));
   return rc ? rc : ok();
 }
 
 int ss_inpaint(float* d_map, float* d_tmp, const uint8_t* d_lut, int H, int W, int iterations, double thr_empty,
                double thr_src, ss_stream stream) {
   SS_REQUIRE(d_map && d_tmp && d_lut && H >= 1 && W >= 1 && iterations >= 0, "bad argument");
-  hipStream_t s = as_stream(stream);
-  float* a = d_map;
-  float* b = d_tmp;
-  const int per = ss::inpaint_steps_per_launch();
-  for (int i = 0; i < iterations; i += per) {
-    StageTimer tm(SS_STAGE_INPAINT, s);
-    SS_HIP(ss::launch_inpaint_multi(a, b, d_lut, H, W, iterations - i < per ? iterations - i : per, thr_empty,
-                                    thr_src, s));
-    float* t = a;
-    a = b;
-    b = t;
-  }
-  if (a != d_map) SS_HIP(hipMemcpyAsync(d_map, a, sizeof(float) * H * W, hipMemcpyDeviceToDevice, s));
-  return ok();
+  const int rc = ss::inpaint_maps(1, &d_map, &d_tmp, &d_lut, H, W, iterations, thr_empty, thr_src, as_stream(stream));
+  return rc ? rc : ok();
 }
 
 int ss_merge(const float* d_sampling, const float* d_weights, const float* const* d_reproj,
@@ -686,9 +750,9 @@ int ss_merge(const float* d_sampling, const float* d_weights, const float* const
 int ss_to_u8(const float* d_in, uint8_t* d_out, size_t n, ss_stream stream) {
   SS_REQUIRE(d_in && d_out, "null pointer argument");
   if (n == 0) return ok();
-  StageTimer t(SS_STAGE_TO_U8, as_stream(stream));
-  SS_HIP(ss::launch_to_u8(d_in, d_out, n, as_stream(stream)));
-  return ok();
+  const ss::ToU8Args a{d_in, d_out};
+  const int rc = ss::to_u8_maps(&a, 1, n, as_stream(stream));
+  return rc ? rc : ok();
 }
 
 int ss_inverse_distance(const float* d_dist, float* d_inv, size_t n, ss_stream stream) {

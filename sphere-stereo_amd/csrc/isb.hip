@@ -16,6 +16,8 @@
 // from level 1, so rounding the coarse levels to 16 bits moved the argmin of
 // several per cent of the pixels, while rounding level 0 changed next to
 // nothing.
+#include <string.h>
+
 #include "ss_kernels.h"
 
 namespace ss {
@@ -125,7 +127,8 @@ __device__ __forceinline__ void down_pixel(const IsbDownArgs& a, int pix, bool a
 }
 
 template <int G, int V, typename T>
-__global__ void __launch_bounds__(kBlock) k_isb_down(IsbDownArgs a) {
+__global__ void __launch_bounds__(kBlock) k_isb_down(Batch<IsbDownArgs> b) {
+  const IsbDownArgs a = b.a[blockIdx.y];
   const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   int pix = (tile * kBlock + (int)threadIdx.x) / G;
@@ -345,7 +348,8 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
 }
 
 template <int G, int V, typename T, bool FUSE_ARGMIN>
-__global__ void __launch_bounds__(kBlock) k_isb_up(IsbUpArgs a) {
+__global__ void __launch_bounds__(kBlock) k_isb_up(Batch<IsbUpArgs> b) {
+  const IsbUpArgs a = b.a[blockIdx.y];
   const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   int pix = (tile * kBlock + (int)threadIdx.x) / G;
@@ -356,7 +360,7 @@ __global__ void __launch_bounds__(kBlock) k_isb_up(IsbUpArgs a) {
   up_pixel<G, V, T, FUSE_ARGMIN>(a, pix, active, lane);
 }
 
-// The coarse end of the pyramid in one workgroup: the down passes into
+// The coarse end of the pyramid in one workgroup per batch entry: the down passes into
 // levels 1..n-1 of the table, then the up passes back into level 0, each
 // separated by a workgroup barrier.  Levels 1..n-1 live only in LDS (they
 // are not needed once the pass is over); level 0 is in global memory.  The
@@ -369,11 +373,13 @@ __global__ void __launch_bounds__(kTailBlock) k_isb_tail(IsbTailArgs t) {
   extern __shared__ __align__(16) char smem[];
   const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
   const int slot = threadIdx.x / G, slots = kTailBlock / G;
+  uint32_t* const g0 = t.g0[blockIdx.x];
+  void* const c0 = t.c0[blockIdx.x];
   auto gp = [&](int k) __attribute__((always_inline)) -> uint32_t* {
-    return k == 0 ? t.g0 : reinterpret_cast<uint32_t*>(smem + t.goff[k]);
+    return k == 0 ? g0 : reinterpret_cast<uint32_t*>(smem + t.goff[k]);
   };
   auto cp = [&](int k) __attribute__((always_inline)) -> void* {
-    return k == 0 ? t.c0 : static_cast<void*>(smem + t.coff[k]);
+    return k == 0 ? c0 : static_cast<void*>(smem + t.coff[k]);
   };
   for (int k = 1; k < t.n; ++k) {
     const IsbDownArgs a{gp(k - 1), cp(k - 1), t.h[k - 1], t.w[k - 1], gp(k), cp(k), t.h[k], t.w[k], t.var_inv_i};
@@ -438,26 +444,35 @@ __global__ void k_packed_to_guide3(const uint32_t* __restrict__ g, uint8_t* __re
 
 inline int blocks_for(long long threads) { return (int)((threads + kBlock - 1) / kBlock); }
 
+template <typename A>
+Batch<A> batch_of(const A* a, int nb) {
+  Batch<A> b;
+  memset(&b, 0, sizeof b);
+  for (int i = 0; i < nb; ++i) b.a[i] = a[i];
+  return b;
+}
+
+// Every entry of a batch has the shape of entry 0.
 template <int G, int V, typename T>
-hipError_t launch_down_gv(const IsbDownArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_isb_down<G, V, T>), dim3(blocks_for((long long)a.hout * a.wout * G)), dim3(kBlock), 0, s,
-                     a);
+hipError_t launch_down_gv(const IsbDownArgs* a, int nb, hipStream_t s) {
+  const dim3 grid(blocks_for((long long)a[0].hout * a[0].wout * G), nb);
+  hipLaunchKernelGGL((k_isb_down<G, V, T>), grid, dim3(kBlock), 0, s, batch_of(a, nb));
   return hipGetLastError();
 }
 
 template <int G, int V>
-hipError_t launch_tail_gv(const IsbTailArgs& t, hipStream_t s) {
-  hipLaunchKernelGGL((k_isb_tail<G, V>), dim3(1), dim3(kTailBlock), t.lds_bytes, s, t);
+hipError_t launch_tail_gv(const IsbTailArgs& t, int nb, hipStream_t s) {
+  hipLaunchKernelGGL((k_isb_tail<G, V>), dim3(nb), dim3(kTailBlock), t.lds_bytes, s, t);
   return hipGetLastError();
 }
 
 template <int G, int V, typename T>
-hipError_t launch_up_gv(const IsbUpArgs& a, bool fuse, hipStream_t s) {
-  const dim3 grid(blocks_for((long long)a.hin * a.win * G));
+hipError_t launch_up_gv(const IsbUpArgs* a, int nb, bool fuse, hipStream_t s) {
+  const dim3 grid(blocks_for((long long)a[0].hin * a[0].win * G), nb);
   if (fuse)
-    hipLaunchKernelGGL((k_isb_up<G, V, T, true>), grid, dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_isb_up<G, V, T, true>), grid, dim3(kBlock), 0, s, batch_of(a, nb));
   else
-    hipLaunchKernelGGL((k_isb_up<G, V, T, false>), grid, dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_isb_up<G, V, T, false>), grid, dim3(kBlock), 0, s, batch_of(a, nb));
   return hipGetLastError();
 }
 
@@ -482,15 +497,15 @@ hipError_t launch_argmin_vt(const void* cost, int npix, const ArgminParams& p, f
   }
 
 template <typename T>
-hipError_t down_t(const IsbDownArgs& a, int V, hipStream_t s) {
-#define SS_CALL(v) launch_down_gv<8, v, T>(a, s)
+hipError_t down_t(const IsbDownArgs* a, int nb, int V, hipStream_t s) {
+#define SS_CALL(v) launch_down_gv<8, v, T>(a, nb, s)
   SS_DISPATCH_V(V, SS_CALL)
 #undef SS_CALL
 }
 
 template <typename T>
-hipError_t up_t(const IsbUpArgs& a, int V, bool fuse, hipStream_t s) {
-#define SS_CALL(v) launch_up_gv<8, v, T>(a, fuse, s)
+hipError_t up_t(const IsbUpArgs* a, int nb, int V, bool fuse, hipStream_t s) {
+#define SS_CALL(v) launch_up_gv<8, v, T>(a, nb, fuse, s)
   SS_DISPATCH_V(V, SS_CALL)
 #undef SS_CALL
 }
@@ -507,25 +522,28 @@ hipError_t argmin_t(const void* cost, int npix, int V, const ArgminParams& p, fl
 // vt: the element type of the level-0 side of the pass (the input of a down
 // pass, the fine level of an up pass); the other side is fp32.  G = 1 (the
 // distance filter) exists for fp32 only.
-hipError_t launch_isb_down(const IsbDownArgs& a, int G, int V, int vt, hipStream_t s) {
-  if (G == 1) return (V == 1 && vt == SS_VOLUME_F32) ? launch_down_gv<1, 1, float>(a, s) : hipErrorInvalidValue;
-#define SS_CALL(T) down_t<T>(a, V, s)
+hipError_t launch_isb_down(const IsbDownArgs* a, int nb, int G, int V, int vt, hipStream_t s) {
+  if (nb < 1 || nb > kMaxBatch) return hipErrorInvalidValue;
+  if (G == 1) return (V == 1 && vt == SS_VOLUME_F32) ? launch_down_gv<1, 1, float>(a, nb, s) : hipErrorInvalidValue;
+#define SS_CALL(T) down_t<T>(a, nb, V, s)
   SS_DISPATCH_T(vt, SS_CALL)
 #undef SS_CALL
 }
 
-hipError_t launch_isb_tail(const IsbTailArgs& t, int G, int V, hipStream_t s) {
-  if (G == 1) return V == 1 ? launch_tail_gv<1, 1>(t, s) : hipErrorInvalidValue;
-#define SS_CALL(v) launch_tail_gv<8, v>(t, s)
+hipError_t launch_isb_tail(const IsbTailArgs& t, int nb, int G, int V, hipStream_t s) {
+  if (nb < 1 || nb > kMaxBatch) return hipErrorInvalidValue;
+  if (G == 1) return V == 1 ? launch_tail_gv<1, 1>(t, nb, s) : hipErrorInvalidValue;
+#define SS_CALL(v) launch_tail_gv<8, v>(t, nb, s)
   SS_DISPATCH_V(V, SS_CALL)
 #undef SS_CALL
 }
 
-hipError_t launch_isb_up(const IsbUpArgs& a, int G, int V, int vt, bool fuse_argmin, hipStream_t s) {
+hipError_t launch_isb_up(const IsbUpArgs* a, int nb, int G, int V, int vt, bool fuse_argmin, hipStream_t s) {
+  if (nb < 1 || nb > kMaxBatch) return hipErrorInvalidValue;
   if (G == 1)
-    return (V == 1 && vt == SS_VOLUME_F32 && !fuse_argmin) ? launch_up_gv<1, 1, float>(a, false, s)
+    return (V == 1 && vt == SS_VOLUME_F32 && !fuse_argmin) ? launch_up_gv<1, 1, float>(a, nb, false, s)
                                                            : hipErrorInvalidValue;
-#define SS_CALL(T) up_t<T>(a, V, fuse_argmin, s)
+#define SS_CALL(T) up_t<T>(a, nb, V, fuse_argmin, s)
   SS_DISPATCH_T(vt, SS_CALL)
 #undef SS_CALL
 }

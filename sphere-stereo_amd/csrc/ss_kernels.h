@@ -39,14 +39,26 @@ struct IsbUpArgs {
   float* fill;  // nullable: with the fused argmin, set to 1e8 at every level-0 pixel
 };
 
+// One launch runs the same pass for up to kMaxBatch references of equal
+// shape (a frame's references): the kernel takes entry blockIdx.y of the
+// table (blockIdx.z for grids of 2-D tiles).  Only the buffers differ between
+// entries, so one launch replaces a chain per reference, and the small
+// coarse-level passes, which cannot fill the GPU, cost one launch per frame.
+constexpr int kMaxBatch = SS_MAX_REFERENCES;
+template <typename A>
+struct Batch {
+  A a[kMaxBatch];
+};
+
 // Levels s0..s0+n-1 of a pyramid for the single-workgroup coarse tail:
-// entry 0 in global memory, entries 1..n-1 at byte offsets in LDS.
+// entry 0 in global memory, entries 1..n-1 at byte offsets in LDS.  Workgroup
+// b runs the tail of batch entry b (its own level-s0 buffers g0[b], c0[b]).
 constexpr int kMaxTailLevels = 16;
 constexpr int kTailLdsBytes = 60 * 1024;
 struct IsbTailArgs {
   int n;
-  uint32_t* g0;
-  void* c0;
+  uint32_t* g0[kMaxBatch];
+  void* c0[kMaxBatch];
   unsigned goff[kMaxTailLevels], coff[kMaxTailLevels];
   int h[kMaxTailLevels], w[kMaxTailLevels];
   float wu[kMaxTailLevels], wd[kMaxTailLevels];  // up pass into entry k
@@ -90,6 +102,23 @@ struct SelectArgs {
   int pow2W, pow2H;
 };
 
+// per-reference buffers of the stitching passes
+struct ReprojectArgs {
+  const float* din;
+  float* dout;
+  ss_intrinsics c;
+  float t[3];
+};
+struct InpaintArgs {
+  const float* din;
+  float* dout;
+  const uint8_t* lut;
+};
+struct ToU8Args {
+  const float* in;
+  uint8_t* out;
+};
+
 struct MergeArgs {
   const float2* sampling;
   const float* weights;
@@ -120,10 +149,10 @@ bool profiler_active();
 
 // isb.hip (vt: ss_volume_type of the level-0 side of a pass, i.e. the input
 // of a down pass or the fine level of an up pass; pyramid levels >= 1 are
-// fp32, and so is everything with G = 1)
-hipError_t launch_isb_down(const IsbDownArgs& a, int G, int V, int vt, hipStream_t s);
-hipError_t launch_isb_up(const IsbUpArgs& a, int G, int V, int vt, bool fuse_argmin, hipStream_t s);
-hipError_t launch_isb_tail(const IsbTailArgs& t, int G, int V, hipStream_t s);
+// fp32, and so is everything with G = 1).  a[0..nb) are the batch entries.
+hipError_t launch_isb_down(const IsbDownArgs* a, int nb, int G, int V, int vt, hipStream_t s);
+hipError_t launch_isb_up(const IsbUpArgs* a, int nb, int G, int V, int vt, bool fuse_argmin, hipStream_t s);
+hipError_t launch_isb_tail(const IsbTailArgs& t, int nb, int G, int V, hipStream_t s);
 hipError_t launch_argmin(const void* cost, int npix, int V, int vt, const ArgminParams& p, float* dist,
                          hipStream_t s);
 hipError_t launch_planar_to_cc(const float* src, float* dst, int HW, int N, int NP, hipStream_t s);
@@ -141,13 +170,20 @@ hipError_t launch_sweep(const SweepArgs& a, int vt, hipStream_t s);
 size_t volume_elem_bytes(int vt);
 // level-0 guide and volume of an estimate workspace
 void estimate_level0(void* ws, int H, int W, int N, int vt, uint32_t** g0, void** c0);
-// everything after the sweep: cost filter + argmin, then the distance filter;
-// fill (nullable, H x W) is set to 1e8 on the way (the reprojection target)
-int estimate_after_sweep(int H, int W, int N, int vt, const float* cands, double sigma_i, double sigma_s,
-                         float* d_dist, void* ws, hipStream_t s, float* fill);
-// the z-tested reprojection, with or without the 1e8 fill of d_out first
-int reproject_into(const float* d_in, float* d_out, int H, int W, const ss_intrinsics* intr, const float* t3,
-                   bool fill, hipStream_t s);
+// everything after the sweep, for nb references in the same launches: cost
+// filter + argmin, then the distance filter; fill (nullable, else nb maps of
+// H x W) is set to 1e8 on the way (the reprojection targets)
+int estimate_after_sweep(int nb, int H, int W, int N, int vt, const float* cands, double sigma_i, double sigma_s,
+                         float* const* d_dist, void* const* ws, hipStream_t s, float* const* fill);
+// the z-tested reprojection of nb maps, with or without the 1e8 fill of the
+// targets first
+int reproject_into(int nb, const float* const* d_in, float* const* d_out, int H, int W, const ss_intrinsics* intr,
+                   const float* const* t3, bool fill, hipStream_t s);
+// uint8 casts of nb images of n values each, in one launch
+int to_u8_maps(const ToU8Args* a, int nb, size_t n, hipStream_t s);
+// `iterations` inpainting passes over nb maps (the result ends in map[b])
+int inpaint_maps(int nb, float* const* map, float* const* tmp, const uint8_t* const* lut, int H, int W,
+                 int iterations, double thr_empty, double thr_src, hipStream_t s);
 // one sweep launch over R references (each with its own workspace)
 int sweep_references(const float* const* d_raylut, const float* const* d_images, int C, const ss_camera* cams,
                      const float* const* rts, const int* refs, int R, const float* cands, int N, int H, int W,
@@ -155,12 +191,11 @@ int sweep_references(const float* const* d_raylut, const float* const* d_images,
 
 // stitch.hip
 hipError_t launch_fill(float* p, float v, size_t n, hipStream_t s);
-hipError_t launch_reproject(const float* din, float* dout, int H, int W, const ss_intrinsics& c, const float* t,
-                            hipStream_t s);
+hipError_t launch_reproject(const ReprojectArgs* a, int nb, int H, int W, hipStream_t s);
 hipError_t launch_inpaint_step(const float* din, float* dout, const uint8_t* lut, int H, int W, double thr_empty,
                                double thr_src, hipStream_t s);
-hipError_t launch_inpaint_multi(const float* din, float* dout, const uint8_t* lut, int H, int W, int steps,
-                                double thr_empty, double thr_src, hipStream_t s);
+hipError_t launch_inpaint_multi(const InpaintArgs* a, int nb, int H, int W, int steps, double thr_empty,
+                                double thr_src, hipStream_t s);
 int inpaint_steps_per_launch();
 hipError_t launch_inpaint_lut(const ss_intrinsics& c, const float* t, int H, int W, float mn, float mx, uint8_t* lut,
                               hipStream_t s);
@@ -169,7 +204,7 @@ hipError_t launch_box_filter(const float* in, float* out, float* tmp, int B, int
 hipError_t launch_blend_lut(const BlendArgs& a, hipStream_t s);
 hipError_t launch_normalize(float* w, int R, int P, hipStream_t s);
 hipError_t launch_merge(const MergeArgs& a, hipStream_t s);
-hipError_t launch_to_u8(const float* in, uint8_t* out, size_t n, hipStream_t s);
+hipError_t launch_to_u8(const ToU8Args* a, int nb, size_t n, hipStream_t s);
 
 // eval.hip
 hipError_t launch_inverse(const float* d, float* inv, size_t n, hipStream_t s);

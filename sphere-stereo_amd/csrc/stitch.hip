@@ -9,6 +9,8 @@
 //   * each inpainting pass reads the previous pass's map and writes a second
 //     buffer (Jacobi) instead of updating the map in place while other threads
 //     read it (stitcher.cu:310, 319).
+#include <string.h>
+
 #include "ss_kernels.h"
 
 namespace ss {
@@ -43,18 +45,21 @@ __global__ void __launch_bounds__(kBlock) k_fill(float* __restrict__ p, float v,
   if (i < n) p[i] = v;
 }
 
-// reprojectDistanceKernel (stitcher.cu:181-204) as an exact atomic z-minimum
-__global__ void __launch_bounds__(kBlock) k_reproject(const float* __restrict__ din, float* __restrict__ dout, int H,
-                                                      int W, ss_intrinsics c, float tx, float ty, float tz) {
+// reprojectDistanceKernel (stitcher.cu:181-204) as an exact atomic z-minimum;
+// map blockIdx.y of the batch
+__global__ void __launch_bounds__(kBlock) k_reproject(Batch<ReprojectArgs> b, int H, int W) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= H * W) return;
+  const ReprojectArgs& a = b.a[blockIdx.y];
+  const ss_intrinsics c = a.c;
+  float* const dout = a.dout;
   float px, py, pz, u, v;
   bool valid = true;
   unproject_c(c, (float)(i % W), (float)(i / W), px, py, pz);
-  const float d = din[i];
-  px = px * d - tx;
-  py = py * d - ty;
-  pz = pz * d - tz;
+  const float d = a.din[i];
+  px = px * d - a.t[0];
+  py = py * d - a.t[1];
+  pz = pz * d - a.t[2];
   project_c<false>(c, px, py, pz, u, v, valid);
   const float dist = sqrtf((px * px + py * py) + pz * pz);
   const long long o = (long long)cvt_rni(v) * W + cvt_rni(u);
@@ -102,14 +107,17 @@ __global__ void __launch_bounds__(kBlock) k_inpaint_step(const float* __restrict
 // kSteps inside the staged region, are exact after kSteps passes.
 // 48 + 2 * 16 = 80: 77 KB of LDS, so the 22 x 22 workgroups of a 1024^2
 // map are all resident at once (2 per CU) and each runs 16 passes.
+// blockIdx.z selects the map of the batch.
 constexpr int kTile = 48;
 constexpr int kSteps = 16;
 constexpr int kRegion = kTile + 2 * kSteps;
 static_assert(kRegion * kRegion % 256 == 0 && kRegion * kRegion <= 65536, "region fits the staging loop and list");
 
-__global__ void __launch_bounds__(kBlock) k_inpaint_multi(const float* __restrict__ din, float* __restrict__ dout,
-                                                          const uchar2* __restrict__ lut, int H, int W, int steps,
+__global__ void __launch_bounds__(kBlock) k_inpaint_multi(Batch<InpaintArgs> bt, int H, int W, int steps,
                                                           double thr_empty, double thr_src) {
+  const float* __restrict__ din = bt.a[blockIdx.z].din;
+  float* __restrict__ dout = bt.a[blockIdx.z].dout;
+  const uchar2* __restrict__ lut = reinterpret_cast<const uchar2*>(bt.a[blockIdx.z].lut);
   __shared__ float bufa[kRegion * kRegion];
   __shared__ float bufb[kRegion * kRegion];
   __shared__ uchar2 tab[kRegion * kRegion];
@@ -369,9 +377,30 @@ __global__ void __launch_bounds__(kBlock) k_merge(MergeArgs a) {
   a.rgb_out[3 * idx + 2] = (uint8_t)c2;
 }
 
-__global__ void __launch_bounds__(kBlock) k_to_u8(const float* __restrict__ in, uint8_t* __restrict__ out, size_t n) {
-  const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) out[i] = (uint8_t)in[i];
+// Four elements per lane (16-byte load, 4-byte store) when both buffers are
+// aligned for it; blockIdx.y selects the image of the batch.
+template <bool VEC4>
+__global__ void __launch_bounds__(kBlock) k_to_u8(Batch<ToU8Args> b, size_t n) {
+  const float* __restrict__ in = b.a[blockIdx.y].in;
+  uint8_t* __restrict__ out = b.a[blockIdx.y].out;
+  const size_t i = ((size_t)blockIdx.x * kBlock + threadIdx.x) * 4;
+  if (i >= n) return;
+  if (VEC4 && i + 4 <= n) {
+    const float4 v = *reinterpret_cast<const float4*>(in + i);
+    const uint32_t q = (uint32_t)(uint8_t)v.x | ((uint32_t)(uint8_t)v.y << 8) | ((uint32_t)(uint8_t)v.z << 16) |
+                       ((uint32_t)(uint8_t)v.w << 24);
+    *reinterpret_cast<uint32_t*>(out + i) = q;
+    return;
+  }
+  for (size_t k = i; k < i + 4 && k < n; ++k) out[k] = (uint8_t)in[k];
+}
+
+template <typename A>
+Batch<A> batch_of(const A* a, int nb) {
+  Batch<A> b;
+  memset(&b, 0, sizeof b);
+  for (int i = 0; i < nb; ++i) b.a[i] = a[i];
+  return b;
 }
 
 }  // namespace
@@ -381,10 +410,9 @@ hipError_t launch_fill(float* p, float v, size_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_reproject(const float* din, float* dout, int H, int W, const ss_intrinsics& c, const float* t,
-                            hipStream_t s) {
-  hipLaunchKernelGGL(k_reproject, dim3(blocks_for((long long)H * W)), dim3(kBlock), 0, s, din, dout, H, W, c, t[0],
-                     t[1], t[2]);
+hipError_t launch_reproject(const ReprojectArgs* a, int nb, int H, int W, hipStream_t s) {
+  if (nb < 1 || nb > kMaxBatch) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_reproject, dim3(blocks_for((long long)H * W), nb), dim3(kBlock), 0, s, batch_of(a, nb), H, W);
   return hipGetLastError();
 }
 
@@ -395,11 +423,11 @@ hipError_t launch_inpaint_step(const float* din, float* dout, const uint8_t* lut
   return hipGetLastError();
 }
 
-hipError_t launch_inpaint_multi(const float* din, float* dout, const uint8_t* lut, int H, int W, int steps,
-                                double thr_empty, double thr_src, hipStream_t s) {
-  const dim3 grid((W + kTile - 1) / kTile, (H + kTile - 1) / kTile);
-  hipLaunchKernelGGL(k_inpaint_multi, grid, dim3(kBlock), 0, s, din, dout, reinterpret_cast<const uchar2*>(lut), H, W,
-                     steps, thr_empty, thr_src);
+hipError_t launch_inpaint_multi(const InpaintArgs* a, int nb, int H, int W, int steps, double thr_empty,
+                                double thr_src, hipStream_t s) {
+  if (nb < 1 || nb > kMaxBatch) return hipErrorInvalidValue;
+  const dim3 grid((W + kTile - 1) / kTile, (H + kTile - 1) / kTile, nb);
+  hipLaunchKernelGGL(k_inpaint_multi, grid, dim3(kBlock), 0, s, batch_of(a, nb), H, W, steps, thr_empty, thr_src);
   return hipGetLastError();
 }
 
@@ -439,8 +467,16 @@ hipError_t launch_merge(const MergeArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_to_u8(const float* in, uint8_t* out, size_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_to_u8, dim3(blocks_for((long long)n)), dim3(kBlock), 0, s, in, out, n);
+hipError_t launch_to_u8(const ToU8Args* a, int nb, size_t n, hipStream_t s) {
+  if (nb < 1 || nb > kMaxBatch) return hipErrorInvalidValue;
+  bool vec = true;
+  for (int i = 0; i < nb; ++i)
+    vec = vec && (reinterpret_cast<uintptr_t>(a[i].in) & 15) == 0 && (reinterpret_cast<uintptr_t>(a[i].out) & 3) == 0;
+  const dim3 grid(blocks_for((long long)((n + 3) / 4)), nb);
+  if (vec)
+    hipLaunchKernelGGL(k_to_u8<true>, grid, dim3(kBlock), 0, s, batch_of(a, nb), n);
+  else
+    hipLaunchKernelGGL(k_to_u8<false>, grid, dim3(kBlock), 0, s, batch_of(a, nb), n);
   return hipGetLastError();
 }
 
