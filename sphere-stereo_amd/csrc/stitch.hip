@@ -100,18 +100,42 @@ __global__ void __launch_bounds__(kBlock) k_inpaint_step(const float* __restrict
 // Several Jacobi passes of inpaintKernel in one launch.  A workgroup owns a
 // kTile x kTile block of the map; it stages the block with a halo of kSteps
 // pixels (map values and table bytes) in LDS, twice, and lists the pixels
-// that can be filled (empty and inside both the map and the staged region).
-// Each pass recomputes only the listed pixels, reading one LDS copy and
-// writing the other; unlisted pixels never change, so both copies keep them.
-// A pass moves information by at most one pixel, so the block's own pixels,
-// kSteps inside the staged region, are exact after kSteps passes.
-// 48 + 2 * 16 = 80: 77 KB of LDS, so the 22 x 22 workgroups of a 1024^2
-// map are all resident at once (2 per CU) and each runs 16 passes.
+// that can change.  Each pass recomputes only the listed pixels, reading one
+// LDS copy and writing the other; unlisted pixels never change, so both
+// copies keep them.  A pass moves information by at most one pixel, so the
+// block's own pixels, kSteps inside the staged region, are exact after
+// kSteps passes.
+//
+// Listed: empty, inside both the map and the staged region, and within
+// `steps` pixels (Chebyshev distance) of a pixel that can act as a source
+// (value <= thr_src).  A pixel changes only when one of its table neighbours,
+// one pixel away, holds a source value; values become sources only by being
+// filled from one; so a pixel farther than `steps` from every source of the
+// staged region keeps its value for the whole launch (the interior of big
+// holes, the area outside a fisheye's field of view).  The launch also stops
+// after a pass that changes nothing, since every later pass would repeat it.
+// 48 + 2 * 16 = 80: 79 KB of LDS, 2 workgroups per CU.
 // blockIdx.z selects the map of the batch.
 constexpr int kTile = 48;
 constexpr int kSteps = 16;
 constexpr int kRegion = kTile + 2 * kSteps;
-static_assert(kRegion * kRegion % 256 == 0 && kRegion * kRegion <= 65536, "region fits the staging loop and list");
+constexpr int kRowWords = (kRegion + 31) / 32;  // a row of the region as a bit mask
+static_assert(kRegion * kRegion % kBlock == 0 && kRegion * kRegion <= 65536 && kRegion <= 96 &&
+                  kRegion * kRowWords <= kBlock,
+              "region fits the staging loop, the list and the row masks");
+
+// A region row's bit mask (bits 0..95 in three words) shifted by k (0..31)
+// towards higher or lower x.
+__device__ __forceinline__ void row_shl(const uint32_t (&m)[3], int k, uint32_t (&o)[3]) {
+  o[0] = m[0] << k;
+  o[1] = (m[1] << k) | (k ? m[0] >> (32 - k) : 0u);
+  o[2] = (m[2] << k) | (k ? m[1] >> (32 - k) : 0u);
+}
+__device__ __forceinline__ void row_shr(const uint32_t (&m)[3], int k, uint32_t (&o)[3]) {
+  o[2] = m[2] >> k;
+  o[1] = (m[1] >> k) | (k ? m[2] << (32 - k) : 0u);
+  o[0] = (m[0] >> k) | (k ? m[1] << (32 - k) : 0u);
+}
 
 __global__ void __launch_bounds__(kBlock) k_inpaint_multi(Batch<InpaintArgs> bt, int H, int W, int steps,
                                                           double thr_empty, double thr_src) {
@@ -122,71 +146,125 @@ __global__ void __launch_bounds__(kBlock) k_inpaint_multi(Batch<InpaintArgs> bt,
   __shared__ float bufb[kRegion * kRegion];
   __shared__ uchar2 tab[kRegion * kRegion];
   __shared__ uint16_t list[kRegion * kRegion];
-  __shared__ unsigned count;
+  __shared__ uint32_t srcm[kRegion][kRowWords];  // sources, then sources dilated along rows
+  __shared__ uint32_t near_[kRegion][kRowWords]; // within `steps` of a source
+  __shared__ unsigned count, changed[2];  // changed[s & 1] == s + 1: pass s changed a pixel
+  const int tid = (int)threadIdx.x;
   const int ox = blockIdx.x * kTile - kSteps, oy = blockIdx.y * kTile - kSteps;
-  if (threadIdx.x == 0) count = 0;
+  const int reach = min(steps, kSteps);
+  if (tid == 0) {
+    count = 0;
+    changed[0] = 0;
+    changed[1] = 0;
+  }
   // stage: all loads of a thread are issued before any is used
   constexpr int kPer = kRegion * kRegion / kBlock;
   float dv[kPer];
   uchar2 tv[kPer];
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
-    const int i = threadIdx.x + k * kBlock;
+    const int i = tid + k * kBlock;
     const int gx = ox + i % kRegion, gy = oy + i / kRegion;
     const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
     const int g = in ? gy * W + gx : 0;
     dv[k] = in ? din[g] : 0.0f;
     tv[k] = in ? lut[g] : make_uchar2(0, 0);
   }
-  __syncthreads();  // count is zero
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
-    const int i = threadIdx.x + k * kBlock;
-    const int lx = i % kRegion, ly = i / kRegion;
-    const int gx = ox + lx, gy = oy + ly;
+    const int i = tid + k * kBlock;
     bufa[i] = dv[k];
     bufb[i] = dv[k];
     tab[i] = tv[k];
+  }
+  __syncthreads();
+  // source bits, one word per thread (pixels outside the map, staged as 0,
+  // count as sources: no listed pixel reads them, so they can only list a
+  // pixel that would not change, never drop one that would)
+  if (tid < kRegion * kRowWords) {
+    const int y = tid / kRowWords, w = tid % kRowWords;
+    uint32_t b = 0;
+    for (int x = 32 * w; x < min(32 * w + 32, kRegion); ++x)
+      if ((double)bufa[y * kRegion + x] <= thr_src) b |= 1u << (x - 32 * w);
+    srcm[y][w] = b;
+  }
+  __syncthreads();
+  // dilate along the row (one thread per row), then across rows
+  uint32_t hrow[3] = {0u, 0u, 0u};
+  if (tid < kRegion) {
+    const uint32_t m[3] = {srcm[tid][0], srcm[tid][1], kRowWords > 2 ? srcm[tid][2] : 0u};
+    hrow[0] = m[0];
+    hrow[1] = m[1];
+    hrow[2] = m[2];
+    for (int k = 1; k <= reach; ++k) {
+      uint32_t l[3], r[3];
+      row_shl(m, k, l);
+      row_shr(m, k, r);
+      hrow[0] |= l[0] | r[0];
+      hrow[1] |= l[1] | r[1];
+      hrow[2] |= l[2] | r[2];
+    }
+  }
+  __syncthreads();  // every row read
+  if (tid < kRegion)
+    for (int w = 0; w < kRowWords; ++w) srcm[tid][w] = hrow[w];
+  __syncthreads();
+  if (tid < kRegion * kRowWords) {
+    const int y = tid / kRowWords, w = tid % kRowWords;
+    uint32_t b = 0;
+    for (int yy = max(y - reach, 0); yy <= min(y + reach, kRegion - 1); ++yy) b |= srcm[yy][w];
+    near_[y][w] = b;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int i = tid + k * kBlock;
+    const int lx = i % kRegion, ly = i / kRegion;
+    const int gx = ox + lx, gy = oy + ly;
     if ((double)dv[k] >= thr_empty && gx >= 1 && gy >= 1 && gx < W - 1 && gy < H - 1 && lx >= 1 && ly >= 1 &&
-        lx < kRegion - 1 && ly < kRegion - 1)
+        lx < kRegion - 1 && ly < kRegion - 1 && ((near_[ly][lx >> 5] >> (lx & 31)) & 1u))
       list[atomicAdd(&count, 1u)] = (uint16_t)i;
   }
   __syncthreads();
   const int n = (int)count;
   float* a = bufa;
   float* b = bufb;
-  if (n > 0) {
-    for (int s = 0; s < steps; ++s) {
-      for (int j = threadIdx.x; j < n; j += kBlock) {
-        const int i = list[j];
-        const float d = a[i];
-        float r = d;
-        if ((double)d >= thr_empty) {
-          const int lx = i % kRegion, ly = i / kRegion;
-          const uchar2 t = tab[i];
-          const int bs[2] = {t.x, t.y};
-          float val = 0.0f, wsum = 0.0f;
+  for (int s = 0; s < steps && n > 0; ++s) {
+    bool moved = false;
+    for (int j = threadIdx.x; j < n; j += kBlock) {
+      const int i = list[j];
+      const float d = a[i];
+      float r = d;
+      if ((double)d >= thr_empty) {
+        const int lx = i % kRegion, ly = i / kRegion;
+        const uchar2 t = tab[i];
+        const int bs[2] = {t.x, t.y};
+        float val = 0.0f, wsum = 0.0f;
 #pragma unroll
-          for (int k = 0; k < 2; ++k) {
-            // table offsets are 0..2 (createInpaintingWeightsKernel), so the
-            // neighbour of a listed pixel lies inside the staged region
-            const int nx = lx + (bs[k] & 3) - 1, ny = ly + ((bs[k] & 12) >> 2) - 1;
-            const float w = (float)(bs[k] & 240);
-            const float sd = a[min(ny, kRegion - 1) * kRegion + min(nx, kRegion - 1)];
-            if ((double)sd <= thr_src && w > 0.0f) {
-              val = val + w * sd;
-              wsum = wsum + w;
-            }
+        for (int k = 0; k < 2; ++k) {
+          // table offsets are 0..2 (createInpaintingWeightsKernel), so the
+          // neighbour of a listed pixel lies inside the staged region
+          const int nx = lx + (bs[k] & 3) - 1, ny = ly + ((bs[k] & 12) >> 2) - 1;
+          const float w = (float)(bs[k] & 240);
+          const float sd = a[min(ny, kRegion - 1) * kRegion + min(nx, kRegion - 1)];
+          if ((double)sd <= thr_src && w > 0.0f) {
+            val = val + w * sd;
+            wsum = wsum + w;
           }
-          if (wsum > 0.0f) r = val / wsum;
         }
-        b[i] = r;
+        if (wsum > 0.0f) r = val / wsum;
       }
-      __syncthreads();
-      float* t = a;
-      a = b;
-      b = t;
+      b[i] = r;
+      moved = moved || __float_as_uint(r) != __float_as_uint(d);
     }
+    if (moved) changed[s & 1] = s + 1;  // every writer stores the same value
+    __syncthreads();
+    float* t = a;
+    a = b;
+    b = t;
+    // uniform: pass s changed nothing, nor would the rest.  (The flag of
+    // pass s is next written in pass s + 2, after every lane has read it.)
+    if (changed[s & 1] != (unsigned)(s + 1)) break;
   }
   for (int i = threadIdx.x; i < kTile * kTile; i += kBlock) {
     const int lx = kSteps + i % kTile, ly = kSteps + i / kTile;

@@ -82,6 +82,40 @@ def test_inpaint_bit_exact(gpu, iters):
     assert_same(to_np(m), want, "inpainted map (%d passes)" % iters)
 
 
+@pytest.mark.parametrize("iters,double_form", [(16, False), (32, True), (40, False)])
+def test_inpaint_large_holes_bit_exact(gpu, iters, double_form):
+    """Holes far wider than the passes can fill (the interior stays empty), a
+    band of NaNs, and maps spanning many workgroup blocks."""
+    H, W = 300, 420
+    intr = [0.3 * W, 0.3 * W, W / 2, H / 2, -0.18, 0.59]
+    t = [0.05, 0.02, -0.03]
+    rec = _lib.ss_intrinsics(*[float(np.float32(v)) for v in intr])
+    lut = torch.empty([H, W, 2], dtype=torch.uint8, device=gpu)
+    _lib.check(_lib.lib.ss_inpaint_lut(rec, _lib.float_array(t), H, W, 0.55, 100, lut.data_ptr(), _stream()))
+    want_lut = O32.inpaint_lut([float(np.float32(v)) for v in intr], [float(np.float32(v)) for v in t], H, W,
+                               0.55, 100)
+    assert_same(to_np(lut), want_lut, "inpaint LUT")
+    rng = np.random.RandomState(100 + iters)
+    d = (1 + rng.rand(H, W) * 50).astype(np.float32)
+    d[20:160, 30:250] = 1e8  # wider than 2 * iters: its middle is never reached
+    d[200:230, 300:410] = 1e8
+    d[250:256, :] = np.nan
+    for _ in range(20):
+        y, x, h, w = rng.randint(0, H), rng.randint(0, W), rng.randint(1, 60), rng.randint(1, 60)
+        d[y:y + h, x:x + w] = 1e8
+    d[rng.rand(H, W) < 0.03] = 1e8
+    if double_form:
+        te, ts = 100.1, 100.0 + float(np.float32(0.1))
+    else:
+        te, ts = 100.1, float(np.float32(100) + np.float32(0.1))
+    want = O32.inpaint(d, want_lut, iters, te, ts)
+    assert np.isnan(want).any() and (want == np.float32(1e8)).any() and (want != d).sum() > 1000
+    m = torch.from_numpy(d).to(gpu)
+    tmp = torch.empty_like(m)
+    _lib.check(_lib.lib.ss_inpaint(m.data_ptr(), tmp.data_ptr(), lut.data_ptr(), H, W, iters, te, ts, _stream()))
+    assert_same(to_np(m), want, "inpainted map (%d passes)" % iters)
+
+
 def test_reproject_bit_exact(gpu):
     H, W = 97, 150
     intr = [float(np.float32(v)) for v in [0.3 * 150, 0.3 * 150, 75.0, 48.0, -0.18, 0.59]]
