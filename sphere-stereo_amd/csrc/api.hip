@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <cmath>
 #include <mutex>
 #include <vector>
 
@@ -258,6 +259,27 @@ int check_sweep(const float* const* d_images, int C, const ss_camera* cams, cons
 // One sweep launch over references refs[0..R) (R * C <= kSweepMaxPairs):
 // rts[r] holds reference r's C transforms, costs[r] / guides[r] its outputs
 // (guides[r] may be null).  Requires N >= 2 (the volume stride is 8 * V).
+// The bounds under which the sweep's projections may take project_t_fast:
+// finite parameters, |fs|, |transform entries|, |candidates| <= 2^20,
+// 2^-16 <= |cs| <= 2^20, |xi|, |alpha|, |1 - alpha| <= 16 (the rays are unit
+// vectors by construction).  Real rigs are far inside them.
+bool sweep_fast_bounds(const ss::SweepArgs& a, int C, int npairs) {
+  auto in = [](float v, float lo, float hi) { return std::isfinite(v) && fabsf(v) >= lo && fabsf(v) <= hi; };
+  const float big = 0x1p20f;
+  for (int k = 0; k < C; ++k) {
+    const ss::TorchCam& c = a.cam[k];
+    if (!in(c.fs0, 0.0f, big) || !in(c.fs1, 0.0f, big) || !in(c.cs0, 0x1p-16f, big) || !in(c.cs1, 0x1p-16f, big) ||
+        !in(c.xi, 0.0f, 16.0f) || !in(c.a, 0.0f, 16.0f) || !in(c.one_m_a, 0.0f, 16.0f))
+      return false;
+  }
+  for (int p = 0; p < npairs; ++p)
+    for (int i = 0; i < 12; ++i)
+      if (!in(a.rt[p][i], 0.0f, big)) return false;
+  for (int n = 0; n < a.N; ++n)
+    if (!in(a.cands[n], 0.0f, big)) return false;
+  return true;
+}
+
 int launch_sweep_refs(const float* const* d_raylut, const float* const* d_images, int C, const ss_camera* cams,
                       const float* const* rts, const int* refs, int R, const float* cands, int N, int H, int W,
                       int vt, void* const* costs, uint32_t* const* guides, hipStream_t s) {
@@ -296,6 +318,7 @@ int launch_sweep_refs(const float* const* d_raylut, const float* const* d_images
   a.halfW = (float)W * 0.5f;
   a.halfH = (float)H * 0.5f;
   a.pow2 = pow2(W) && pow2(H);
+  a.fastproj = sweep_fast_bounds(a, C, R * C);
   StageTimer t(SS_STAGE_SWEEP, s);
   SS_HIP(ss::launch_sweep(a, vt, s));
   return SS_OK;

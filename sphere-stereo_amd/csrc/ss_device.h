@@ -39,6 +39,49 @@ __device__ __forceinline__ float expf_cw(float x) {
 }
 
 // ---------------------------------------------------------------------------
+// Fast forms of the correctly rounded fp32 square root and division, for
+// operands in the normal range.  They are the sequences the compiler emits
+// for sqrtf and '/' in this build (-fhip-fp32-correctly-rounded-divide-
+// sqrt, read from its gfx950 assembly) without the guards that only act on
+// extreme operands, so they return the same bits wherever those guards are
+// idle.
+// ---------------------------------------------------------------------------
+// v_sqrt_f32, then s - ulp and s + ulp tested with fused residuals; the full
+// form also scales inputs below 2^-96 by 2^32 and passes +-0 and +inf
+// through.  Equal to sqrtf(x) for x in [2^-96, +inf).
+__device__ __forceinline__ float sqrt_rn_normal(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+  const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+  float t = (0.0f >= __builtin_fmaf(-sm, s, x)) ? sm : s;
+  t = (0.0f < __builtin_fmaf(-sp, s, x)) ? sp : t;
+  return t;
+}
+
+// Division: a Newton-refined reciprocal of d, then two fused residual
+// corrections of n * r.  The full form first scales the operands by 2^+-64
+// (v_div_scale: only for denormal or near-overflow operands, an exponent
+// gap of 96 or more, or a numerator below 2^-103) and ends with v_div_fixup
+// (special values; for finite non-zero results it only restates the sign).
+// The reciprocal refinement depends on d alone, so divisions by one d share
+// it.
+struct RcpRN {
+  float d, r;
+};
+__device__ __forceinline__ RcpRN rcp_refined(float d) {
+  const float r0 = __builtin_amdgcn_rcpf(d);
+  const float e = __builtin_fmaf(-d, r0, 1.0f);
+  return RcpRN{d, __builtin_fmaf(e, r0, r0)};
+}
+__device__ __forceinline__ float div_rn_normal(float n, const RcpRN& q) {
+  const float q0 = n * q.r;
+  const float e1 = __builtin_fmaf(-q.d, q0, n);
+  const float q1 = __builtin_fmaf(e1, q.r, q0);
+  const float e2 = __builtin_fmaf(-q.d, q1, n);
+  return __builtin_fmaf(e2, q.r, q1);
+}
+
+// ---------------------------------------------------------------------------
 // Double-sphere camera, torch form (utils.py:57-94): fp32 tensor arithmetic
 // whose Python-double scalars were rounded to fp32 once on the host.
 // ---------------------------------------------------------------------------
@@ -68,6 +111,29 @@ __device__ __forceinline__ void project_t(const TorchCam& c, float qx, float qy,
   valid = qz > c.mw2 * d1;
   u = (c.fs0 * qx) / nrm + c.cs0;
   v = (c.fs1 * qy) / nrm + c.cs1;
+}
+
+// project_t (without the validity flag) with the fast square roots and one
+// shared reciprocal for its two divisions.  Returns whether the result is
+// certainly project_t's: |nrm| >= 2^-30, given camera parameters, transforms
+// and candidate distances within sweep_fast_bounds (api.hip) and unit rays.
+// Under those bounds the numerators stay below 2^62 and both square-root
+// inputs below 2^96, so only a tiny nrm can wake a guard: it needs a point
+// within 2^-48 of the camera centre (a square-root input below 2^-96) or
+// makes the exponent gap reach 96.  A numerator so small that the division
+// would scale it gives a quotient below 2^-73, which leaves u = q + cs0 equal
+// to cs0 either way because |cs| >= 2^-16; a zero numerator gives a zero
+// quotient whose sign cs0 absorbs.
+__device__ __forceinline__ bool project_t_fast(const TorchCam& c, float qx, float qy, float qz, float& u, float& v) {
+  float xy2 = qx * qx + qy * qy;
+  float d1 = sqrt_rn_normal(xy2 + qz * qz);
+  float cc = c.xi * d1 + qz;
+  float d2 = sqrt_rn_normal(xy2 + cc * cc);
+  float nrm = c.a * d2 + c.one_m_a * cc;
+  const RcpRN q = rcp_refined(nrm);
+  u = div_rn_normal(c.fs0 * qx, q) + c.cs0;
+  v = div_rn_normal(c.fs1 * qy, q) + c.cs1;
+  return fabsf(nrm) >= 0x1p-30f;
 }
 
 // torch.matmul(cat([p, 1]), rt.T) for one 3x4 row-major transform, as the

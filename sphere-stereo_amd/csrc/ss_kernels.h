@@ -87,7 +87,8 @@ struct SweepArgs {
   int tiles_x, blocks_per_ref;   // 64 x 4 pixel tiles per row of tiles / per reference
   // grid_sample's coordinate round trip: size, 1/size, 2/size, size/2
   float Wf, Hf, invW, invH, inv2W, inv2H, halfW, halfH;
-  int pow2;  // both sizes are powers of two
+  int pow2;      // both sizes are powers of two
+  int fastproj;  // the parameters are within sweep_fast_bounds: project_t_fast applies
 };
 
 struct SelectArgs {

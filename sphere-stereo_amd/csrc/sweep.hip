@@ -232,7 +232,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
             float qx, qy, qz, u, w;
             bool valid;
             rigid(rt, d * rl.x, d * rl.y, d * rl.z, qx, qy, qz);
-            project_t(cam, qx, qy, qz, u, w, valid);
+            // the fast form where it is certainly exact, else the full one
+            if (!a.fastproj || !project_t_fast(cam, qx, qy, qz, u, w)) project_t(cam, qx, qy, qz, u, w, valid);
             if constexpr (POW2) {
               ix[v] = grid_coord_pow2(u, a.inv2W, a.halfW);
               iy[v] = grid_coord_pow2(w, a.inv2H, a.halfH);
