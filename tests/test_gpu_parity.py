@@ -178,6 +178,52 @@ def test_sweep_bit_exact(small, gpu, vtype):
         assert_same(got_g, O32.guide(to_np(imgs[r])), "guide written by the sweep, ref %d" % r)
 
 
+def _fma32(a, b, c):
+    """fmaf in float32 (the fp64 sum of an exact fp32 product and an fp32
+    value, rounded to fp32; double rounding is too rare to matter here)."""
+    return np.float32(np.float64(a) * np.float64(b) + np.float64(c))
+
+
+@pytest.mark.parametrize("case", ["point_at_camera_centre", "parameters_out_of_fast_bounds"])
+def test_sweep_projection_edge_cases(small, gpu, case):
+    """The sweep projects with exact fast square roots and divisions where the
+    projection's denominator is not tiny and the camera parameters are within
+    the bounds the host checks.  Here one sample lands exactly on the other
+    camera's centre (a zero point: the full forms' special cases), or a
+    camera's xi leaves the bounds (every projection takes the full forms)."""
+    fr, est, ora = small
+    H, W, N = 72, 80, 32
+    NP = _lib.lib.ss_padded_candidates(N)
+    r, slot = fr["refs"][0], 0
+    rts_gpu = _lib.float_array(list(est._rts[r]))
+    rts_ora = np.array(ora.rts[r], dtype=np.float64).copy()
+    cams_gpu = (_lib.ss_camera * len(est._cams))()
+    C.memmove(cams_gpu, est._cams, C.sizeof(cams_gpu))
+    cams_ora = np.array(ora.cams, dtype=np.float64).copy()
+    if case == "point_at_camera_centre":
+        lut = to_np(est._raylut[slot]).reshape(H, W, 4)
+        ys, xs = np.nonzero(ora.sel[r] >= 0)
+        y, x = int(ys[len(ys) // 2]), int(xs[len(xs) // 2])
+        k = int(ora.sel[r][y, x])
+        d = np.float32(est._cands_list[5])
+        p = [np.float32(d * np.float32(lut[y, x, i])) for i in range(3)]
+        for row in range(3):
+            m = [np.float32(rts_gpu[12 * k + 4 * row + i]) for i in range(3)]
+            q = _fma32(p[2], m[2], _fma32(p[1], m[1], np.float32(p[0] * m[0])))
+            rts_gpu[12 * k + 4 * row + 3] = float(-q)
+            rts_ora[12 * k + 4 * row + 3] = float(-q)
+    else:
+        k = [c for c in range(len(est._cams)) if c != r][0]
+        cams_gpu[k].xi = 20.0
+        cams_ora[k][4] = 20.0
+    cost = torch.empty([H, W, NP], dtype=torch.float32, device=gpu)
+    imgs = [i.contiguous() for i in fr["to_match"]]
+    _lib.check(_lib.lib.ss_sweep(est._raylut[slot].data_ptr(), _lib.ptr_array(imgs), len(imgs), cams_gpu, rts_gpu, r,
+                                 est._cands, N, H, W, 0, cost.data_ptr(), None, _stream()))
+    want = O32.sweep(np.stack([to_np(i) for i in imgs]), ora.sel[r], ora.cands, cams_ora, rts_ora, r, 0)
+    assert_same(np.transpose(to_np(cost)[..., :N], (2, 0, 1)), want, "cost volume (%s)" % case)
+
+
 @pytest.mark.parametrize("vtype", [1, 2])
 @pytest.mark.parametrize("shape,N", [((33, 17), 8), ((48, 64), 32), ((31, 45), 48), ((24, 40), 128)])
 def test_isb_filter_precision_form_bit_exact(gpu, vtype, shape, N):
