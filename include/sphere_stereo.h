@@ -241,19 +241,16 @@ typedef struct {
 } ss_frame_desc;
 
 /* No counterpart in the reference (its per-frame loop is Python).  Creates an
- * executor for desc (copied) on the current device.  Graphs are captured on a
- * stream, and the second group of references runs on another, that every
- * executor of the device shares (created on first use, at default priority,
- * kept until the process ends).  Calls of ss_frame_run on one device are
- * serialised while they enqueue work. */
+ * executor for desc (copied) on the current device.  Graphs are captured on
+ * one stream per device that every executor shares (created on first use, at
+ * default priority, kept until the process ends).  Calls of ss_frame_run on
+ * one device are serialised while they enqueue work. */
 int ss_frame_create(const ss_frame_desc *desc, void **frame);
-/* Enqueues one frame after the work already on `stream` and makes `stream`
- * wait for it.  The references form two groups (even and odd indices; one
- * group when R = 1), each on a stream of its own, and every pass is one
- * launch per group: the sweep (which also writes the guides), each level of
- * the cost filter, the fused argmin, each level of the distance filter, the
- * reprojection, the inpainting passes and the uint8 casts of the stitch
- * images; the merge then runs on `stream`.  d_match: C float32 H x W x 3
+/* Enqueues one frame on `stream`, after the work already there.  Every pass
+ * is one launch for all references: the sweep (which also writes the
+ * guides), each level of the cost filter, the fused argmin, each level of
+ * the distance filter, the reprojection, the inpainting passes, the uint8
+ * casts of the stitch images, then the merge.  d_match: C float32 H x W x 3
  * images; d_stitch: R float32 Hs x Ws x 3 images.  With use_graph, the frame
  * is captured into a HIP graph on the first call (and again whenever the
  * image pointers change) and replayed on `stream` afterwards. */

@@ -1,23 +1,20 @@
 // Per-frame executor: estimate_RGBD_panorama (depth_estimation.py:205-243 and
-// stitcher.py:166-220) as one call.  Each pass runs once for a whole group of
-// references: one launch sweeps every reference of the group (and writes its
-// guide), and each later pass (cost filter levels, fused argmin, distance
-// filter, reprojection, inpainting, uint8 cast) is one launch whose grid
-// covers the group.  The full-size passes then fill the GPU, and the
-// latency-bound coarse-level passes cost one launch per group instead of one
-// per reference.
-//
-// With two or more references, the odd ones run as a second group on a
-// second stream, which waits only for the work that preceded the frame on
-// the caller's stream; the merge waits for both groups.  On the MI355X boxes
-// measured, the command processor drains one queue's kernels for a few
-// hundred microseconds before it serves the other, so the two groups overlap
-// only in part (the second group's sweep overlaps the first group's
-// reprojection and inpainting): 744 us per frame against 765 us with one
-// group on one stream (2 references, 1024^2, 32 candidates).  (Chains per
-// reference that waited for a shared sweep started up to 250 us after their
-// inputs were ready.)  The frame can be captured once into a HIP graph and
-// replayed, which removes the per-launch host cost of its ~60 kernels.
+// stitcher.py:166-220) as one call.  Every pass runs once for all of the
+// frame's references: one sweep launch writes every reference's volume and
+// guide, and each later pass (cost filter levels, fused argmin, distance
+// filter, uint8 cast, reprojection, inpainting) is one launch whose grid
+// covers every reference.  The full-size passes then fill the GPU with the
+// work of all references at once, and the latency-bound coarse-level passes
+// cost one launch per frame instead of one per reference.  (Per-reference
+// chains on separate streams were measured to run one after the other: the
+// second chain started about 250 us after the sweep, well after its work had
+// been queued.  A variant with the references in two groups, each with its
+// own sweep on its own stream, ran 3 % faster, but only because the command
+// processor drained one queue for a few hundred microseconds before it
+// served the other, so the overlap it got was incidental, and timing a
+// kernel with events then counted the waits on the other queue.)  The frame
+// can be captured once into a HIP graph and replayed, which removes the
+// per-launch host cost of its ~30 kernels.
 #include <stdio.h>
 #include <string.h>
 
@@ -28,31 +25,27 @@
 
 namespace {
 
-// The capture stream and the second group's stream, one pair per device,
-// shared by every executor on it, created on first use (default priority)
-// and kept until the process ends.  The mutex serialises the enqueueing
-// (host work only) of frames on the device, and with it the use of both.
+// One capture stream per device, shared by every executor on it, created on
+// first use (default priority) and kept until the process ends.  The mutex
+// serialises the enqueueing (host work only) of frames on the device, and
+// with it the use of the capture stream.
 constexpr int kMaxDevices = 64;
 struct StreamPool {
   std::once_flag once;
   hipError_t err = hipSuccess;
-  hipStream_t capture = nullptr, lane = nullptr;
+  hipStream_t capture = nullptr;
   std::mutex mu;
 };
 StreamPool g_pool[kMaxDevices];
 
 hipError_t pool_init(StreamPool& p) {
-  std::call_once(p.once, [&p] {
-    p.err = hipStreamCreateWithFlags(&p.capture, hipStreamNonBlocking);
-    if (p.err == hipSuccess) p.err = hipStreamCreateWithFlags(&p.lane, hipStreamNonBlocking);
-  });
+  std::call_once(p.once, [&p] { p.err = hipStreamCreateWithFlags(&p.capture, hipStreamNonBlocking); });
   return p.err;
 }
 
 struct Frame {
   ss_frame_desc d;
   StreamPool* pool = nullptr;
-  hipEvent_t start = nullptr, join = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   const float* cap_match[SS_MAX_CAMERAS] = {};
@@ -75,64 +68,32 @@ struct Frame {
     if (rc_ != SS_OK) return rc_; \
   } while (0)
 
-// One half of the references (r = l, l + 2, ...) from sweep to uint8 cast.
-int enqueue_half(const ss_frame_desc& d, int l, int L, const float* const* match, const float* const* stitch,
-                 hipStream_t s) {
-  const float* raylut[SS_MAX_REFERENCES];
-  const float* rts[SS_MAX_REFERENCES];
-  int refs[SS_MAX_REFERENCES];
-  void* ws[SS_MAX_REFERENCES];
-  float* dist[SS_MAX_REFERENCES];
-  const float* cdist[SS_MAX_REFERENCES];
-  float* reproj[SS_MAX_REFERENCES];
-  float* scratch[SS_MAX_REFERENCES];
-  const uint8_t* lut[SS_MAX_REFERENCES];
-  ss_intrinsics intr[SS_MAX_REFERENCES];
-  const float* trans[SS_MAX_REFERENCES];
-  ss::ToU8Args cast[SS_MAX_REFERENCES];
-  int n = 0;
-  for (int r = l; r < d.R; r += L, ++n) {
-    raylut[n] = d.d_raylut[r];
-    rts[n] = d.rts[r];
-    refs[n] = d.refs[r];
-    ws[n] = d.d_workspace[r];
-    dist[n] = d.d_dist[r];
-    cdist[n] = d.d_dist[r];
-    reproj[n] = d.d_reproj[r];
-    scratch[n] = d.d_scratch[r];
-    lut[n] = d.d_inpaint_lut[r];
-    intr[n] = d.intr[r];
-    trans[n] = d.trans[r];
-    cast[n] = ss::ToU8Args{stitch[r], d.d_stitch_u8[r]};
-  }
-  FR_CALL(ss::sweep_references(raylut, match, d.C, d.cams, rts, refs, n, d.cands, d.N, d.H, d.W, d.volume_type, ws,
-                               s));
-  // the fused argmin pass also fills the reprojection targets with 1e8
-  FR_CALL(ss::estimate_after_sweep(n, d.H, d.W, d.N, d.volume_type, d.cands, d.sigma_i, d.sigma_s, dist, ws, s,
-                                   reproj));
-  FR_CALL(ss::reproject_into(n, cdist, reproj, d.H, d.W, intr, trans, false, s));
-  FR_CALL(ss::inpaint_maps(n, reproj, scratch, lut, d.H, d.W, d.inpaint_iterations, d.thr_empty, d.thr_src, s));
-  return ss::to_u8_maps(cast, n, (size_t)d.Hs * d.Ws * 3, s);
-}
-
-// The frame's work on `s` (half of the references on the pool's lane).
+// The frame's work, in order on `s`.
 int enqueue(Frame* f, const float* const* match, const float* const* stitch, hipStream_t s) {
   const ss_frame_desc& d = f->d;
   const int R = d.R;
-  const int L = R >= 2 ? 2 : 1;
-  if (L == 2) {
-    FR_HIP(hipEventRecord(f->start, s));
-    FR_HIP(hipStreamWaitEvent(f->pool->lane, f->start, 0));
-    FR_CALL(enqueue_half(d, 1, L, match, stitch, f->pool->lane));
-    FR_HIP(hipEventRecord(f->join, f->pool->lane));
-  }
-  FR_CALL(enqueue_half(d, 0, L, match, stitch, s));
-  if (L == 2) FR_HIP(hipStreamWaitEvent(s, f->join, 0));
+  const float* rts[SS_MAX_REFERENCES];
   const float* dists[SS_MAX_REFERENCES];
+  const float* trans[SS_MAX_REFERENCES];
+  ss::ToU8Args cast[SS_MAX_REFERENCES];
+  for (int r = 0; r < R; ++r) {
+    rts[r] = d.rts[r];
+    dists[r] = d.d_dist[r];
+    trans[r] = d.trans[r];
+    cast[r] = ss::ToU8Args{stitch[r], d.d_stitch_u8[r]};
+  }
+  FR_CALL(ss::sweep_references(d.d_raylut, match, d.C, d.cams, rts, d.refs, R, d.cands, d.N, d.H, d.W,
+                               d.volume_type, d.d_workspace, s));
+  // the fused argmin pass also fills the reprojection targets with 1e8
+  FR_CALL(ss::estimate_after_sweep(R, d.H, d.W, d.N, d.volume_type, d.cands, d.sigma_i, d.sigma_s, d.d_dist,
+                                   d.d_workspace, s, d.d_reproj));
+  FR_CALL(ss::reproject_into(R, dists, d.d_reproj, d.H, d.W, d.intr, trans, false, s));
+  FR_CALL(ss::inpaint_maps(R, d.d_reproj, d.d_scratch, d.d_inpaint_lut, d.H, d.W, d.inpaint_iterations, d.thr_empty,
+                           d.thr_src, s));
+  FR_CALL(ss::to_u8_maps(cast, R, (size_t)d.Hs * d.Ws * 3, s));
   const float* reproj[SS_MAX_REFERENCES];
   const uint8_t* imgs[SS_MAX_REFERENCES];
   for (int r = 0; r < R; ++r) {
-    dists[r] = d.d_dist[r];
     reproj[r] = d.d_reproj[r];
     imgs[r] = d.d_stitch_u8[r];
   }
@@ -176,11 +137,6 @@ int ss_frame_create(const ss_frame_desc* desc, void** frame) {
   if (!f) return ss::report_error(SS_ERR_ARG, "out of host memory");
   f->d = d;
   f->pool = &g_pool[dev];
-  if (hipEventCreateWithFlags(&f->start, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&f->join, hipEventDisableTiming) != hipSuccess) {
-    ss_frame_destroy(f);
-    return ss::report_error(SS_ERR_HIP, "hipEventCreateWithFlags failed");
-  }
   *frame = f;
   return SS_OK;
 }
@@ -230,8 +186,6 @@ int ss_frame_destroy(void* frame) {
   Frame* f = static_cast<Frame*>(frame);
   if (!f) return SS_OK;
   drop_graph(f);
-  if (f->start) (void)hipEventDestroy(f->start);
-  if (f->join) (void)hipEventDestroy(f->join);
   delete f;
   return SS_OK;
 }
