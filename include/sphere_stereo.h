@@ -36,6 +36,7 @@ extern "C" {
 #define SS_MAX_CAMERAS 8
 #define SS_MAX_CANDIDATES 128
 #define SS_MAX_REFERENCES 8
+#define SS_MAX_BATCH_FRAMES 64   /* frames of one ss_frame_batch_create */
 
 typedef void *ss_stream; /* hipStream_t */
 
@@ -256,6 +257,21 @@ int ss_frame_create(const ss_frame_desc *desc, void **frame);
  * image pointers change) and replayed on `stream` afterwards. */
 int ss_frame_run(void *frame, const float *const *d_match, const float *const *d_stitch, int use_graph,
                  ss_stream stream);
+/* No counterpart in the reference (the batching form; the reference runs one
+ * frame per call).  Creates an executor for nframes frames (descs copied)
+ * that share one configuration: C, R, N, H, W, Hs, Ws, PH, PW, the volume
+ * type, the candidates, sigma_i, sigma_s and the inpainting settings must be
+ * equal; the rigs (cameras, transforms, viewpoint tables), the buffers and the
+ * images are each frame's own.  Every pass after the sweep runs once for a
+ * chunk of frames (as many as fit 16 frame-reference pairs), so the
+ * latency-bound coarse pyramid levels cost one launch per chunk.  Free with
+ * ss_frame_destroy. */
+int ss_frame_batch_create(const ss_frame_desc *descs, int nframes, void **frame);
+/* Enqueues the batch's frames on `stream`, after the work already there.
+ * d_match: nframes * C images (frame-major), d_stitch: nframes * R images,
+ * as ss_frame_run's for each frame; use_graph as ss_frame_run. */
+int ss_frame_batch_run(void *frame, const float *const *d_match, const float *const *d_stitch, int use_graph,
+                       ss_stream stream);
 int ss_frame_destroy(void *frame);
 
 /* Kernel timing ------------------------------------------------------------- */

@@ -1,6 +1,7 @@
 // Per-frame executor: estimate_RGBD_panorama (depth_estimation.py:205-243 and
-// stitcher.py:166-220) as one call.  Every pass runs once for all of the
-// frame's references: one sweep launch writes every reference's volume and
+// stitcher.py:166-220) as one call, for one frame or a batch of frames with
+// one configuration.  Every pass runs once for all of the frame's references
+// (of all the batch's frames, in chunks of up to 16 pairs): one sweep launch writes every reference's volume and
 // guide, and each later pass (cost filter levels, fused argmin, distance
 // filter, uint8 cast, reprojection, inpainting) is one launch whose grid
 // covers every reference.  The full-size passes then fill the GPU with the
@@ -18,8 +19,10 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <new>
+#include <vector>
 
 #include "ss_kernels.h"
 
@@ -43,13 +46,14 @@ hipError_t pool_init(StreamPool& p) {
   return p.err;
 }
 
+// A batch of F frames (F = 1 for ss_frame_create) with one configuration:
+// the rigs, tables, buffers and images of the frames differ.
 struct Frame {
-  ss_frame_desc d;
+  std::vector<ss_frame_desc> d;
   StreamPool* pool = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  const float* cap_match[SS_MAX_CAMERAS] = {};
-  const float* cap_stitch[SS_MAX_REFERENCES] = {};
+  std::vector<const float*> cap_match, cap_stitch;
 };
 
 #define FR_HIP(expr)                                                            \
@@ -68,37 +72,93 @@ struct Frame {
     if (rc_ != SS_OK) return rc_; \
   } while (0)
 
-// The frame's work, in order on `s`.
+// The batch's work, in order on `s`.  Frames go in chunks of as many as fit
+// kMaxBatch (frame, reference) pairs: each frame's sweep (one launch for its
+// references), then every later pass once for the whole chunk, then each
+// frame's merge.  A chunk's volumes are produced and consumed together, so
+// its passes meet them in the last-level cache where they fit.
 int enqueue(Frame* f, const float* const* match, const float* const* stitch, hipStream_t s) {
-  const ss_frame_desc& d = f->d;
-  const int R = d.R;
-  const float* rts[SS_MAX_REFERENCES];
-  const float* dists[SS_MAX_REFERENCES];
-  const float* trans[SS_MAX_REFERENCES];
-  ss::ToU8Args cast[SS_MAX_REFERENCES];
-  for (int r = 0; r < R; ++r) {
-    rts[r] = d.rts[r];
-    dists[r] = d.d_dist[r];
-    trans[r] = d.trans[r];
-    cast[r] = ss::ToU8Args{stitch[r], d.d_stitch_u8[r]};
+  const int F = (int)f->d.size();
+  const ss_frame_desc& c = f->d[0];  // the shared configuration
+  const int R = c.R, C = c.C;
+  const int per = ss::kMaxBatch / R;  // frames per chunk (R <= kMaxBatch)
+  for (int f0 = 0; f0 < F; f0 += per) {
+    const int nf = F - f0 < per ? F - f0 : per, n = nf * R;
+    float* dist[ss::kMaxBatch];
+    const float* cdist[ss::kMaxBatch];
+    void* ws[ss::kMaxBatch];
+    float* reproj[ss::kMaxBatch];
+    float* scratch[ss::kMaxBatch];
+    const uint8_t* lut[ss::kMaxBatch];
+    ss_intrinsics intr[ss::kMaxBatch];
+    const float* trans[ss::kMaxBatch];
+    ss::ToU8Args cast[ss::kMaxBatch];
+    for (int fi = f0; fi < f0 + nf; ++fi) {
+      const ss_frame_desc& d = f->d[fi];
+      const float* rts[SS_MAX_REFERENCES];
+      for (int r = 0; r < R; ++r) {
+        rts[r] = d.rts[r];
+        const int j = (fi - f0) * R + r;
+        dist[j] = d.d_dist[r];
+        cdist[j] = d.d_dist[r];
+        ws[j] = d.d_workspace[r];
+        reproj[j] = d.d_reproj[r];
+        scratch[j] = d.d_scratch[r];
+        lut[j] = d.d_inpaint_lut[r];
+        intr[j] = d.intr[r];
+        trans[j] = d.trans[r];
+        cast[j] = ss::ToU8Args{stitch[(size_t)fi * R + r], d.d_stitch_u8[r]};
+      }
+      FR_CALL(ss::sweep_references(d.d_raylut, match + (size_t)fi * C, C, d.cams, rts, d.refs, R, d.cands, d.N, d.H,
+                                   d.W, d.volume_type, d.d_workspace, s));
+    }
+    // the fused argmin pass also fills the reprojection targets with 1e8
+    FR_CALL(ss::estimate_after_sweep(n, c.H, c.W, c.N, c.volume_type, c.cands, c.sigma_i, c.sigma_s, dist, ws, s,
+                                     reproj));
+    FR_CALL(ss::reproject_into(n, cdist, reproj, c.H, c.W, intr, trans, false, s));
+    FR_CALL(ss::inpaint_maps(n, reproj, scratch, lut, c.H, c.W, c.inpaint_iterations, c.thr_empty, c.thr_src, s));
+    FR_CALL(ss::to_u8_maps(cast, n, (size_t)c.Hs * c.Ws * 3, s));
+    for (int fi = f0; fi < f0 + nf; ++fi) {
+      const ss_frame_desc& d = f->d[fi];
+      const float* rp[SS_MAX_REFERENCES];
+      const float* ds[SS_MAX_REFERENCES];
+      const uint8_t* imgs[SS_MAX_REFERENCES];
+      for (int r = 0; r < R; ++r) {
+        rp[r] = d.d_reproj[r];
+        ds[r] = d.d_dist[r];
+        imgs[r] = d.d_stitch_u8[r];
+      }
+      FR_CALL(ss_merge(d.d_sampling, d.d_weights, rp, ds, imgs, d.Hs, d.Ws, &d.trans[0][0], d.intr, R, d.H, d.W, d.PH,
+                       d.PW, d.d_dist_pano, d.d_rgb_pano, s));
+    }
   }
-  FR_CALL(ss::sweep_references(d.d_raylut, match, d.C, d.cams, rts, d.refs, R, d.cands, d.N, d.H, d.W,
-                               d.volume_type, d.d_workspace, s));
-  // the fused argmin pass also fills the reprojection targets with 1e8
-  FR_CALL(ss::estimate_after_sweep(R, d.H, d.W, d.N, d.volume_type, d.cands, d.sigma_i, d.sigma_s, d.d_dist,
-                                   d.d_workspace, s, d.d_reproj));
-  FR_CALL(ss::reproject_into(R, dists, d.d_reproj, d.H, d.W, d.intr, trans, false, s));
-  FR_CALL(ss::inpaint_maps(R, d.d_reproj, d.d_scratch, d.d_inpaint_lut, d.H, d.W, d.inpaint_iterations, d.thr_empty,
-                           d.thr_src, s));
-  FR_CALL(ss::to_u8_maps(cast, R, (size_t)d.Hs * d.Ws * 3, s));
-  const float* reproj[SS_MAX_REFERENCES];
-  const uint8_t* imgs[SS_MAX_REFERENCES];
-  for (int r = 0; r < R; ++r) {
-    reproj[r] = d.d_reproj[r];
-    imgs[r] = d.d_stitch_u8[r];
-  }
-  return ss_merge(d.d_sampling, d.d_weights, reproj, dists, imgs, d.Hs, d.Ws, &d.trans[0][0], d.intr, R, d.H, d.W,
-                  d.PH, d.PW, d.d_dist_pano, d.d_rgb_pano, s);
+  return SS_OK;
+}
+
+// The checks of one frame descriptor.
+int check_desc(const ss_frame_desc& d) {
+  if (d.C < 1 || d.C > SS_MAX_CAMERAS || d.R < 1 || d.R > SS_MAX_REFERENCES || d.N < 1 || d.N > SS_MAX_CANDIDATES ||
+      d.H < 4 || d.W < 4 || d.Hs < 2 || d.Ws < 2 || d.PH < 1 || d.PW < 1 || d.inpaint_iterations < 0)
+    return ss::report_error(SS_ERR_ARG, "frame descriptor sizes out of range");
+  if (d.volume_type != SS_VOLUME_F32 && d.volume_type != SS_VOLUME_F16 && d.volume_type != SS_VOLUME_BF16)
+    return ss::report_error(SS_ERR_ARG, "frame descriptor: unknown volume type");
+  if (d.workspace_bytes < ss_estimate_workspace_bytes(d.H, d.W, d.N, d.volume_type))
+    return ss::report_error(SS_ERR_WORKSPACE, "frame descriptor: workspace_bytes below ss_estimate_workspace_bytes");
+  for (int r = 0; r < d.R; ++r)
+    if (d.refs[r] < 0 || d.refs[r] >= d.C || !d.d_raylut[r] || !d.d_workspace[r] || !d.d_dist[r] ||
+        !d.d_inpaint_lut[r] || !d.d_reproj[r] || !d.d_scratch[r] || !d.d_stitch_u8[r])
+      return ss::report_error(SS_ERR_ARG, "frame descriptor: bad reference index or null buffer");
+  if (!d.d_sampling || !d.d_weights || !d.d_dist_pano || !d.d_rgb_pano)
+    return ss::report_error(SS_ERR_ARG, "frame descriptor: null stitching table or output");
+  return SS_OK;
+}
+
+// Whether b has a's configuration (everything a batch's frames share).
+bool same_config(const ss_frame_desc& a, const ss_frame_desc& b) {
+  return a.C == b.C && a.R == b.R && a.N == b.N && a.H == b.H && a.W == b.W && a.Hs == b.Hs && a.Ws == b.Ws &&
+         a.PH == b.PH && a.PW == b.PW && a.inpaint_iterations == b.inpaint_iterations &&
+         a.volume_type == b.volume_type && a.sigma_i == b.sigma_i && a.sigma_s == b.sigma_s &&
+         a.thr_empty == b.thr_empty && a.thr_src == b.thr_src && memcmp(a.cands, b.cands, sizeof(float) * a.N) == 0;
 }
 
 int drop_graph(Frame* f) {
@@ -113,41 +173,39 @@ int drop_graph(Frame* f) {
 
 extern "C" {
 
-int ss_frame_create(const ss_frame_desc* desc, void** frame) {
-  if (!desc || !frame) return ss::report_error(SS_ERR_ARG, "null descriptor or output handle");
-  const ss_frame_desc& d = *desc;
-  if (d.C < 1 || d.C > SS_MAX_CAMERAS || d.R < 1 || d.R > SS_MAX_REFERENCES || d.N < 1 || d.N > SS_MAX_CANDIDATES ||
-      d.H < 4 || d.W < 4 || d.Hs < 2 || d.Ws < 2 || d.PH < 1 || d.PW < 1 || d.inpaint_iterations < 0)
-    return ss::report_error(SS_ERR_ARG, "frame descriptor sizes out of range");
-  if (d.volume_type != SS_VOLUME_F32 && d.volume_type != SS_VOLUME_F16 && d.volume_type != SS_VOLUME_BF16)
-    return ss::report_error(SS_ERR_ARG, "frame descriptor: unknown volume type");
-  if (d.workspace_bytes < ss_estimate_workspace_bytes(d.H, d.W, d.N, d.volume_type))
-    return ss::report_error(SS_ERR_WORKSPACE, "frame descriptor: workspace_bytes below ss_estimate_workspace_bytes");
-  for (int r = 0; r < d.R; ++r)
-    if (d.refs[r] < 0 || d.refs[r] >= d.C || !d.d_raylut[r] || !d.d_workspace[r] || !d.d_dist[r] ||
-        !d.d_inpaint_lut[r] || !d.d_reproj[r] || !d.d_scratch[r] || !d.d_stitch_u8[r])
-      return ss::report_error(SS_ERR_ARG, "frame descriptor: bad reference index or null buffer");
-  if (!d.d_sampling || !d.d_weights || !d.d_dist_pano || !d.d_rgb_pano)
-    return ss::report_error(SS_ERR_ARG, "frame descriptor: null stitching table or output");
+int ss_frame_batch_create(const ss_frame_desc* descs, int nframes, void** frame) {
+  if (!descs || !frame) return ss::report_error(SS_ERR_ARG, "null descriptor array or output handle");
+  if (nframes < 1 || nframes > SS_MAX_BATCH_FRAMES)
+    return ss::report_error(SS_ERR_ARG, "frame count outside [1, SS_MAX_BATCH_FRAMES]");
+  for (int i = 0; i < nframes; ++i) {
+    const int rc = check_desc(descs[i]);
+    if (rc != SS_OK) return rc;
+    if (!same_config(descs[0], descs[i]))
+      return ss::report_error(SS_ERR_ARG, "frame descriptors of a batch differ in sizes, volume type, candidates, "
+                                          "sigmas or inpainting settings");
+  }
   int dev = 0;
   FR_HIP(hipGetDevice(&dev));
   if (dev < 0 || dev >= kMaxDevices) return ss::report_error(SS_ERR_UNSUPPORTED, "device index beyond the pool");
   FR_HIP(pool_init(g_pool[dev]));
   Frame* f = new (std::nothrow) Frame();
   if (!f) return ss::report_error(SS_ERR_ARG, "out of host memory");
-  f->d = d;
+  f->d.assign(descs, descs + nframes);
   f->pool = &g_pool[dev];
   *frame = f;
   return SS_OK;
 }
 
-int ss_frame_run(void* frame, const float* const* d_match, const float* const* d_stitch, int use_graph,
-                 ss_stream stream) {
+int ss_frame_create(const ss_frame_desc* desc, void** frame) { return ss_frame_batch_create(desc, 1, frame); }
+
+int ss_frame_batch_run(void* frame, const float* const* d_match, const float* const* d_stitch, int use_graph,
+                       ss_stream stream) {
   Frame* f = static_cast<Frame*>(frame);
   if (!f || !d_match || !d_stitch) return ss::report_error(SS_ERR_ARG, "null frame or image array");
-  for (int k = 0; k < f->d.C; ++k)
+  const size_t nm = f->d.size() * (size_t)f->d[0].C, ns = f->d.size() * (size_t)f->d[0].R;
+  for (size_t k = 0; k < nm; ++k)
     if (!d_match[k]) return ss::report_error(SS_ERR_ARG, "null matching image");
-  for (int r = 0; r < f->d.R; ++r)
+  for (size_t r = 0; r < ns; ++r)
     if (!d_stitch[r]) return ss::report_error(SS_ERR_ARG, "null stitch image");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   std::lock_guard<std::mutex> lk(f->pool->mu);
@@ -159,9 +217,8 @@ int ss_frame_run(void* frame, const float* const* d_match, const float* const* d
   // replay routed through another stream paid two cross-queue waits per
   // frame, measured at ~40 us
   hipStream_t main = f->pool->capture;
-  bool same = f->exec != nullptr;
-  for (int k = 0; k < f->d.C && same; ++k) same = f->cap_match[k] == d_match[k];
-  for (int r = 0; r < f->d.R && same; ++r) same = f->cap_stitch[r] == d_stitch[r];
+  const bool same = f->exec != nullptr && std::equal(d_match, d_match + nm, f->cap_match.begin()) &&
+                    std::equal(d_stitch, d_stitch + ns, f->cap_stitch.begin());
   if (!same) {
     FR_CALL(drop_graph(f));
     FR_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeRelaxed));
@@ -175,11 +232,16 @@ int ss_frame_run(void* frame, const float* const* d_match, const float* const* d
     FR_HIP(e);
     f->graph = g;
     FR_HIP(hipGraphInstantiate(&f->exec, f->graph, nullptr, nullptr, 0));
-    for (int k = 0; k < f->d.C; ++k) f->cap_match[k] = d_match[k];
-    for (int r = 0; r < f->d.R; ++r) f->cap_stitch[r] = d_stitch[r];
+    f->cap_match.assign(d_match, d_match + nm);
+    f->cap_stitch.assign(d_stitch, d_stitch + ns);
   }
   FR_HIP(hipGraphLaunch(f->exec, s));
   return SS_OK;
+}
+
+int ss_frame_run(void* frame, const float* const* d_match, const float* const* d_stitch, int use_graph,
+                 ss_stream stream) {
+  return ss_frame_batch_run(frame, d_match, d_stitch, use_graph, stream);
 }
 
 int ss_frame_destroy(void* frame) {

@@ -309,9 +309,11 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
     float cD[9][CH];
 #pragma unroll
     for (int t = 0; t < 9; ++t) load_n<CH>(cin + (size_t)tap[t] * NP + base_c + cb, cD[t]);
+    // every fine pixel is computed (at the last row or column two of them
+    // coincide, as the clamped indices make them) and only stores are
+    // guarded: straight-line code, no branch per fine pixel
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      if (!doit[s]) continue;
       float o[CH];
       load_n<CH>(cio + (size_t)fine[s] * NP + base_c + cb, o);
 #pragma unroll
@@ -325,7 +327,7 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
 #pragma unroll
         for (int v = 0; v < CH; ++v) res[s][cb + v] = o[v];
       } else {
-        if (active) store_n<CH>(cio + (size_t)fine[s] * NP + base_c + cb, o);
+        if (active && doit[s]) store_n<CH>(cio + (size_t)fine[s] * NP + base_c + cb, o);
       }
     }
   }
@@ -336,7 +338,6 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
     ArgminGroup mine = {};
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      if (!doit[s]) continue;  // uniform within the group: same coarse pixel
       const ArgminGroup g = argmin_reduce_group<G, V>(res[s], lane, a.am);
       if (lane == s) mine = g;
     }

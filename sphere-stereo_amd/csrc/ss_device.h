@@ -21,11 +21,13 @@ namespace ss {
 // The scaling by 2^n is one correctly rounded v_ldexp_f32, so subnormal
 // results keep their value (fp32 denormals are not flushed in this build).
 // ---------------------------------------------------------------------------
+// Branch-free: the polynomial runs on the argument clamped to the finite
+// range (unchanged inside it) and the out-of-range results are selected at
+// the end, so the lanes of a wavefront never diverge here.
 __device__ __forceinline__ float expf_cw(float x) {
-  if (!(x >= -104.0f)) return (x != x) ? x : 0.0f;
-  if (x > 88.7228394f) return __builtin_huge_valf();
-  float n = floorf(x * 1.44269504088896341f + 0.5f);
-  float r = x - n * 0.693359375f;
+  const float xc = fminf(fmaxf(x, -104.0f), 88.7228394f);
+  float n = floorf(xc * 1.44269504088896341f + 0.5f);
+  float r = xc - n * 0.693359375f;
   r = r - n * -2.12194440e-4f;
   float z = r * r;
   float p = 1.9875691500e-4f;
@@ -35,7 +37,9 @@ __device__ __forceinline__ float expf_cw(float x) {
   p = p * r + 1.6666665459e-1f;
   p = p * r + 5.0000001201e-1f;
   p = p * z + r + 1.0f;
-  return ldexpf(p, (int)n);
+  float e = ldexpf(p, (int)n);
+  e = x > 88.7228394f ? __builtin_huge_valf() : e;
+  return x >= -104.0f ? e : ((x != x) ? x : 0.0f);
 }
 
 // ---------------------------------------------------------------------------

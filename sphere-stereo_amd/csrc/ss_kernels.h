@@ -40,11 +40,14 @@ struct IsbUpArgs {
 };
 
 // One launch runs the same pass for up to kMaxBatch references of equal
-// shape (a frame's references): the kernel takes entry blockIdx.y of the
-// table (blockIdx.z for grids of 2-D tiles).  Only the buffers differ between
-// entries, so one launch replaces a chain per reference, and the small
-// coarse-level passes, which cannot fill the GPU, cost one launch per frame.
-constexpr int kMaxBatch = SS_MAX_REFERENCES;
+// shape (a frame's references, or those of a batch of frames): the kernel
+// takes entry blockIdx.y of the table (blockIdx.z for grids of 2-D tiles).
+// Only the buffers differ between entries, so one launch replaces a chain per
+// reference, and the small coarse-level passes, which cannot fill the GPU,
+// cost one launch per batch.  (16 entries keep the largest table, 16 up-pass
+// records, at 1.5 KB of kernel arguments.)
+constexpr int kMaxBatch = 16;
+static_assert(kMaxBatch >= SS_MAX_REFERENCES, "a frame's references fit one launch");
 template <typename A>
 struct Batch {
   A a[kMaxBatch];

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libsphere_stereo.so")
 SS_MAX_CAMERAS = 8
 SS_MAX_CANDIDATES = 128
 SS_MAX_REFERENCES = 8
+SS_MAX_BATCH_FRAMES = 64
 
 # ss_volume_type: element type of the stored cost volume
 SS_VOLUME_F32, SS_VOLUME_F16, SS_VOLUME_BF16 = 0, 1, 2
@@ -76,6 +77,8 @@ SIGNATURES = {
     "ss_error_stats": (_i, [_p, _p, _z, _p, _i, _i, _p, _p, _z, _p]),
     "ss_frame_create": (_i, [C.POINTER(ss_frame_desc), C.POINTER(C.c_void_p)]),
     "ss_frame_run": (_i, [_p, _p, _p, _i, _p]),
+    "ss_frame_batch_create": (_i, [C.POINTER(ss_frame_desc), _i, C.POINTER(C.c_void_p)]),
+    "ss_frame_batch_run": (_i, [_p, _p, _p, _i, _p]),
     "ss_frame_destroy": (_i, [_p]),
     "ss_profile_enable": (_i, [_i, C.c_uint]),
     "ss_profile_collect": (_i, [_p, _p]),

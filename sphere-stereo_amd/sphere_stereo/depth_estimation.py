@@ -91,6 +91,14 @@ class RGBD_Estimator:
         self._frame = self._make_frame()
 
     def _make_frame(self):
+        d = self._frame_desc()
+        handle = _lib.C.c_void_p()
+        _lib.check(_lib.lib.ss_frame_create(_lib.C.byref(d), _lib.C.byref(handle)))
+        return handle
+
+    def _frame_desc(self):
+        """The ss_frame_desc of this estimator's frame (its rig, tables and
+        buffers)."""
         st = self.fishey_stitcher
         d = _lib.ss_frame_desc()
         d.C, d.R, d.N, d.H, d.W = len(self.calibrations), len(self.references_indices), self.candidate_count, \
@@ -121,9 +129,7 @@ class RGBD_Estimator:
             d.d_reproj[slot] = st.reprojected_distances[slot].data_ptr()
             d.d_scratch[slot] = st.inpaint_scratch[slot].data_ptr()
             d.d_stitch_u8[slot] = self._stitch_u8[slot].data_ptr()
-        handle = _lib.C.c_void_p()
-        _lib.check(_lib.lib.ss_frame_create(_lib.C.byref(d), _lib.C.byref(handle)))
-        return handle
+        return d
 
     def __del__(self):
         # at interpreter shutdown the module globals may already be cleared
@@ -194,6 +200,15 @@ class RGBD_Estimator:
             rgb: uint8 [pano rows, pano cols, 3]; distance: float32 [pano rows, pano cols]
             (buffers reused by the next call)
         """
+        imgs, stitch = self._check_inputs(images_to_match, images_to_stitch)
+        self._inputs = (imgs, stitch)  # kept alive until the next call (the frame may still read them)
+        _lib.check(_lib.lib.ss_frame_run(self._frame, _lib.ptr_array(imgs), _lib.ptr_array(stitch),
+                                         int(bool(self.use_graph)), self._stream()))
+        return self.fishey_stitcher.RGB_panorama, self.fishey_stitcher.distance_panorama
+
+    def _check_inputs(self, images_to_match, images_to_stitch):
+        """The frame's images as contiguous HWC tensors, after the checks of
+        estimate_RGBD_panorama."""
         imgs = [_hwc(i) for i in images_to_match]
         stitch = [i.contiguous() for i in images_to_stitch]
         if len(imgs) != len(self.calibrations) or len(stitch) != len(self.references_indices):
@@ -207,7 +222,56 @@ class RGBD_Estimator:
             if t.dtype != torch.float32 or tuple(t.shape) != self._stitch_res + (3,) or t.device != self.device:
                 raise ValueError("images to stitch must be float32 [%d, %d, 3] on %s" % (self._stitch_res + (
                     self.device,)))
-        self._inputs = (imgs, stitch)  # kept alive until the next call (the frame may still read them)
-        _lib.check(_lib.lib.ss_frame_run(self._frame, _lib.ptr_array(imgs), _lib.ptr_array(stitch),
-                                         int(bool(self.use_graph)), self._stream()))
-        return self.fishey_stitcher.RGB_panorama, self.fishey_stitcher.distance_panorama
+        return imgs, stitch
+
+
+class FrameBatch:
+    """The batching form (no counterpart in the reference, which runs one frame
+    per call): several estimators with one configuration (resolutions,
+    candidates, sigmas, volume type, panorama) run their frames together.
+    Their rigs and images may differ.  Every pass after the sweep is one
+    launch for a chunk of frames (up to 16 frame-reference pairs), so the
+    latency-bound coarse pyramid levels and the small stitching passes are
+    paid once per chunk instead of once per frame.  Each estimator keeps its
+    own buffers: after ``estimate_RGBD_panoramas`` its ``fishey_stitcher``
+    holds its frame's panorama, as after ``estimate_RGBD_panorama``.
+    """
+
+    def __init__(self, estimators):
+        self.estimators = list(estimators)
+        F = len(self.estimators)
+        if not 1 <= F <= _lib.SS_MAX_BATCH_FRAMES:
+            raise ValueError("a batch holds 1 to %d estimators" % _lib.SS_MAX_BATCH_FRAMES)
+        devs = {e.device for e in self.estimators}
+        if len(devs) != 1:
+            raise ValueError("the estimators of a batch must share one device")
+        descs = (_lib.ss_frame_desc * F)(*[e._frame_desc() for e in self.estimators])
+        self._frame = None
+        handle = _lib.C.c_void_p()
+        _lib.check(_lib.lib.ss_frame_batch_create(descs, F, _lib.C.byref(handle)))
+        self._frame = handle
+        self.use_graph = False
+        self._inputs = None
+
+    def estimate_RGBD_panoramas(self, frames):
+        """frames: one (images_to_match, images_to_stitch) pair per estimator,
+        as estimate_RGBD_panorama takes them.  Returns one (rgb, distance)
+        pair per estimator (its reused buffers)."""
+        frames = list(frames)
+        if len(frames) != len(self.estimators):
+            raise ValueError("expected %d frames" % len(self.estimators))
+        match, stitch = [], []
+        for e, (m, s) in zip(self.estimators, frames):
+            imgs, st = e._check_inputs(m, s)
+            match += imgs
+            stitch += st
+        self._inputs = (match, stitch)  # kept alive until the next call
+        _lib.check(_lib.lib.ss_frame_batch_run(self._frame, _lib.ptr_array(match), _lib.ptr_array(stitch),
+                                               int(bool(self.use_graph)), self.estimators[0]._stream()))
+        return [(e.fishey_stitcher.RGB_panorama, e.fishey_stitcher.distance_panorama) for e in self.estimators]
+
+    def __del__(self):
+        lib = getattr(_lib, "lib", None)
+        if getattr(self, "_frame", None) and lib is not None:
+            lib.ss_frame_destroy(self._frame)
+            self._frame = None

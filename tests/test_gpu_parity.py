@@ -12,7 +12,7 @@ import torch
 from helpers import first_mismatch, frame, to_np
 from oracle import Oracle
 from pipeline import OraclePipeline
-from sphere_stereo import ISB_Filter, RGBD_Estimator, _lib
+from sphere_stereo import FrameBatch, ISB_Filter, RGBD_Estimator, _lib
 from sphere_stereo.utils import camera_array, distance_candidates, relative_transforms
 
 pytestmark = pytest.mark.gpu
@@ -353,3 +353,43 @@ def test_graph_replay_matches_eager(small):
         assert torch.equal(rgb, rgb0) and torch.equal(dist.view(torch.int32), dist0.view(torch.int32))
     finally:
         est.use_graph = False
+
+
+@pytest.mark.parametrize("volume_dtype", ["float32", "float16"])
+def test_frame_batch_matches_single_frames(gpu, volume_dtype):
+    """Nine rigs (own seeds and baselines, one configuration) through the
+    batching form: two chunks (8 + 1 frames of 2 references), eager and
+    replayed from a graph, each frame bit-exact with its single-frame run."""
+    ests, frames, want, kws = [], [], [], []
+    for seed in range(9):
+        fr = frame(seed=10 + seed, resolution=160, matching=(80, 72), stitch=(96, 96), refs=(0, 2), device=gpu,
+                   baseline=0.1 + 0.02 * seed)
+        kw = dict(min_dist=0.55, max_dist=100, candidate_count=32, references_indices=fr["refs"],
+                  reprojection_viewpoint=fr["viewpoint"], masks=fr["masks"], matching_resolution=fr["matching"],
+                  rgb_to_stitch_resolution=fr["stitch"], panorama_resolution=(128, 64), sigma_i=10, sigma_s=12.5)
+        est = RGBD_Estimator(fr["calibs"], device=gpu, volume_dtype=volume_dtype, **kw)
+        rgb, dist = est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])
+        want.append((rgb.clone(), dist.clone()))
+        kws.append((fr["calibs"], kw))
+        ests.append(est)
+        frames.append(([t.contiguous() for t in fr["to_match"]], [t.contiguous() for t in fr["to_stitch"]]))
+    for e in ests:  # clear the outputs so that stale values cannot pass
+        e.fishey_stitcher.RGB_panorama.zero_()
+        e.fishey_stitcher.distance_panorama.zero_()
+    batch = FrameBatch(ests)
+    for mode in ("eager", "graph", "graph", "graph"):
+        batch.use_graph = mode == "graph"
+        out = batch.estimate_RGBD_panoramas(frames)
+        torch.cuda.synchronize()
+        for i, ((rgb, dist), (wr, wd)) in enumerate(zip(out, want)):
+            assert torch.equal(rgb, wr), "%s rgb panorama of frame %d" % (mode, i)
+            assert torch.equal(dist.view(torch.int32), wd.view(torch.int32)), "%s distance panorama %d" % (mode, i)
+        junk = torch.randn(1 << 20, device=gpu).sum().item()  # eager work between replays
+        assert junk == junk
+        for e in ests:
+            e.fishey_stitcher.distance_panorama.fill_(-1.0)
+    # frames of a batch share one configuration
+    calibs, kw = kws[1]
+    other = RGBD_Estimator(calibs, device=gpu, volume_dtype=volume_dtype, **dict(kw, candidate_count=16))
+    with pytest.raises(_lib.SphereStereoError, match="differ"):
+        FrameBatch([ests[0], other])
