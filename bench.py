@@ -8,7 +8,9 @@ sigma_i 10, sigma_s 25, 1216x1216 stitch input, 2048x1024 RGB + float32
 distance panorama, fp32 (the reference's type).  One step = one
 estimate_RGBD_panorama call on inputs already resident in HBM.  The same
 frames are then timed with the cost volume stored in fp16 and bf16
-("precision_forms") and with HIP-graph replay ("graph_replay").
+("precision_forms"), with HIP-graph replay ("graph_replay"), and in the
+batching form: --batch frames (default 8) from as many seeded rigs of the
+same configuration, fp32 and fp16 volumes ("batching_form").
 
 Prints one JSON line (see the roofline / cpu_baseline fields) and a per-stage
 split on stderr.
@@ -105,6 +107,45 @@ def setup(wl, device, frames_cache=None, build_estimator=True):
               sigma_s=wl["sigma_s"])
     est = RGBD_Estimator(calibs, device=device, **kw) if build_estimator else None
     return est, to_match, to_stitch, calibs, kw
+
+
+def batching_form(args, wl, device, plain_fp32):
+    """The batching form: args.batch frames of the workload's configuration,
+    each from its own seeded rig (seed, seed + 1, ...), through FrameBatch
+    (every pass after the sweep one launch per chunk of 16 frame-reference
+    pairs), timed like the plain form; fp32 and fp16 volumes.  value =
+    panoramas/s over all frames of the batch."""
+    import torch
+
+    from sphere_stereo import FrameBatch, RGBD_Estimator
+
+    sets = []
+    for b in range(args.batch):
+        wb = dict(wl, seed=wl["seed"] + b)
+        cache = "%s.rig%d" % (args.frames_cache, b) if args.frames_cache and b else args.frames_cache
+        _, tm, ts, calibs, kw = setup(wb, device, cache, build_estimator=False)
+        sets.append(([t.contiguous() for t in tm], [t.contiguous() for t in ts], calibs, kw))
+    out = {"frames_per_batch": args.batch, "rigs": "seeds %d..%d" % (wl["seed"], wl["seed"] + args.batch - 1)}
+    for dt in ("float32", "float16"):
+        ests = [RGBD_Estimator(c, device=device, volume_dtype=dt, **kw) for (_, _, c, kw) in sets]
+        batch = FrameBatch(ests)
+        frames = [(tm, ts) for (tm, ts, _, _) in sets]
+        for _ in range(args.warmup):
+            batch.estimate_RGBD_panoramas(frames)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            batch.estimate_RGBD_panoramas(frames)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        v = args.steps * args.batch / el
+        out[dt] = {"value": round(v, 3), "unit": "panoramas/s", "ms_per_batch": round(1e3 * el / args.steps, 4),
+                   "vs_plain_fp32": round(v / plain_fp32, 4)}
+        print("[bench] batching form, %d frames, %s volume: %.1f panoramas/s" % (args.batch, dt, v), file=sys.stderr,
+              flush=True)
+        del batch, ests
+        torch.cuda.empty_cache()
+    return out
 
 
 def profile(lib, est, to_match, to_stitch, frames, mask):
@@ -227,6 +268,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--frames-cache", default=None, help="reuse rendered fisheyes saved at this path")
+    ap.add_argument("--batch", type=int, default=8,
+                    help="frames per batch of the batching form (0: skip it); each from its own seeded rig")
     ap.add_argument("--precision-report", action="store_true",
                     help="instead of the throughput line: inverse-distance error of the fp32/fp16/bf16 volume forms "
                          "against the fp64 oracle (use --workload c2)")
@@ -321,6 +364,8 @@ def main():
         print("[bench] %s volume: %.1f panoramas/s" % (dt, args.steps / h_el), file=sys.stderr, flush=True)
         del est_h
 
+    batching = batching_form(args, wl, device, value) if args.batch > 0 else None
+
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
     if os.path.exists(tpath):
@@ -354,6 +399,7 @@ def main():
         "graph_replay": {"value": round(args.steps / g_elapsed, 3), "unit": "panoramas/s",
                          "ms_per_step": round(1e3 * g_elapsed / args.steps, 4)},
         "precision_forms": forms,
+        "batching_form": batching,
     }
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
