@@ -191,6 +191,23 @@ int ss_merge(const float *d_sampling, const float *d_weights, const float *const
 /* Truncating float32 -> uint8 cast of n values (depth_estimation.py:239-240). */
 int ss_to_u8(const float *d_in, uint8_t *d_out, size_t n, ss_stream stream);
 
+/* Input stage ----------------------------------------------------------------- */
+
+/* Element types of a decoded input image (ss_resize_linear). */
+typedef enum { SS_INPUT_U8 = 0, SS_INPUT_U16 = 1, SS_INPUT_F32 = 2 } ss_input_type;
+
+/* Replaces read_input_images' per-image preparation (utils.py:157-204): the
+ * HWC image (channels 1..4) converted to float32 in [0, 255] (uint8 as is,
+ * uint16 / 255 as utils.py:176; a float32 image is taken as already in
+ * [0, 255]) and resized to Hd x Wd as its cv2.resize(image, (cols, rows),
+ * cv2.INTER_AREA) calls do (utils.py:198-201; that flag lands in the dst
+ * slot, so the interpolation is INTER_LINEAR: half-pixel centres, source
+ * coordinate (d + 0.5) * (S / D) - 0.5 in double rounded to float, clamped
+ * to the first / last sample, a horizontal then a vertical a*w0 + b*w1).
+ * d_dst: Hd x Wd x channels float32. */
+int ss_resize_linear(const void *d_src, int src_type, int Hs, int Ws, int channels, float *d_dst, int Hd, int Wd,
+                     ss_stream stream);
+
 /* Output conversion and evaluation -------------------------------------------- */
 
 /* 1 / distance (main.py:114): the inverse-distance map the reference saves and

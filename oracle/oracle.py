@@ -202,6 +202,17 @@ class Oracle:
                                      _d(thr_empty), _d(thr_src))
         return dout
 
+    def resize_linear(self, img, cols, rows):
+        """The input stage's resize (utils.py:198-201) of an HWC or HW image
+        already converted to float32 in [0, 255]."""
+        img = np.ascontiguousarray(img, np.float32)
+        hw = img.ndim == 2
+        x = img[..., None] if hw else img
+        Hs, Ws, Cn = x.shape
+        out = np.zeros((rows, cols, Cn), self.real)
+        self._fn("ora_resize_linear")(_ptr(x), _f(Hs), _f(Ws), _f(Cn), _ptr(out), _f(rows), _f(cols))
+        return out[..., 0] if hw else out
+
     def inpaint(self, d, lut, iterations, thr_empty, thr_src):
         for _ in range(iterations):
             d = self.inpaint_step(d, lut, thr_empty, thr_src)
@@ -270,4 +281,5 @@ _SIGNATURES = {
     "ora_blend_lut": [_p, _p, _p, _f, _f, _f, _f, _f, _p, _p, _p, _d, _d],
     "ora_normalize_weights": [_p, _f, _f],
     "ora_merge": [_p, _p, _p, _p, _p, _f, _f, _p, _p, _f, _f, _f, _f, _f, _p, _p],
+    "ora_resize_linear": [_p, _f, _f, _f, _p, _f, _f],
 }

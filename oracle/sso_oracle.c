@@ -35,6 +35,11 @@
  *   stitcher functions - closed forms and invariants (identity reprojection,
  *                      inpaint of isolated holes, LUT packing round trip, blend
  *                      weights summing to 1, merge of constant maps).
+ *   ora_resize_linear - PARITY UNPINNED against cv2 (absent here, and its
+ *                      float resize varies with the platform's FMA use):
+ *                      closed forms (identity size, 2:1 block means, constant
+ *                      and ramp images, border clamps) and agreement with
+ *                      torch's bilinear interpolation within float tolerance.
  */
 #include <math.h>
 #include <stdint.h>
@@ -941,4 +946,52 @@ void SFX(ora_merge)(const real *sampling, const real *weights, const real *repro
         for (int k = 0; k < 3; k++) rgb_out[3 * idx + k] = (uint8_t)rgb[k];
     }
     free(c);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Input stage (utils.py:157-204): the image as float32 in [0, 255], then     */
+/* cv2.resize(image, (Wd, Hd), cv2.INTER_AREA), whose flag lands in the dst   */
+/* slot, i.e. INTER_LINEAR: for each destination coordinate d, f = (float)    */
+/* ((d + 0.5) * (1 / (D / S)) - 0.5) in double, s = floor(f), f -= s, clamped */
+/* to the first / last sample (f = 0 there), weights 1 - f and f; then a      */
+/* horizontal pass and a vertical pass, each a * w0 + b * w1.                 */
+/* src: Hs x Ws x C float32 (already converted), dst: Hd x Wd x C.            */
+/* ------------------------------------------------------------------------ */
+static void resize_axis(int d, int S, int D, int *s0, int *s1, real *w0, real *w1)
+{
+    double scale = 1.0 / ((double)D / (double)S);
+    real f = (real)((d + 0.5) * scale - 0.5);
+    int s = (int)RFLOOR(f);
+    f = f - (real)s;
+    if (s < 0) {
+        f = 0;
+        s = 0;
+    }
+    if (s >= S - 1) {
+        f = 0;
+        s = S - 1;
+    }
+    *s0 = s;
+    *s1 = s + 1 < S ? s + 1 : S - 1;
+    *w0 = K(1.0) - f;
+    *w1 = f;
+}
+
+void SFX(ora_resize_linear)(const float *src, int Hs, int Ws, int C, real *dst, int Hd, int Wd)
+{
+    for (int dy = 0; dy < Hd; dy++) {
+        int y0, y1;
+        real b0, b1;
+        resize_axis(dy, Hs, Hd, &y0, &y1, &b0, &b1);
+        for (int dx = 0; dx < Wd; dx++) {
+            int x0, x1;
+            real a0, a1;
+            resize_axis(dx, Ws, Wd, &x0, &x1, &a0, &a1);
+            for (int c = 0; c < C; c++) {
+                real h0 = (real)src[((size_t)y0 * Ws + x0) * C + c] * a0 + (real)src[((size_t)y0 * Ws + x1) * C + c] * a1;
+                real h1 = (real)src[((size_t)y1 * Ws + x0) * C + c] * a0 + (real)src[((size_t)y1 * Ws + x1) * C + c] * a1;
+                dst[((size_t)dy * Wd + dx) * C + c] = h0 * b0 + h1 * b1;
+            }
+        }
+    }
 }

@@ -778,6 +778,19 @@ int ss_to_u8(const float* d_in, uint8_t* d_out, size_t n, ss_stream stream) {
   return rc ? rc : ok();
 }
 
+int ss_resize_linear(const void* d_src, int src_type, int Hs, int Ws, int channels, float* d_dst, int Hd, int Wd,
+                     ss_stream stream) {
+  SS_REQUIRE(d_src && d_dst, "null pointer argument");
+  SS_REQUIRE(src_type == SS_INPUT_U8 || src_type == SS_INPUT_U16 || src_type == SS_INPUT_F32, "unknown input type %d",
+             src_type);
+  SS_REQUIRE(channels >= 1 && channels <= 4, "channel count %d outside [1, 4]", channels);
+  SS_REQUIRE(Hs >= 1 && Ws >= 1 && Hd >= 1 && Wd >= 1, "bad sizes %dx%d -> %dx%d", Ws, Hs, Wd, Hd);
+  SS_REQUIRE((long long)Hd * Wd <= 0x7fffffffLL && (long long)Hs * Ws * channels <= 0x7fffffffLL,
+             "image too large");
+  SS_HIP(ss::launch_resize_linear(d_src, src_type, Hs, Ws, channels, d_dst, Hd, Wd, as_stream(stream)));
+  return ok();
+}
+
 int ss_inverse_distance(const float* d_dist, float* d_inv, size_t n, ss_stream stream) {
   SS_REQUIRE(d_dist && d_inv, "null pointer argument");
   if (n == 0) return ok();

@@ -210,6 +210,10 @@ hipError_t launch_normalize(float* w, int R, int P, hipStream_t s);
 hipError_t launch_merge(const MergeArgs& a, hipStream_t s);
 hipError_t launch_to_u8(const ToU8Args* a, int nb, size_t n, hipStream_t s);
 
+// input.hip
+hipError_t launch_resize_linear(const void* src, int src_type, int Hs, int Ws, int C, float* dst, int Hd, int Wd,
+                                hipStream_t s);
+
 // eval.hip
 hipError_t launch_inverse(const float* d, float* inv, size_t n, hipStream_t s);
 int error_stats_blocks(size_t n);

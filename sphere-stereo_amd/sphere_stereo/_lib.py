@@ -13,6 +13,7 @@ SS_MAX_CAMERAS = 8
 SS_MAX_CANDIDATES = 128
 SS_MAX_REFERENCES = 8
 SS_MAX_BATCH_FRAMES = 64
+SS_INPUT_U8, SS_INPUT_U16, SS_INPUT_F32 = 0, 1, 2
 
 # ss_volume_type: element type of the stored cost volume
 SS_VOLUME_F32, SS_VOLUME_F16, SS_VOLUME_BF16 = 0, 1, 2
@@ -75,6 +76,7 @@ SIGNATURES = {
     "ss_inverse_distance": (_i, [_p, _p, _z, _p]),
     "ss_error_stats_workspace_bytes": (_z, [_z]),
     "ss_error_stats": (_i, [_p, _p, _z, _p, _i, _i, _p, _p, _z, _p]),
+    "ss_resize_linear": (_i, [_p, _i, _i, _i, _i, _p, _i, _i, _p]),
     "ss_frame_create": (_i, [C.POINTER(ss_frame_desc), C.POINTER(C.c_void_p)]),
     "ss_frame_run": (_i, [_p, _p, _p, _i, _p]),
     "ss_frame_batch_create": (_i, [C.POINTER(ss_frame_desc), _i, C.POINTER(C.c_void_p)]),

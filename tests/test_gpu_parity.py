@@ -393,3 +393,23 @@ def test_frame_batch_matches_single_frames(gpu, volume_dtype):
     other = RGBD_Estimator(calibs, device=gpu, volume_dtype=volume_dtype, **dict(kw, candidate_count=16))
     with pytest.raises(_lib.SphereStereoError, match="differ"):
         FrameBatch([ests[0], other])
+
+
+@pytest.mark.parametrize("dtype", [torch.uint8, torch.uint16, torch.float32])
+@pytest.mark.parametrize("shape,size", [((1216, 1216, 3), (1024, 1024)), ((1216, 1216, 3), (512, 512)),
+                                        ((100, 90, 3), (137, 121)), ((7, 9, 1), (3, 4)), ((64, 48), (64, 48))])
+def test_resize_linear_bit_exact(gpu, dtype, shape, size):
+    """The input stage (utils.py:157-204) on the GPU against the oracle: the
+    conversion to float32 in [0, 255] and the INTER_LINEAR resize."""
+    from sphere_stereo.utils import resize_linear
+
+    rows, cols = size
+    g = torch.Generator().manual_seed(sum(shape) + rows)
+    hi = {torch.uint8: 256, torch.uint16: 65536, torch.float32: 256}[dtype]
+    img = torch.randint(0, hi, shape, generator=g, dtype=torch.int64)
+    img = img.to(torch.float32) if dtype == torch.float32 else img.to(dtype)
+    got = resize_linear(img.to(gpu), cols, rows)
+    f = img.to(torch.float32).numpy()
+    if dtype == torch.uint16:
+        f = f / np.float32(255)  # utils.py:176, a float32 division
+    assert_same(to_np(got), O32.resize_linear(f.astype(np.float32), cols, rows), "resized %s %s" % (dtype, shape))

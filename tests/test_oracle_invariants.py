@@ -444,3 +444,46 @@ def test_merge_of_constant_maps_and_images():
     dist, rgb = O32.merge(samp, w, maps, maps, imgs, trans, intr)
     assert np.abs(dist - 5.0).max() < 5e-6
     assert rgb.min() >= 99 and rgb.max() <= 100
+
+
+def test_resize_linear_closed_forms():
+    """The input stage's cv2 INTER_LINEAR resize (utils.py:198-201): same size
+    is the identity; 2:1 takes means of 2 x 2 blocks (weights 1/2 exactly);
+    a constant stays constant; upscaled borders repeat the edge samples; a
+    ramp along x is reproduced at the mapped source coordinates."""
+    rng = np.random.RandomState(0)
+    x = (rng.rand(17, 23, 3) * 255).astype(np.float32)
+    assert np.array_equal(O32.resize_linear(x, 23, 17), x)
+    y = (rng.rand(12, 16, 3) * 255).astype(np.float32)
+    h = y[:, 0::2] * np.float32(0.5) + y[:, 1::2] * np.float32(0.5)
+    want = h[0::2] * np.float32(0.5) + h[1::2] * np.float32(0.5)
+    assert np.array_equal(O32.resize_linear(y, 8, 6), want)
+    c = np.full((31, 29), 87.0, np.float32)
+    for cols, rows in ((12, 17), (50, 40), (29, 31)):
+        out = O32.resize_linear(c, cols, rows)
+        assert np.abs(out - 87.0).max() <= 87.0 * 2 ** -22
+    up = O32.resize_linear(y, 64, 48)
+    assert np.array_equal(up[:, 0], up[:, 1]) and np.array_equal(up[0], up[1])  # (d+0.5)/4-0.5 < 0 for d <= 1
+    assert np.array_equal(up[:, -1], up[:, -2])
+    ramp = np.tile(np.arange(40, dtype=np.float32) * 3.0, (5, 1))
+    out = O64.resize_linear(ramp, 25, 5)
+    src_x = (np.arange(25) + 0.5) * (40 / 25) - 0.5
+    assert np.abs(out[2] - 3.0 * np.clip(src_x, 0, 39)).max() < 1e-6
+
+
+def test_resize_linear_matches_torch_bilinear():
+    """torch's bilinear interpolation (align_corners=False) maps coordinates
+    the same way; it forms the scale and source coordinate in float32 instead
+    of cv2's double, which moves a coordinate by up to ~1e-5 pixel, i.e. the
+    values by up to ~1e-5 x the local gradient (<= 255 here).  A wrong mapping
+    (pixel corners instead of centres, no clamp) is off by tens."""
+    import torch
+    import torch.nn.functional as F
+
+    rng = np.random.RandomState(1)
+    for (h, w), (rows, cols) in (((96, 96), (80, 80)), ((61, 47), (23, 37)), ((20, 30), (53, 71))):
+        x = (rng.rand(h, w, 3) * 255).astype(np.float32)
+        t = F.interpolate(torch.from_numpy(x).permute(2, 0, 1)[None], size=(rows, cols), mode="bilinear",
+                          align_corners=False, antialias=False)[0].permute(1, 2, 0).numpy()
+        d = np.abs(O32.resize_linear(x, cols, rows) - t)
+        assert d.max() < 1e-2 and d.mean() < 1e-3
