@@ -8,7 +8,8 @@ from . import _lib  # noqa: F401  (fails loudly if the library is not built)
 from .depth_estimation import FrameBatch, RGBD_Estimator
 from .isb_filter import ISB_Filter
 from .stitcher import Stitcher, vectorize_calibration
+from .streaming import StreamingDriver
 from .utils import Calibration, parse_json_calib, project, rgb2yCbCr, unproject
 
-__all__ = ["RGBD_Estimator", "FrameBatch", "ISB_Filter", "Stitcher", "vectorize_calibration", "Calibration", "parse_json_calib",
+__all__ = ["RGBD_Estimator", "FrameBatch", "ISB_Filter", "Stitcher", "StreamingDriver", "vectorize_calibration", "Calibration", "parse_json_calib",
            "project", "unproject", "rgb2yCbCr"]

@@ -158,33 +158,41 @@ def distance_candidates(min_dist, max_dist, candidate_count):
 _INPUT_TYPES = {torch.uint8: _lib.SS_INPUT_U8, torch.uint16: _lib.SS_INPUT_U16, torch.float32: _lib.SS_INPUT_F32}
 
 
-def resize_linear(image, cols, rows, stream=None):
+def resize_linear(image, cols, rows, stream=None, out=None):
     """One image of read_input_images (utils.py:174-177, 198-201) on the GPU:
     a decoded HWC (or HW) uint8 / uint16 / float32 device tensor converted to
     float32 in [0, 255] (uint16 / 255; float32 taken as already in [0, 255])
     and resized to rows x cols as cv2.resize(..., cv2.INTER_AREA) does there
-    (INTER_LINEAR, that flag being in the dst slot).  Returns a new float32
-    tensor (rows, cols[, C])."""
+    (INTER_LINEAR, that flag being in the dst slot).  Returns a float32 tensor
+    (rows, cols[, C]): `out` when given (contiguous, that shape), else new."""
     if image.dtype not in _INPUT_TYPES:
         raise ValueError("input images must be uint8, uint16 or float32")
     if image.dim() not in (2, 3) or (image.dim() == 3 and not 1 <= image.shape[2] <= 4):
         raise ValueError("input images must be HW or HWC with 1 to 4 channels")
     src = image.contiguous()
     C = 1 if src.dim() == 2 else src.shape[2]
-    out = torch.empty((rows, cols) if src.dim() == 2 else (rows, cols, C), dtype=torch.float32, device=src.device)
+    shape = (rows, cols) if src.dim() == 2 else (rows, cols, C)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float32, device=src.device)
+    elif tuple(out.shape) != shape or out.dtype != torch.float32 or not out.is_contiguous() or out.device != src.device:
+        raise ValueError("out must be a contiguous float32 tensor of shape %s on %s" % (shape, src.device))
     st = _lib.C.c_void_p(stream.cuda_stream) if stream is not None else _lib.stream_of(out)
     _lib.check(_lib.lib.ss_resize_linear(src.data_ptr(), _INPUT_TYPES[src.dtype], src.shape[0], src.shape[1], C,
                                          out.data_ptr(), rows, cols, st))
     return out
 
 
-def prepare_inputs(images, matching_resolution, rgb_to_stitch_resolution, references_indices, stream=None):
+def prepare_inputs(images, matching_resolution, rgb_to_stitch_resolution, references_indices, stream=None,
+                   out=None):
     """read_input_images (utils.py:157-204) after decoding, on the GPU: the C
     decoded fisheyes (device tensors, as cv2.imread gives them: HWC uint8 /
     uint16 / float32) -> (images_to_match, images_to_stitch), float32 HWC in
-    [0, 255] at the matching and stitch resolutions ((cols, rows) pairs)."""
+    [0, 255] at the matching and stitch resolutions ((cols, rows) pairs).
+    `out`, a previous result, is written in place (stable buffers let a
+    captured frame graph be replayed)."""
     mc, mr = matching_resolution
     sc, sr = rgb_to_stitch_resolution
-    to_match = [resize_linear(img, mc, mr, stream) for img in images]
-    to_stitch = [resize_linear(images[r], sc, sr, stream) for r in references_indices]
+    om, os_ = out if out is not None else ([None] * len(images), [None] * len(references_indices))
+    to_match = [resize_linear(img, mc, mr, stream, o) for img, o in zip(images, om)]
+    to_stitch = [resize_linear(images[r], sc, sr, stream, o) for r, o in zip(references_indices, os_)]
     return to_match, to_stitch

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import first_mismatch, frame, to_np
+from helpers import first_mismatch, frame, rendered, to_np
 from oracle import Oracle
 from pipeline import OraclePipeline
 from sphere_stereo import FrameBatch, ISB_Filter, RGBD_Estimator, _lib
@@ -413,3 +413,35 @@ def test_resize_linear_bit_exact(gpu, dtype, shape, size):
     if dtype == torch.uint16:
         f = f / np.float32(255)  # utils.py:176, a float32 division
     assert_same(to_np(got), O32.resize_linear(f.astype(np.float32), cols, rows), "resized %s %s" % (dtype, shape))
+
+
+def test_streaming_driver_matches_frame_by_frame(gpu):
+    """The streaming driver (double-buffered uploads, GPU input stage, graph
+    replay, double-buffered downloads) over five frames of decoded uint8
+    fisheyes gives, in order, exactly the panoramas of the frame-by-frame
+    path (input stage then estimate_RGBD_panorama, then 1 / distance)."""
+    from sphere_stereo import StreamingDriver
+    from sphere_stereo.evaluation import inverse_distance
+    from sphere_stereo.utils import prepare_inputs
+
+    fr = frame(seed=3, resolution=160, matching=(80, 72), stitch=(96, 96), refs=(0, 2), device=gpu, baseline=0.12)
+    _, imgs, _ = rendered(3, 160, 0.12)
+    kw = dict(min_dist=0.55, max_dist=100, candidate_count=32, references_indices=fr["refs"],
+              reprojection_viewpoint=fr["viewpoint"], masks=fr["masks"], matching_resolution=fr["matching"],
+              rgb_to_stitch_resolution=fr["stitch"], panorama_resolution=(128, 64), sigma_i=10, sigma_s=12.5)
+    est = RGBD_Estimator(fr["calibs"], device=gpu, **kw)
+    raws = [[np.ascontiguousarray(np.roll(imgs[k].numpy(), 5 * i, axis=1)) for k in range(imgs.shape[0])]
+            for i in range(5)]
+    want = []
+    for raw in raws:
+        tm, ts = prepare_inputs([torch.from_numpy(a).to(gpu) for a in raw], fr["matching"], fr["stitch"], fr["refs"])
+        rgb, dist = est.estimate_RGBD_panorama(tm, ts)
+        want.append((to_np(rgb).copy(), to_np(inverse_distance(dist)).copy()))
+    assert not np.array_equal(want[0][0], want[1][0])
+    drv = StreamingDriver(est, raws[0][0].shape, np.uint8, use_graph=True)
+    got = list(drv.run(raws))
+    assert [g[0] for g in got] == list(range(5))
+    for (i, rgb, inv), (wr, wi) in zip(got, want):
+        assert np.array_equal(rgb, wr), "rgb panorama of frame %d" % i
+        assert np.array_equal(inv.view(np.int32), wi.view(np.int32)), "inverse distance of frame %d" % i
+    est.use_graph = False
