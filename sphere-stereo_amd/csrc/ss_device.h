@@ -156,6 +156,31 @@ __device__ __forceinline__ float grid_coord(float u, float size) {
   return ((g + 1.0f) * size - 1.0f) / 2.0f;
 }
 
+// grid_coord with the division by a power-of-two size done as an exact
+// multiply by its reciprocal (a compile-time choice, so that the other
+// form is not evaluated as well).
+template <bool POW2>
+__device__ __forceinline__ float grid_coord2(float u, float size, float inv) {
+  const float t = u + 0.5f;
+  const float q = POW2 ? t * inv : t / size;
+  const float g = q * 2.0f - 1.0f;
+  return ((g + 1.0f) * size - 1.0f) * 0.5f;
+}
+
+// grid_coord2 in four operations for a power-of-two size, with inv2 = 2/size
+// and half = size/2 (both exact).  Each step equals the corresponding steps
+// of the long form exactly: t*(2/size) is the exact product 2*(t/size), so
+// fma(t, inv2, -1) = RN(2*(t/size) - 1); (g+1)*size is exact, so
+// fma(s, half, -0.5) = RN((s*size - 1)/2).  (Where t/size would be a
+// subnormal, both forms give g = -1.  Where s*size would overflow, both
+// coordinates fall outside the image and sample zero.)
+__device__ __forceinline__ float grid_coord_pow2(float u, float inv2, float half) {
+  const float t = u + 0.5f;
+  const float g = __builtin_fmaf(t, inv2, -1.0f);
+  const float s = g + 1.0f;
+  return __builtin_fmaf(s, half, -0.5f);
+}
+
 // ---------------------------------------------------------------------------
 // Double-sphere camera, stitcher form (stitcher.cu:121-168): fp32 throughout,
 // no clamp in unproject; float2/float divides by reciprocal multiply.
