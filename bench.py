@@ -10,7 +10,11 @@ estimate_RGBD_panorama call on inputs already resident in HBM.  The same
 frames are then timed with the cost volume stored in fp16 and bf16
 ("precision_forms"), with HIP-graph replay ("graph_replay"), and in the
 batching form: --batch frames (default 8) from as many seeded rigs of the
-same configuration, fp32 and fp16 volumes ("batching_form").
+same configuration, fp32 and fp16 volumes ("batching_form").  Last, the
+differentiation form (4 frames of one rig at 768^2, 48 candidates): forward
+and forward + backward time per batch and a 50-step Gauss-Newton calibration
+refinement ("differentiation_form"; --no-diff-form skips it).
+--diff-report checks its gradient against fp64 central differences instead.
 
 Prints one JSON line (see the roofline / cpu_baseline fields) and a per-stage
 split on stderr.
@@ -148,6 +152,159 @@ def batching_form(args, wl, device, plain_fp32):
     return out
 
 
+DIFF_FORM = {"matching": (768, 768), "N": 48, "frames": 4, "refs": (0, 2), "sigma_i": 10.0, "sigma_s": 12.5,
+             "temperature": 0.05}
+
+
+def diff_setup(device):
+    """The differentiation form's batch: 4 frames (scene seeds 0..3) of the
+    config-0 rig at 768^2 matching, rendered on the GPU."""
+    from sphere_stereo import synth
+    from sphere_stereo.differentiable import DifferentiableDepth
+    from sphere_stereo.utils import parse_json_calib
+
+    d = DIFF_FORM
+    frames = []
+    for seed in range(d["frames"]):
+        srig = synth.SyntheticRig(seed=seed, resolution=1216)
+        imgs, masks = srig.render(device)
+        tm, _, mk = synth.make_frame(srig, d["matching"], (1216, 1216), list(d["refs"]), device=device,
+                                     rendered=(imgs, masks))
+        frames.append([t.contiguous() for t in tm])
+        calibs = parse_json_calib(srig.basalt_json()["value0"], list(d["matching"]), device)
+    rig = DifferentiableDepth(calibs, 0.55, 100, d["N"], list(d["refs"]), mk, d["matching"], d["sigma_i"],
+                              d["sigma_s"], device, temperature=d["temperature"])
+    return rig, frames
+
+
+def differentiation_form(args, device):
+    """The differentiation form: soft-argmin inverse distance of both
+    references of 4 frames, forward and forward + backward (gradients to the
+    images, all intrinsics and all poses) timed per batch; then the
+    Gauss-Newton refinement of camera 1 from a perturbation of 0.5 degrees,
+    5 mm and 1 % of fx over 50 steps."""
+    import numpy as np
+    import torch
+
+    from sphere_stereo.differentiable import pose_exp, rotation_angle_deg
+
+    rig, frames = diff_setup(device)
+    gen = torch.Generator(device="cpu").manual_seed(7)
+    wmap = torch.rand((rig.rows, rig.cols), generator=gen).to(device)
+    req = [[t.clone().requires_grad_(True) for t in f] for f in frames]
+
+    def fwd(grad):
+        cams6 = rig.cams6.clone().requires_grad_(grad)
+        rts = rig.rts.clone().requires_grad_(grad)
+        loss = 0.0
+        for imgs in (req if grad else frames):
+            for inv in rig(imgs, cams6, rts):
+                loss = loss + (inv * wmap).sum()
+        if grad:
+            loss.backward()
+        return loss
+
+    out = {"frames_per_batch": DIFF_FORM["frames"], "matching": list(DIFF_FORM["matching"]), "candidates": DIFF_FORM["N"],
+           "references": list(DIFF_FORM["refs"]), "temperature": DIFF_FORM["temperature"], "dtype": "f32"}
+    for name, grad in (("forward", False), ("forward_backward", True)):
+        ctx = torch.no_grad() if not grad else torch.enable_grad()
+        with ctx:
+            for _ in range(2):
+                fwd(grad)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            reps = 5
+            for _ in range(reps):
+                fwd(grad)
+            torch.cuda.synchronize()
+            el = (time.perf_counter() - t0) / reps
+        out[name] = {"ms_per_batch": round(1e3 * el, 3), "frames_per_s": round(DIFF_FORM["frames"] / el, 2)}
+        print("[bench] differentiation form %s: %.2f ms per batch of %d frames" % (name, 1e3 * el,
+              DIFF_FORM["frames"]), file=sys.stderr, flush=True)
+    ct, rt = rig.cams6.clone(), rig.rts.clone()
+    ax = torch.tensor([0.3, 0.8, -0.5], dtype=torch.float64)
+    ax = ax / ax.norm() * np.radians(0.5)
+    tr = torch.tensor([0.004, -0.002, 0.0022], dtype=torch.float64)
+    tr = tr / tr.norm() * 0.005
+    r0, c0 = rt.clone(), ct.clone()
+    r0[1] = rt[1] @ pose_exp(torch.cat([ax, tr]))
+    c0[1, 0] *= 1.01
+
+    def errors(c6, rts):
+        D = torch.linalg.inv(rt[1]) @ rts[1]
+        return {"rotation_deg": round(rotation_angle_deg(D[:3, :3]), 4), "translation_mm": round(float(
+            D[:3, 3].norm()) * 1e3, 3), "fx_rel": round(float(c6[1, 0] / ct[1, 0]) - 1, 5)}
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cams, rts, hist = rig.refine_calibration(frames, 1, steps=50, cams6=c0, rts=r0)
+    el = time.perf_counter() - t0
+    out["calibration_refinement"] = {"camera": 1, "steps": len(hist), "seconds": round(el, 3),
+                                     "before": errors(c0, r0), "after": errors(cams, rts),
+                                     "loss_first": round(hist[0]["loss"], 3), "loss_last": round(hist[-1]["loss"], 3)}
+    print("[bench] calibration refinement: %s -> %s in %.2f s" % (errors(c0, r0), errors(cams, rts), el),
+          file=sys.stderr, flush=True)
+    return out
+
+
+def diff_report(args):
+    """Gradient check at the differentiation form's size: the GPU gradient of
+    L = sum_p w_p / d_p (reference 0 of frame 0, soft-argmin readout) with
+    respect to the camera parameters that L depends on most, against fp64
+    central differences (h = 1e-4, and h = 1e-6, where fewer samples cross
+    the kinks of the bilinear cells and of |.| within the step) of the
+    oracle's forward (sweep, ISB filter, soft-argmin) with the same camera
+    selection."""
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    from sphere_stereo.differentiable import relative_poses
+    from sphere_stereo.utils import distance_candidates
+
+    device = torch.device("cuda:0")
+    rig, frames = diff_setup(device)
+    images = frames[0]
+    gen = torch.Generator(device="cpu").manual_seed(7)
+    wmap = torch.rand((rig.rows, rig.cols), generator=gen, dtype=torch.float64)
+    cams6 = rig.cams6.clone().requires_grad_(True)
+    inv = rig(images, cams6, rig.rts)[0]
+    (inv.double() * wmap.to(device)).sum().backward()
+    g = cams6.grad.numpy()
+    o64, o32 = Oracle("f64"), Oracle("f32")
+    imgs32 = np.stack([t.cpu().numpy() for t in images]).astype(np.float32)
+    sel = rig._sel[0].cpu().numpy().astype(np.int32).reshape(rig.rows, rig.cols)
+    rel = relative_poses(rig.rts, 0).reshape(-1, 12).numpy()
+    cands = np.array(distance_candidates(0.55, 100, rig.N), np.float64)
+    guide = o32.guide(imgs32[0])
+    wnp = wmap.numpy()
+
+    def L(c6):
+        cost = o64.sweep(imgs32, sel, cands, c6, rel, 0)
+        Fv, _ = o64.isb_filter(guide, cost, rig.sigma_i, rig.sigma_s)
+        return float((o64.soft_argmin(Fv, rig.beta, rig.inv0, rig.inv1) * wnp).sum())
+
+    rows = []
+    names = ["fs0", "fs1", "cs0", "cs1", "xi", "alpha"]
+    for flat in np.argsort(-np.abs(g).ravel())[:args.diff_params]:
+        k, i = divmod(int(flat), 6)
+        row = {"camera": k, "parameter": names[i], "gpu": round(float(g[k, i]), 5)}
+        for h in (1e-4, 1e-6):
+            cp, cm = rig.cams6.numpy().copy(), rig.cams6.numpy().copy()
+            cp[k, i] += h
+            cm[k, i] -= h
+            fd = (L(cp) - L(cm)) / (2 * h)
+            tag = "h%.0e" % h
+            row["fd_" + tag] = round(fd, 5)
+            row["rel_err_" + tag] = round(abs(float(g[k, i]) - fd) / max(abs(fd), 1e-12), 6)
+        rows.append(row)
+        print("[bench] d/d %s[%d]: %s" % (names[i], k, row), file=sys.stderr, flush=True)
+    print(json.dumps({"differentiation_gradient_check": {"workload": "frame 0, reference 0, 768^2, 48 candidates",
+                                                         "loss": "sum_p w_p / d_p, w ~ U(0,1) seed 7",
+                                                         "rows": rows}}), flush=True)
+
+
 def profile(lib, est, to_match, to_stitch, frames, mask):
     import ctypes as C
 
@@ -270,10 +427,19 @@ def main():
     ap.add_argument("--frames-cache", default=None, help="reuse rendered fisheyes saved at this path")
     ap.add_argument("--batch", type=int, default=8,
                     help="frames per batch of the batching form (0: skip it); each from its own seeded rig")
+    ap.add_argument("--no-diff-form", action="store_true", help="skip the differentiation form's timing and "
+                    "calibration refinement")
     ap.add_argument("--precision-report", action="store_true",
                     help="instead of the throughput line: inverse-distance error of the fp32/fp16/bf16 volume forms "
                          "against the fp64 oracle (use --workload c2)")
+    ap.add_argument("--diff-report", action="store_true",
+                    help="instead of the throughput line: the differentiation form's GPU gradient against fp64 "
+                         "central differences (h = 1e-4) of the oracle at full size")
+    ap.add_argument("--diff-params", type=int, default=6, help="parameters checked by --diff-report")
     args = ap.parse_args()
+    if args.diff_report:
+        diff_report(args)
+        return
     if args.precision_report:
         precision_report(args)
         return
@@ -365,6 +531,7 @@ def main():
         del est_h
 
     batching = batching_form(args, wl, device, value) if args.batch > 0 else None
+    diff_form = differentiation_form(args, device) if not args.no_diff_form else None
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
@@ -400,6 +567,7 @@ def main():
                          "ms_per_step": round(1e3 * g_elapsed / args.steps, 4)},
         "precision_forms": forms,
         "batching_form": batching,
+        "differentiation_form": diff_form,
     }
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)

@@ -291,6 +291,71 @@ int ss_frame_batch_run(void *frame, const float *const *d_match, const float *co
                        ss_stream stream);
 int ss_frame_destroy(void *frame);
 
+/* Differentiation form --------------------------------------------------------
+ * No counterpart in the reference (it has no backward).  The forward is the
+ * plain path's ss_ray_lut + ss_sweep + ss_isb_filter on an fp32 volume, read
+ * out by ss_soft_argmin instead of the hard argmin + parabola of
+ * depth_estimation.py:180-198; the calls below are the adjoints of those
+ * stages, in reverse order, and the Gauss-Newton pieces of a photometric
+ * calibration refinement.  All require N >= 2. */
+
+/* Soft-argmin readout: iota = sum_n n softmax_n(-beta * F_n) over the filtered
+ * fp32 volume d_cost (H x W x Np), d_invd[p] = inv0 + (inv1 - inv0) * iota /
+ * (N - 1), where inv0 = 1/d_0 and inv1 = 1/d_{N-1} (the candidates are uniform
+ * in inverse distance, depth_estimation.py:145-146). */
+int ss_soft_argmin(const float *d_cost, int H, int W, int N, float beta, float inv0, float inv1, float *d_invd,
+                   ss_stream stream);
+
+/* Its adjoint: d_grad_cost (H x W x Np, pads written 0) = dL/dF given
+ * d_grad_invd = dL/d(invd) (H x W) and the same filtered volume. */
+int ss_soft_argmin_backward(const float *d_cost, const float *d_grad_invd, int H, int W, int N, float beta,
+                            float inv0, float inv1, float *d_grad_cost, ss_stream stream);
+
+/* Adjoint of ss_isb_filter (ISB_Filter.apply, isb_filter.py:72-136) on an fp32
+ * volume, which for a fixed guide is linear in the cost.  d_guide: the level-0
+ * packed guide the forward filter was given (before the filter updated it);
+ * d_grad_out: dL/d(filtered volume); d_grad_in (not aliasing it): dL/d(input
+ * volume).  The guide pyramid is recomputed, no forward level is needed. */
+size_t ss_isb_backward_workspace_bytes(int H, int W, int N);
+int ss_isb_filter_backward(const uint32_t *d_guide, const float *d_grad_out, float *d_grad_in, int H, int W, int N,
+                           double sigma_i, double sigma_s, void *d_workspace, size_t workspace_bytes,
+                           ss_stream stream);
+
+/* Adjoint of ss_sweep (depth_estimation.py:141-175, with the clamp at 500)
+ * for reference `ref`, arguments as ss_sweep plus d_grad_cost = dL/d(clamped
+ * cost volume).  d_grad_images (nullable; entries nullable): C HWC float32
+ * buffers to which dL/d(image) is ADDED.  d_grad_params receives (C + 1) x 18
+ * floats: record k < C is camera k's (fs0, fs1, cs0, cs1, xi, alpha) followed
+ * by its 3 x 4 (camera <- reference) transform row-major, as sampled from this
+ * reference; record C holds the reference camera's own 6 parameters through
+ * its rays.  The selection and the bilinear cell are held fixed. */
+size_t ss_sweep_backward_workspace_bytes(int H, int W, int C);
+int ss_sweep_backward(const float *d_raylut, const float *const *d_images, int C, const ss_camera *cams,
+                      const float *rts, int ref, const float *cands, int N, int H, int W, const float *d_grad_cost,
+                      float *const *d_grad_images, float *d_grad_params, void *d_workspace, size_t workspace_bytes,
+                      ss_stream stream);
+
+/* Photometric Gauss-Newton normal equations of one (reference, target camera)
+ * pair: each reference pixel with d_invd > 0 and a selected camera (d_raylut
+ * as ss_ray_lut writes it; with skip_selected, one other than `target`) is placed at ray / invd, moved into the
+ * target by rt (3 x 4, target <- reference), projected with *cam and compared
+ * with the target image (residual per channel, Huber weight at `huber`);
+ * pixels outside the image interior or below 0.9 of d_mask (nullable) are
+ * skipped.  Parameters: a right perturbation exp(delta) of the target's pose
+ * (rotation 3, translation 3) and relative scales of fs0 and fs1.  Writes one
+ * record of ss_photometric_record_floats() floats per block of
+ * ss_photometric_blocks(H, W) blocks at d_partial. */
+int ss_photometric_blocks(int H, int W);
+int ss_photometric_record_floats(void);
+int ss_photometric_normal_equations(const float *d_raylut, const float *d_invd, const float *d_ref_image,
+                                    const float *d_image, const float *d_mask, const ss_camera *cam, const float *rt,
+                                    int target, int skip_selected, int H, int W, float huber, float *d_partial,
+                                    ss_stream stream);
+/* Sums nblocks records and solves (H + lambda diag H) delta = -b in fp64 on the
+ * GPU (Cholesky).  d_out: 11 doubles = delta[8], loss, residual count, and 1.0
+ * if the factorisation succeeded (else delta = 0). */
+int ss_gauss_newton_solve(const float *d_partial, int nblocks, double lambda, double *d_out, ss_stream stream);
+
 /* Kernel timing ------------------------------------------------------------- */
 /* No counterpart in the reference (it has no timers).  While enabled, every
  * launch of a stage whose bit is set in stage_mask is bracketed by a pair of

@@ -173,6 +173,15 @@ class Oracle:
         self._fn("ora_argmin_refine")(_ptr(cost), _f(N), _f(H * W), _d(d0), _d(dN1), _d(A), _ptr(dist), _ptr(idx))
         return dist, idx
 
+    def soft_argmin(self, cost, beta, inv0, inv1):
+        """The differentiation form's readout: inverse distance per pixel of a
+        planar [N][H][W] filtered volume."""
+        cost = np.ascontiguousarray(cost, self.real)
+        N, H, W = cost.shape
+        invd = np.zeros((H, W), self.real)
+        self._fn("ora_soft_argmin")(_ptr(cost), _f(N), _f(H * W), _d(beta), _d(inv0), _d(inv1), _ptr(invd))
+        return invd
+
     def round_volume(self, x, vtype):
         """Values rounded as the precision form stores them (1 fp16, 2 bf16)."""
         x = np.array(x, self.real, copy=True, order="C")
@@ -274,6 +283,7 @@ _SIGNATURES = {
     "ora_isb_filter": [_p, _p, _f, _f, _f, _d, _d, _f, _f],
     "ora_round_volume": [_p, C.c_long, _f],
     "ora_argmin_refine": [_p, _f, _f, _d, _d, _d, _p, _p],
+    "ora_soft_argmin": [_p, _f, _f, _d, _d, _d, _p],
     "ora_reproject_zmin": [_p, _p, _f, _f, _p, _p],
     "ora_inpaint_lut": [_p, _f, _f, _p, _p, _d, _d],
     "ora_inpaint_step": [_p, _p, _p, _f, _f, _d, _d],

@@ -35,6 +35,9 @@
  *   stitcher functions - closed forms and invariants (identity reprojection,
  *                      inpaint of isolated holes, LUT packing round trip, blend
  *                      weights summing to 1, merge of constant maps).
+ *   ora_soft_argmin  - the differentiation form's readout (no reference
+ *                      counterpart): closed forms (constant cost, a single
+ *                      dominant minimum, geometric weights of a linear cost).
  *   ora_resize_linear - PARITY UNPINNED against cv2 (absent here, and its
  *                      float resize varies with the platform's FMA use):
  *                      closed forms (identity size, 2:1 block means, constant
@@ -594,6 +597,32 @@ void SFX(ora_argmin_refine)(const real *cost, int N, int HW, double d0_, double 
         if (RFABS(M - m) < K(1e-8)) d = dN1;
         dist[p] = d;
         if (idx_out) idx_out[p] = i;
+    }
+}
+
+/* Soft-argmin readout of the differentiation form (no counterpart in the
+ * reference, whose hard argmin + parabola, depth_estimation.py:180-198, has
+ * no useful derivative): iota = sum_n n * softmax_n(-beta * cost_n) with the
+ * minimum subtracted first, and the inverse distance inv0 + (inv1 - inv0) *
+ * iota / (N - 1), the candidates being uniform in inverse distance
+ * (depth_estimation.py:145-146).  cost: N x HW planar. */
+void SFX(ora_soft_argmin)(const real *cost, int N, int HW, double beta_, double inv0_, double inv1_, real *invd)
+{
+    const real beta = (real)beta_, inv0 = (real)inv0_, dinv = (real)(inv1_ - inv0_);
+    for (int p = 0; p < HW; p++) {
+        real m = cost[p];
+        for (int n = 1; n < N; n++) m = RFMIN(m, cost[(size_t)n * HW + p]);
+        real z = 0, s = 0;
+        for (int n = 0; n < N; n++) {
+#if ORA_F64
+            real e = exp(-beta * (cost[(size_t)n * HW + p] - m));
+#else
+            real e = expf(-beta * (cost[(size_t)n * HW + p] - m));
+#endif
+            z = z + e;
+            s = s + (real)n * e;
+        }
+        invd[p] = inv0 + dinv * ((s / z) / (real)(N - 1));
     }
 }
 

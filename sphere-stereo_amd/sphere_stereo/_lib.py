@@ -82,6 +82,16 @@ SIGNATURES = {
     "ss_frame_batch_create": (_i, [C.POINTER(ss_frame_desc), _i, C.POINTER(C.c_void_p)]),
     "ss_frame_batch_run": (_i, [_p, _p, _p, _i, _p]),
     "ss_frame_destroy": (_i, [_p]),
+    "ss_soft_argmin": (_i, [_p, _i, _i, _i, _fl, _fl, _fl, _p, _p]),
+    "ss_soft_argmin_backward": (_i, [_p, _p, _i, _i, _i, _fl, _fl, _fl, _p, _p]),
+    "ss_isb_backward_workspace_bytes": (_z, [_i, _i, _i]),
+    "ss_isb_filter_backward": (_i, [_p, _p, _p, _i, _i, _i, _d, _d, _p, _z, _p]),
+    "ss_sweep_backward_workspace_bytes": (_z, [_i, _i, _i]),
+    "ss_sweep_backward": (_i, [_p, _p, _i, _CAM, _p, _i, _p, _i, _i, _i, _p, _p, _p, _p, _z, _p]),
+    "ss_photometric_blocks": (_i, [_i, _i]),
+    "ss_photometric_record_floats": (_i, []),
+    "ss_photometric_normal_equations": (_i, [_p, _p, _p, _p, _p, _CAM, _p, _i, _i, _i, _i, _fl, _p, _p]),
+    "ss_gauss_newton_solve": (_i, [_p, _i, _d, _p, _p]),
     "ss_profile_enable": (_i, [_i, C.c_uint]),
     "ss_profile_collect": (_i, [_p, _p]),
     "ss_stage_name": (C.c_char_p, [_i]),

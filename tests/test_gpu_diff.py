@@ -1,0 +1,266 @@
+"""Differentiation form on the GPU, through the C ABI: the soft-argmin readout
+and the adjoint kernels against the float64 torch autograd reference
+(tests/diff_reference.py), the filter adjoint against the dot-product
+identity <M x, y> = <x, M^T y> with the library's own forward filter, the
+whole gradient against fp64 central differences of the oracle, and the
+Gauss-Newton calibration refinement on a perturbed rig."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import diff_reference as dr
+from helpers import frame
+from oracle import Oracle
+from sphere_stereo import _lib
+from sphere_stereo.differentiable import (DifferentiableDepth, camera_parameters, camera_poses, pose_exp,
+                                          relative_poses, rotation_angle_deg)
+from test_diff_cpu import oracle_forward, problem
+
+pytestmark = pytest.mark.gpu
+
+O32, O64 = Oracle("f32"), Oracle("f64")
+F32 = _lib.VOLUME_TYPES["float32"]
+
+
+def stream(dev):
+    return C.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+def cam_array(cams6):
+    arr = (_lib.ss_camera * cams6.shape[0])()
+    for i, r in enumerate(cams6.tolist()):
+        arr[i].fs[0], arr[i].fs[1], arr[i].cs[0], arr[i].cs[1] = r[0], r[1], r[2], r[3]
+        arr[i].xi, arr[i].alpha = r[4], r[5]
+    return arr
+
+
+def unpack_guide(packed, H, W):
+    g = packed.cpu().numpy().view(np.uint32).reshape(H, W)
+    return np.stack([(g >> (8 * c)) & 0xFF for c in range(3)], -1).astype(np.uint8)
+
+
+def test_soft_argmin_forward_and_backward(gpu):
+    H, W, N = 37, 53, 24
+    NP = _lib.lib.ss_padded_candidates(N)
+    gen = torch.Generator().manual_seed(0)
+    vol = torch.rand((H, W, NP), generator=gen) * 300
+    beta, inv0, inv1 = 0.04, 1 / 0.55, 0.01
+    d = vol.to(gpu)
+    inv = torch.empty((H, W), device=gpu)
+    s = stream(gpu)
+    print("soft-argmin forward", H, W, N, flush=True)
+    _lib.check(_lib.lib.ss_soft_argmin(d.data_ptr(), H, W, N, beta, inv0, inv1, inv.data_ptr(), s))
+    want = O32.soft_argmin(vol[..., :N].permute(2, 0, 1).numpy(), beta, inv0, inv1)
+    np.testing.assert_allclose(inv.cpu().numpy(), want, rtol=2e-6, atol=2e-6)
+    g = torch.rand((H, W), generator=gen) - 0.5
+    gF = torch.full_like(d, 7.0)
+    _lib.check(_lib.lib.ss_soft_argmin_backward(d.data_ptr(), g.to(gpu).data_ptr(), H, W, N, beta, inv0, inv1,
+                                                gF.data_ptr(), s))
+    Fv = vol[..., :N].double().permute(2, 0, 1).clone().requires_grad_(True)
+    (dr.soft_argmin(Fv, beta, inv0, inv1) * g.double()).sum().backward()
+    got = gF.cpu()[..., :N].permute(2, 0, 1).double()
+    scale = float(Fv.grad.abs().max())
+    assert float((got - Fv.grad).abs().max()) <= 1e-4 * scale
+    assert bool((gF.cpu()[..., N:] == 0).all())
+
+
+def _random_guide(gpu, H, W, seed):
+    gen = torch.Generator().manual_seed(seed)
+    # smooth random colours with edges, so that the guide weights vary
+    base = torch.rand((H // 4 + 2, W // 4 + 2, 3), generator=gen) * 255
+    img = torch.nn.functional.interpolate(base.permute(2, 0, 1)[None], size=(H, W), mode="nearest")[0]
+    img = (img.permute(1, 2, 0) + 8 * torch.rand((H, W, 3), generator=gen)).contiguous()
+    guide = torch.empty(H * W, dtype=torch.int32, device=gpu)
+    _lib.check(_lib.lib.ss_guide(img.to(gpu).data_ptr(), H, W, guide.data_ptr(), stream(gpu)))
+    return guide
+
+
+@pytest.mark.parametrize("H,W,N", [(48, 40, 16), (37, 29, 8), (64, 64, 32), (33, 70, 48)])
+def test_isb_filter_backward(gpu, H, W, N):
+    NP = _lib.lib.ss_padded_candidates(N)
+    s = stream(gpu)
+    guide = _random_guide(gpu, H, W, seed=H * W)
+    gen = torch.Generator().manual_seed(N)
+    x = torch.zeros((H, W, NP))
+    x[..., :N] = torch.rand((H, W, N), generator=gen) * 500
+    y = torch.zeros((H, W, NP))
+    y[..., :N] = torch.rand((H, W, N), generator=gen) - 0.5
+    sigma_i, sigma_s = 10.0, 12.5
+    print("isb forward + backward", H, W, N, flush=True)
+    g_fwd = guide.clone()
+    Mx = x.to(gpu)
+    ws = torch.empty(_lib.lib.ss_isb_workspace_bytes(H, W, N, F32, 0), dtype=torch.uint8, device=gpu)
+    _lib.check(_lib.lib.ss_isb_filter(g_fwd.data_ptr(), Mx.data_ptr(), H, W, N, F32, sigma_i, sigma_s,
+                                      ws.data_ptr(), ws.numel(), s))
+    nb = _lib.lib.ss_isb_backward_workspace_bytes(H, W, N)
+    wsb = torch.empty(nb, dtype=torch.uint8, device=gpu)
+    yd = y.to(gpu)
+    MTy = torch.full_like(yd, 3.0)
+    _lib.check(_lib.lib.ss_isb_filter_backward(guide.data_ptr(), yd.data_ptr(), MTy.data_ptr(), H, W, N, sigma_i,
+                                               sigma_s, wsb.data_ptr(), nb, s))
+    Mx, MTy = Mx.cpu().double(), MTy.cpu().double()
+    dot1 = float((Mx[..., :N] * y[..., :N].double()).sum())
+    dot2 = float((x[..., :N].double() * MTy[..., :N]).sum())
+    assert abs(dot1 - dot2) <= 2e-6 * abs(dot1), (dot1, dot2)
+    assert bool((MTy[..., N:] == 0).all())
+    # against autograd of the torch reference built on the same guide bytes
+    tab = dr.filter_tables(unpack_guide(guide, H, W), sigma_i, sigma_s, O32)
+    xin = x[..., :N].permute(2, 0, 1).double().clone().requires_grad_(True)
+    out = dr.isb_filter(xin, tab)
+    np.testing.assert_allclose(out.detach().numpy(), Mx[..., :N].permute(2, 0, 1).numpy(), rtol=2e-5, atol=2e-4)
+    (out * y[..., :N].permute(2, 0, 1).double()).sum().backward()
+    got = MTy[..., :N].permute(2, 0, 1)
+    assert float((got - xin.grad).abs().max()) <= 1e-4 * float(xin.grad.abs().max())
+
+
+def _sweep_inputs(gpu, p):
+    H, W = p["imgs32"].shape[1:3]
+    imgs = [torch.from_numpy(im).to(gpu).contiguous() for im in p["imgs32"]]
+    sel = torch.from_numpy(p["sel"].astype(np.int8)).to(gpu).contiguous()
+    arr = cam_array(p["cams6"])
+    raylut = torch.empty((H, W, 4), device=gpu)
+    _lib.check(_lib.lib.ss_ray_lut(C.byref(arr[p["ref"]]), sel.data_ptr(), H, W, raylut.data_ptr(), stream(gpu)))
+    return imgs, arr, raylut
+
+
+@pytest.mark.parametrize("ref", [0, 2])
+def test_sweep_backward_matches_torch(gpu, ref):
+    p = problem(matching=(40, 36), N=8, ref=ref)
+    H, W = p["imgs32"].shape[1:3]
+    N, Cn = p["N"], len(p["imgs32"])
+    NP = _lib.lib.ss_padded_candidates(N)
+    imgs, arr, raylut = _sweep_inputs(gpu, p)
+    gen = torch.Generator().manual_seed(11 + ref)
+    gC = torch.zeros((H, W, NP))
+    gC[..., :N] = torch.rand((H, W, N), generator=gen) - 0.3
+    g_imgs = [torch.zeros_like(im) for im in imgs]
+    gp = torch.empty((Cn + 1) * 18, device=gpu)
+    nb = _lib.lib.ss_sweep_backward_workspace_bytes(H, W, Cn)
+    ws = torch.empty(nb, dtype=torch.uint8, device=gpu)
+    cands = _lib.float_array(p["cands"].tolist())
+    rts = _lib.float_array(p["rel"].reshape(-1).tolist())
+    print("sweep backward ref", ref, flush=True)
+    _lib.check(_lib.lib.ss_sweep_backward(raylut.data_ptr(), _lib.ptr_array(imgs), Cn, arr, rts, ref, cands, N, H, W,
+                                          gC.to(gpu).data_ptr(), _lib.ptr_array(g_imgs), gp.data_ptr(), ws.data_ptr(),
+                                          nb, stream(gpu)))
+    gp = gp.cpu().double().view(Cn + 1, 18)
+    images = [torch.from_numpy(im).double().requires_grad_(True) for im in p["imgs32"]]
+    cams6 = p["cams6"].clone().requires_grad_(True)
+    rel = p["rel"].clone().requires_grad_(True)
+    cost = dr.sweep_cost(images, cams6, rel, ref, torch.from_numpy(p["sel"]).long(), torch.from_numpy(p["cands"]))
+    (cost * gC[..., :N].permute(2, 0, 1).double()).sum().backward()
+    g_cams = gp[:Cn, :6].clone()
+    g_cams[ref] += gp[Cn, :6]
+    for k in range(Cn):
+        tol = 2e-3 * float(cams6.grad[k].abs().max()) + 1e-3
+        assert float((g_cams[k] - cams6.grad[k]).abs().max()) <= tol, (k, g_cams[k], cams6.grad[k])
+        tol = 2e-3 * float(rel.grad[k].abs().max()) + 1e-3
+        assert float((gp[k, 6:].view(3, 4) - rel.grad[k]).abs().max()) <= tol, (k, gp[k, 6:], rel.grad[k])
+    for k in range(Cn):
+        got = g_imgs[k].cpu().double()
+        want = images[k].grad if images[k].grad is not None else torch.zeros_like(got)  # never sampled
+        assert float((got - want).abs().max()) <= 2e-3 * float(want.abs().max()) + 1e-6, k
+
+
+def test_full_gradient_matches_torch_and_central_differences(gpu):
+    """DifferentiableDepth (soft-argmin over the filtered volume, all adjoint
+    kernels) against autograd of the torch reference on the same rig, and
+    against fp64 central differences of the oracle's forward.  The step is
+    h = 1e-6: at this 40 x 36 size a step of 1e-4 in alpha or xi moves enough
+    samples across bilinear cells to shift the difference quotient by up to
+    25 % (the oracle's and the torch reference's quotients agree with each
+    other at every step; bench.py --diff-report gives the h = 1e-4 figures
+    at full size)."""
+    ref = 0
+    p = problem(matching=(40, 36), N=8, ref=ref)
+    f = frame(seed=0, resolution=160, matching=(40, 36), refs=(0, 2))
+    rig = DifferentiableDepth(f["calibs"], 0.55, 100, p["N"], [ref], f["masks"], (40, 36), p["sigma_i"],
+                              p["sigma_s"], gpu)
+    rig.set_calibration(p["cams6"], p["rts"])
+    print("full gradient", flush=True)
+    images = [torch.from_numpy(im).to(gpu).requires_grad_(True) for im in p["imgs32"]]
+    cams6 = p["cams6"].clone().requires_grad_(True)
+    rts = p["rts"].clone().requires_grad_(True)
+    inv = rig(images, cams6, rts)[0]
+    want_inv = oracle_forward(O32, p)
+    np.testing.assert_allclose(inv.detach().cpu().numpy(), want_inv, rtol=1e-4, atol=1e-5)
+    gen = torch.Generator().manual_seed(5)
+    wmap = torch.rand(inv.shape, generator=gen, dtype=torch.float64)
+    (inv.double() * wmap.to(gpu)).sum().backward()
+    # torch reference
+    tab = dr.filter_tables(O32.guide(p["imgs32"][ref]), p["sigma_i"], p["sigma_s"], O32)
+    images_t = [torch.from_numpy(im).double().requires_grad_(True) for im in p["imgs32"]]
+    cams6_t = p["cams6"].clone().requires_grad_(True)
+    rts_t = p["rts"].clone().requires_grad_(True)
+    inv_t = dr.forward(images_t, cams6_t, relative_poses(rts_t, ref), ref, torch.from_numpy(p["sel"]).long(),
+                       torch.from_numpy(p["cands"]), tab, p["beta"], p["inv0"], p["inv1"])
+    (inv_t * wmap).sum().backward()
+    for name, got, want in [("cams6", cams6.grad, cams6_t.grad), ("rts", rts.grad, rts_t.grad)]:
+        err = float((got.cpu() - want).abs().max())
+        assert err <= 5e-3 * float(want.abs().max()), (name, err, float(want.abs().max()))
+    for k in range(len(images)):
+        got = images[k].grad.cpu().double()
+        want = images_t[k].grad if images_t[k].grad is not None else torch.zeros_like(got)
+        assert float((got - want).abs().max()) <= 5e-3 * float(want.abs().max()) + 1e-9, k
+    # fp64 central differences of the oracle on the camera parameters the
+    # loss depends on most
+    wnp = wmap.numpy()
+    rel0 = p["rel"].reshape(-1, 12).numpy()
+
+    def L(c6):
+        return float((oracle_forward(O64, p, cams6=c6, rel=rel0) * wnp).sum())
+
+    g = cams6.grad.cpu().numpy()
+    order = np.argsort(-np.abs(g).ravel())[:5]
+    for flat in order:
+        k, i = divmod(int(flat), 6)
+        cp, cm = p["cams6"].numpy().copy(), p["cams6"].numpy().copy()
+        cp[k, i] += 1e-6
+        cm[k, i] -= 1e-6
+        fd = (L(cp) - L(cm)) / 2e-6
+        assert abs(g[k, i] - fd) <= 2e-3 * abs(fd) + 2e-3 * np.abs(g).max(), (k, i, g[k, i], fd)
+
+
+def test_calibration_refinement_reduces_the_error(gpu):
+    """Gauss-Newton on the photometric reprojection, at the differentiation
+    form's size (4 frames of one rig, 768^2 matching, 48 candidates): camera 1
+    perturbed by 0.5 degrees of rotation, 5 mm of translation and 1 % of fx.
+    Rotation and focal length come back to within a tenth of their
+    perturbation.  A few millimetres of translation are barely observable at
+    this baseline (5 mm moves a point at 3 m by ~0.4 px, less than the depth
+    quantisation moves it): started from the true calibration the same
+    refinement settles ~3 mm away, so translation is only required not to
+    drift beyond its starting error."""
+    from sphere_stereo import synth
+    from sphere_stereo.utils import parse_json_calib
+
+    match = (768, 768)
+    frames = []
+    for seed in range(4):
+        srig = synth.SyntheticRig(seed=seed, resolution=1216)
+        imgs, masks = srig.render(gpu)
+        tm, _, mk = synth.make_frame(srig, match, (1216, 1216), [0, 2], device=gpu, rendered=(imgs, masks))
+        frames.append([t.contiguous() for t in tm])
+        calibs = parse_json_calib(srig.basalt_json()["value0"], list(match), gpu)
+    rig = DifferentiableDepth(calibs, 0.55, 100, 48, [0, 2], mk, match, 10.0, 12.5, gpu)
+    cams_true, rts_true = rig.cams6.clone(), rig.rts.clone()
+    ax = torch.tensor([0.3, 0.8, -0.5], dtype=torch.float64)
+    ax = ax / ax.norm() * np.radians(0.5)
+    tr = torch.tensor([0.004, -0.002, 0.0022], dtype=torch.float64)
+    tr = tr / tr.norm() * 0.005
+    rts0, cams0 = rts_true.clone(), cams_true.clone()
+    rts0[1] = rts_true[1] @ pose_exp(torch.cat([ax, tr]))
+    cams0[1, 0] *= 1.01
+    print("calibration refinement", flush=True)
+    cams, rts, hist = rig.refine_calibration(frames, 1, steps=50, cams6=cams0, rts=rts0)
+
+    def errors(c6, rt):
+        D = torch.linalg.inv(rts_true[1]) @ rt[1]
+        return rotation_angle_deg(D[:3, :3]), float(D[:3, 3].norm()) * 1e3, abs(float(c6[1, 0] / cams_true[1, 0]) - 1)
+
+    e0, e1 = errors(cams0, rts0), errors(cams, rts)
+    print("before", e0, "after", e1, "loss", hist[0]["loss"], hist[-1]["loss"], flush=True)
+    assert hist[-1]["loss"] < 0.5 * hist[0]["loss"]
+    assert e1[0] < 0.1 * e0[0] and e1[2] < 0.1 * e0[2] and e1[1] < e0[1], (e0, e1)
