@@ -351,9 +351,10 @@ int ss_photometric_normal_equations(const float *d_raylut, const float *d_invd, 
                                     const float *d_image, const float *d_mask, const ss_camera *cam, const float *rt,
                                     int target, int skip_selected, int H, int W, float huber, float *d_partial,
                                     ss_stream stream);
-/* Sums nblocks records and solves (H + lambda diag H) delta = -b in fp64 on the
- * GPU (Cholesky).  d_out: 11 doubles = delta[8], loss, residual count, and 1.0
- * if the factorisation succeeded (else delta = 0). */
+/* Sums nblocks records (in fp64) and solves (H + lambda diag H) delta = -b in
+ * fp64 on the GPU (Cholesky).  d_out: 64 doubles; d_out[0..11) = delta[8],
+ * loss, residual count, and 1.0 if the factorisation succeeded (else delta =
+ * 0); the rest is scratch. */
 int ss_gauss_newton_solve(const float *d_partial, int nblocks, double lambda, double *d_out, ss_stream stream);
 
 /* Kernel timing ------------------------------------------------------------- */

@@ -60,46 +60,70 @@ struct SoftArgs {
   float* out;              // forward: 1/d [P]; backward: dL/dF [P][NP]
 };
 
-__device__ __forceinline__ float soft_min(const float* f, int N) {
-  float m = f[0];
-  for (int n = 1; n < N; ++n) m = fminf(m, f[n]);
-  return m;
+// A group of 8 lanes per pixel, lane j holding candidates [j*V, j*V + V) of
+// the NP = 8 * V stride, so that a wavefront reads 8 whole pixel rows.
+constexpr int kSoftGroup = 8;
+
+__device__ __forceinline__ float group_min8(float v) {
+#pragma unroll
+  for (int off = 4; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, kSoftGroup));
+  return v;
+}
+__device__ __forceinline__ float group_sum8(float v) {
+#pragma unroll
+  for (int off = 4; off > 0; off >>= 1) v += __shfl_xor(v, off, kSoftGroup);
+  return v;
+}
+
+// The softmax statistics of one pixel: minimum m, z = sum_n e_n and
+// s = sum_n n e_n with e_n = exp(-beta (F_n - m)), in every lane of the group.
+struct SoftStats {
+  float m, z, s;
+};
+__device__ __forceinline__ SoftStats soft_stats(const float* f, int n0, int V, int N, float beta) {
+  float m = __builtin_huge_valf();
+  for (int v = 0; v < V; ++v)
+    if (n0 + v < N) m = fminf(m, f[v]);
+  m = group_min8(m);
+  float z = 0.0f, s = 0.0f;
+  for (int v = 0; v < V; ++v)
+    if (n0 + v < N) {
+      const float e = expf(-beta * (f[v] - m));
+      z += e;
+      s += (float)(n0 + v) * e;
+    }
+  return SoftStats{m, group_sum8(z), group_sum8(s)};
 }
 
 __global__ void __launch_bounds__(kBlock) k_soft_argmin(SoftArgs a) {
-  const int p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= a.P) return;
-  const float* f = a.F + (size_t)p * a.NP;
-  const float m = soft_min(f, a.N);
-  float z = 0.0f, s = 0.0f;
-  for (int n = 0; n < a.N; ++n) {
-    const float e = expf(-a.beta * (f[n] - m));
-    z += e;
-    s += (float)n * e;
-  }
-  a.out[p] = a.inv0 + a.dinv * ((s / z) / (float)(a.N - 1));
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const int lane = (int)(threadIdx.x & (kSoftGroup - 1));
+  long long p = t / kSoftGroup;
+  const bool active = p < a.P;
+  if (!active) p = a.P - 1;  // the whole group takes part in the shuffles
+  const int V = a.NP / kSoftGroup, n0 = lane * V;
+  const SoftStats st = soft_stats(a.F + (size_t)p * a.NP + n0, n0, V, a.N, a.beta);
+  if (active && lane == 0) a.out[p] = a.inv0 + a.dinv * ((st.s / st.z) / (float)(a.N - 1));
 }
 
 // dL/dF_n = dL/diota * softmax_n * (n - iota) * (-beta)
 __global__ void __launch_bounds__(kBlock) k_soft_argmin_bwd(SoftArgs a) {
-  const int p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= a.P) return;
-  const float* f = a.F + (size_t)p * a.NP;
-  const float m = soft_min(f, a.N);
-  float z = 0.0f, s = 0.0f;
-  for (int n = 0; n < a.N; ++n) {
-    const float e = expf(-a.beta * (f[n] - m));
-    z += e;
-    s += (float)n * e;
-  }
-  const float iota = s / z;
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const int lane = (int)(threadIdx.x & (kSoftGroup - 1));
+  long long p = t / kSoftGroup;
+  const bool active = p < a.P;
+  if (!active) p = a.P - 1;
+  const int V = a.NP / kSoftGroup, n0 = lane * V;
+  const float* f = a.F + (size_t)p * a.NP + n0;
+  const SoftStats st = soft_stats(f, n0, V, a.N, a.beta);
+  if (!active) return;
+  const float iota = st.s / st.z;
   const float gi = a.ginvd[p] * a.dinv / (float)(a.N - 1);
-  const float c = -a.beta * gi / z;
-  float* g = a.out + (size_t)p * a.NP;
-  for (int n = 0; n < a.NP; ++n) {
-    float v = 0.0f;
-    if (n < a.N) v = c * expf(-a.beta * (f[n] - m)) * ((float)n - iota);
-    g[n] = v;
+  const float c = -a.beta * gi / st.z;
+  float* g = a.out + (size_t)p * a.NP + n0;
+  for (int v = 0; v < V; ++v) {
+    const int n = n0 + v;
+    g[v] = n < a.N ? c * expf(-a.beta * (f[v] - st.m)) * ((float)n - iota) : 0.0f;
   }
 }
 
@@ -620,18 +644,23 @@ __global__ void __launch_bounds__(kBlock) k_sweep_bwd(SweepBwdArgs a) {
   }
 }
 
-// Fixed-order (per column, compensated) sum of nb block records.
-__global__ void __launch_bounds__(kBlock) k_sum_records(const float* partial, int nb, int width, float* out) {
-  const int col = blockIdx.x * kBlock + threadIdx.x;
-  if (col >= width) return;
-  float s = 0.0f, comp = 0.0f;
-  for (int b = 0; b < nb; ++b) {
-    const float y = partial[(size_t)b * width + col] - comp;
-    const float t = s + y;
-    comp = (t - s) - y;
-    s = t;
+// Column sums of nb block records of `width` values: one workgroup per
+// column, each lane a fixed stride of records, then a fixed-order tree, so
+// the result does not depend on scheduling.  T is the accumulation type.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_sum_records(const float* partial, int nb, int width, T* out) {
+  __shared__ T red[kBlock];
+  const int col = blockIdx.x, t = threadIdx.x;
+  T s = 0;
+  for (int b = t; b < nb; b += kBlock) s += (T)partial[(size_t)b * width + col];
+  red[t] = s;
+  __syncthreads();
+#pragma unroll
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (t < off) red[t] += red[t + off];
+    __syncthreads();
   }
-  out[col] = s;
+  if (t == 0) out[col] = red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -775,19 +804,11 @@ __global__ void __launch_bounds__(kBlock) k_photo_normal(PhotoArgs a) {
   }
 }
 
-// The normal-equation solve, in fp64 (one workgroup): sums the block records,
-// damps the diagonal (Levenberg-Marquardt, H + lambda * diag(H)), factorises
-// by Cholesky and writes delta = -H^-1 b, then the loss and the residual count.
-__global__ void __launch_bounds__(64) k_gn_solve(const float* partial, int nb, double lambda, double* out) {
-  __shared__ double sum[kNE];
-  const int t = threadIdx.x;
-  if (t < kNE) {
-    double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += (double)partial[(size_t)b * kNE + t];
-    sum[t] = s;
-  }
-  __syncthreads();
-  if (t != 0) return;
+// The normal-equation solve, in fp64 (one lane): the summed records (fp64),
+// a damped diagonal (Levenberg-Marquardt, H + lambda * diag(H)), Cholesky,
+// delta = -H^-1 b, then the loss and the residual count.
+__global__ void __launch_bounds__(64) k_gn_solve(const double* sum, double lambda, double* out) {
+  if (threadIdx.x != 0) return;
   double H[kGN][kGN], b[kGN];
   int o = 0;
   for (int i = 0; i < kGN; ++i)
@@ -917,7 +938,7 @@ int ss_soft_argmin(const float* d_cost, int H, int W, int N, float beta, float i
   const long long P = (long long)H * W;
   if (P >= (1ll << 31)) return fail_arg("image too large");
   SoftArgs a{d_cost, (int)P, N, padded_np(N), beta, inv0, inv1 - inv0, nullptr, d_invd};
-  hipLaunchKernelGGL(k_soft_argmin, dim3(blocks_of(P)), dim3(kBlock), 0, as_stream(stream), a);
+  hipLaunchKernelGGL(k_soft_argmin, dim3(blocks_of(P * kSoftGroup)), dim3(kBlock), 0, as_stream(stream), a);
   GR_HIP(hipGetLastError());
   return SS_OK;
 }
@@ -929,7 +950,7 @@ int ss_soft_argmin_backward(const float* d_cost, const float* d_grad_invd, int H
   const long long P = (long long)H * W;
   if (P >= (1ll << 31)) return fail_arg("image too large");
   SoftArgs a{d_cost, (int)P, N, padded_np(N), beta, inv0, inv1 - inv0, d_grad_invd, d_grad_cost};
-  hipLaunchKernelGGL(k_soft_argmin_bwd, dim3(blocks_of(P)), dim3(kBlock), 0, as_stream(stream), a);
+  hipLaunchKernelGGL(k_soft_argmin_bwd, dim3(blocks_of(P * kSoftGroup)), dim3(kBlock), 0, as_stream(stream), a);
   GR_HIP(hipGetLastError());
   return SS_OK;
 }
@@ -1061,7 +1082,7 @@ int ss_sweep_backward(const float* d_raylut, const float* const* d_images, int C
   const int nb = blocks_of((long long)H * W);
   hipLaunchKernelGGL(k_sweep_bwd, dim3(nb), dim3(kBlock), 0, s, a);
   const int width = (C + 1) * kRec;
-  hipLaunchKernelGGL(k_sum_records, dim3(blocks_of(width)), dim3(kBlock), 0, s, a.partial, nb, width, d_grad_params);
+  hipLaunchKernelGGL(k_sum_records<float>, dim3(width), dim3(kBlock), 0, s, a.partial, nb, width, d_grad_params);
   GR_HIP(hipGetLastError());
   return SS_OK;
 }
@@ -1103,7 +1124,11 @@ int ss_photometric_record_floats(void) { return kNE; }
 
 int ss_gauss_newton_solve(const float* d_partial, int nblocks, double lambda, double* d_out, ss_stream stream) {
   if (!d_partial || !d_out || nblocks < 1) return fail_arg("null pointer or no records");
-  hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(64), 0, as_stream(stream), d_partial, nblocks, lambda, d_out);
+  // the sums go to d_out[16..16+kNE) (the caller's buffer holds 64 doubles)
+  double* sums = d_out + 16;
+  hipLaunchKernelGGL(k_sum_records<double>, dim3(kNE), dim3(kBlock), 0, as_stream(stream), d_partial, nblocks, kNE,
+                     sums);
+  hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(64), 0, as_stream(stream), sums, lambda, d_out);
   GR_HIP(hipGetLastError());
   return SS_OK;
 }

@@ -2,6 +2,7 @@
 // by the forward passes (isb.hip) and their adjoints (grad.hip).
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace ss {

@@ -273,7 +273,7 @@ class DifferentiableDepth:
         rec = _lib.lib.ss_photometric_record_floats()
         pairs = len(frames) * len(self.refs)
         partial = torch.empty(pairs * nb * rec, dtype=torch.float32, device=self.device)
-        out = torch.empty(11, dtype=torch.float64, device=self.device)
+        out = torch.empty(64, dtype=torch.float64, device=self.device)
         s = self._stream()
         history = []
         for _ in range(steps):
