@@ -538,6 +538,14 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get(dominant)
+    # the dominant kernel's vector-ALU occupancy (rocprofv3 VALUBusy and
+    # VALU instructions per launch, profiles/valu_<workload>.json): the sweep
+    # is bound by its fp32 VALU issue, not by HBM
+    valu = None
+    vpath = os.path.join(ROOT, "profiles", "valu_%s.json" % args.workload)
+    if os.path.exists(vpath):
+        with open(vpath) as f:
+            valu = json.load(f).get(dominant)
 
     out = {
         "metric": METRIC,
@@ -561,7 +569,8 @@ def main():
                    "global_batch": 1, "parallelism": "none (1 GPU)"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "avg_launch_ms": round(avg_launch_ms, 5), "algorithmic_bytes_per_launch": int(bytes_per_launch)},
+                     "avg_launch_ms": round(avg_launch_ms, 5), "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                     "valu": valu},
         "stage_ms_per_frame": {k: round(v, 4) for k, v in per_frame.items() if v > 0},
         "graph_replay": {"value": round(args.steps / g_elapsed, 3), "unit": "panoramas/s",
                          "ms_per_step": round(1e3 * g_elapsed / args.steps, 4)},
