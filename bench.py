@@ -10,7 +10,9 @@ estimate_RGBD_panorama call on inputs already resident in HBM.  The same
 frames are then timed with the cost volume stored in fp16 and bf16
 ("precision_forms"), with HIP-graph replay ("graph_replay"), and in the
 batching form: --batch frames (default 8) from as many seeded rigs of the
-same configuration, fp32 and fp16 volumes ("batching_form").  Last, the
+same configuration, fp32 and fp16 volumes ("batching_form"), and a ragged
+batch of 16 rigs of mixed resolutions, candidate counts and reference sets
+against the same frames run one at a time ("batching_form_ragged").  Last, the
 differentiation form (4 frames of one rig at 768^2, 48 candidates): forward
 and forward + backward time per batch and a 50-step Gauss-Newton calibration
 refinement ("differentiation_form"; --no-diff-form skips it).
@@ -149,6 +151,79 @@ def batching_form(args, wl, device, plain_fp32):
               flush=True)
         del batch, ests
         torch.cuda.empty_cache()
+    return out
+
+
+def batching_ragged(args, device):
+    """The ragged batching form: 16 rigs, each a seeded scene with its own
+    baseline drawn from U(0.15, 0.40) m, matching resolution from {512, 768,
+    1024}^2, candidate count from {16, 32, 48, 64} and 2 to 4 reference
+    cameras (1216^2 stitch input, 1024x512 panorama), fp16 volume, one
+    FrameBatch (grouped by configuration, each group one launch per pass and
+    chunk), timed against the same 16 frames run one at a time in the plain
+    fp32 form."""
+    import random
+
+    import torch
+
+    from sphere_stereo import FrameBatch, RGBD_Estimator, synth
+    from sphere_stereo.utils import parse_json_calib
+
+    rng = random.Random(3)
+    sets = []
+    for i in range(16):
+        res = rng.choice([512, 768, 1024])
+        N = rng.choice([16, 32, 48, 64])
+        refs = {2: [0, 2], 3: [0, 1, 2], 4: [0, 1, 2, 3]}[rng.choice([2, 3, 4])]
+        srig = synth.SyntheticRig(seed=100 + i, resolution=1216, baseline=rng.uniform(0.15, 0.40))
+        imgs, masks = srig.render(device)
+        calibs = parse_json_calib(srig.basalt_json()["value0"], [res, res], device)
+        viewpoint = torch.zeros([3], device=device)
+        for r in refs:
+            viewpoint += calibs[r].rt[:3, 3] / len(refs)
+        tm, ts, mk = synth.make_frame(srig, (res, res), (1216, 1216), refs, device=device, rendered=(imgs, masks))
+        kw = dict(min_dist=0.55, max_dist=100, candidate_count=N, references_indices=refs,
+                  reprojection_viewpoint=viewpoint, masks=mk, matching_resolution=(res, res),
+                  rgb_to_stitch_resolution=(1216, 1216), panorama_resolution=(1024, 512), sigma_i=10.0,
+                  sigma_s=12.5 * res / 512)
+        sets.append(([t.contiguous() for t in tm], [t.contiguous() for t in ts], calibs, kw, (res, N, len(refs))))
+    print("[bench] ragged batch rendered: %s" % [c for (_, _, _, _, c) in sets], file=sys.stderr, flush=True)
+    frames = [(tm, ts) for (tm, ts, _, _, _) in sets]
+    plain = [RGBD_Estimator(c, device=device, **kw) for (_, _, c, kw, _) in sets]
+    for _ in range(args.warmup):
+        for e, (tm, ts) in zip(plain, frames):
+            e.estimate_RGBD_panorama(tm, ts)
+    torch.cuda.synchronize()
+    reps = max(1, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for e, (tm, ts) in zip(plain, frames):
+            e.estimate_RGBD_panorama(tm, ts)
+    torch.cuda.synchronize()
+    v_plain = reps * len(frames) / (time.perf_counter() - t0)
+    del plain
+    torch.cuda.empty_cache()
+    ests = [RGBD_Estimator(c, device=device, volume_dtype="float16", **kw) for (_, _, c, kw, _) in sets]
+    batch = FrameBatch(ests)
+    out = {"rigs": len(sets), "configurations": len(batch.groups),
+           "shapes": ["%d^2 N%d R%d" % c for (_, _, _, _, c) in sets],
+           "plain_fp32_sequential": {"value": round(v_plain, 3), "unit": "panoramas/s"}}
+    for mode in ("eager", "graph"):
+        batch.use_graph = mode == "graph"
+        for _ in range(args.warmup):
+            batch.estimate_RGBD_panoramas(frames)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            batch.estimate_RGBD_panoramas(frames)
+        torch.cuda.synchronize()
+        v = reps * len(frames) / (time.perf_counter() - t0)
+        out["float16_batched_" + mode] = {"value": round(v, 3), "unit": "panoramas/s",
+                                          "vs_plain_fp32": round(v / v_plain, 4)}
+        print("[bench] ragged batch (%d rigs, %d configurations), fp16 %s: %.1f panoramas/s (plain fp32 one at a "
+              "time: %.1f)" % (len(sets), len(batch.groups), mode, v, v_plain), file=sys.stderr, flush=True)
+    del batch, ests
+    torch.cuda.empty_cache()
     return out
 
 
@@ -427,6 +502,8 @@ def main():
     ap.add_argument("--frames-cache", default=None, help="reuse rendered fisheyes saved at this path")
     ap.add_argument("--batch", type=int, default=8,
                     help="frames per batch of the batching form (0: skip it); each from its own seeded rig")
+    ap.add_argument("--no-ragged", action="store_true", help="skip the ragged batching form (16 rigs of mixed "
+                    "resolutions, candidate counts and reference sets)")
     ap.add_argument("--no-diff-form", action="store_true", help="skip the differentiation form's timing and "
                     "calibration refinement")
     ap.add_argument("--precision-report", action="store_true",
@@ -531,6 +608,7 @@ def main():
         del est_h
 
     batching = batching_form(args, wl, device, value) if args.batch > 0 else None
+    ragged = batching_ragged(args, device) if not args.no_ragged else None
     diff_form = differentiation_form(args, device) if not args.no_diff_form else None
 
     traffic = None
@@ -576,6 +654,7 @@ def main():
                          "ms_per_step": round(1e3 * g_elapsed / args.steps, 4)},
         "precision_forms": forms,
         "batching_form": batching,
+        "batching_form_ragged": ragged,
         "differentiation_form": diff_form,
     }
     if not args.no_cpu_baseline:

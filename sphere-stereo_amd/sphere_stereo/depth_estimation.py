@@ -96,6 +96,14 @@ class RGBD_Estimator:
         _lib.check(_lib.lib.ss_frame_create(_lib.C.byref(d), _lib.C.byref(handle)))
         return handle
 
+    def _config_key(self):
+        """What the frames of one native batch share (ss_frame_batch_create
+        checks the same fields)."""
+        st = self.fishey_stitcher
+        return (len(self.calibrations), len(self.references_indices), self.candidate_count, self.rows, self.cols,
+                st.srows, st.scols, st.prows, st.pcols, st.inpainting_iterations, self._vtype, float(self.sigma_i),
+                float(self.sigma_s), st.thr_empty, st.thr_src, tuple(self._cands_list))
+
     def _frame_desc(self):
         """The ss_frame_desc of this estimator's frame (its rig, tables and
         buffers)."""
@@ -227,14 +235,17 @@ class RGBD_Estimator:
 
 class FrameBatch:
     """The batching form (no counterpart in the reference, which runs one frame
-    per call): several estimators with one configuration (resolutions,
-    candidates, sigmas, volume type, panorama) run their frames together.
-    Their rigs and images may differ.  Every pass after the sweep is one
-    launch for a chunk of frames (up to 16 frame-reference pairs), so the
-    latency-bound coarse pyramid levels and the small stitching passes are
-    paid once per chunk instead of once per frame.  Each estimator keeps its
-    own buffers: after ``estimate_RGBD_panoramas`` its ``fishey_stitcher``
-    holds its frame's panorama, as after ``estimate_RGBD_panorama``.
+    per call): several estimators run their frames together.  Their rigs and
+    images differ, and so may their configurations (matching resolution,
+    candidate count, references, sigmas, volume type, panorama): the batch is
+    split into groups of one configuration, and each group is one native
+    batch in which every pass after the sweep is one launch for a chunk of
+    frames (up to 16 frame-reference pairs), so the latency-bound coarse
+    pyramid levels and the small stitching passes are paid once per chunk
+    instead of once per frame.  The groups are enqueued one after the other
+    on the same stream.  Each estimator keeps its own buffers: after
+    ``estimate_RGBD_panoramas`` its ``fishey_stitcher`` holds its frame's
+    panorama, as after ``estimate_RGBD_panorama``.
     """
 
     def __init__(self, estimators):
@@ -245,11 +256,18 @@ class FrameBatch:
         devs = {e.device for e in self.estimators}
         if len(devs) != 1:
             raise ValueError("the estimators of a batch must share one device")
-        descs = (_lib.ss_frame_desc * F)(*[e._frame_desc() for e in self.estimators])
-        self._frame = None
-        handle = _lib.C.c_void_p()
-        _lib.check(_lib.lib.ss_frame_batch_create(descs, F, _lib.C.byref(handle)))
-        self._frame = handle
+        # group by configuration, in order of first appearance
+        groups = {}
+        for i, e in enumerate(self.estimators):
+            groups.setdefault(e._config_key(), []).append(i)
+        self.groups = []
+        self._frames = []
+        for idx in groups.values():
+            descs = (_lib.ss_frame_desc * len(idx))(*[self.estimators[i]._frame_desc() for i in idx])
+            handle = _lib.C.c_void_p()
+            _lib.check(_lib.lib.ss_frame_batch_create(descs, len(idx), _lib.C.byref(handle)))
+            self._frames.append(handle)
+            self.groups.append(idx)
         self.use_graph = False
         self._inputs = None
 
@@ -260,18 +278,23 @@ class FrameBatch:
         frames = list(frames)
         if len(frames) != len(self.estimators):
             raise ValueError("expected %d frames" % len(self.estimators))
-        match, stitch = [], []
-        for e, (m, s) in zip(self.estimators, frames):
-            imgs, st = e._check_inputs(m, s)
-            match += imgs
-            stitch += st
-        self._inputs = (match, stitch)  # kept alive until the next call
-        _lib.check(_lib.lib.ss_frame_batch_run(self._frame, _lib.ptr_array(match), _lib.ptr_array(stitch),
-                                               int(bool(self.use_graph)), self.estimators[0]._stream()))
+        checked = [e._check_inputs(m, s) for e, (m, s) in zip(self.estimators, frames)]
+        inputs = []
+        stream = self.estimators[0]._stream()
+        for idx, handle in zip(self.groups, self._frames):
+            match, stitch = [], []
+            for i in idx:
+                match += checked[i][0]
+                stitch += checked[i][1]
+            inputs.append((match, stitch))
+            _lib.check(_lib.lib.ss_frame_batch_run(handle, _lib.ptr_array(match), _lib.ptr_array(stitch),
+                                                   int(bool(self.use_graph)), stream))
+        self._inputs = inputs  # kept alive until the next call
         return [(e.fishey_stitcher.RGB_panorama, e.fishey_stitcher.distance_panorama) for e in self.estimators]
 
     def __del__(self):
         lib = getattr(_lib, "lib", None)
-        if getattr(self, "_frame", None) and lib is not None:
-            lib.ss_frame_destroy(self._frame)
-            self._frame = None
+        for h in getattr(self, "_frames", None) or []:
+            if h and lib is not None:
+                lib.ss_frame_destroy(h)
+        self._frames = []

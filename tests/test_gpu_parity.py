@@ -388,11 +388,49 @@ def test_frame_batch_matches_single_frames(gpu, volume_dtype):
         assert junk == junk
         for e in ests:
             e.fishey_stitcher.distance_panorama.fill_(-1.0)
-    # frames of a batch share one configuration
+    # the frames of one native batch share one configuration
     calibs, kw = kws[1]
     other = RGBD_Estimator(calibs, device=gpu, volume_dtype=volume_dtype, **dict(kw, candidate_count=16))
-    with pytest.raises(_lib.SphereStereoError, match="differ"):
-        FrameBatch([ests[0], other])
+    descs = (_lib.ss_frame_desc * 2)(ests[0]._frame_desc(), other._frame_desc())
+    handle = _lib.C.c_void_p()
+    assert _lib.lib.ss_frame_batch_create(descs, 2, _lib.C.byref(handle)) == 1  # SS_ERR_ARG
+    assert b"differ" in _lib.lib.ss_last_error()
+
+
+def test_frame_batch_ragged_matches_single_frames(gpu):
+    """A ragged batch: rigs of different matching resolutions, candidate
+    counts, reference sets and volume types (FrameBatch groups them by
+    configuration); every frame bit-exact with its single-frame run, eager and
+    replayed from graphs."""
+    cases = [((80, 72), 32, (0, 2), "float32"), ((64, 64), 16, (0, 1, 2), "float16"), ((80, 72), 32, (0, 2), "float32"),
+             ((96, 80), 48, (0, 1, 2, 3), "float32"), ((64, 64), 16, (0, 1, 2), "float16"), ((80, 72), 24, (1, 3),
+                                                                                         "bfloat16")]
+    ests, frames, want = [], [], []
+    for i, (matching, N, refs, vt) in enumerate(cases):
+        fr = frame(seed=30 + i, resolution=160, matching=matching, stitch=(96, 96), refs=refs, device=gpu,
+                   baseline=0.15 + 0.05 * i)
+        kw = dict(min_dist=0.55, max_dist=100, candidate_count=N, references_indices=fr["refs"],
+                  reprojection_viewpoint=fr["viewpoint"], masks=fr["masks"], matching_resolution=fr["matching"],
+                  rgb_to_stitch_resolution=fr["stitch"], panorama_resolution=(128, 64), sigma_i=10, sigma_s=12.5)
+        est = RGBD_Estimator(fr["calibs"], device=gpu, volume_dtype=vt, **kw)
+        rgb, dist = est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])
+        want.append((rgb.clone(), dist.clone()))
+        ests.append(est)
+        frames.append(([t.contiguous() for t in fr["to_match"]], [t.contiguous() for t in fr["to_stitch"]]))
+    for e in ests:
+        e.fishey_stitcher.RGB_panorama.zero_()
+        e.fishey_stitcher.distance_panorama.zero_()
+    batch = FrameBatch(ests)
+    assert sorted(len(g) for g in batch.groups) == [1, 1, 2, 2]
+    for mode in ("eager", "graph", "graph"):
+        batch.use_graph = mode == "graph"
+        out = batch.estimate_RGBD_panoramas(frames)
+        torch.cuda.synchronize()
+        for i, ((rgb, dist), (wr, wd)) in enumerate(zip(out, want)):
+            assert torch.equal(rgb, wr), "%s rgb panorama of frame %d" % (mode, i)
+            assert torch.equal(dist.view(torch.int32), wd.view(torch.int32)), "%s distance panorama %d" % (mode, i)
+        for e in ests:
+            e.fishey_stitcher.distance_panorama.fill_(-1.0)
 
 
 @pytest.mark.parametrize("dtype", [torch.uint8, torch.uint16, torch.float32])
