@@ -399,55 +399,84 @@ __global__ void __launch_bounds__(kBlock) k_normalize(float* __restrict__ w, int
   for (int r = 0; r < R; ++r) w[(size_t)r * P + p] = w[(size_t)r * P + p] / s;
 }
 
-// mergeRGBDPanoramaKernel (stitcher.cu:423-474)
+// mergeRGBDPanoramaKernel (stitcher.cu:423-474), one lane per panorama pixel.
+// Each reference's work is a chain of three dependent gathers (blend tables,
+// then the two distance maps at the table's position, then the stitch image
+// at the reprojected position).  With RN references known at compile time
+// the chains of all references are issued side by side, stage by stage, so
+// their memory latencies overlap; the sums still run in reference order, as
+// the reference kernel accumulates them.  RN = 0 is the generic loop.
+template <int RN>
 __global__ void __launch_bounds__(kBlock) k_merge(MergeArgs a) {
+  constexpr int RM = RN > 0 ? RN : 1;
   const int idx = blockIdx.x * kBlock + threadIdx.x;
   const int P = a.PH * a.PW;
   if (idx >= P) return;
   const float rx = (float)a.Ws / (float)a.W, ry = (float)a.Hs / (float)a.H;
   const float Wsm = (float)a.Ws - 1.1f, Hsm = (float)a.Hs - 1.1f;
   float inv_sum = 0.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+  const int R = RN > 0 ? RN : a.R;
+  for (int r0 = 0; r0 < R; r0 += RM) {
+    float bw[RM], rd[RM], dc[RM], u[RM], v[RM];
+    float2 uv[RM];
 #pragma unroll
-  for (int r = 0; r < SS_MAX_REFERENCES; ++r) {
-    if (r >= a.R) break;
-    const float bw = a.weights[(size_t)r * P + idx];
-    const float2 uv = a.sampling[(size_t)r * P + idx];
-    // the reference offsets v into the stacked maps (an fp32 add), then samples
-    const float vo = uv.y + (float)(r * a.H);
-    const float* rep = a.reproj[r] - (size_t)r * a.H * a.W;
-    const float* dst = a.dists[r] - (size_t)r * a.H * a.W;
-    const float rd = interp_f(rep, a.W, uv.x, vo);
-    inv_sum = inv_sum + (bw * 1.0f) / rd;
-    float dc = interp_f(dst, a.W, uv.x, vo);
-    dc = fminf(rd, dc);
-    float px, py, pz, u, v;
-    bool valid = true;
-    unproject_c(a.intr[r], uv.x, uv.y, px, py, pz);
-    px = px * dc + a.trans[r][0];
-    py = py * dc + a.trans[r][1];
-    pz = pz * dc + a.trans[r][2];
-    project_c<false>(a.intr[r], px, py, pz, u, v, valid);
-    u = u * rx;
-    v = v * ry;
-    u = fminf(fmaxf(u, 0.1f), Wsm);
-    v = fminf(fmaxf(v, 0.1f), Hsm);
-    v = v + (float)(r * a.Hs);
-    // bilinear uint8 read (stitcher.cu:56-83) on the stacked images
-    const uint8_t* img = a.imgs[r] - (size_t)r * a.Hs * a.Ws * 3;
-    const int u1 = (int)floorf(u), v1 = (int)floorf(v), u2 = u1 + 1, v2 = v1 + 1;
-    const float u1f = (float)u1, u2f = (float)u2, v1f = (float)v1, v2f = (float)v2;
-    const float w1 = (u2f - u) * (v2f - v), w2 = (u2f - u) * (v - v1f), w3 = (u - u1f) * (v2f - v),
-                w4 = (u - u1f) * (v - v1f);
-    const uint8_t* p1 = img + ((size_t)v1 * a.Ws + u1) * 3;
-    const uint8_t* p2 = img + ((size_t)v2 * a.Ws + u1) * 3;
-    const uint8_t* p3 = img + ((size_t)v1 * a.Ws + u2) * 3;
-    const uint8_t* p4 = img + ((size_t)v2 * a.Ws + u2) * 3;
-    float s0 = (((float)p1[0] * w1 + (float)p2[0] * w2) + (float)p3[0] * w3) + (float)p4[0] * w4;
-    float s1 = (((float)p1[1] * w1 + (float)p2[1] * w2) + (float)p3[1] * w3) + (float)p4[1] * w4;
-    float s2 = (((float)p1[2] * w1 + (float)p2[2] * w2) + (float)p3[2] * w3) + (float)p4[2] * w4;
-    c0 = c0 + s0 * bw;
-    c1 = c1 + s1 * bw;
-    c2 = c2 + s2 * bw;
+    for (int j = 0; j < RM; ++j) {
+      const int r = r0 + j;
+      bw[j] = a.weights[(size_t)r * P + idx];
+      uv[j] = a.sampling[(size_t)r * P + idx];
+    }
+#pragma unroll
+    for (int j = 0; j < RM; ++j) {
+      const int r = r0 + j;
+      // the reference offsets v into the stacked maps (an fp32 add), then samples
+      const float vo = uv[j].y + (float)(r * a.H);
+      rd[j] = interp_f(a.reproj[r] - (size_t)r * a.H * a.W, a.W, uv[j].x, vo);
+      dc[j] = interp_f(a.dists[r] - (size_t)r * a.H * a.W, a.W, uv[j].x, vo);
+    }
+#pragma unroll
+    for (int j = 0; j < RM; ++j) {
+      const int r = r0 + j;
+      inv_sum = inv_sum + (bw[j] * 1.0f) / rd[j];
+      const float d = fminf(rd[j], dc[j]);
+      float px, py, pz;
+      bool valid = true;
+      unproject_c(a.intr[r], uv[j].x, uv[j].y, px, py, pz);
+      px = px * d + a.trans[r][0];
+      py = py * d + a.trans[r][1];
+      pz = pz * d + a.trans[r][2];
+      project_c<false>(a.intr[r], px, py, pz, u[j], v[j], valid);
+      u[j] = fminf(fmaxf(u[j] * rx, 0.1f), Wsm);
+      v[j] = fminf(fmaxf(v[j] * ry, 0.1f), Hsm) + (float)(r * a.Hs);
+    }
+    float t[RM][4][3];
+#pragma unroll
+    for (int j = 0; j < RM; ++j) {
+      // bilinear uint8 read (stitcher.cu:56-83) on the stacked images
+      const int r = r0 + j;
+      const uint8_t* img = a.imgs[r] - (size_t)r * a.Hs * a.Ws * 3;
+      const int u1 = (int)floorf(u[j]), v1 = (int)floorf(v[j]);
+      const uint8_t* p1 = img + ((size_t)v1 * a.Ws + u1) * 3;
+      const uint8_t* p2 = p1 + (size_t)a.Ws * 3;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        t[j][0][c] = (float)p1[c];
+        t[j][1][c] = (float)p2[c];
+        t[j][2][c] = (float)p1[3 + c];
+        t[j][3][c] = (float)p2[3 + c];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < RM; ++j) {
+      const float u1f = floorf(u[j]), v1f = floorf(v[j]), u2f = u1f + 1.0f, v2f = v1f + 1.0f;
+      const float w1 = (u2f - u[j]) * (v2f - v[j]), w2 = (u2f - u[j]) * (v[j] - v1f),
+                  w3 = (u[j] - u1f) * (v2f - v[j]), w4 = (u[j] - u1f) * (v[j] - v1f);
+      float sc[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sc[c] = ((t[j][0][c] * w1 + t[j][1][c] * w2) + t[j][2][c] * w3) + t[j][3][c] * w4;
+      c0 = c0 + sc[0] * bw[j];
+      c1 = c1 + sc[1] * bw[j];
+      c2 = c2 + sc[2] * bw[j];
+    }
   }
   a.dist_out[idx] = 1.0f / inv_sum;
   a.rgb_out[3 * idx] = (uint8_t)c0;
@@ -541,7 +570,14 @@ hipError_t launch_normalize(float* w, int R, int P, hipStream_t s) {
 }
 
 hipError_t launch_merge(const MergeArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_merge, dim3(blocks_for((long long)a.PH * a.PW)), dim3(kBlock), 0, s, a);
+  const dim3 grid(blocks_for((long long)a.PH * a.PW));
+  switch (a.R) {
+    case 1: hipLaunchKernelGGL(k_merge<1>, grid, dim3(kBlock), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_merge<2>, grid, dim3(kBlock), 0, s, a); break;
+    case 3: hipLaunchKernelGGL(k_merge<3>, grid, dim3(kBlock), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_merge<4>, grid, dim3(kBlock), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_merge<0>, grid, dim3(kBlock), 0, s, a); break;
+  }
   return hipGetLastError();
 }
 
