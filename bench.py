@@ -154,6 +154,26 @@ def batching_form(args, wl, device, plain_fp32):
     return out
 
 
+def sweep_filter_summary(wl, per_frame):
+    """Sweep + cost filter as one stage against the HBM peak: their
+    algorithmic bytes per frame (the sweep; every down pass; every up pass
+    including level 0's filter part, without the argmin the last pass fuses)
+    over the summed per-frame time of the sweep, down, up and fused
+    up + argmin launches (the fused pass counted whole)."""
+    W, H = wl["matching"]
+    N, R = wl["N"], len(wl["refs"])
+    L = levels(H, W)
+    P = [math.ceil(H / 2 ** s) * math.ceil(W / 2 ** s) for s in range(L)]
+    up0 = R * (P[1] + 2 * P[0]) * (3 + N * 4) if L > 1 else 0
+    b = algorithmic_bytes("sweep", wl) + algorithmic_bytes("isb_down", wl) + algorithmic_bytes("isb_up", wl) + up0
+    ms = sum(per_frame.get(k, 0.0) for k in ("sweep", "isb_down", "isb_up", "isb_up_argmin"))
+    if ms <= 0:
+        return None
+    gbs = b / (ms * 1e-3) / 1e9
+    return {"algorithmic_bytes": int(b), "ms_per_frame": round(ms, 4), "achieved_gbs": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def batching_ragged(args, device):
     """The ragged batching form: 16 rigs, each a seeded scene with its own
     baseline drawn from U(0.15, 0.40) m, matching resolution from {512, 768,
@@ -650,6 +670,7 @@ def main():
                      "avg_launch_ms": round(avg_launch_ms, 5), "algorithmic_bytes_per_launch": int(bytes_per_launch),
                      "valu": valu},
         "stage_ms_per_frame": {k: round(v, 4) for k, v in per_frame.items() if v > 0},
+        "sweep_filter": sweep_filter_summary(wl, per_frame),
         "graph_replay": {"value": round(args.steps / g_elapsed, 3), "unit": "panoramas/s",
                          "ms_per_step": round(1e3 * g_elapsed / args.steps, 4)},
         "precision_forms": forms,
