@@ -132,13 +132,13 @@ void carve_levels(Carver& cv, Level* lv, int L, int H, int W, int NP) {
 // With fuse_last, the level-0 up-pass computes the argmin into dist[b]
 // instead of writing the filtered volume.  vt is the element type of level
 // 0; levels >= 1 are fp32.
-int run_pyramid(Level* const* lv, int nb, int L, int G, int V, int vt, double sigma_i, double sigma_s,
+// wtab: the filter's guide weight table (launch_weight_tables with
+// var_inv_i = 1 / (2 sigma_i^2), isb_filter.py:88), already enqueued on s.
+int run_pyramid(Level* const* lv, int nb, int L, int G, int V, int vt, const float* wtab, double sigma_s,
                 bool fuse_last, const ss::ArgminParams& am, float* const* dist, hipStream_t s, int stage_down,
                 int stage_up, float* const* fill = nullptr) {
   if (nb < 1 || nb > ss::kMaxBatch) return fail(SS_ERR_ARG, "batch of %d pyramids outside [1, %d]", nb, ss::kMaxBatch);
   const double var_inv_s = 1 / (2 * sigma_s * sigma_s);
-  const double var_inv_i = 1 / (2 * sigma_i * sigma_i);
-  const float vi = (float)var_inv_i;
   const Level* l0 = lv[0];  // the shapes (identical for every entry)
   // up-pass weights into level k (isb_filter.py:110-112)
   auto weights = [&](int k, double& wu, double& wd) {
@@ -180,7 +180,7 @@ int run_pyramid(Level* const* lv, int nb, int L, int G, int V, int vt, double si
   for (int k = 1; k <= last_down; ++k) {
     for (int b = 0; b < nb; ++b) {
       const Level* v = lv[k == 1 ? nb - 1 - b : b];
-      da[b] = ss::IsbDownArgs{v[k - 1].g, v[k - 1].c, v[k - 1].h, v[k - 1].w, v[k].g, v[k].c, v[k].h, v[k].w, vi};
+      da[b] = ss::IsbDownArgs{v[k - 1].g, v[k - 1].c, v[k - 1].h, v[k - 1].w, v[k].g, v[k].c, v[k].h, v[k].w, wtab};
     }
     StageTimer t(stage_down, s);
     SS_HIP(ss::launch_isb_down(da, nb, G, V, k == 1 ? vt : SS_VOLUME_F32, s));
@@ -189,7 +189,7 @@ int run_pyramid(Level* const* lv, int nb, int L, int G, int V, int vt, double si
     ss::IsbTailArgs ta;
     memset(&ta, 0, sizeof ta);
     ta.n = L - s0;
-    ta.var_inv_i = vi;
+    ta.wtab = wtab;
     for (int b = 0; b < nb; ++b) {
       ta.g0[b] = lv[b][s0].g;
       ta.c0[b] = lv[b][s0].c;
@@ -221,7 +221,7 @@ int run_pyramid(Level* const* lv, int nb, int L, int G, int V, int vt, double si
     for (int b = 0; b < nb; ++b) {
       const Level* v = lv[b];
       ua[b] = ss::IsbUpArgs{v[k + 1].g, v[k + 1].c, v[k + 1].h, v[k + 1].w, v[k].g, v[k].c, v[k].h, v[k].w,
-                            (float)wu,  (float)wd,  vi,         fuse ? dist[b] : nullptr, am,
+                            (float)wu,  (float)wd,  wtab,       fuse ? dist[b] : nullptr, am,
                             fuse && fill ? fill[b] : nullptr};
     }
     StageTimer t(fuse ? SS_STAGE_ISB_UP_ARGMIN : stage_up, s);
@@ -460,21 +460,34 @@ size_t ss_isb_workspace_bytes(int H, int W, int N, int vtype, int planar) {
     cv.take<uint32_t>((size_t)H * W);
     cv.take<float>((size_t)H * W * NP);
   }
+  cv.take<float>(ss::kWeightTable);
   carve_levels(cv, lv, L, H, W, NP);
   return cv.off + 256;
+}
+
+// The guide weight table of a filter with sigma_i (isb_filter.py:88), built on s.
+static int build_weight_table(Carver& cv, double sigma_i, hipStream_t s, float** tab) {
+  *tab = cv.take<float>(ss::kWeightTable);
+  const float vi = (float)(1 / (2 * sigma_i * sigma_i));
+  float* const t[1] = {*tab};
+  SS_HIP(ss::launch_weight_tables(&vi, t, 1, s));
+  return SS_OK;
 }
 
 static int isb_filter_impl(uint32_t* d_guide, void* d_cost, int H, int W, int N, int vt, double sigma_i,
                            double sigma_s, Carver& cv, hipStream_t s) {
   const int L = levels_of(H, W);
   const int NP = padded(N);
+  float* wtab = nullptr;
+  const int rc = build_weight_table(cv, sigma_i, s, &wtab);
+  if (rc) return rc;
   Level lv[32];
   lv[0] = Level{H, W, d_guide, d_cost};
   carve_levels(cv, lv, L, H, W, NP);
   const int G = N == 1 ? 1 : 8, V = N == 1 ? 1 : NP / 8;
   ss::ArgminParams am{};
   Level* const one[1] = {lv};
-  return run_pyramid(one, 1, L, G, V, vt, sigma_i, sigma_s, false, am, nullptr, s, SS_STAGE_ISB_DOWN,
+  return run_pyramid(one, 1, L, G, V, vt, wtab, sigma_s, false, am, nullptr, s, SS_STAGE_ISB_DOWN,
                      SS_STAGE_ISB_UP);
 }
 
@@ -532,11 +545,13 @@ int ss_argmin_refine(const void* d_cost, int H, int W, int N, int vtype, float d
 // Workspace of ss_estimate_distance: level-0 guide and volume, the cost
 // pyramid, and the distance pyramid (which shares the guide levels).
 static size_t estimate_layout(Carver& cv, int H, int W, int N, int vt, uint32_t** g0, void** c0, Level* lv,
-                              Level* dl) {
+                              Level* dl, float** wtab = nullptr) {
   const int L = levels_of(H, W), NP = padded(N < 2 ? 2 : N);
   const size_t eb = elem_bytes(vt);
   *g0 = cv.take<uint32_t>((size_t)H * W);
   *c0 = cv.take<char>((size_t)H * W * NP * eb);
+  float* t = cv.take<float>(2 * ss::kWeightTable);  // cost filter, distance filter
+  if (wtab) *wtab = t;
   carve_levels(cv, lv, L, H, W, NP);
   for (int s = 1; s < L; ++s) {
     dl[s].h = lv[s].h;
@@ -577,15 +592,23 @@ int estimate_after_sweep(int nb, int H, int W, int N, int vt, const float* cands
   Level* dlp[kMaxBatch];
   uint32_t* g0[kMaxBatch];
   void* c0[kMaxBatch];
+  float* wtab = nullptr;  // entry 0's tables serve the whole batch (one sigma_i)
   for (int b = 0; b < nb; ++b) {
     Carver cv(ws[b]);
-    estimate_layout(cv, H, W, N, vt, &g0[b], &c0[b], lv[b], dl[b]);
+    estimate_layout(cv, H, W, N, vt, &g0[b], &c0[b], lv[b], dl[b], b == 0 ? &wtab : nullptr);
     lv[b][0] = Level{H, W, g0[b], c0[b]};
     dl[b][0] = Level{H, W, g0[b], d_dist[b]};
     lvp[b] = lv[b];
     dlp[b] = dl[b];
   }
   const float d0 = cands[0], dN1 = cands[N - 1];
+  {
+    // the cost filter's table (sigma_i) and the distance filter's (sigma_i / 2,
+    // depth_estimation.py:201-202), one launch
+    const float vi[2] = {(float)(1 / (2 * sigma_i * sigma_i)), (float)(1 / (2 * (sigma_i / 2) * (sigma_i / 2)))};
+    float* const t[2] = {wtab, wtab + ss::kWeightTable};
+    SS_HIP(ss::launch_weight_tables(vi, t, 2, s));
+  }
   if (N == 1) {
     // one candidate: max == min everywhere, so the reference returns d_{N-1}
     // (depth_estimation.py:198) whatever the cost
@@ -595,14 +618,14 @@ int estimate_after_sweep(int nb, int H, int W, int N, int vt, const float* cands
     }
   } else {
     ss::ArgminParams am{N, d0, dN1, (float)(d0 / dN1) - 1.0f};
-    const int rc = run_pyramid(lvp, nb, L, 8, ss::lanes_v(N), vt, sigma_i, sigma_s, true, am, d_dist, s,
+    const int rc = run_pyramid(lvp, nb, L, 8, ss::lanes_v(N), vt, wtab, sigma_s, true, am, d_dist, s,
                                SS_STAGE_ISB_DOWN, SS_STAGE_ISB_UP, L > 1 ? fill : nullptr);
     if (rc) return rc;
     if (fill && L == 1)
       for (int b = 0; b < nb; ++b) SS_HIP(ss::launch_fill(fill[b], 1e8f, (size_t)H * W, s));
   }
-  return run_pyramid(dlp, nb, L, 1, 1, SS_VOLUME_F32, sigma_i / 2, sigma_s / 2, false, ss::ArgminParams{}, nullptr, s,
-                     SS_STAGE_DIST_DOWN, SS_STAGE_DIST_UP);
+  return run_pyramid(dlp, nb, L, 1, 1, SS_VOLUME_F32, wtab + ss::kWeightTable, sigma_s / 2, false, ss::ArgminParams{},
+                     nullptr, s, SS_STAGE_DIST_DOWN, SS_STAGE_DIST_UP);
 }
 
 // Sweeps (and guides) of R references into their estimate workspaces, in as

@@ -56,8 +56,7 @@ __device__ __forceinline__ void down_pixel(const IsbDownArgs& a, int pix, bool a
     const int row = pick3((j == 0 || j == 6 || j == 7) ? 0 : ((j == 1 || j == 5) ? 1 : 2), ym, y2, yp);
     const int col = pick3((j <= 2) ? 0 : ((j == 3 || j == 7) ? 1 : 2), xm, x2, xp);
     const float base = (j & 1) ? 0.5f : 0.25f;
-    const float d = guide_sad(gc, a.gin[row * a.win + col]);
-    return base * (expf_cw(-d * d * a.var_inv_i) + 0.01f);
+    return base * (a.wtab[guide_sad_u(gc, a.gin[row * a.win + col])] + 0.01f);
   };
 
   float w[8];
@@ -232,8 +231,7 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
     const int col = pick3(cc, c0, x, c2);
     const int s = up_stencil(k);
     const uint32_t gf = pick4(s, gF[0], gF[1], gF[2], gF[3]);
-    const float d = guide_sad(a.gin[row * a.win + col], gf);
-    return up_base(k) * (a.weight_down * (expf_cw(-d * d * a.var_inv_i) + 0.01f));
+    return up_base(k) * (a.weight_down * (a.wtab[guide_sad_u(a.gin[row * a.win + col], gf)] + 0.01f));
   };
 
   float w[25];
@@ -360,7 +358,7 @@ __global__ void __launch_bounds__(kTailBlock) k_isb_tail(IsbTailArgs t) {
     return k == 0 ? c0 : static_cast<void*>(smem + t.coff[k]);
   };
   for (int k = 1; k < t.n; ++k) {
-    const IsbDownArgs a{gp(k - 1), cp(k - 1), t.h[k - 1], t.w[k - 1], gp(k), cp(k), t.h[k], t.w[k], t.var_inv_i};
+    const IsbDownArgs a{gp(k - 1), cp(k - 1), t.h[k - 1], t.w[k - 1], gp(k), cp(k), t.h[k], t.w[k], t.wtab};
     const int npix = t.h[k] * t.w[k];
     for (int base = 0; base < npix; base += slots)
       if (base + slot < npix) down_pixel<G, V, float>(a, base + slot, true, lane);
@@ -368,12 +366,27 @@ __global__ void __launch_bounds__(kTailBlock) k_isb_tail(IsbTailArgs t) {
   }
   for (int k = t.n - 2; k >= 0; --k) {
     const IsbUpArgs a{gp(k + 1), cp(k + 1), t.h[k + 1], t.w[k + 1], gp(k), cp(k), t.h[k], t.w[k],
-                      t.wu[k],   t.wd[k],   t.var_inv_i, nullptr,  ArgminParams{}, nullptr};
+                      t.wu[k],   t.wd[k],   t.wtab, nullptr,  ArgminParams{}, nullptr};
     const int npix = t.h[k + 1] * t.w[k + 1];
     for (int base = 0; base < npix; base += slots)
       if (base + slot < npix) up_pixel<G, V, float, false>(a, base + slot, true, lane);
     __syncthreads();
   }
+}
+
+// Weight tables: entry d of table i is the guide weight factor
+// exp(-d * d * vi[i]) with the operations and exp of the passes' own
+// expression (isb_filter.cu:87 and 229-248: __expf(-d*d*var_inv)), so a
+// lookup returns the bits the expression would.
+struct WeightTableArgs {
+  float vi[kMaxBatch];
+  float* tab[kMaxBatch];
+};
+__global__ void __launch_bounds__(kBlock) k_weight_tables(WeightTableArgs t) {
+  const int d = blockIdx.x * kBlock + threadIdx.x;
+  if (d >= kWeightTable) return;
+  const float df = (float)d;
+  t.tab[blockIdx.y][d] = expf_cw(-df * df * t.vi[blockIdx.y]);
 }
 
 // Argmin over an already filtered level-0 volume (unfused form).
@@ -506,6 +519,18 @@ hipError_t launch_isb_down(const IsbDownArgs* a, int nb, int G, int V, int vt, h
 #define SS_CALL(T) down_t<T>(a, nb, V, s)
   SS_DISPATCH_T(vt, SS_CALL)
 #undef SS_CALL
+}
+
+hipError_t launch_weight_tables(const float* vi, float* const* tab, int n, hipStream_t s) {
+  if (n < 1 || n > kMaxBatch) return hipErrorInvalidValue;
+  WeightTableArgs t;
+  memset(&t, 0, sizeof t);
+  for (int i = 0; i < n; ++i) {
+    t.vi[i] = vi[i];
+    t.tab[i] = tab[i];
+  }
+  hipLaunchKernelGGL(k_weight_tables, dim3((kWeightTable + kBlock - 1) / kBlock, n), dim3(kBlock), 0, s, t);
+  return hipGetLastError();
 }
 
 hipError_t launch_isb_tail(const IsbTailArgs& t, int nb, int G, int V, hipStream_t s) {

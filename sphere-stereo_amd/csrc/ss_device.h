@@ -233,6 +233,9 @@ __device__ __forceinline__ float guide_sad(uint32_t a, uint32_t b) {
   return (float)__builtin_amdgcn_sad_u8(a, b, 0u);
 }
 
+// The same distance as an integer, the index of a filter's weight table.
+__device__ __forceinline__ uint32_t guide_sad_u(uint32_t a, uint32_t b) { return __builtin_amdgcn_sad_u8(a, b, 0u); }
+
 __device__ __forceinline__ float gch(uint32_t g, int c) { return (float)((g >> (8 * c)) & 0xffu); }
 
 // float -> uint8 by truncation (vec_utils.cuh:111-114); inputs are convex

@@ -11,6 +11,12 @@
 namespace ss {
 
 // Volumes are passed untyped; the kernels cast them to their storage type.
+// The guide weights exp(-d^2 var_inv_i) of a filter depend on the guide
+// distance d alone, an integer in [0, 765] (a sum of three byte
+// differences): each filter call tabulates them once (kWeightTable floats,
+// launch_weight_tables) and its passes look them up.
+constexpr int kWeightTable = 768;
+
 struct IsbDownArgs {
   const uint32_t* gin;
   const void* cin;
@@ -18,7 +24,7 @@ struct IsbDownArgs {
   uint32_t* gout;
   void* cout;
   int hout, wout;
-  float var_inv_i;
+  const float* wtab;
 };
 
 struct ArgminParams {
@@ -33,7 +39,8 @@ struct IsbUpArgs {
   uint32_t* gio;
   void* cio;
   int hio, wio;
-  float weight_up, weight_down, var_inv_i;
+  float weight_up, weight_down;
+  const float* wtab;
   float* dist;  // fused argmin output (level-0 distance map)
   ArgminParams am;
   float* fill;  // nullable: with the fused argmin, set to 1e8 at every level-0 pixel
@@ -65,7 +72,7 @@ struct IsbTailArgs {
   unsigned goff[kMaxTailLevels], coff[kMaxTailLevels];
   int h[kMaxTailLevels], w[kMaxTailLevels];
   float wu[kMaxTailLevels], wd[kMaxTailLevels];  // up pass into entry k
-  float var_inv_i;
+  const float* wtab;
   unsigned lds_bytes;
 };
 
@@ -157,6 +164,8 @@ bool profiler_active();
 hipError_t launch_isb_down(const IsbDownArgs* a, int nb, int G, int V, int vt, hipStream_t s);
 hipError_t launch_isb_up(const IsbUpArgs* a, int nb, int G, int V, int vt, bool fuse_argmin, hipStream_t s);
 hipError_t launch_isb_tail(const IsbTailArgs& t, int nb, int G, int V, hipStream_t s);
+// n weight tables in one launch: tab[i][d] = exp(-d^2 vi[i]) for d < kWeightTable
+hipError_t launch_weight_tables(const float* vi, float* const* tab, int n, hipStream_t s);
 hipError_t launch_argmin(const void* cost, int npix, int V, int vt, const ArgminParams& p, float* dist,
                          hipStream_t s);
 hipError_t launch_planar_to_cc(const float* src, float* dst, int HW, int N, int NP, hipStream_t s);
