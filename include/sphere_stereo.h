@@ -133,7 +133,8 @@ int ss_sweep(const float *d_raylut, const float *const *d_images, int C, const s
 
 /* Bytes of workspace for ss_isb_filter (vtype volume) / ss_isb_filter_planar
  * (planar = 1, vtype must be SS_VOLUME_F32): pyramid levels 1..L-1 of guides
- * and volume, plus the level-0 repack for the planar form. */
+ * and volume, the guide-weight table of the call (768 floats), plus the
+ * level-0 repack for the planar form. */
 size_t ss_isb_workspace_bytes(int H, int W, int N, int vtype, int planar);
 
 /* ISB_Filter.apply (isb_filter.py:72-136) on the library layout: d_guide
