@@ -119,8 +119,9 @@ def batching_form(args, wl, device, plain_fp32):
     """The batching form: args.batch frames of the workload's configuration,
     each from its own seeded rig (seed, seed + 1, ...), through FrameBatch
     (every pass after the sweep one launch per chunk of 16 frame-reference
-    pairs), timed like the plain form; fp32 and fp16 volumes.  value =
-    panoramas/s over all frames of the batch."""
+    pairs, the sweep's variant picked by the autotuner first), timed like the
+    plain form; fp32 and fp16 volumes.  value = panoramas/s over all frames of
+    the batch."""
     import torch
 
     from sphere_stereo import FrameBatch, RGBD_Estimator
@@ -136,6 +137,7 @@ def batching_form(args, wl, device, plain_fp32):
         ests = [RGBD_Estimator(c, device=device, volume_dtype=dt, **kw) for (_, _, c, kw) in sets]
         batch = FrameBatch(ests)
         frames = [(tm, ts) for (tm, ts, _, _) in sets]
+        tuned = batch.autotune(frames)  # the sweep autotuner, per configuration
         for _ in range(args.warmup):
             batch.estimate_RGBD_panoramas(frames)
         torch.cuda.synchronize()
@@ -146,7 +148,7 @@ def batching_form(args, wl, device, plain_fp32):
         el = time.perf_counter() - t0
         v = args.steps * args.batch / el
         out[dt] = {"value": round(v, 3), "unit": "panoramas/s", "ms_per_batch": round(1e3 * el / args.steps, 4),
-                   "vs_plain_fp32": round(v / plain_fp32, 4)}
+                   "vs_plain_fp32": round(v / plain_fp32, 4), "sweep_autotune": tuned[0]}
         print("[bench] batching form, %d frames, %s volume: %.1f panoramas/s" % (args.batch, dt, v), file=sys.stderr,
               flush=True)
         del batch, ests
@@ -225,7 +227,9 @@ def batching_ragged(args, device):
     torch.cuda.empty_cache()
     ests = [RGBD_Estimator(c, device=device, volume_dtype="float16", **kw) for (_, _, c, kw, _) in sets]
     batch = FrameBatch(ests)
+    tuned = batch.autotune(frames)
     out = {"rigs": len(sets), "configurations": len(batch.groups),
+           "sweep_variants": [t["variant"] for t in tuned],
            "shapes": ["%d^2 N%d R%d" % c for (_, _, _, _, c) in sets],
            "plain_fp32_sequential": {"value": round(v_plain, 3), "unit": "panoramas/s"}}
     for mode in ("eager", "graph"):

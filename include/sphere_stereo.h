@@ -131,6 +131,25 @@ int ss_sweep(const float *d_raylut, const float *const *d_images, int C, const s
              int ref, const float *cands, int N, int H, int W, int vtype, void *d_cost, uint32_t *d_guide,
              ss_stream stream);
 
+/* Sweep autotuning (no counterpart in the reference).  The sweep has
+ * ss_sweep_variant_count() scheduling variants that compute identical bits:
+ * variant v projects `chunk` candidates before loading their taps and stages
+ * `group` candidates in LDS per flush (ss_sweep_variant_info).  Every sweep
+ * of a configuration (H, W, N, vtype) uses the variant recorded for it, 0 if
+ * none.  ss_sweep_autotune times every variant on the given inputs (one
+ * warm-up and `repeats` timed launches each, synchronising `stream`; not
+ * while it is being captured), records the fastest for the configuration
+ * and returns it; ms_out (nullable) receives the per-launch milliseconds of
+ * each variant.  d_cost is scratch of ss_sweep's size.  The record is per
+ * process; graphs captured before it keep their variant. */
+int ss_sweep_variant_count(void);
+int ss_sweep_variant_info(int variant, int *chunk, int *group);
+int ss_sweep_set_variant(int H, int W, int N, int vtype, int variant); /* -1 forgets the record */
+int ss_sweep_get_variant(int H, int W, int N, int vtype);
+int ss_sweep_autotune(const float *d_raylut, const float *const *d_images, int C, const ss_camera *cams,
+                      const float *rts, int ref, const float *cands, int N, int H, int W, int vtype, void *d_cost,
+                      int repeats, ss_stream stream, float *ms_out, int *best_out);
+
 /* Bytes of workspace for ss_isb_filter (vtype volume) / ss_isb_filter_planar
  * (planar = 1, vtype must be SS_VOLUME_F32): pyramid levels 1..L-1 of guides
  * and volume, the guide-weight table of the call (768 floats), plus the

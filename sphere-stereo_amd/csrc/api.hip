@@ -280,9 +280,38 @@ bool sweep_fast_bounds(const ss::SweepArgs& a, int C, int npairs) {
   return true;
 }
 
+// The sweep variant chosen per configuration by ss_sweep_autotune (or
+// ss_sweep_set_variant); configurations never tuned use variant 0.
+struct SweepTune {
+  int H, W, N, vt, variant;
+};
+std::mutex g_tune_mu;
+std::vector<SweepTune> g_tune;
+
+int sweep_variant_for(int H, int W, int N, int vt) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (const SweepTune& t : g_tune)
+    if (t.H == H && t.W == W && t.N == N && t.vt == vt) return t.variant;
+  return 0;
+}
+
+void set_sweep_variant(int H, int W, int N, int vt, int variant) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (size_t i = 0; i < g_tune.size(); ++i)
+    if (g_tune[i].H == H && g_tune[i].W == W && g_tune[i].N == N && g_tune[i].vt == vt) {
+      if (variant < 0)
+        g_tune.erase(g_tune.begin() + (long)i);
+      else
+        g_tune[i].variant = variant;
+      return;
+    }
+  if (variant >= 0) g_tune.push_back(SweepTune{H, W, N, vt, variant});
+}
+
+// variant < 0: the configuration's tuned variant
 int launch_sweep_refs(const float* const* d_raylut, const float* const* d_images, int C, const ss_camera* cams,
                       const float* const* rts, const int* refs, int R, const float* cands, int N, int H, int W,
-                      int vt, void* const* costs, uint32_t* const* guides, hipStream_t s) {
+                      int vt, void* const* costs, uint32_t* const* guides, hipStream_t s, int variant = -1) {
   ss::SweepArgs a;
   memset(&a, 0, sizeof a);
   for (int k = 0; k < C; ++k) {
@@ -320,7 +349,7 @@ int launch_sweep_refs(const float* const* d_raylut, const float* const* d_images
   a.pow2 = pow2(W) && pow2(H);
   a.fastproj = sweep_fast_bounds(a, C, R * C);
   StageTimer t(SS_STAGE_SWEEP, s);
-  SS_HIP(ss::launch_sweep(a, vt, s));
+  SS_HIP(ss::launch_sweep(a, vt, variant < 0 ? sweep_variant_for(H, W, N, vt) : variant, s));
   return SS_OK;
 }
 
@@ -448,6 +477,79 @@ int ss_sweep(const float* d_raylut, const float* const* d_images, int C, const s
   rc = launch_sweep_refs(&d_raylut, d_images, C, cams, &rts, &ref, 1, cands, N, H, W, vtype, &d_cost, &d_guide,
                          as_stream(stream));
   return rc ? rc : ok();
+}
+
+int ss_sweep_variant_count(void) { return ss::sweep_variant_count(); }
+
+int ss_sweep_variant_info(int variant, int* chunk, int* group) {
+  SS_REQUIRE(chunk && group, "null output");
+  SS_REQUIRE(variant >= 0 && variant < ss::sweep_variant_count(), "sweep variant %d outside [0, %d)", variant,
+             ss::sweep_variant_count());
+  ss::sweep_variant_info(variant, chunk, group);
+  return ok();
+}
+
+int ss_sweep_set_variant(int H, int W, int N, int vtype, int variant) {
+  SS_REQUIRE(variant >= -1 && variant < ss::sweep_variant_count(), "sweep variant %d outside [-1, %d)", variant,
+             ss::sweep_variant_count());
+  set_sweep_variant(H, W, N, vtype, variant);
+  return ok();
+}
+
+int ss_sweep_get_variant(int H, int W, int N, int vtype) { return sweep_variant_for(H, W, N, vtype); }
+
+int ss_sweep_autotune(const float* d_raylut, const float* const* d_images, int C, const ss_camera* cams,
+                      const float* rts, int ref, const float* cands, int N, int H, int W, int vtype, void* d_cost,
+                      int repeats, ss_stream stream, float* ms_out, int* best_out) {
+  int rc = check_cams(cams, rts, C, ref);
+  if (rc) return rc;
+  rc = check_sweep(d_images, C, cams, cands, N, H, W, vtype);
+  if (rc) return rc;
+  SS_REQUIRE(d_raylut && d_cost, "null pointer argument");
+  SS_REQUIRE(N >= 2, "the sweep needs at least two candidates");
+  SS_REQUIRE(repeats >= 1 && repeats <= 1000, "repeats %d outside [1, 1000]", repeats);
+  hipStream_t s = as_stream(stream);
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  SS_HIP(hipStreamIsCapturing(s, &cs));
+  SS_REQUIRE(cs == hipStreamCaptureStatusNone, "ss_sweep_autotune times launches: not while capturing a graph");
+  hipEvent_t e0, e1;
+  SS_HIP(hipEventCreate(&e0));
+  SS_HIP(hipEventCreate(&e1));
+  const float* lut[1] = {d_raylut};
+  const float* rt[1] = {rts};
+  void* cost[1] = {d_cost};
+  const int nv = ss::sweep_variant_count();
+  int best = 0;
+  float best_ms = 0.0f;
+  for (int v = 0; v < nv && rc == SS_OK; ++v) {
+    rc = launch_sweep_refs(lut, d_images, C, cams, rt, &ref, 1, cands, N, H, W, vtype, cost, nullptr, s, v);
+    if (rc) break;
+    if (hipEventRecord(e0, s) != hipSuccess) {
+      rc = fail(SS_ERR_HIP, "hipEventRecord failed");
+      break;
+    }
+    for (int i = 0; i < repeats && rc == SS_OK; ++i)
+      rc = launch_sweep_refs(lut, d_images, C, cams, rt, &ref, 1, cands, N, H, W, vtype, cost, nullptr, s, v);
+    if (rc) break;
+    float ms = 0.0f;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      rc = fail(SS_ERR_HIP, "timing the sweep variants failed");
+      break;
+    }
+    ms /= (float)repeats;
+    if (ms_out) ms_out[v] = ms;
+    if (v == 0 || ms < best_ms) {
+      best = v;
+      best_ms = ms;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  set_sweep_variant(H, W, N, vtype, best);
+  if (best_out) *best_out = best;
+  return ok();
 }
 
 size_t ss_isb_workspace_bytes(int H, int W, int N, int vtype, int planar) {

@@ -177,7 +177,10 @@ hipError_t launch_packed_to_guide3(const uint32_t* g, uint8_t* g3, int HW, hipSt
 hipError_t launch_guide(const float* img, int HW, uint32_t* guide, hipStream_t s);
 hipError_t launch_ray_lut(const TorchCam& ref, const int8_t* sel, int H, int W, float4* lut, hipStream_t s);
 hipError_t launch_select_camera(const SelectArgs& a, hipStream_t s);
-hipError_t launch_sweep(const SweepArgs& a, int vt, hipStream_t s);
+hipError_t launch_sweep(const SweepArgs& a, int vt, int variant, hipStream_t s);
+// the sweep's scheduling variants (candidate chunk, flush group); 0 is the default
+int sweep_variant_count();
+void sweep_variant_info(int v, int* chunk, int* group);
 
 // api.hip: pieces of ss_estimate_distance the frame executor composes
 size_t volume_elem_bytes(int vt);

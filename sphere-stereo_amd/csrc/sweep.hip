@@ -137,12 +137,18 @@ __global__ void __launch_bounds__(kBlock) k_select_camera(SelectArgs a) {
 // store would touch 64 lines partially, and lines evicted half-written
 // multiplied the HBM write traffic about three times.  The lane also writes
 // its pixel's guide (utils.py:144-155), which needs the same reference colour.
+//
+// Two scheduling parameters are template arguments, picked per configuration
+// by ss_sweep_autotune: CH candidates are projected before their taps are
+// loaded (more loads in flight per lane against more registers), and GRP
+// candidates are staged per flush (LDS per workgroup, hence occupancy,
+// against flush count).  The arithmetic of a sample does not depend on them.
 constexpr int kTileW = 64, kTileH = kBlock / kTileW;
-constexpr int kGroup = 32;               // candidates staged per flush
-constexpr int kStride = kGroup + 1;      // LDS words per pixel (conflict-free)
 
-template <typename T, bool POW2>
+template <typename T, bool POW2, int CH, int GRP>
 __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
+  constexpr int kChunk = CH, kGroup = GRP;
+  constexpr int kStride = kGroup + 1;  // LDS words per pixel (conflict-free)
   __shared__ float stage[kTileH][kTileW * kStride];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int r = tile / a.blocks_per_ref;
@@ -187,19 +193,19 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
   for (int g0 = 0; g0 < a.NP; g0 += kGroup) {
     const int cnt = a.NP - g0 < kGroup ? a.NP - g0 : kGroup;  // a multiple of 8
 #pragma unroll 1
-    for (int j = 0; j < cnt; j += 4) {
+    for (int j = 0; j < cnt; j += kChunk) {
       const int n0 = g0 + j;
-      float out[4];
+      float out[kChunk];
       if (k < 0) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) out[v] = (n0 + v < a.N) ? cnone : 0.0f;
+        for (int v = 0; v < kChunk; ++v) out[v] = (n0 + v < a.N) ? cnone : 0.0f;
       } else {
-        // sample coordinates of the chunk's 4 candidates (pads sample a
+        // sample coordinates of the chunk's candidates (pads sample a
         // dummy interior point and are stored as 0)
-        float ix[4], iy[4];
+        float ix[kChunk], iy[kChunk];
         bool inner = true;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
+        for (int v = 0; v < kChunk; ++v) {
           ix[v] = 0.5f;
           iy[v] = 0.5f;
           if (n0 + v < a.N) {
@@ -221,11 +227,11 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
           inner = inner && x0f >= 0.0f && x0f < wlim && y0f >= 0.0f && y0f < hlim;
         }
         if (inner) {
-          // all 16 taps inside the image: issue every load, then combine
+          // all taps of the chunk inside the image: issue every load, then combine
           // with the arithmetic of sample_zero (no tap clamped or masked)
-          float tn[4][6], ts[4][6];
+          float tn[kChunk][6], ts[kChunk][6];
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
+          for (int v = 0; v < kChunk; ++v) {
             const float* pn = img + (unsigned)((int)floorf(iy[v]) * a.W + (int)floorf(ix[v])) * 3u;
             const float* ps = pn + (unsigned)a.W * 3u;
 #pragma unroll
@@ -235,7 +241,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
             }
           }
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
+          for (int v = 0; v < kChunk; ++v) {
             const float x0f = floorf(ix[v]), y0f = floorf(iy[v]);
             const float x1f = x0f + 1.0f, y1f = y0f + 1.0f;
             const float wnw = (x1f - ix[v]) * (y1f - iy[v]), wne = (ix[v] - x0f) * (y1f - iy[v]);
@@ -255,7 +261,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
           }
         } else {
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
+          for (int v = 0; v < kChunk; ++v) {
             float c = 0.0f;
             if (n0 + v < a.N) {
               float sv[3];
@@ -268,7 +274,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
         }
       }
 #pragma unroll
-      for (int v = 0; v < 4; ++v) st[lane * kStride + j + v] = out[v];
+      for (int v = 0; v < kChunk; ++v) st[lane * kStride + j + v] = out[v];
     }
     // the wavefront's block: all its lanes' LDS writes precede the reads
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -292,13 +298,24 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
   }
 }
 
-template <typename T>
-hipError_t launch_sweep_t(const SweepArgs& a, hipStream_t s) {
+template <typename T, int CH, int GRP>
+void launch_sweep_v(const SweepArgs& a, hipStream_t s) {
   const dim3 grid(a.blocks_per_ref * a.R);
   if (a.pow2)
-    hipLaunchKernelGGL((k_sweep<T, true>), grid, dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_sweep<T, true, CH, GRP>), grid, dim3(kBlock), 0, s, a);
   else
-    hipLaunchKernelGGL((k_sweep<T, false>), grid, dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_sweep<T, false, CH, GRP>), grid, dim3(kBlock), 0, s, a);
+}
+
+// the variants: (candidate chunk, flush group)
+template <typename T>
+hipError_t launch_sweep_t(const SweepArgs& a, int variant, hipStream_t s) {
+  switch (variant) {
+    case 1: launch_sweep_v<T, 2, 32>(a, s); break;
+    case 2: launch_sweep_v<T, 4, 16>(a, s); break;
+    case 3: launch_sweep_v<T, 2, 16>(a, s); break;
+    default: launch_sweep_v<T, 4, 32>(a, s); break;
+  }
   return hipGetLastError();
 }
 
@@ -319,8 +336,15 @@ hipError_t launch_select_camera(const SelectArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_sweep(const SweepArgs& a, int vt, hipStream_t s) {
-#define SS_CALL(T) launch_sweep_t<T>(a, s)
+int sweep_variant_count() { return 4; }
+void sweep_variant_info(int v, int* chunk, int* group) {
+  static const int kInfo[4][2] = {{4, 32}, {2, 32}, {4, 16}, {2, 16}};
+  *chunk = kInfo[v][0];
+  *group = kInfo[v][1];
+}
+
+hipError_t launch_sweep(const SweepArgs& a, int vt, int variant, hipStream_t s) {
+#define SS_CALL(T) launch_sweep_t<T>(a, variant, s)
   SS_DISPATCH_T(vt, SS_CALL)
 #undef SS_CALL
 }

@@ -63,6 +63,11 @@ SIGNATURES = {
     "ss_normalize_weights": (_i, [_p, _i, _i, _p]),
     "ss_guide": (_i, [_p, _i, _i, _p, _p]),
     "ss_sweep": (_i, [_p, _p, _i, _CAM, _p, _i, _p, _i, _i, _i, _i, _p, _p, _p]),
+    "ss_sweep_variant_count": (_i, []),
+    "ss_sweep_variant_info": (_i, [_i, _p, _p]),
+    "ss_sweep_set_variant": (_i, [_i, _i, _i, _i, _i]),
+    "ss_sweep_get_variant": (_i, [_i, _i, _i, _i]),
+    "ss_sweep_autotune": (_i, [_p, _p, _i, _CAM, _p, _i, _p, _i, _i, _i, _i, _p, _i, _p, _p, _p]),
     "ss_isb_workspace_bytes": (_z, [_i, _i, _i, _i, _i]),
     "ss_isb_filter": (_i, [_p, _p, _i, _i, _i, _i, _d, _d, _p, _z, _p]),
     "ss_isb_filter_planar": (_i, [_p, _p, _i, _i, _i, _d, _d, _p, _z, _p]),

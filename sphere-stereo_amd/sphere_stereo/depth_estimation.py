@@ -96,6 +96,30 @@ class RGBD_Estimator:
         _lib.check(_lib.lib.ss_frame_create(_lib.C.byref(d), _lib.C.byref(handle)))
         return handle
 
+    def autotune(self, images_to_match, repeats=5):
+        """Times the sweep's scheduling variants (ss_sweep_autotune: identical
+        results, different candidate chunking and LDS staging) on these
+        images for the first reference and records the fastest for this
+        configuration; every later sweep of the configuration uses it.
+        Returns {"variant": v, "ms": [per-launch ms of each variant]}."""
+        imgs = [_hwc(i) for i in images_to_match]
+        if len(imgs) != len(self.calibrations):
+            raise ValueError("expected %d images to match" % len(self.calibrations))
+        if self.candidate_count < 2:
+            return {"variant": 0, "ms": []}
+        nv = _lib.lib.ss_sweep_variant_count()
+        NP = _lib.lib.ss_padded_candidates(self.candidate_count)
+        eb = 4 if self.volume_dtype == "float32" else 2
+        scratch = torch.empty(self.rows * self.cols * NP * eb, dtype=torch.uint8, device=self.device)
+        ms = (_lib.C.c_float * nv)()
+        best = _lib.C.c_int(0)
+        r = self.references_indices[0]
+        _lib.check(_lib.lib.ss_sweep_autotune(self._raylut[0].data_ptr(), _lib.ptr_array(imgs), len(imgs), self._cams,
+                                              self._rts[r], r, self._cands, self.candidate_count, self.rows,
+                                              self.cols, self._vtype, scratch.data_ptr(), int(repeats),
+                                              self._stream(), ms, _lib.C.byref(best)))
+        return {"variant": int(best.value), "ms": [round(float(v), 4) for v in ms]}
+
     def _config_key(self):
         """What the frames of one native batch share (ss_frame_batch_create
         checks the same fields)."""
@@ -270,6 +294,25 @@ class FrameBatch:
             self.groups.append(idx)
         self.use_graph = False
         self._inputs = None
+
+    def autotune(self, frames, repeats=5):
+        """The sweep autotuner (RGBD_Estimator.autotune) for each configuration
+        group of the batch, on its first frame; then drops captured graphs so
+        that the next graph replay records the chosen variants.  Returns one
+        result per group."""
+        frames = list(frames)
+        out = []
+        for idx in self.groups:
+            out.append(self.estimators[idx[0]].autotune(frames[idx[0]][0], repeats))
+        for i, h in enumerate(self._frames):
+            # recreate the native batches: a captured graph keeps its kernels
+            _lib.lib.ss_frame_destroy(h)
+            descs = (_lib.ss_frame_desc * len(self.groups[i]))(*[self.estimators[j]._frame_desc()
+                                                                   for j in self.groups[i]])
+            handle = _lib.C.c_void_p()
+            _lib.check(_lib.lib.ss_frame_batch_create(descs, len(self.groups[i]), _lib.C.byref(handle)))
+            self._frames[i] = handle
+        return out
 
     def estimate_RGBD_panoramas(self, frames):
         """frames: one (images_to_match, images_to_stitch) pair per estimator,

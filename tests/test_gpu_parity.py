@@ -483,3 +483,52 @@ def test_streaming_driver_matches_frame_by_frame(gpu):
         assert np.array_equal(rgb, wr), "rgb panorama of frame %d" % i
         assert np.array_equal(inv.view(np.int32), wi.view(np.int32)), "inverse distance of frame %d" % i
     est.use_graph = False
+
+
+@pytest.mark.parametrize("N,vtype", [(32, "float32"), (20, "float16"), (48, "bfloat16")])
+def test_sweep_variants_identical_and_autotune(gpu, N, vtype):
+    """Every scheduling variant of the sweep writes the same volume and guide
+    bits; the autotuner times them all, records the fastest for the
+    configuration, and frames then run with it unchanged in result."""
+    fr = frame(seed=4, resolution=160, matching=(80, 72), stitch=(96, 96), refs=(0, 2), device=gpu)
+    kw = dict(min_dist=0.55, max_dist=100, candidate_count=N, references_indices=fr["refs"],
+              reprojection_viewpoint=fr["viewpoint"], masks=fr["masks"], matching_resolution=fr["matching"],
+              rgb_to_stitch_resolution=fr["stitch"], panorama_resolution=(128, 64), sigma_i=10, sigma_s=12.5)
+    est = RGBD_Estimator(fr["calibs"], device=gpu, volume_dtype=vtype, **kw)
+    lib = _lib.lib
+    H, W, vt = est.rows, est.cols, _lib.VOLUME_TYPES[vtype]
+    NP = lib.ss_padded_candidates(N)
+    eb = 4 if vtype == "float32" else 2
+    imgs = [t.contiguous() for t in fr["to_match"]]
+    r = est.references_indices[0]
+    stream = _lib.C.c_void_p(torch.cuda.current_stream(gpu).cuda_stream)
+    nv = lib.ss_sweep_variant_count()
+    assert nv >= 2
+    outs = []
+    for v in range(nv):
+        assert lib.ss_sweep_set_variant(H, W, N, vt, v) == 0
+        cost = torch.full((H * W * NP * eb,), 0xAB, dtype=torch.uint8, device=gpu)
+        guide = torch.full((H * W,), -1, dtype=torch.int32, device=gpu)
+        _lib.check(lib.ss_sweep(est._raylut[0].data_ptr(), _lib.ptr_array(imgs), len(imgs), est._cams, est._rts[r], r,
+                                est._cands, N, H, W, vt, cost.data_ptr(), guide.data_ptr(), stream))
+        torch.cuda.synchronize()
+        outs.append((cost, guide))
+    for v in range(1, nv):
+        chunk, group = _lib.C.c_int(), _lib.C.c_int()
+        assert lib.ss_sweep_variant_info(v, _lib.C.byref(chunk), _lib.C.byref(group)) == 0
+        assert torch.equal(outs[v][0], outs[0][0]), "volume of variant %d (chunk %d, group %d)" % (v, chunk.value,
+                                                                                                   group.value)
+        assert torch.equal(outs[v][1], outs[0][1]), "guide of variant %d" % v
+    assert lib.ss_sweep_set_variant(H, W, N, vt, -1) == 0
+    assert lib.ss_sweep_get_variant(H, W, N, vt) == 0
+    want = [t.clone() for t in est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])]
+    res = est.autotune(imgs, repeats=2)
+    try:
+        assert 0 <= res["variant"] < nv and len(res["ms"]) == nv and all(m > 0 for m in res["ms"]), res
+        assert lib.ss_sweep_get_variant(H, W, N, vt) == res["variant"]
+        got = est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])
+        torch.cuda.synchronize()
+        assert torch.equal(got[0], want[0])
+        assert torch.equal(got[1].view(torch.int32), want[1].view(torch.int32))
+    finally:
+        lib.ss_sweep_set_variant(H, W, N, vt, -1)
