@@ -23,6 +23,7 @@
 // in a fixed order, so the parameter gradients are deterministic too.
 // Piecewise-constant parts have zero derivative: the camera selection, the
 // uint8 guide and the floor of the bilinear cell.
+#include <stdio.h>
 #include <string.h>
 
 #include <cmath>
