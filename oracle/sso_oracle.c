@@ -12,7 +12,10 @@
  *   ORA_F64=1 -> symbols *_f64: the same method evaluated in fp64 (the error
  *                yardstick: results of the fp32 forms are reported against it).
  * Both builds use -ffp-contract=off and no -ffast-math, so every operation is
- * one IEEE-754 round-to-nearest operation in the written order.
+ * one IEEE-754 round-to-nearest operation in the written order.  The loops
+ * over output pixels run on OpenMP threads (every pixel is computed by one
+ * thread from inputs no other thread writes, so results do not depend on the
+ * thread count); the scatter of ora_reproject_zmin stays serial.
  *
  * Layouts follow the reference: images HWC (BGR channel order, as cv2 reads
  * them), cost volumes planar [N][H][W], guides uint8 [H][W][3].
@@ -301,6 +304,7 @@ void SFX(ora_select_camera)(int C, int H, int W, int ref, const double *cams, co
     for (int k = 0; k < C; k++) tcam_init(&cc[k], cams + 6 * k);
     for (int i = 0; i < 12 * C; i++) rt[i] = (real)rts[i];
     real mn = (real)min_dist, mx = (real)max_dist;
+    #pragma omp parallel for schedule(dynamic, 8)
     for (int y = 0; y < H; y++)
         for (int x = 0; x < W; x++) {
             real ray[3];
@@ -358,6 +362,7 @@ void SFX(ora_sweep)(int C, int H, int W, int ref, const float *imgs, const int32
     for (int i = 0; i < 12 * C; i++) rt[i] = (real)rts[i];
     for (int n = 0; n < N; n++) cd[n] = (real)cands[n];
     const size_t HW = (size_t)H * W;
+    #pragma omp parallel for schedule(dynamic, 4)
     for (int y = 0; y < H; y++)
         for (int x = 0; x < W; x++) {
             size_t pix = (size_t)y * W + x;
@@ -389,6 +394,7 @@ void SFX(ora_sweep)(int C, int H, int W, int ref, const float *imgs, const int32
  * applied to channel indices 0,1,2 of the BGR data as the reference does. */
 void SFX(ora_guide)(const float *img, int HW, uint8_t *guide)
 {
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < HW; i++) {
         real c0 = img[3 * i], c1 = img[3 * i + 1], c2 = img[3 * i + 2];
         real y = ((K(16.0) + K(0.1826) * c0) + K(0.6142) * c1) + K(0.062) * c2;
@@ -418,6 +424,7 @@ void SFX(ora_isb_down)(const uint8_t *gin, const real *cin, int hin, int win, ui
     static const real base[8] = {K(0.25), K(0.5), K(0.25), K(0.5), K(0.25), K(0.5), K(0.25), K(0.5)};
     const real vi = (real)var_inv_i;
     const size_t pin = (size_t)hin * win, pout = (size_t)hout * wout;
+    #pragma omp parallel for schedule(static)
     for (int y = 0; y < hout; y++)
         for (int x = 0; x < wout; x++) {
             int x2 = 2 * x, y2 = 2 * y;
@@ -468,6 +475,7 @@ void SFX(ora_isb_up)(const uint8_t *gin, const real *cin, int hin, int win, uint
     const int (*taps[4])[2] = {t1, t2, t3, t4};
     const real *bases[4] = {b1, b23, b23, b4};
     const int ntap[4] = {9, 6, 6, 4};
+    #pragma omp parallel for schedule(static)
     for (int y = 0; y < hin; y++)
         for (int x = 0; x < win; x++) {
             int rows[3] = {y - 1 >= 0 ? y - 1 : 0, y, y + 1 < hin ? y + 1 : hin - 1};
@@ -575,6 +583,7 @@ void SFX(ora_argmin_refine)(const real *cost, int N, int HW, double d0_, double 
                             int32_t *idx_out)
 {
     const real d0 = (real)d0_, dN1 = (real)dN1_, A = (real)A_;
+    #pragma omp parallel for schedule(static)
     for (int p = 0; p < HW; p++) {
         real m = cost[p], M = cost[p];
         int i = 0;
@@ -609,6 +618,7 @@ void SFX(ora_argmin_refine)(const real *cost, int N, int HW, double d0_, double 
 void SFX(ora_soft_argmin)(const real *cost, int N, int HW, double beta_, double inv0_, double inv1_, real *invd)
 {
     const real beta = (real)beta_, inv0 = (real)inv0_, dinv = (real)(inv1_ - inv0_);
+    #pragma omp parallel for schedule(static)
     for (int p = 0; p < HW; p++) {
         real m = cost[p];
         for (int n = 1; n < N; n++) m = RFMIN(m, cost[(size_t)n * HW + p]);
@@ -766,6 +776,7 @@ void SFX(ora_inpaint_lut)(uint8_t *lut, int H, int W, const double *intr, const 
     cintr_init(&c, intr);
     real t[3] = {(real)t3[0], (real)t3[1], (real)t3[2]};
     real mn = (real)min_dist, mx = (real)max_dist;
+    #pragma omp parallel for schedule(static)
     for (int y = 0; y < H; y++)
         for (int x = 0; x < W; x++) {
             real u[3], pc[2], pf[2];
@@ -811,6 +822,7 @@ void SFX(ora_inpaint_lut)(uint8_t *lut, int H, int W, const double *intr, const 
 void SFX(ora_inpaint_step)(const real *din, real *dout, const uint8_t *lut, int H, int W, double thr_empty,
                            double thr_src)
 {
+    #pragma omp parallel for schedule(static)
     for (int y = 0; y < H; y++)
         for (int x = 0; x < W; x++) {
             size_t i = (size_t)y * W + x;
@@ -876,13 +888,15 @@ void SFX(ora_blend_lut)(real *sampling, real *weights, const real *masks, int R,
                         const double *intr, const double *rot, const double *trans, double min_dist,
                         double max_dist)
 {
+    if (R > 16) return; /* the per-pixel weights live on the stack */
     CIntr *c = (CIntr *)malloc(sizeof(CIntr) * R);
-    real *bw = (real *)malloc(sizeof(real) * R);
     for (int r = 0; r < R; r++) cintr_init(&c[r], intr + 6 * r);
     real mn = (real)min_dist, mx = (real)max_dist;
     const size_t P = (size_t)PH * PW;
+    #pragma omp parallel for schedule(dynamic, 8)
     for (int y = 0; y < PH; y++)
         for (int x = 0; x < PW; x++) {
+            real bw[16];
             size_t idx = (size_t)y * PW + x;
             real phi = (real)(((double)((float)y + 0.5f) * PI_REF) / PH - PI_REF / 2.0);
             real theta = (real)(((double)(((float)x + 0.5f) * 2.0f) * PI_REF) / PW + PI_REF);
@@ -924,7 +938,6 @@ void SFX(ora_blend_lut)(real *sampling, real *weights, const real *masks, int R,
             for (int r = 0; r < R; r++) weights[(size_t)r * P + idx] = bw[r] / bsum;
         }
     free(c);
-    free(bw);
 }
 
 /* blending_weights /= sum over references (stitcher.py:164) */
@@ -947,6 +960,7 @@ void SFX(ora_merge)(const real *sampling, const real *weights, const real *repro
     for (int r = 0; r < R; r++) cintr_init(&c[r], intr + 6 * r);
     const size_t P = (size_t)PH * PW;
     const real rx = (real)Ws / (real)W, ry = (real)Hs / (real)H;
+    #pragma omp parallel for schedule(static)
     for (size_t idx = 0; idx < P; idx++) {
         real inv_sum = 0, rgb[3] = {0, 0, 0};
         for (int r = 0; r < R; r++) {

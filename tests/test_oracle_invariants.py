@@ -487,3 +487,26 @@ def test_resize_linear_matches_torch_bilinear():
                           align_corners=False, antialias=False)[0].permute(1, 2, 0).numpy()
         d = np.abs(O32.resize_linear(x, cols, rows) - t)
         assert d.max() < 1e-2 and d.mean() < 1e-3
+
+
+def test_oracle_results_do_not_depend_on_the_thread_count():
+    """The oracle's per-pixel loops run on OpenMP threads; every output is
+    computed by one thread, so 1 and 4 threads give the same bits."""
+    import ctypes
+
+    gomp = ctypes.CDLL("libgomp.so.1")
+    before = gomp.omp_get_max_threads()
+    o = Oracle("f32")
+    rng = np.random.default_rng(9)
+    cost = (rng.random((16, 37, 29)) * 500).astype(np.float32)
+    guide = rng.integers(0, 256, (37, 29, 3), dtype=np.uint8)
+    try:
+        outs = []
+        for n in (1, 4):
+            gomp.omp_set_num_threads(n)
+            f, g = o.isb_filter(guide, cost, 10.0, 12.5)
+            outs.append((f, g, o.soft_argmin(f, 0.04, 1 / 0.55, 0.01)))
+    finally:
+        gomp.omp_set_num_threads(before)
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
