@@ -445,7 +445,7 @@ def cpu_baseline(wl, calibs, kw, to_match, to_stitch, budget_s):
         ora.estimate_RGBD_panorama(tm, ts)
         frames += 1
         el = time.perf_counter() - t0
-        if el > budget_s or frames >= 20:
+        if el > budget_s or frames >= 40:
             break
     import ctypes
 
