@@ -246,6 +246,23 @@ def batching_ragged(args, device):
                                           "vs_plain_fp32": round(v / v_plain, 4)}
         print("[bench] ragged batch (%d rigs, %d configurations), fp16 %s: %.1f panoramas/s (plain fp32 one at a "
               "time: %.1f)" % (len(sets), len(batch.groups), mode, v, v_plain), file=sys.stderr, flush=True)
+    del batch
+    # the ragged persistent form: one executor for all 16 rigs, one launch per pass
+    batch = FrameBatch(ests, persistent=True)
+    for mode in ("eager", "graph"):
+        batch.use_graph = mode == "graph"
+        for _ in range(args.warmup):
+            batch.estimate_RGBD_panoramas(frames)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            batch.estimate_RGBD_panoramas(frames)
+        torch.cuda.synchronize()
+        v = reps * len(frames) / (time.perf_counter() - t0)
+        out["float16_persistent_" + mode] = {"value": round(v, 3), "unit": "panoramas/s",
+                                             "vs_plain_fp32": round(v / v_plain, 4)}
+        print("[bench] ragged batch (%d rigs), fp16 persistent %s: %.1f panoramas/s" % (len(sets), mode, v),
+              file=sys.stderr, flush=True)
     del batch, ests
     torch.cuda.empty_cache()
     return out

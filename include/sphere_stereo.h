@@ -309,6 +309,26 @@ int ss_frame_batch_create(const ss_frame_desc *descs, int nframes, void **frame)
  * as ss_frame_run's for each frame; use_graph as ss_frame_run. */
 int ss_frame_batch_run(void *frame, const float *const *d_match, const float *const *d_stitch, int use_graph,
                        ss_stream stream);
+/* No counterpart in the reference (the batching form).  The ragged persistent
+ * form of ss_frame_batch_create: the nframes frames may differ in every size
+ * and setting (C, R, N, H, W, Hs, Ws, PH, PW, candidates, sigmas, inpainting
+ * settings); they must share the volume type and have N >= 2 and a matching
+ * resolution of at least 8 x 8.  Each pass of the frame (the sweep, each
+ * level of the two filters, the coarse tails, the reprojection, each group of
+ * 16 inpainting passes, the uint8 casts, the merge) is one launch for all the
+ * frames, sized to the workgroups the GPU holds resident, whose wavefronts
+ * take work items (row segments, pixel blocks, tiles) from a work queue.  The
+ * launch records, the filters' weight tables and the queue live in d_table, a
+ * device buffer of ss_frame_ragged_table_bytes(descs, nframes) bytes (256-byte
+ * aligned) that the caller keeps until ss_frame_destroy; it is written on
+ * `stream` during the call, which returns once it is ready.  Runs go through
+ * ss_frame_batch_run (d_match: every frame's C images, frame after frame;
+ * d_stitch: every frame's R images), with or without a graph.  Runs of one
+ * executor use one queue: they must not overlap on different streams.
+ * Returns 0 from ss_frame_ragged_table_bytes for descriptors it rejects. */
+size_t ss_frame_ragged_table_bytes(const ss_frame_desc *descs, int nframes);
+int ss_frame_ragged_create(const ss_frame_desc *descs, int nframes, void *d_table, size_t table_bytes,
+                           ss_stream stream, void **frame);
 int ss_frame_destroy(void *frame);
 
 /* Differentiation form --------------------------------------------------------

@@ -107,11 +107,7 @@ int padded(int N) {
   return v < 0 ? -1 : 8 * v;
 }
 
-struct Level {
-  int h, w;
-  uint32_t* g;
-  void* c;
-};
+using Level = ss::PyrLevel;
 
 size_t elem_bytes(int vt) { return vt == SS_VOLUME_F32 ? 4 : 2; }
 bool valid_vtype(int vt) { return vt == SS_VOLUME_F32 || vt == SS_VOLUME_F16 || vt == SS_VOLUME_BF16; }
@@ -127,6 +123,93 @@ void carve_levels(Carver& cv, Level* lv, int L, int H, int W, int NP) {
   }
 }
 
+}  // namespace
+
+namespace ss {
+
+// Up-pass weights into level k (isb_filter.py:110-112), in double as the
+// reference's Python computes them.
+void up_weights(double sigma_s, int k, double* wu, double* wd) {
+  const double var_inv_s = 1 / (2 * sigma_s * sigma_s);
+  const double d = pow(2.0, k) - 0.5;
+  *wd = exp(-(d * d) * var_inv_s);
+  *wu = 1 - *wd;
+}
+
+// The coarse end of a pyramid runs in one workgroup from level s0 on: every
+// level it produces holds at most kTailPixels pixels (one CU then does each
+// pass in about a microsecond) and all of them (guide + NP values per pixel)
+// fit in the LDS budget.  The tail starts at level 1 or deeper (its levels
+// are all fp32, and a fused level-0 pass stays a regular launch).  Returns 0
+// when the pyramid has no such tail.
+int tail_start(const PyrLevel* l0, int L, int NP) {
+  const long long kTailPixels = 256;
+  auto lds_need = [&](int from) {
+    size_t b = 0;
+    for (int k = from + 1; k < L; ++k) {
+      const size_t px = (size_t)l0[k].h * l0[k].w;
+      b += ((px * 4 + 15) & ~size_t(15)) + ((px * NP * 4 + 15) & ~size_t(15));
+    }
+    return b;
+  };
+  int s0 = L - 1;
+  for (int k = 1; k < L - 1; ++k)
+    if ((long long)l0[k + 1].h * l0[k + 1].w <= kTailPixels && lds_need(k) <= (size_t)kTailLdsBytes) {
+      s0 = k;
+      break;
+    }
+  return (L - 1 - s0 >= 2 && L - s0 <= kMaxTailLevels) ? s0 : 0;
+}
+
+// The level table of a tail from s0 (levels s0..L-1 of l0); returns its LDS bytes.
+unsigned tail_levels(const PyrLevel* l0, int s0, int L, int NP, const float* wtab, double sigma_s, TailLevels* t) {
+  memset(t, 0, sizeof *t);
+  t->n = L - s0;
+  t->wtab = wtab;
+  unsigned off = 0;
+  for (int e = 0; e < t->n; ++e) {
+    t->h[e] = l0[s0 + e].h;
+    t->w[e] = l0[s0 + e].w;
+    if (e > 0) {
+      const unsigned px = (unsigned)(t->h[e] * t->w[e]);
+      t->goff[e] = off;
+      off += (px * 4 + 15) & ~15u;
+      t->coff[e] = off;
+      off += (px * (unsigned)(NP * 4) + 15) & ~15u;
+    }
+    double wu, wd;
+    up_weights(sigma_s, s0 + e, &wu, &wd);
+    t->wu[e] = (float)wu;
+    t->wd[e] = (float)wd;
+  }
+  return off;
+}
+
+// The bounds under which the sweep's projections may take project_t_fast:
+// finite parameters, |fs|, |transform entries|, |candidates| <= 2^20,
+// 2^-16 <= |cs| <= 2^20, |xi|, |alpha|, |1 - alpha| <= 16 (the rays are unit
+// vectors by construction).  Real rigs are far inside them.  rt: npairs
+// transforms of 12 floats.
+bool sweep_fast_bounds(const TorchCam* cams, int C, const float* rt, int npairs, const float* cands, int N) {
+  auto in = [](float v, float lo, float hi) { return std::isfinite(v) && fabsf(v) >= lo && fabsf(v) <= hi; };
+  const float big = 0x1p20f;
+  for (int k = 0; k < C; ++k) {
+    const TorchCam& c = cams[k];
+    if (!in(c.fs0, 0.0f, big) || !in(c.fs1, 0.0f, big) || !in(c.cs0, 0x1p-16f, big) || !in(c.cs1, 0x1p-16f, big) ||
+        !in(c.xi, 0.0f, 16.0f) || !in(c.a, 0.0f, 16.0f) || !in(c.one_m_a, 0.0f, 16.0f))
+      return false;
+  }
+  for (int i = 0; i < 12 * npairs; ++i)
+    if (!in(rt[i], 0.0f, big)) return false;
+  for (int n = 0; n < N; ++n)
+    if (!in(cands[n], 0.0f, big)) return false;
+  return true;
+}
+
+}  // namespace ss
+
+namespace {
+
 // ISB_Filter.apply's two launch loops (isb_filter.py:87-134) for nb pyramids
 // of the same shape in the same launches (lv[b] are the levels of entry b).
 // With fuse_last, the level-0 up-pass computes the argmin into dist[b]
@@ -138,37 +221,10 @@ int run_pyramid(Level* const* lv, int nb, int L, int G, int V, int vt, const flo
                 bool fuse_last, const ss::ArgminParams& am, float* const* dist, hipStream_t s, int stage_down,
                 int stage_up, float* const* fill = nullptr) {
   if (nb < 1 || nb > ss::kMaxBatch) return fail(SS_ERR_ARG, "batch of %d pyramids outside [1, %d]", nb, ss::kMaxBatch);
-  const double var_inv_s = 1 / (2 * sigma_s * sigma_s);
   const Level* l0 = lv[0];  // the shapes (identical for every entry)
-  // up-pass weights into level k (isb_filter.py:110-112)
-  auto weights = [&](int k, double& wu, double& wd) {
-    const double d = pow(2.0, k) - 0.5;
-    wd = exp(-(d * d) * var_inv_s);
-    wu = 1 - wd;
-  };
-  // The coarse end runs in one workgroup (per entry) from level s0 on: every
-  // level it produces holds at most kTailPixels pixels (one CU then does each
-  // pass in about a microsecond) and all of them (guide + NP values per
-  // pixel) fit in the LDS budget.
-  const long long kTailPixels = 256;
   const int NP = G * V;
-  auto lds_need = [&](int from) {
-    size_t b = 0;
-    for (int k = from + 1; k < L; ++k) {
-      const size_t px = (size_t)l0[k].h * l0[k].w;
-      b += ((px * 4 + 15) & ~size_t(15)) + ((px * NP * 4 + 15) & ~size_t(15));
-    }
-    return b;
-  };
-  // the tail starts at level 1 or deeper (its levels are all fp32, and a
-  // fused level-0 pass stays a regular launch)
-  int s0 = L - 1;
-  for (int k = 1; k < L - 1; ++k)
-    if ((long long)l0[k + 1].h * l0[k + 1].w <= kTailPixels && lds_need(k) <= (size_t)ss::kTailLdsBytes) {
-      s0 = k;
-      break;
-    }
-  const bool tail = L - 1 - s0 >= 2 && L - s0 <= ss::kMaxTailLevels;
+  const int s0 = ss::tail_start(l0, L, NP);
+  const bool tail = s0 > 0;
   const int last_down = tail ? s0 : L - 1;
   ss::IsbDownArgs da[ss::kMaxBatch];
   ss::IsbUpArgs ua[ss::kMaxBatch];
@@ -188,35 +244,17 @@ int run_pyramid(Level* const* lv, int nb, int L, int G, int V, int vt, const flo
   if (tail) {
     ss::IsbTailArgs ta;
     memset(&ta, 0, sizeof ta);
-    ta.n = L - s0;
-    ta.wtab = wtab;
     for (int b = 0; b < nb; ++b) {
       ta.g0[b] = lv[b][s0].g;
       ta.c0[b] = lv[b][s0].c;
     }
-    unsigned off = 0;
-    for (int e = 0; e < ta.n; ++e) {
-      ta.h[e] = l0[s0 + e].h;
-      ta.w[e] = l0[s0 + e].w;
-      if (e > 0) {
-        const unsigned px = (unsigned)(ta.h[e] * ta.w[e]);
-        ta.goff[e] = off;
-        off += (px * 4 + 15) & ~15u;
-        ta.coff[e] = off;
-        off += (px * (unsigned)(NP * 4) + 15) & ~15u;
-      }
-      double wu, wd;
-      weights(s0 + e, wu, wd);
-      ta.wu[e] = (float)wu;
-      ta.wd[e] = (float)wd;
-    }
-    ta.lds_bytes = off;
+    ta.lds_bytes = tail_levels(l0, s0, L, NP, wtab, sigma_s, &ta.t);
     StageTimer t(stage_down, s);
     SS_HIP(ss::launch_isb_tail(ta, nb, G, V, s));
   }
   for (int k = (tail ? s0 : L - 1) - 1; k >= 0; --k) {
     double wu, wd;
-    weights(k, wu, wd);
+    ss::up_weights(sigma_s, k, &wu, &wd);
     const bool fuse = fuse_last && k == 0;
     for (int b = 0; b < nb; ++b) {
       const Level* v = lv[b];
@@ -256,28 +294,9 @@ int check_sweep(const float* const* d_images, int C, const ss_camera* cams, cons
   return SS_OK;
 }
 
-// One sweep launch over references refs[0..R) (R * C <= kSweepMaxPairs):
-// rts[r] holds reference r's C transforms, costs[r] / guides[r] its outputs
-// (guides[r] may be null).  Requires N >= 2 (the volume stride is 8 * V).
-// The bounds under which the sweep's projections may take project_t_fast:
-// finite parameters, |fs|, |transform entries|, |candidates| <= 2^20,
-// 2^-16 <= |cs| <= 2^20, |xi|, |alpha|, |1 - alpha| <= 16 (the rays are unit
-// vectors by construction).  Real rigs are far inside them.
+// ss::sweep_fast_bounds on a launch's argument record
 bool sweep_fast_bounds(const ss::SweepArgs& a, int C, int npairs) {
-  auto in = [](float v, float lo, float hi) { return std::isfinite(v) && fabsf(v) >= lo && fabsf(v) <= hi; };
-  const float big = 0x1p20f;
-  for (int k = 0; k < C; ++k) {
-    const ss::TorchCam& c = a.cam[k];
-    if (!in(c.fs0, 0.0f, big) || !in(c.fs1, 0.0f, big) || !in(c.cs0, 0x1p-16f, big) || !in(c.cs1, 0x1p-16f, big) ||
-        !in(c.xi, 0.0f, 16.0f) || !in(c.a, 0.0f, 16.0f) || !in(c.one_m_a, 0.0f, 16.0f))
-      return false;
-  }
-  for (int p = 0; p < npairs; ++p)
-    for (int i = 0; i < 12; ++i)
-      if (!in(a.rt[p][i], 0.0f, big)) return false;
-  for (int n = 0; n < a.N; ++n)
-    if (!in(a.cands[n], 0.0f, big)) return false;
-  return true;
+  return ss::sweep_fast_bounds(a.cam, C, &a.rt[0][0], npairs, a.cands, a.N);
 }
 
 // The sweep variant chosen per configuration by ss_sweep_autotune (or
@@ -308,6 +327,9 @@ void set_sweep_variant(int H, int W, int N, int vt, int variant) {
   if (variant >= 0) g_tune.push_back(SweepTune{H, W, N, vt, variant});
 }
 
+// One sweep launch over references refs[0..R) (R * C <= kSweepMaxPairs):
+// rts[r] holds reference r's C transforms, costs[r] / guides[r] its outputs
+// (guides[r] may be null).  Requires N >= 2 (the volume stride is 8 * V).
 // variant < 0: the configuration's tuned variant
 int launch_sweep_refs(const float* const* d_raylut, const float* const* d_images, int C, const ss_camera* cams,
                       const float* const* rts, const int* refs, int R, const float* cands, int N, int H, int W,
@@ -683,6 +705,18 @@ void estimate_level0(void* ws, int H, int W, int N, int vt, uint32_t** g0, void*
   Carver cv(ws);
   Level lv[32], dl[32];
   estimate_layout(cv, H, W, N, vt, g0, c0, lv, dl);
+}
+
+int isb_levels(int H, int W) { return levels_of(H, W); }
+
+int estimate_levels(void* ws, int H, int W, int N, int vt, float* dist, PyrLevel* lv, PyrLevel* dl) {
+  Carver cv(ws);
+  uint32_t* g0;
+  void* c0;
+  estimate_layout(cv, H, W, N, vt, &g0, &c0, lv, dl);
+  lv[0] = PyrLevel{H, W, g0, c0};
+  dl[0] = PyrLevel{H, W, g0, dist};
+  return levels_of(H, W);
 }
 
 int estimate_after_sweep(int nb, int H, int W, int N, int vt, const float* cands, double sigma_i, double sigma_s,

@@ -54,6 +54,8 @@ struct Frame {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<const float*> cap_match, cap_stitch;
+  ss::RaggedPlan* ragged = nullptr;  // the ragged persistent form (ss_frame_ragged_create)
+  size_t n_match = 0, n_stitch = 0;  // images per run
 };
 
 #define FR_HIP(expr)                                                            \
@@ -78,6 +80,7 @@ struct Frame {
 // frame's merge.  A chunk's volumes are produced and consumed together, so
 // its passes meet them in the last-level cache where they fit.
 int enqueue(Frame* f, const float* const* match, const float* const* stitch, hipStream_t s) {
+  if (f->ragged) return ss::ragged_enqueue(f->ragged, match, stitch, s);
   const int F = (int)f->d.size();
   const ss_frame_desc& c = f->d[0];  // the shared configuration
   const int R = c.R, C = c.C;
@@ -192,6 +195,46 @@ int ss_frame_batch_create(const ss_frame_desc* descs, int nframes, void** frame)
   if (!f) return ss::report_error(SS_ERR_ARG, "out of host memory");
   f->d.assign(descs, descs + nframes);
   f->pool = &g_pool[dev];
+  f->n_match = (size_t)nframes * descs[0].C;
+  f->n_stitch = (size_t)nframes * descs[0].R;
+  *frame = f;
+  return SS_OK;
+}
+
+size_t ss_frame_ragged_table_bytes(const ss_frame_desc* descs, int nframes) {
+  if (!descs || nframes < 1 || nframes > SS_MAX_BATCH_FRAMES) return 0;
+  for (int i = 0; i < nframes; ++i)
+    if (check_desc(descs[i]) != SS_OK) return 0;
+  return ss::ragged_table_bytes(descs, nframes);
+}
+
+int ss_frame_ragged_create(const ss_frame_desc* descs, int nframes, void* d_table, size_t table_bytes,
+                           ss_stream stream, void** frame) {
+  if (!descs || !frame) return ss::report_error(SS_ERR_ARG, "null descriptor array or output handle");
+  if (nframes < 1 || nframes > SS_MAX_BATCH_FRAMES)
+    return ss::report_error(SS_ERR_ARG, "frame count outside [1, SS_MAX_BATCH_FRAMES]");
+  for (int i = 0; i < nframes; ++i) {
+    const int rc = check_desc(descs[i]);
+    if (rc != SS_OK) return rc;
+  }
+  int dev = 0;
+  FR_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return ss::report_error(SS_ERR_UNSUPPORTED, "device index beyond the pool");
+  FR_HIP(pool_init(g_pool[dev]));
+  ss::RaggedPlan* plan = nullptr;
+  FR_CALL(ss::ragged_create(descs, nframes, d_table, table_bytes, reinterpret_cast<hipStream_t>(stream), &plan));
+  Frame* f = new (std::nothrow) Frame();
+  if (!f) {
+    ss::ragged_free(plan);
+    return ss::report_error(SS_ERR_ARG, "out of host memory");
+  }
+  f->d.assign(descs, descs + nframes);
+  f->pool = &g_pool[dev];
+  f->ragged = plan;
+  int nm = 0, ns = 0;
+  ss::ragged_images(plan, &nm, &ns);
+  f->n_match = (size_t)nm;
+  f->n_stitch = (size_t)ns;
   *frame = f;
   return SS_OK;
 }
@@ -202,7 +245,7 @@ int ss_frame_batch_run(void* frame, const float* const* d_match, const float* co
                        ss_stream stream) {
   Frame* f = static_cast<Frame*>(frame);
   if (!f || !d_match || !d_stitch) return ss::report_error(SS_ERR_ARG, "null frame or image array");
-  const size_t nm = f->d.size() * (size_t)f->d[0].C, ns = f->d.size() * (size_t)f->d[0].R;
+  const size_t nm = f->n_match, ns = f->n_stitch;
   for (size_t k = 0; k < nm; ++k)
     if (!d_match[k]) return ss::report_error(SS_ERR_ARG, "null matching image");
   for (size_t r = 0; r < ns; ++r)
@@ -248,6 +291,7 @@ int ss_frame_destroy(void* frame) {
   Frame* f = static_cast<Frame*>(frame);
   if (!f) return SS_OK;
   drop_graph(f);
+  ss::ragged_free(f->ragged);
   delete f;
   return SS_OK;
 }

@@ -18,8 +18,11 @@
 // nothing.
 #include <string.h>
 
+#include <type_traits>
+
 #include "ss_isb_stencil.h"
 #include "ss_kernels.h"
+#include "ss_queue.h"
 
 namespace ss {
 
@@ -345,12 +348,9 @@ __global__ void __launch_bounds__(kBlock) k_isb_up(Batch<IsbUpArgs> b) {
 constexpr int kTailBlock = 1024;
 
 template <int G, int V>
-__global__ void __launch_bounds__(kTailBlock) k_isb_tail(IsbTailArgs t) {
-  extern __shared__ __align__(16) char smem[];
+__device__ __forceinline__ void tail_body(const TailLevels& t, uint32_t* const g0, void* const c0, char* smem) {
   const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
   const int slot = threadIdx.x / G, slots = kTailBlock / G;
-  uint32_t* const g0 = t.g0[blockIdx.x];
-  void* const c0 = t.c0[blockIdx.x];
   auto gp = [&](int k) __attribute__((always_inline)) -> uint32_t* {
     return k == 0 ? g0 : reinterpret_cast<uint32_t*>(smem + t.goff[k]);
   };
@@ -372,6 +372,97 @@ __global__ void __launch_bounds__(kTailBlock) k_isb_tail(IsbTailArgs t) {
       if (base + slot < npix) up_pixel<G, V, float, false>(a, base + slot, true, lane);
     __syncthreads();
   }
+}
+
+template <int G, int V>
+__global__ void __launch_bounds__(kTailBlock) k_isb_tail(IsbTailArgs t) {
+  extern __shared__ __align__(16) char smem[];
+  tail_body<G, V>(t.t, t.g0[blockIdx.x], t.c0[blockIdx.x], smem);
+}
+
+// ---------------------------------------------------------------------------
+// Ragged persistent forms: a wavefront's item is kIsbItemPix8 (G = 8) or
+// kIsbItemPix1 (G = 1) consecutive coarse pixels of one entry, 64 / G per
+// step; the entry's V picks the instantiation.
+// ---------------------------------------------------------------------------
+template <int G>
+constexpr int item_pix() {
+  return G == 1 ? kIsbItemPix1 : kIsbItemPix8;
+}
+
+template <int G, int V, typename T>
+__device__ __forceinline__ void down_item(const IsbDownArgs& a, int item, int lane, int slot) {
+  const int npix = a.hout * a.wout;
+  const int base = item * item_pix<G>();
+#pragma unroll 1
+  for (int it = 0; it < item_pix<G>() && base + it < npix; it += 64 / G) {
+    int pix = base + it + slot;
+    const bool active = pix < npix;
+    if (!active) pix = npix - 1;
+    down_pixel<G, V, T>(a, pix, active, lane);
+  }
+}
+
+template <int G, int V, typename T, bool FUSE>
+__device__ __forceinline__ void up_item(const IsbUpArgs& a, int item, int lane, int slot) {
+  const int npix = a.hin * a.win;
+  const int base = item * item_pix<G>();
+#pragma unroll 1
+  for (int it = 0; it < item_pix<G>() && base + it < npix; it += 64 / G) {
+    int pix = base + it + slot;
+    const bool active = pix < npix;
+    if (!active) pix = npix - 1;
+    up_pixel<G, V, T, FUSE>(a, pix, active, lane);
+  }
+}
+
+// Calls f(integral_constant<V>) for the entry's V.  A launch instantiates
+// the V <= VMAX only (VMAX: the smallest of 1, 2, 4, 8, 16 that covers its
+// entries), so that entries with few candidates do not run at the register
+// budget, hence the occupancy, of the largest V.
+template <int VMAX, typename F>
+__device__ __forceinline__ void with_v(int V, F&& f) {
+  switch (V) {
+    case 1: f(std::integral_constant<int, 1>()); break;
+    case 2: if constexpr (VMAX >= 2) f(std::integral_constant<int, 2>()); break;
+    case 3: if constexpr (VMAX >= 3) f(std::integral_constant<int, 3>()); break;
+    case 4: if constexpr (VMAX >= 4) f(std::integral_constant<int, 4>()); break;
+    case 6: if constexpr (VMAX >= 6) f(std::integral_constant<int, 6>()); break;
+    case 8: if constexpr (VMAX >= 8) f(std::integral_constant<int, 8>()); break;
+    case 12: if constexpr (VMAX >= 12) f(std::integral_constant<int, 12>()); break;
+    case 16: if constexpr (VMAX >= 16) f(std::integral_constant<int, 16>()); break;
+    default: break;
+  }
+}
+
+template <int G, typename T, int VMAX>
+__global__ void __launch_bounds__(kBlock) k_isb_down_ragged(RaggedSpan<IsbDownEntry> sp) {
+  const int l64 = (int)(threadIdx.x & 63);
+  const int lane = G == 1 ? 0 : (l64 & (G - 1)), slot = l64 / G;
+  for_each_wave_item(sp, [&](int e, int t) {
+    const IsbDownEntry en = load_const(sp.e + e);
+    const IsbDownArgs& a = en.a;
+    with_v<VMAX>(en.V, [&](auto v) { down_item<G, decltype(v)::value, T>(a, t, lane, slot); });
+  });
+}
+
+template <int G, typename T, bool FUSE, int VMAX>
+__global__ void __launch_bounds__(kBlock) k_isb_up_ragged(RaggedSpan<IsbUpEntry> sp) {
+  const int l64 = (int)(threadIdx.x & 63);
+  const int lane = G == 1 ? 0 : (l64 & (G - 1)), slot = l64 / G;
+  for_each_wave_item(sp, [&](int e, int t) {
+    const IsbUpEntry en = load_const(sp.e + e);
+    const IsbUpArgs& a = en.a;
+    with_v<VMAX>(en.V, [&](auto v) { up_item<G, decltype(v)::value, T, FUSE>(a, t, lane, slot); });
+  });
+}
+
+// one workgroup per entry, as k_isb_tail
+template <int G>
+__global__ void __launch_bounds__(kTailBlock) k_isb_tail_ragged(const TailEntry* __restrict__ te) {
+  extern __shared__ __align__(16) char smem[];
+  const TailEntry& e = te[blockIdx.x];
+  with_v<G == 1 ? 1 : 16>(e.V, [&](auto v) { tail_body<G, decltype(v)::value>(e.t, e.g0, e.c0, smem); });
 }
 
 // Weight tables: entry d of table i is the guide weight factor
@@ -575,6 +666,85 @@ hipError_t launch_guide3_to_packed(const uint8_t* g3, uint32_t* g, int HW, hipSt
 
 hipError_t launch_packed_to_guide3(const uint32_t* g, uint8_t* g3, int HW, hipStream_t s) {
   hipLaunchKernelGGL(k_packed_to_guide3, dim3(blocks_for(HW)), dim3(kBlock), 0, s, g, g3, HW);
+  return hipGetLastError();
+}
+
+template <int G, typename T, int VMAX>
+hipError_t down_ragged_t(const RaggedSpan<IsbDownEntry>& sp, hipStream_t s) {
+  const auto k = &k_isb_down_ragged<G, T, VMAX>;
+  const int grid = persistent_grid(reinterpret_cast<const void*>(k), kBlock, 0, sp.total, kBlock / 64);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, sp);
+  return hipGetLastError();
+}
+
+template <int G, typename T, bool FUSE, int VMAX>
+hipError_t up_ragged_t(const RaggedSpan<IsbUpEntry>& sp, hipStream_t s) {
+  const auto k = &k_isb_up_ragged<G, T, FUSE, VMAX>;
+  const int grid = persistent_grid(reinterpret_cast<const void*>(k), kBlock, 0, sp.total, kBlock / 64);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, sp);
+  return hipGetLastError();
+}
+
+#define SS_DISPATCH_VMAX(vmax, CALL)            \
+  switch (vmax) {                               \
+    case 1: return CALL(1);                     \
+    case 2: return CALL(2);                     \
+    case 4: return CALL(4);                     \
+    case 8: return CALL(8);                     \
+    case 16: return CALL(16);                   \
+    default: return hipErrorInvalidValue;       \
+  }
+
+template <typename T>
+hipError_t down_ragged_vt(const RaggedSpan<IsbDownEntry>& sp, int vmax, hipStream_t s) {
+#define SS_CALL(m) down_ragged_t<8, T, m>(sp, s)
+  SS_DISPATCH_VMAX(vmax, SS_CALL)
+#undef SS_CALL
+}
+
+template <typename T, bool FUSE>
+hipError_t up_ragged_vt(const RaggedSpan<IsbUpEntry>& sp, int vmax, hipStream_t s) {
+#define SS_CALL(m) up_ragged_t<8, T, FUSE, m>(sp, s)
+  SS_DISPATCH_VMAX(vmax, SS_CALL)
+#undef SS_CALL
+}
+
+int ragged_vmax(int V) {
+  int m = 1;
+  while (m < V) m *= 2;
+  return m;
+}
+
+hipError_t launch_isb_down_ragged(const RaggedSpan<IsbDownEntry>& sp, int G, int vt, int vmax, hipStream_t s) {
+  if (sp.n < 1 || sp.total < 1) return hipErrorInvalidValue;
+  if (G == 1) return (vt == SS_VOLUME_F32 && vmax == 1) ? down_ragged_t<1, float, 1>(sp, s) : hipErrorInvalidValue;
+#define SS_CALL(T) down_ragged_vt<T>(sp, vmax, s)
+  SS_DISPATCH_T(vt, SS_CALL)
+#undef SS_CALL
+}
+
+hipError_t launch_isb_up_ragged(const RaggedSpan<IsbUpEntry>& sp, int G, int vt, int vmax, bool fuse_argmin,
+                                hipStream_t s) {
+  if (sp.n < 1 || sp.total < 1) return hipErrorInvalidValue;
+  if (G == 1)
+    return (vt == SS_VOLUME_F32 && vmax == 1 && !fuse_argmin) ? up_ragged_t<1, float, false, 1>(sp, s)
+                                                              : hipErrorInvalidValue;
+  if (fuse_argmin) {
+#define SS_CALL(T) up_ragged_vt<T, true>(sp, vmax, s)
+    SS_DISPATCH_T(vt, SS_CALL)
+#undef SS_CALL
+  }
+#define SS_CALL(T) up_ragged_vt<T, false>(sp, vmax, s)
+  SS_DISPATCH_T(vt, SS_CALL)
+#undef SS_CALL
+}
+
+hipError_t launch_isb_tail_ragged(const TailEntry* t, int n, int G, unsigned lds_bytes, hipStream_t s) {
+  if (n < 1) return hipErrorInvalidValue;
+  if (G == 1)
+    hipLaunchKernelGGL(k_isb_tail_ragged<1>, dim3(n), dim3(kTailBlock), lds_bytes, s, t);
+  else
+    hipLaunchKernelGGL(k_isb_tail_ragged<8>, dim3(n), dim3(kTailBlock), lds_bytes, s, t);
   return hipGetLastError();
 }
 

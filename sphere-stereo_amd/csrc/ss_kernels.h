@@ -65,14 +65,17 @@ struct Batch {
 // b runs the tail of batch entry b (its own level-s0 buffers g0[b], c0[b]).
 constexpr int kMaxTailLevels = 16;
 constexpr int kTailLdsBytes = 60 * 1024;
-struct IsbTailArgs {
+struct TailLevels {
   int n;
-  uint32_t* g0[kMaxBatch];
-  void* c0[kMaxBatch];
   unsigned goff[kMaxTailLevels], coff[kMaxTailLevels];
   int h[kMaxTailLevels], w[kMaxTailLevels];
   float wu[kMaxTailLevels], wd[kMaxTailLevels];  // up pass into entry k
   const float* wtab;
+};
+struct IsbTailArgs {
+  TailLevels t;
+  uint32_t* g0[kMaxBatch];
+  void* c0[kMaxBatch];
   unsigned lds_bytes;
 };
 
@@ -154,9 +157,111 @@ struct BlendArgs {
   float mn, mx;
 };
 
+// ---------------------------------------------------------------------------
+// Ragged persistent launches (ragged.hip): one launch per pass covers the
+// frame-reference pairs of many frames whatever their shapes.  Each launch is
+// sized to what the GPU holds resident; its wavefronts (or workgroups) take
+// work items from a WorkQueue until it is empty.  The records below live in
+// a device table built once per batch; a launch gets a span of them.
+// ---------------------------------------------------------------------------
+// The items of a launch are split into kQueueParts contiguous parts, one per
+// XCD: a worker starts on the part of its XCD (neighbouring items share
+// cache lines of the same images) and then takes from the others.  The
+// counters are zero between launches: the last worker to finish resets them.
+constexpr int kQueueParts = 8;
+struct WorkQueue {
+  unsigned next[kQueueParts];
+  unsigned done;
+  unsigned pad[7];
+};
+
+// Entries e[0..n) of a launch; entry k owns items [first[k], first[k+1]),
+// first[n] = total.
+template <typename E>
+struct RaggedSpan {
+  const E* e;
+  const int* first;
+  int n, total;
+  WorkQueue* q;
+};
+
+// Items of the ragged ISB passes: coarse pixels per item for G = 8 (groups of
+// 8 lanes) and G = 1.
+constexpr int kIsbItemPix8 = 64, kIsbItemPix1 = 256;
+
+// One reference of one frame in the ragged sweep: an item is a 64-pixel row
+// segment, swept by one wavefront.
+struct SweepPair {
+  const float4* raylut;
+  void* cost;
+  uint32_t* guide;
+  int img0;    // index of the frame's camera 0 in the run's image table
+  int refcam;  // camera index of the reference
+  int H, W, N, NP, tiles_x, pow2, fastproj;
+  float Wf, Hf, invW, invH, inv2W, inv2H, halfW, halfH;
+  TorchCam cam[SS_MAX_CAMERAS];
+  float rt[SS_MAX_CAMERAS][12];
+  float cands[SS_MAX_CANDIDATES];
+};
+
+struct IsbDownEntry {
+  IsbDownArgs a;
+  int V;
+};
+struct IsbUpEntry {
+  IsbUpArgs a;
+  int V;
+};
+
+// One pyramid's coarse tail in the ragged tail launch (one workgroup each).
+struct TailEntry {
+  TailLevels t;
+  uint32_t* g0;
+  void* c0;
+  int V;
+};
+
+struct ReprojEntry {
+  ReprojectArgs a;
+  int H, W;
+};
+// steps == 0: copy din to dout (the last launch of an odd count)
+struct InpaintEntry {
+  InpaintArgs a;
+  int H, W, steps, tiles_x;
+  double thr_empty, thr_src;
+};
+// per-run pointers written into the device table before a ragged run
+constexpr int kPublishMax = 64;
+struct PublishArgs {
+  const void* src[kPublishMax];
+  const void** dst;
+  int n;
+};
+struct CastEntry {
+  int src;  // index in the run's stitch-image table
+  uint8_t* out;
+  long long n;
+};
+
 // api.hip: error text of the calling thread; whether kernel timing is on
 int report_error(int code, const char* what);
 bool profiler_active();
+
+// api.hip: pyramid levels (level s: ceil(H / 2^s) x ceil(W / 2^s)) and their buffers
+struct PyrLevel {
+  int h, w;
+  uint32_t* g;
+  void* c;
+};
+int isb_levels(int H, int W);
+void up_weights(double sigma_s, int k, double* wu, double* wd);
+int tail_start(const PyrLevel* l0, int L, int NP);
+unsigned tail_levels(const PyrLevel* l0, int s0, int L, int NP, const float* wtab, double sigma_s, TailLevels* t);
+// an estimate workspace's cost pyramid lv[0..L) (level 0 = guide + volume)
+// and distance pyramid dl[0..L) (level 0 = guide + dist); returns L
+int estimate_levels(void* ws, int H, int W, int N, int vt, float* dist, PyrLevel* lv, PyrLevel* dl);
+bool sweep_fast_bounds(const TorchCam* cams, int C, const float* rt, int npairs, const float* cands, int N);
 
 // isb.hip (vt: ss_volume_type of the level-0 side of a pass, i.e. the input
 // of a down pass or the fine level of an up pass; pyramid levels >= 1 are
@@ -233,5 +338,40 @@ size_t error_stats_partial_bytes(size_t n);
 int error_stats_max_thresholds();
 hipError_t launch_error_stats(const float* a, const float* b, size_t n, const float* thr, int nthr, int finite_only,
                               double* partial, double* out, hipStream_t s);
+
+// ragged persistent launches (grid: workgroups, sized by the caller with
+// persistent_grid; every workgroup takes items until the queue is empty)
+int persistent_grid(const void* kernel, int block, size_t lds, long long workers_needed, int waves_per_worker);
+hipError_t launch_publish(const void* const* src, int n, const void** dst, hipStream_t s);
+hipError_t launch_sweep_ragged(const RaggedSpan<SweepPair>& sp, const float* const* img, int vt, hipStream_t s);
+// vmax: ragged_vmax of the largest V among the span's entries
+int ragged_vmax(int V);
+hipError_t launch_isb_down_ragged(const RaggedSpan<IsbDownEntry>& sp, int G, int vt, int vmax, hipStream_t s);
+hipError_t launch_isb_up_ragged(const RaggedSpan<IsbUpEntry>& sp, int G, int vt, int vmax, bool fuse_argmin,
+                                hipStream_t s);
+hipError_t launch_isb_tail_ragged(const TailEntry* t, int n, int G, unsigned lds_bytes, hipStream_t s);
+hipError_t launch_reproject_ragged(const RaggedSpan<ReprojEntry>& sp, hipStream_t s);
+hipError_t launch_inpaint_ragged(const RaggedSpan<InpaintEntry>& sp, hipStream_t s);
+hipError_t launch_to_u8_ragged(const RaggedSpan<CastEntry>& sp, const float* const* src, hipStream_t s);
+hipError_t launch_merge_ragged(const RaggedSpan<MergeArgs>& sp, hipStream_t s);
+// items of one entry of each ragged pass
+inline int sweep_items(int H, int W) { return H * ((W + 63) / 64); }
+inline int isb_items(int npix, int G) {
+  const int per = G == 1 ? kIsbItemPix1 : kIsbItemPix8;
+  return (npix + per - 1) / per;
+}
+constexpr int kPixItem = 256;    // reprojection and merge: pixels per item
+constexpr int kCastItem = 1024;  // uint8 cast: values per item
+inline int pix_items(long long n) { return (int)((n + kPixItem - 1) / kPixItem); }
+inline int cast_items(long long n) { return (int)((n + kCastItem - 1) / kCastItem); }
+int inpaint_tile();  // the side of an inpainting item (a tile of the map)
+
+// ragged.hip: the ragged persistent executor of a batch of frames
+struct RaggedPlan;
+size_t ragged_table_bytes(const ss_frame_desc* d, int F);
+int ragged_create(const ss_frame_desc* d, int F, void* table, size_t bytes, hipStream_t s, RaggedPlan** out);
+int ragged_enqueue(const RaggedPlan* pl, const float* const* match, const float* const* stitch, hipStream_t s);
+int ragged_images(const RaggedPlan* pl, int* n_match, int* n_stitch);
+void ragged_free(RaggedPlan* pl);
 
 }  // namespace ss

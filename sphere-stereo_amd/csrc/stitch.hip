@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include "ss_kernels.h"
+#include "ss_queue.h"
 
 namespace ss {
 
@@ -47,10 +48,7 @@ __global__ void __launch_bounds__(kBlock) k_fill(float* __restrict__ p, float v,
 
 // reprojectDistanceKernel (stitcher.cu:181-204) as an exact atomic z-minimum;
 // map blockIdx.y of the batch
-__global__ void __launch_bounds__(kBlock) k_reproject(Batch<ReprojectArgs> b, int H, int W) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= H * W) return;
-  const ReprojectArgs& a = b.a[blockIdx.y];
+__device__ __forceinline__ void reproject_pixel(const ReprojectArgs& a, int H, int W, int i) {
   const ss_intrinsics c = a.c;
   float* const dout = a.dout;
   float px, py, pz, u, v;
@@ -67,6 +65,22 @@ __global__ void __launch_bounds__(kBlock) k_reproject(Batch<ReprojectArgs> b, in
   // bit patterns is the float order; 1e8 is the fill value
   if (o >= 0 && o < (long long)H * W && dist < 1e8f)
     atomicMin(reinterpret_cast<unsigned int*>(dout) + o, __float_as_uint(dist));
+}
+
+__global__ void __launch_bounds__(kBlock) k_reproject(Batch<ReprojectArgs> b, int H, int W) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= H * W) return;
+  reproject_pixel(b.a[blockIdx.y], H, W, i);
+}
+
+// ragged: a wavefront's item is kPixItem consecutive pixels of one map
+__global__ void __launch_bounds__(kBlock) k_reproject_ragged(RaggedSpan<ReprojEntry> sp) {
+  const int lane = (int)(threadIdx.x & 63);
+  for_each_wave_item(sp, [&](int e, int t) {
+    const ReprojEntry r = load_const(sp.e + e);
+    const int n = r.H * r.W;
+    for (int i = t * kPixItem + lane; i < (t + 1) * kPixItem && i < n; i += 64) reproject_pixel(r.a, r.H, r.W, i);
+  });
 }
 
 // one Jacobi pass of inpaintKernel (stitcher.cu:283-323)
@@ -137,11 +151,11 @@ __device__ __forceinline__ void row_shr(const uint32_t (&m)[3], int k, uint32_t 
   o[0] = (m[0] >> k) | (k ? m[1] << (32 - k) : 0u);
 }
 
-__global__ void __launch_bounds__(kBlock) k_inpaint_multi(Batch<InpaintArgs> bt, int H, int W, int steps,
-                                                          double thr_empty, double thr_src) {
-  const float* __restrict__ din = bt.a[blockIdx.z].din;
-  float* __restrict__ dout = bt.a[blockIdx.z].dout;
-  const uchar2* __restrict__ lut = reinterpret_cast<const uchar2*>(bt.a[blockIdx.z].lut);
+// The passes of one tile (bx, by) of a map: a workgroup's whole body (it
+// returns with every thread past its last barrier).
+__device__ __forceinline__ void inpaint_tile(const float* __restrict__ din, float* __restrict__ dout,
+                                             const uchar2* __restrict__ lut, int H, int W, int steps,
+                                             double thr_empty, double thr_src, int bx, int by) {
   __shared__ float bufa[kRegion * kRegion];
   __shared__ float bufb[kRegion * kRegion];
   __shared__ uchar2 tab[kRegion * kRegion];
@@ -150,7 +164,7 @@ __global__ void __launch_bounds__(kBlock) k_inpaint_multi(Batch<InpaintArgs> bt,
   __shared__ uint32_t near_[kRegion][kRowWords]; // within `steps` of a source
   __shared__ unsigned count, changed[2];  // changed[s & 1] == s + 1: pass s changed a pixel
   const int tid = (int)threadIdx.x;
-  const int ox = blockIdx.x * kTile - kSteps, oy = blockIdx.y * kTile - kSteps;
+  const int ox = bx * kTile - kSteps, oy = by * kTile - kSteps;
   const int reach = min(steps, kSteps);
   if (tid == 0) {
     count = 0;
@@ -271,6 +285,24 @@ __global__ void __launch_bounds__(kBlock) k_inpaint_multi(Batch<InpaintArgs> bt,
     const int gx = ox + lx, gy = oy + ly;
     if (gx < W && gy < H) dout[gy * W + gx] = a[ly * kRegion + lx];
   }
+}
+
+__global__ void __launch_bounds__(kBlock) k_inpaint_multi(Batch<InpaintArgs> bt, int H, int W, int steps,
+                                                          double thr_empty, double thr_src) {
+  const InpaintArgs& a = bt.a[blockIdx.z];
+  inpaint_tile(a.din, a.dout, reinterpret_cast<const uchar2*>(a.lut), H, W, steps, thr_empty, thr_src, blockIdx.x,
+               blockIdx.y);
+}
+
+// ragged: a workgroup's item is one tile of one map, with the map's own
+// step count (0: the tile is copied)
+__global__ void __launch_bounds__(kBlock) k_inpaint_ragged(RaggedSpan<InpaintEntry> sp) {
+  for_each_block_item(sp, [&](int e, int t) {
+    const InpaintEntry m = load_const(sp.e + e);
+    const int by = t / m.tiles_x, bx = t - by * m.tiles_x;
+    inpaint_tile(m.a.din, m.a.dout, reinterpret_cast<const uchar2*>(m.a.lut), m.H, m.W, m.steps, m.thr_empty,
+                 m.thr_src, bx, by);
+  });
 }
 
 // createInpaintingWeightsKernel (stitcher.cu:212-272)
@@ -407,11 +439,9 @@ __global__ void __launch_bounds__(kBlock) k_normalize(float* __restrict__ w, int
 // their memory latencies overlap; the sums still run in reference order, as
 // the reference kernel accumulates them.  RN = 0 is the generic loop.
 template <int RN>
-__global__ void __launch_bounds__(kBlock) k_merge(MergeArgs a) {
+__device__ __forceinline__ void merge_pixel(const MergeArgs& a, int idx) {
   constexpr int RM = RN > 0 ? RN : 1;
-  const int idx = blockIdx.x * kBlock + threadIdx.x;
   const int P = a.PH * a.PW;
-  if (idx >= P) return;
   const float rx = (float)a.Ws / (float)a.W, ry = (float)a.Hs / (float)a.H;
   const float Wsm = (float)a.Ws - 1.1f, Hsm = (float)a.Hs - 1.1f;
   float inv_sum = 0.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
@@ -484,6 +514,31 @@ __global__ void __launch_bounds__(kBlock) k_merge(MergeArgs a) {
   a.rgb_out[3 * idx + 2] = (uint8_t)c2;
 }
 
+template <int RN>
+__global__ void __launch_bounds__(kBlock) k_merge(MergeArgs a) {
+  const int idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= a.PH * a.PW) return;
+  merge_pixel<RN>(a, idx);
+}
+
+// ragged: a wavefront's item is kPixItem consecutive panorama pixels of one
+// frame, whose reference count picks the instantiation
+__global__ void __launch_bounds__(kBlock) k_merge_ragged(RaggedSpan<MergeArgs> sp) {
+  const int lane = (int)(threadIdx.x & 63);
+  for_each_wave_item(sp, [&](int e, int t) {
+    const MergeArgs& a = sp.e[e];
+    const int P = a.PH * a.PW;
+    const int end = min((t + 1) * kPixItem, P);
+    switch (a.R) {
+      case 1: for (int i = t * kPixItem + lane; i < end; i += 64) merge_pixel<1>(a, i); break;
+      case 2: for (int i = t * kPixItem + lane; i < end; i += 64) merge_pixel<2>(a, i); break;
+      case 3: for (int i = t * kPixItem + lane; i < end; i += 64) merge_pixel<3>(a, i); break;
+      case 4: for (int i = t * kPixItem + lane; i < end; i += 64) merge_pixel<4>(a, i); break;
+      default: for (int i = t * kPixItem + lane; i < end; i += 64) merge_pixel<0>(a, i); break;
+    }
+  });
+}
+
 // Four elements per lane (16-byte load, 4-byte store) when both buffers are
 // aligned for it; blockIdx.y selects the image of the batch.
 template <bool VEC4>
@@ -500,6 +555,32 @@ __global__ void __launch_bounds__(kBlock) k_to_u8(Batch<ToU8Args> b, size_t n) {
     return;
   }
   for (size_t k = i; k < i + 4 && k < n; ++k) out[k] = (uint8_t)in[k];
+}
+
+// ragged: a wavefront's item is kCastItem consecutive values of one image
+// (4 per lane per step); src is the run's stitch-image table
+__global__ void __launch_bounds__(kBlock) k_to_u8_ragged(RaggedSpan<CastEntry> sp, const float* const* src) {
+  const int lane = (int)(threadIdx.x & 63);
+  for_each_wave_item(sp, [&](int e, int t) {
+    const CastEntry c = load_const(sp.e + e);
+    const float* __restrict__ in = src[c.src];
+    uint8_t* __restrict__ out = c.out;
+    const bool vec = (reinterpret_cast<uintptr_t>(in) & 15) == 0 && (reinterpret_cast<uintptr_t>(out) & 3) == 0;
+    const long long end = min((long long)(t + 1) * kCastItem, c.n);
+    for (long long i = (long long)t * kCastItem + 4 * lane; i < end; i += 256) {
+      if (vec && i + 4 <= end) {
+        const float4 v = *reinterpret_cast<const float4*>(in + i);
+        *reinterpret_cast<uint32_t*>(out + i) = (uint32_t)(uint8_t)v.x | ((uint32_t)(uint8_t)v.y << 8) |
+                                                ((uint32_t)(uint8_t)v.z << 16) | ((uint32_t)(uint8_t)v.w << 24);
+      } else {
+        for (long long k = i; k < i + 4 && k < end; ++k) out[k] = (uint8_t)in[k];
+      }
+    }
+  });
+}
+
+__global__ void k_publish(PublishArgs a) {
+  if ((int)threadIdx.x < a.n) a.dst[threadIdx.x] = a.src[threadIdx.x];
 }
 
 template <typename A>
@@ -591,6 +672,51 @@ hipError_t launch_to_u8(const ToU8Args* a, int nb, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_to_u8<true>, grid, dim3(kBlock), 0, s, batch_of(a, nb), n);
   else
     hipLaunchKernelGGL(k_to_u8<false>, grid, dim3(kBlock), 0, s, batch_of(a, nb), n);
+  return hipGetLastError();
+}
+
+hipError_t launch_publish(const void* const* src, int n, const void** dst, hipStream_t s) {
+  for (int i0 = 0; i0 < n; i0 += kPublishMax) {
+    PublishArgs a;
+    memset(&a, 0, sizeof a);
+    a.n = n - i0 < kPublishMax ? n - i0 : kPublishMax;
+    a.dst = dst + i0;
+    for (int i = 0; i < a.n; ++i) a.src[i] = src[i0 + i];
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(kPublishMax), 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_reproject_ragged(const RaggedSpan<ReprojEntry>& sp, hipStream_t s) {
+  if (sp.n < 1 || sp.total < 1) return hipErrorInvalidValue;
+  const int grid = persistent_grid(reinterpret_cast<const void*>(&k_reproject_ragged), kBlock, 0, sp.total,
+                                   kBlock / 64);
+  hipLaunchKernelGGL(k_reproject_ragged, dim3(grid), dim3(kBlock), 0, s, sp);
+  return hipGetLastError();
+}
+
+hipError_t launch_inpaint_ragged(const RaggedSpan<InpaintEntry>& sp, hipStream_t s) {
+  if (sp.n < 1 || sp.total < 1) return hipErrorInvalidValue;
+  const int grid = persistent_grid(reinterpret_cast<const void*>(&k_inpaint_ragged), kBlock, 0, sp.total, 1);
+  hipLaunchKernelGGL(k_inpaint_ragged, dim3(grid), dim3(kBlock), 0, s, sp);
+  return hipGetLastError();
+}
+
+int inpaint_tile() { return kTile; }
+
+hipError_t launch_to_u8_ragged(const RaggedSpan<CastEntry>& sp, const float* const* src, hipStream_t s) {
+  if (sp.n < 1 || sp.total < 1) return hipErrorInvalidValue;
+  const int grid = persistent_grid(reinterpret_cast<const void*>(&k_to_u8_ragged), kBlock, 0, sp.total,
+                                   kBlock / 64);
+  hipLaunchKernelGGL(k_to_u8_ragged, dim3(grid), dim3(kBlock), 0, s, sp, src);
+  return hipGetLastError();
+}
+
+hipError_t launch_merge_ragged(const RaggedSpan<MergeArgs>& sp, hipStream_t s) {
+  if (sp.n < 1 || sp.total < 1) return hipErrorInvalidValue;
+  const int grid = persistent_grid(reinterpret_cast<const void*>(&k_merge_ragged), kBlock, 0, sp.total,
+                                   kBlock / 64);
+  hipLaunchKernelGGL(k_merge_ragged, dim3(grid), dim3(kBlock), 0, s, sp);
   return hipGetLastError();
 }
 

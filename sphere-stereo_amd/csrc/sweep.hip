@@ -9,6 +9,7 @@
 // every camera and keeps the selected one), which leaves the result
 // unchanged.
 #include "ss_kernels.h"
+#include "ss_queue.h"
 
 namespace ss {
 
@@ -145,50 +146,47 @@ __global__ void __launch_bounds__(kBlock) k_select_camera(SelectArgs a) {
 // against flush count).  The arithmetic of a sample does not depend on them.
 constexpr int kTileW = 64, kTileH = kBlock / kTileW;
 
-template <typename T, bool POW2, int CH, int GRP>
-__global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
+// One wavefront's 64-pixel row segment (y, x0..x0+63) of a reference: P
+// holds the sizes, candidates and grid_sample constants (SweepArgs or
+// SweepPair), cams / rts the cameras and the reference's transforms into
+// them, img_of(k) camera k's image; st is the wavefront's LDS block.
+template <typename T, bool POW2, int CH, int GRP, typename P, typename ImgOf>
+__device__ __forceinline__ void sweep_row(const P& a, const float4* __restrict__ raylut,
+                                          const float* __restrict__ refimg, void* cost, uint32_t* guide,
+                                          const TorchCam* cams, const float (*rts)[12], ImgOf img_of, int y, int x0,
+                                          int lane, float* st) {
   constexpr int kChunk = CH, kGroup = GRP;
   constexpr int kStride = kGroup + 1;  // LDS words per pixel (conflict-free)
-  __shared__ float stage[kTileH][kTileW * kStride];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int r = tile / a.blocks_per_ref;
-  const int t = tile - r * a.blocks_per_ref;
-  const int ty = t / a.tiles_x, tx = t - ty * a.tiles_x;
-  const int lane = (int)(threadIdx.x & (kTileW - 1)), wv = (int)(threadIdx.x / kTileW);
-  const int x0 = tx * kTileW, y = ty * kTileH + wv;
-  if (y >= a.H) return;  // a whole wavefront
   // lanes past the right edge compute on a valid pixel and store nothing, so
   // that the whole wavefront takes part in the flush
   const bool active = x0 + lane < a.W;
   const int x = active ? x0 + lane : a.W - 1;
-  const SweepRef& rf = a.ref[r];
   const int pix = y * a.W + x;
-  const float4 rl = rf.raylut[pix];
+  const float4 rl = raylut[pix];
   const int k = (int)rl.w;
-  const float r0 = rf.img[3 * pix], r1 = rf.img[3 * pix + 1], r2 = rf.img[3 * pix + 2];
-  if (rf.guide && active) {
+  const float r0 = refimg[3 * pix], r1 = refimg[3 * pix + 1], r2 = refimg[3 * pix + 2];
+  if (guide && active) {
     float gy = ((16.0f + 0.1826f * r0) + 0.6142f * r1) + 0.062f * r2;
     float cb = ((128.0f - 0.1006f * r0) - 0.3386f * r1) + 0.4392f * r2;
     float cr = ((128.0f + 0.4392f * r0) - 0.3989f * r1) - 0.0403f * r2;
     gy = gy < 16.0f ? 16.0f : (gy > 235.0f ? 235.0f : gy);
     cb = cb < 16.0f ? 16.0f : (cb > 240.0f ? 240.0f : cb);
     cr = cr < 16.0f ? 16.0f : (cr > 240.0f ? 240.0f : cr);
-    rf.guide[pix] = pack_guide(gy, cb, cr);
+    guide[pix] = pack_guide(gy, cb, cr);
   }
   // register copies: read through pointers, the camera and transform would
   // be reloaded after every store (the compiler cannot rule out aliasing)
   const int kk = k < 0 ? 0 : k;
-  const TorchCam cam = a.cam[kk];
+  const TorchCam cam = cams[kk];
   float rt[12];
 #pragma unroll
-  for (int i = 0; i < 12; ++i) rt[i] = a.rt[rf.rt0 + kk][i];
-  const float* img = a.img[kk];
+  for (int i = 0; i < 12; ++i) rt[i] = rts[kk][i];
+  const float* img = img_of(kk);
   // nothing sampled: the volume holds zeros there (depth_estimation.py:150-152)
   float cnone = (r0 + r1) + r2;
   cnone = cnone > 500.0f ? 500.0f : cnone;
   const float wlim = (float)(a.W - 1), hlim = (float)(a.H - 1);
-  float* st = stage[wv];
-  T* const row = static_cast<T*>(rf.cost) + (size_t)(y * a.W + x0) * a.NP;
+  T* const row = static_cast<T*>(cost) + (size_t)(y * a.W + x0) * a.NP;
 
   for (int g0 = 0; g0 < a.NP; g0 += kGroup) {
     const int cnt = a.NP - g0 < kGroup ? a.NP - g0 : kGroup;  // a multiple of 8
@@ -298,6 +296,66 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
   }
 }
 
+template <typename T, bool POW2, int CH, int GRP>
+__global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
+  __shared__ float stage[kTileH][kTileW * (GRP + 1)];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int r = tile / a.blocks_per_ref;
+  const int t = tile - r * a.blocks_per_ref;
+  const int ty = t / a.tiles_x, tx = t - ty * a.tiles_x;
+  const int lane = (int)(threadIdx.x & (kTileW - 1)), wv = (int)(threadIdx.x / kTileW);
+  const int y = ty * kTileH + wv;
+  if (y >= a.H) return;  // a whole wavefront
+  const SweepRef& rf = a.ref[r];
+  sweep_row<T, POW2, CH, GRP>(
+      a, rf.raylut, rf.img, rf.cost, rf.guide, a.cam, a.rt + rf.rt0, [&](int k) { return a.img[k]; }, y,
+      tx * kTileW, lane, stage[wv]);
+}
+
+// The uniform part of a SweepPair as sweep_row reads it (in registers; the
+// candidates through the constant address space).
+#if defined(__HIP_DEVICE_COMPILE__)
+using ConstFloatPtr = const SS_CONST_AS float*;
+using ConstPairPtr = const SS_CONST_AS SweepPair*;
+#else
+using ConstFloatPtr = const float*;
+using ConstPairPtr = const SweepPair*;
+#endif
+struct RowGeom {
+  int H, W, N, NP, fastproj;
+  float Wf, Hf, invW, invH, inv2W, inv2H, halfW, halfH;
+  ConstFloatPtr cands;
+};
+
+// The ragged form: items are the row segments of every pair of the span
+// (pair-major, then row, then segment); img is the run's image table.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_sweep_ragged(RaggedSpan<SweepPair> sp, const float* const* img) {
+  constexpr int kGrp = 32;
+  __shared__ float stage[kTileH][kTileW * (kGrp + 1)];
+  const int lane = (int)(threadIdx.x & (kTileW - 1)), wv = (int)(threadIdx.x / kTileW);
+  for_each_wave_item(sp, [&](int e, int t) {
+    const SweepPair* pp = sp.e + e;
+    const ConstPairPtr pc = (ConstPairPtr)pp;
+    const RowGeom g{pc->H,     pc->W,    pc->N,     pc->NP,    pc->fastproj, pc->Wf,    pc->Hf,
+                    pc->invW,  pc->invH, pc->inv2W, pc->inv2H, pc->halfW,    pc->halfH, pc->cands};
+    const int tiles_x = pc->tiles_x;
+    const int y = t / tiles_x, tx = t - y * tiles_x;
+    const float* const* im = img + pc->img0;
+    const float* refimg = im[pc->refcam];
+    const float4* raylut = pc->raylut;
+    void* cost = pc->cost;
+    uint32_t* guide = pc->guide;
+    auto img_of = [&](int k) { return im[k]; };
+    if (pc->pow2)
+      sweep_row<T, true, 4, kGrp>(g, raylut, refimg, cost, guide, pp->cam, pp->rt, img_of, y, tx * kTileW, lane,
+                                  stage[wv]);
+    else
+      sweep_row<T, false, 4, kGrp>(g, raylut, refimg, cost, guide, pp->cam, pp->rt, img_of, y, tx * kTileW, lane,
+                                   stage[wv]);
+  });
+}
+
 template <typename T, int CH, int GRP>
 void launch_sweep_v(const SweepArgs& a, hipStream_t s) {
   const dim3 grid(a.blocks_per_ref * a.R);
@@ -341,6 +399,20 @@ void sweep_variant_info(int v, int* chunk, int* group) {
   static const int kInfo[4][2] = {{4, 32}, {2, 32}, {4, 16}, {2, 16}};
   *chunk = kInfo[v][0];
   *group = kInfo[v][1];
+}
+
+template <typename T>
+hipError_t sweep_ragged_t(const RaggedSpan<SweepPair>& sp, const float* const* img, hipStream_t s) {
+  const int grid = persistent_grid(reinterpret_cast<const void*>(&k_sweep_ragged<T>), kBlock, 0, sp.total,
+                                   kBlock / 64);
+  hipLaunchKernelGGL(k_sweep_ragged<T>, dim3(grid), dim3(kBlock), 0, s, sp, img);
+  return hipGetLastError();
+}
+
+hipError_t launch_sweep_ragged(const RaggedSpan<SweepPair>& sp, const float* const* img, int vt, hipStream_t s) {
+#define SS_CALL(T) sweep_ragged_t<T>(sp, img, s)
+  SS_DISPATCH_T(vt, SS_CALL)
+#undef SS_CALL
 }
 
 hipError_t launch_sweep(const SweepArgs& a, int vt, int variant, hipStream_t s) {

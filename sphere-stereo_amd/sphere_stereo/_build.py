@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 LIB = os.path.join(HERE, "libsphere_stereo.so")
-SOURCES = ["api.hip", "frame.hip", "isb.hip", "sweep.hip", "stitch.hip", "eval.hip", "input.hip", "grad.hip"]
+SOURCES = ["api.hip", "frame.hip", "ragged.hip", "isb.hip", "sweep.hip", "stitch.hip", "eval.hip", "input.hip", "grad.hip"]
 
 # -ffp-contract=off and correctly rounded fp32 division/sqrt keep every
 # operation a single IEEE round-to-nearest step (bit-exact with the oracle);

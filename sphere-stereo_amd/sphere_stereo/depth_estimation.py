@@ -261,18 +261,29 @@ class FrameBatch:
     """The batching form (no counterpart in the reference, which runs one frame
     per call): several estimators run their frames together.  Their rigs and
     images differ, and so may their configurations (matching resolution,
-    candidate count, references, sigmas, volume type, panorama): the batch is
-    split into groups of one configuration, and each group is one native
-    batch in which every pass after the sweep is one launch for a chunk of
-    frames (up to 16 frame-reference pairs), so the latency-bound coarse
-    pyramid levels and the small stitching passes are paid once per chunk
-    instead of once per frame.  The groups are enqueued one after the other
-    on the same stream.  Each estimator keeps its own buffers: after
-    ``estimate_RGBD_panoramas`` its ``fishey_stitcher`` holds its frame's
-    panorama, as after ``estimate_RGBD_panorama``.
+    candidate count, references, sigmas, panorama, inpainting settings).
+
+    ``persistent=True`` (the ragged persistent form): one native executor
+    runs every pass once for all the frames, whatever their configurations,
+    in launches sized to what the GPU holds resident whose wavefronts take
+    work items (row segments, pixel blocks, tiles) from a work queue.  The
+    estimators must share the volume type and have at least 2 candidates and
+    an 8 x 8 matching resolution.  The launch records live in a device table
+    the batch allocates once.
+
+    ``persistent=False``: the batch is split into groups of one
+    configuration, and each group is one native batch in which every pass
+    after the sweep is one launch for a chunk of frames (up to 16
+    frame-reference pairs), so the latency-bound coarse pyramid levels and
+    the small stitching passes are paid once per chunk instead of once per
+    frame.  The groups are enqueued one after the other on the same stream.
+
+    Each estimator keeps its own buffers: after ``estimate_RGBD_panoramas`` its
+    ``fishey_stitcher`` holds its frame's panorama, as after
+    ``estimate_RGBD_panorama``.
     """
 
-    def __init__(self, estimators):
+    def __init__(self, estimators, persistent=False):
         self.estimators = list(estimators)
         F = len(self.estimators)
         if not 1 <= F <= _lib.SS_MAX_BATCH_FRAMES:
@@ -280,26 +291,51 @@ class FrameBatch:
         devs = {e.device for e in self.estimators}
         if len(devs) != 1:
             raise ValueError("the estimators of a batch must share one device")
-        # group by configuration, in order of first appearance
-        groups = {}
-        for i, e in enumerate(self.estimators):
-            groups.setdefault(e._config_key(), []).append(i)
+        self.persistent = bool(persistent)
+        self._table = None
         self.groups = []
         self._frames = []
-        for idx in groups.values():
-            descs = (_lib.ss_frame_desc * len(idx))(*[self.estimators[i]._frame_desc() for i in idx])
-            handle = _lib.C.c_void_p()
-            _lib.check(_lib.lib.ss_frame_batch_create(descs, len(idx), _lib.C.byref(handle)))
-            self._frames.append(handle)
-            self.groups.append(idx)
+        if self.persistent:
+            self.groups.append(list(range(F)))
+            self._frames.append(self._create_ragged())
+        else:
+            # group by configuration, in order of first appearance
+            groups = {}
+            for i, e in enumerate(self.estimators):
+                groups.setdefault(e._config_key(), []).append(i)
+            for idx in groups.values():
+                self._frames.append(self._create_group(idx))
+                self.groups.append(idx)
         self.use_graph = False
         self._inputs = None
+
+    def _create_group(self, idx):
+        descs = (_lib.ss_frame_desc * len(idx))(*[self.estimators[i]._frame_desc() for i in idx])
+        handle = _lib.C.c_void_p()
+        _lib.check(_lib.lib.ss_frame_batch_create(descs, len(idx), _lib.C.byref(handle)))
+        return handle
+
+    def _create_ragged(self):
+        F = len(self.estimators)
+        descs = (_lib.ss_frame_desc * F)(*[e._frame_desc() for e in self.estimators])
+        nbytes = _lib.lib.ss_frame_ragged_table_bytes(descs, F)
+        if nbytes == 0:
+            raise ValueError("these estimators cannot form a persistent batch: they must share the volume type "
+                             "and have >= 2 candidates and an 8 x 8 or larger matching resolution")
+        self._table = torch.empty(int(nbytes), dtype=torch.uint8, device=self.estimators[0].device)
+        handle = _lib.C.c_void_p()
+        _lib.check(_lib.lib.ss_frame_ragged_create(descs, F, self._table.data_ptr(), int(nbytes),
+                                                   self.estimators[0]._stream(), _lib.C.byref(handle)))
+        return handle
 
     def autotune(self, frames, repeats=5):
         """The sweep autotuner (RGBD_Estimator.autotune) for each configuration
         group of the batch, on its first frame; then drops captured graphs so
         that the next graph replay records the chosen variants.  Returns one
-        result per group."""
+        result per group.  (The persistent form's sweep has one scheduling;
+        it is not tuned.)"""
+        if self.persistent:
+            return []
         frames = list(frames)
         out = []
         for idx in self.groups:
@@ -307,11 +343,7 @@ class FrameBatch:
         for i, h in enumerate(self._frames):
             # recreate the native batches: a captured graph keeps its kernels
             _lib.lib.ss_frame_destroy(h)
-            descs = (_lib.ss_frame_desc * len(self.groups[i]))(*[self.estimators[j]._frame_desc()
-                                                                   for j in self.groups[i]])
-            handle = _lib.C.c_void_p()
-            _lib.check(_lib.lib.ss_frame_batch_create(descs, len(self.groups[i]), _lib.C.byref(handle)))
-            self._frames[i] = handle
+            self._frames[i] = self._create_group(self.groups[i])
         return out
 
     def estimate_RGBD_panoramas(self, frames):

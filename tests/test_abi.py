@@ -133,3 +133,50 @@ def test_frame_desc_layout_matches_header(tmp_path):
     assert int(out["camera"]) == C.sizeof(_lib.ss_camera)
     for f in fields:
         assert int(out[f]) == getattr(_lib.ss_frame_desc, f).offset, f
+
+
+def _fake_desc(H, W, N, R, vt, iters=32):
+    """A frame descriptor with placeholder device addresses (the table sizing
+    is host arithmetic: it never reads them)."""
+    lib = _lib.lib
+    d = _lib.ss_frame_desc()
+    d.C, d.R, d.N, d.H, d.W = 4, R, N, H, W
+    d.Hs, d.Ws, d.PH, d.PW = 96, 96, 64, 128
+    d.inpaint_iterations = iters
+    d.volume_type = vt
+    d.workspace_bytes = lib.ss_estimate_workspace_bytes(H, W, N, vt)
+    addr = 0x100000
+    for r in range(R):
+        d.refs[r] = r
+        for arr in (d.d_raylut, d.d_workspace, d.d_dist, d.d_inpaint_lut, d.d_reproj, d.d_scratch, d.d_stitch_u8):
+            arr[r] = addr
+            addr += 0x100000
+    for n in range(N):
+        d.cands[n] = 1.0 / (1 / 0.55 + (1 / 100 - 1 / 0.55) * n / max(N - 1, 1))
+    d.d_sampling = d.d_weights = d.d_dist_pano = d.d_rgb_pano = addr
+    return d
+
+
+def test_ragged_table_sizing_and_rejections():
+    """ss_frame_ragged_table_bytes: the plan of a ragged batch (host logic,
+    no device memory touched) grows with the batch and is refused for mixed
+    volume types, one candidate and resolutions without two pyramid levels."""
+    lib = _lib.lib
+    F32, F16 = _lib.SS_VOLUME_F32, _lib.SS_VOLUME_F16
+
+    def size(*descs):
+        arr = (_lib.ss_frame_desc * len(descs))(*descs)
+        return lib.ss_frame_ragged_table_bytes(arr, len(descs))
+
+    a = _fake_desc(512, 512, 32, 2, F16)
+    b = _fake_desc(768, 768, 48, 4, F16, iters=16)
+    c = _fake_desc(96, 80, 16, 3, F16, iters=0)
+    one = size(a)
+    three = size(a, b, c)
+    assert one > 0 and three > one
+    assert size(a, b, c, a, b, c) > three
+    assert size(a, _fake_desc(512, 512, 32, 2, F32)) == 0      # volume types differ
+    assert size(_fake_desc(512, 512, 1, 2, F16)) == 0          # one candidate
+    assert size(_fake_desc(6, 6, 32, 2, F16)) == 0             # a single pyramid level
+    assert size(_fake_desc(8, 8, 32, 2, F16)) > 0
+    assert lib.ss_frame_ragged_table_bytes(None, 1) == 0

@@ -433,6 +433,71 @@ def test_frame_batch_ragged_matches_single_frames(gpu):
             e.fishey_stitcher.distance_panorama.fill_(-1.0)
 
 
+@pytest.mark.parametrize("vt", ["float32", "float16", "bfloat16"])
+def test_frame_batch_persistent_matches_single_frames(gpu, vt):
+    """The ragged persistent form: frames of different matching resolutions
+    (pyramids of different depths, with and without a coarse tail),
+    candidate counts (V = 2, 3, 4, 6, 8), reference and camera sets, sigmas,
+    inpainting pass counts (an odd and an even number of 16-pass launches,
+    and none), stitch sizes and panoramas in one executor, one launch per
+    pass; every frame bit-exact with its own single-frame run, eager and over
+    graph replays."""
+    from sphere_stereo import _lib
+
+    cases = [((80, 72), 32, (0, 2), 12.5, 32, (96, 96), (128, 64)),
+             ((64, 64), 16, (0, 1, 2), 10.0, 16, (96, 96), (96, 48)),
+             ((96, 80), 48, (0, 1, 2, 3), 15.0, 40, (80, 88), (128, 64)),
+             ((32, 32), 24, (1, 3), 12.5, 0, (96, 96), (64, 32)),
+             ((128, 112), 64, (0, 2), 25.0, 20, (112, 112), (160, 80)),
+             ((80, 72), 32, (0, 2), 12.5, 32, (96, 96), (128, 64))]
+    ests, frames, want = [], [], []
+    for i, (matching, N, refs, sig_s, iters, stitch, pano) in enumerate(cases):
+        fr = frame(seed=50 + i, resolution=160, matching=matching, stitch=stitch, refs=refs, device=gpu,
+                   baseline=0.15 + 0.04 * i)
+        kw = dict(min_dist=0.55, max_dist=100, candidate_count=N, references_indices=fr["refs"],
+                  reprojection_viewpoint=fr["viewpoint"], masks=fr["masks"], matching_resolution=fr["matching"],
+                  rgb_to_stitch_resolution=fr["stitch"], panorama_resolution=pano, sigma_i=10, sigma_s=sig_s)
+        est = RGBD_Estimator(fr["calibs"], device=gpu, volume_dtype=vt, **kw)
+        if iters != est.fishey_stitcher.inpainting_iterations:
+            est.fishey_stitcher.inpainting_iterations = iters
+            _lib.lib.ss_frame_destroy(est._frame)
+            est._frame = est._make_frame()
+        print("persistent case %d: %s N=%d refs=%s iters=%d %s" % (i, matching, N, refs, iters, vt), flush=True)
+        rgb, dist = est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])
+        want.append((rgb.clone(), dist.clone()))
+        ests.append(est)
+        frames.append(([t.contiguous() for t in fr["to_match"]], [t.contiguous() for t in fr["to_stitch"]]))
+    for e in ests:
+        e.fishey_stitcher.RGB_panorama.zero_()
+        e.fishey_stitcher.distance_panorama.zero_()
+    batch = FrameBatch(ests, persistent=True)
+    assert batch.groups == [list(range(len(cases)))]
+    for mode in ("eager", "eager", "graph", "graph", "graph"):
+        batch.use_graph = mode == "graph"
+        out = batch.estimate_RGBD_panoramas(frames)
+        torch.cuda.synchronize()
+        for i, ((rgb, dist), (wr, wd)) in enumerate(zip(out, want)):
+            assert torch.equal(rgb, wr), "%s rgb panorama of frame %d" % (mode, i)
+            assert torch.equal(dist.view(torch.int32), wd.view(torch.int32)), "%s distance panorama %d" % (mode, i)
+        for e in ests:
+            e.fishey_stitcher.distance_panorama.fill_(-1.0)
+            e.fishey_stitcher.RGB_panorama.fill_(7)
+
+
+def test_frame_batch_persistent_rejects(gpu):
+    """The persistent form needs one volume type and N >= 2."""
+    ests = []
+    for i, (N, vt) in enumerate([(32, "float32"), (16, "float16")]):
+        fr = frame(seed=70 + i, resolution=160, matching=(64, 64), refs=(0, 2), device=gpu)
+        kw = dict(min_dist=0.55, max_dist=100, candidate_count=N, references_indices=fr["refs"],
+                  reprojection_viewpoint=fr["viewpoint"], masks=fr["masks"], matching_resolution=fr["matching"],
+                  rgb_to_stitch_resolution=fr["stitch"], panorama_resolution=(64, 32), sigma_i=10, sigma_s=12.5)
+        ests.append(RGBD_Estimator(fr["calibs"], device=gpu, volume_dtype=vt, **kw))
+    with pytest.raises(ValueError):
+        FrameBatch(ests, persistent=True)
+    FrameBatch(ests)  # the grouped form takes them
+
+
 @pytest.mark.parametrize("dtype", [torch.uint8, torch.uint16, torch.float32])
 @pytest.mark.parametrize("shape,size", [((1216, 1216, 3), (1024, 1024)), ((1216, 1216, 3), (512, 512)),
                                         ((100, 90, 3), (137, 121)), ((7, 9, 1), (3, 4)), ((64, 48), (64, 48))])
