@@ -316,18 +316,28 @@ int ss_frame_batch_run(void *frame, const float *const *d_match, const float *co
  * resolution of at least 8 x 8.  Each pass of the frame (the sweep, each
  * level of the two filters, the coarse tails, the reprojection, each group of
  * 16 inpainting passes, the uint8 casts, the merge) is one launch for all the
- * frames, sized to the workgroups the GPU holds resident, whose wavefronts
- * take work items (row segments, pixel blocks, tiles) from a work queue.  The
- * launch records, the filters' weight tables and the queue live in d_table, a
- * device buffer of ss_frame_ragged_table_bytes(descs, nframes) bytes (256-byte
- * aligned) that the caller keeps until ss_frame_destroy; it is written on
- * `stream` during the call, which returns once it is ready.  Runs go through
+ * frames (the filter levels one per class of candidate counts present).  In
+ * the persistent form a launch is sized to the workgroups the GPU holds
+ * resident and each worker loops over work items (row segments, pixel
+ * blocks, tiles); in the dispatched form each worker takes one item.  Each
+ * XCD takes an eighth of every frame-reference pair, pair after pair.  The launch records, the filters' weight tables and
+ * the per-run image pointer slots live in d_table, a device buffer of
+ * ss_frame_ragged_table_bytes(descs, nframes) bytes (256-byte aligned) that
+ * the caller keeps until ss_frame_destroy; it is written on `stream` during
+ * the call, which returns once it is ready.  Runs go through
  * ss_frame_batch_run (d_match: every frame's C images, frame after frame;
  * d_stitch: every frame's R images), with or without a graph.  Runs of one
- * executor use one queue: they must not overlap on different streams.
- * Returns 0 from ss_frame_ragged_table_bytes for descriptors it rejects. */
+ * executor share the pointer slots: they must not overlap on different
+ * streams.  Returns 0 from ss_frame_ragged_table_bytes for descriptors it
+ * rejects. */
 size_t ss_frame_ragged_table_bytes(const ss_frame_desc *descs, int nframes);
-int ss_frame_ragged_create(const ss_frame_desc *descs, int nframes, void *d_table, size_t table_bytes,
+/* schedule: SS_RAGGED_PERSISTENT (every pass in the persistent form),
+ * SS_RAGGED_DISPATCHED (every pass with a worker per item, balanced by the
+ * hardware dispatcher), SS_RAGGED_AUTO (per pass, the form measured faster). */
+#define SS_RAGGED_PERSISTENT 0
+#define SS_RAGGED_DISPATCHED 1
+#define SS_RAGGED_AUTO 2
+int ss_frame_ragged_create(const ss_frame_desc *descs, int nframes, int schedule, void *d_table, size_t table_bytes,
                            ss_stream stream, void **frame);
 int ss_frame_destroy(void *frame);
 

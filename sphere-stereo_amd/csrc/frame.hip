@@ -208,7 +208,7 @@ size_t ss_frame_ragged_table_bytes(const ss_frame_desc* descs, int nframes) {
   return ss::ragged_table_bytes(descs, nframes);
 }
 
-int ss_frame_ragged_create(const ss_frame_desc* descs, int nframes, void* d_table, size_t table_bytes,
+int ss_frame_ragged_create(const ss_frame_desc* descs, int nframes, int schedule, void* d_table, size_t table_bytes,
                            ss_stream stream, void** frame) {
   if (!descs || !frame) return ss::report_error(SS_ERR_ARG, "null descriptor array or output handle");
   if (nframes < 1 || nframes > SS_MAX_BATCH_FRAMES)
@@ -222,7 +222,8 @@ int ss_frame_ragged_create(const ss_frame_desc* descs, int nframes, void* d_tabl
   if (dev < 0 || dev >= kMaxDevices) return ss::report_error(SS_ERR_UNSUPPORTED, "device index beyond the pool");
   FR_HIP(pool_init(g_pool[dev]));
   ss::RaggedPlan* plan = nullptr;
-  FR_CALL(ss::ragged_create(descs, nframes, d_table, table_bytes, reinterpret_cast<hipStream_t>(stream), &plan));
+  FR_CALL(ss::ragged_create(descs, nframes, schedule, d_table, table_bytes, reinterpret_cast<hipStream_t>(stream),
+                            &plan));
   Frame* f = new (std::nothrow) Frame();
   if (!f) {
     ss::ragged_free(plan);

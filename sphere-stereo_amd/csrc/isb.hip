@@ -22,7 +22,7 @@
 
 #include "ss_isb_stencil.h"
 #include "ss_kernels.h"
-#include "ss_queue.h"
+#include "ss_ragged.h"
 
 namespace ss {
 
@@ -381,9 +381,9 @@ __global__ void __launch_bounds__(kTailBlock) k_isb_tail(IsbTailArgs t) {
 }
 
 // ---------------------------------------------------------------------------
-// Ragged persistent forms: a wavefront's item is kIsbItemPix8 (G = 8) or
-// kIsbItemPix1 (G = 1) consecutive coarse pixels of one entry, 64 / G per
-// step; the entry's V picks the instantiation.
+// Ragged forms: a wavefront's item is kIsbItemPix8 (G = 8) or kIsbItemPix1
+// (G = 1) consecutive coarse pixels of one entry, one step; the entry's V
+// picks the instantiation.
 // ---------------------------------------------------------------------------
 template <int G>
 constexpr int item_pix() {
@@ -393,64 +393,70 @@ constexpr int item_pix() {
 template <int G, int V, typename T>
 __device__ __forceinline__ void down_item(const IsbDownArgs& a, int item, int lane, int slot) {
   const int npix = a.hout * a.wout;
-  const int base = item * item_pix<G>();
-#pragma unroll 1
-  for (int it = 0; it < item_pix<G>() && base + it < npix; it += 64 / G) {
-    int pix = base + it + slot;
-    const bool active = pix < npix;
-    if (!active) pix = npix - 1;
-    down_pixel<G, V, T>(a, pix, active, lane);
-  }
+  int pix = item * item_pix<G>() + slot;
+  const bool active = pix < npix;
+  if (!active) pix = npix - 1;
+  down_pixel<G, V, T>(a, pix, active, lane);
 }
 
 template <int G, int V, typename T, bool FUSE>
 __device__ __forceinline__ void up_item(const IsbUpArgs& a, int item, int lane, int slot) {
   const int npix = a.hin * a.win;
-  const int base = item * item_pix<G>();
-#pragma unroll 1
-  for (int it = 0; it < item_pix<G>() && base + it < npix; it += 64 / G) {
-    int pix = base + it + slot;
-    const bool active = pix < npix;
-    if (!active) pix = npix - 1;
-    up_pixel<G, V, T, FUSE>(a, pix, active, lane);
-  }
+  int pix = item * item_pix<G>() + slot;
+  const bool active = pix < npix;
+  if (!active) pix = npix - 1;
+  up_pixel<G, V, T, FUSE>(a, pix, active, lane);
 }
 
-// Calls f(integral_constant<V>) for the entry's V.  A launch instantiates
-// the V <= VMAX only (VMAX: the smallest of 1, 2, 4, 8, 16 that covers its
-// entries), so that entries with few candidates do not run at the register
-// budget, hence the occupancy, of the largest V.
+// Calls f(integral_constant<V>) for the entry's V.  A launch takes the
+// entries of one V class (VMAX = ragged_vmax(V): {1}, {2}, {3, 4}, {6, 8},
+// {12, 16}) and instantiates those V only, so that its register budget, hence
+// its occupancy, is that of its own candidate counts.
 template <int VMAX, typename F>
 __device__ __forceinline__ void with_v(int V, F&& f) {
   switch (V) {
-    case 1: f(std::integral_constant<int, 1>()); break;
-    case 2: if constexpr (VMAX >= 2) f(std::integral_constant<int, 2>()); break;
-    case 3: if constexpr (VMAX >= 3) f(std::integral_constant<int, 3>()); break;
-    case 4: if constexpr (VMAX >= 4) f(std::integral_constant<int, 4>()); break;
-    case 6: if constexpr (VMAX >= 6) f(std::integral_constant<int, 6>()); break;
-    case 8: if constexpr (VMAX >= 8) f(std::integral_constant<int, 8>()); break;
-    case 12: if constexpr (VMAX >= 12) f(std::integral_constant<int, 12>()); break;
-    case 16: if constexpr (VMAX >= 16) f(std::integral_constant<int, 16>()); break;
+    case 1: if constexpr (VMAX == 1) f(std::integral_constant<int, 1>()); break;
+    case 2: if constexpr (VMAX == 2) f(std::integral_constant<int, 2>()); break;
+    case 3: if constexpr (VMAX == 4) f(std::integral_constant<int, 3>()); break;
+    case 4: if constexpr (VMAX == 4) f(std::integral_constant<int, 4>()); break;
+    case 6: if constexpr (VMAX == 8) f(std::integral_constant<int, 6>()); break;
+    case 8: if constexpr (VMAX == 8) f(std::integral_constant<int, 8>()); break;
+    case 12: if constexpr (VMAX == 16) f(std::integral_constant<int, 12>()); break;
+    case 16: if constexpr (VMAX == 16) f(std::integral_constant<int, 16>()); break;
     default: break;
   }
 }
 
-template <int G, typename T, int VMAX>
-__global__ void __launch_bounds__(kBlock) k_isb_down_ragged(RaggedSpan<IsbDownEntry> sp) {
+// Occupancy targets (waves per SIMD, the second __launch_bounds__ argument)
+// of the ragged passes per V class, each the most the class reaches without
+// register spills.  The persistent loop makes the compiler's own choice
+// lower than the plain kernels' for the same V.
+template <int G, int VMAX>
+constexpr int down_waves() {
+  return G == 1 ? 7 : (VMAX <= 2 ? 6 : VMAX == 4 ? 5 : 4);
+}
+template <int G, int VMAX, bool FUSE>
+constexpr int up_waves() {
+  return G == 1 ? 4 : (FUSE ? (VMAX == 1 ? 5 : VMAX == 2 ? 4 : VMAX == 4 ? 3 : VMAX == 8 ? 2 : 1)
+                            : (VMAX <= 8 ? 3 : 2));
+}
+
+template <int G, typename T, int VMAX, bool LOOP>
+__global__ void __launch_bounds__(kBlock, (down_waves<G, VMAX>())) k_isb_down_ragged(RaggedSpan<IsbDownEntry> sp) {
   const int l64 = (int)(threadIdx.x & 63);
   const int lane = G == 1 ? 0 : (l64 & (G - 1)), slot = l64 / G;
-  for_each_wave_item(sp, [&](int e, int t) {
+  for_each_wave_item<LOOP>(sp, [&](int e, int t) {
     const IsbDownEntry en = load_const(sp.e + e);
     const IsbDownArgs& a = en.a;
     with_v<VMAX>(en.V, [&](auto v) { down_item<G, decltype(v)::value, T>(a, t, lane, slot); });
   });
 }
 
-template <int G, typename T, bool FUSE, int VMAX>
-__global__ void __launch_bounds__(kBlock) k_isb_up_ragged(RaggedSpan<IsbUpEntry> sp) {
+template <int G, typename T, bool FUSE, int VMAX, bool LOOP>
+__global__ void __launch_bounds__(kBlock, (up_waves<G, VMAX, FUSE>())) k_isb_up_ragged(RaggedSpan<IsbUpEntry> sp) {
   const int l64 = (int)(threadIdx.x & 63);
   const int lane = G == 1 ? 0 : (l64 & (G - 1)), slot = l64 / G;
-  for_each_wave_item(sp, [&](int e, int t) {
+  for_each_wave_item<LOOP>(sp, [&](int e, int t) {
     const IsbUpEntry en = load_const(sp.e + e);
     const IsbUpArgs& a = en.a;
     with_v<VMAX>(en.V, [&](auto v) { up_item<G, decltype(v)::value, T, FUSE>(a, t, lane, slot); });
@@ -462,7 +468,16 @@ template <int G>
 __global__ void __launch_bounds__(kTailBlock) k_isb_tail_ragged(const TailEntry* __restrict__ te) {
   extern __shared__ __align__(16) char smem[];
   const TailEntry& e = te[blockIdx.x];
-  with_v<G == 1 ? 1 : 16>(e.V, [&](auto v) { tail_body<G, decltype(v)::value>(e.t, e.g0, e.c0, smem); });
+  auto body = [&](auto v) { tail_body<G, decltype(v)::value>(e.t, e.g0, e.c0, smem); };
+  if constexpr (G == 1) {
+    with_v<1>(e.V, body);
+  } else {
+    with_v<1>(e.V, body);
+    with_v<2>(e.V, body);
+    with_v<4>(e.V, body);
+    with_v<8>(e.V, body);
+    with_v<16>(e.V, body);
+  }
 }
 
 // Weight tables: entry d of table i is the guide weight factor
@@ -669,18 +684,18 @@ hipError_t launch_packed_to_guide3(const uint32_t* g, uint8_t* g3, int HW, hipSt
   return hipGetLastError();
 }
 
-template <int G, typename T, int VMAX>
+template <int G, typename T, int VMAX, bool LOOP>
 hipError_t down_ragged_t(const RaggedSpan<IsbDownEntry>& sp, hipStream_t s) {
-  const auto k = &k_isb_down_ragged<G, T, VMAX>;
-  const int grid = persistent_grid(reinterpret_cast<const void*>(k), kBlock, 0, sp.total, kBlock / 64);
+  const auto k = &k_isb_down_ragged<G, T, VMAX, LOOP>;
+  const int grid = ragged_grid(reinterpret_cast<const void*>(k), kBlock, 0, sp, kBlock / 64, LOOP);
   hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, sp);
   return hipGetLastError();
 }
 
-template <int G, typename T, bool FUSE, int VMAX>
+template <int G, typename T, bool FUSE, int VMAX, bool LOOP>
 hipError_t up_ragged_t(const RaggedSpan<IsbUpEntry>& sp, hipStream_t s) {
-  const auto k = &k_isb_up_ragged<G, T, FUSE, VMAX>;
-  const int grid = persistent_grid(reinterpret_cast<const void*>(k), kBlock, 0, sp.total, kBlock / 64);
+  const auto k = &k_isb_up_ragged<G, T, FUSE, VMAX, LOOP>;
+  const int grid = ragged_grid(reinterpret_cast<const void*>(k), kBlock, 0, sp, kBlock / 64, LOOP);
   hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, sp);
   return hipGetLastError();
 }
@@ -696,15 +711,15 @@ hipError_t up_ragged_t(const RaggedSpan<IsbUpEntry>& sp, hipStream_t s) {
   }
 
 template <typename T>
-hipError_t down_ragged_vt(const RaggedSpan<IsbDownEntry>& sp, int vmax, hipStream_t s) {
-#define SS_CALL(m) down_ragged_t<8, T, m>(sp, s)
+hipError_t down_ragged_vt(const RaggedSpan<IsbDownEntry>& sp, int vmax, bool loop, hipStream_t s) {
+#define SS_CALL(m) (loop ? down_ragged_t<8, T, m, true>(sp, s) : down_ragged_t<8, T, m, false>(sp, s))
   SS_DISPATCH_VMAX(vmax, SS_CALL)
 #undef SS_CALL
 }
 
 template <typename T, bool FUSE>
-hipError_t up_ragged_vt(const RaggedSpan<IsbUpEntry>& sp, int vmax, hipStream_t s) {
-#define SS_CALL(m) up_ragged_t<8, T, FUSE, m>(sp, s)
+hipError_t up_ragged_vt(const RaggedSpan<IsbUpEntry>& sp, int vmax, bool loop, hipStream_t s) {
+#define SS_CALL(m) (loop ? up_ragged_t<8, T, FUSE, m, true>(sp, s) : up_ragged_t<8, T, FUSE, m, false>(sp, s))
   SS_DISPATCH_VMAX(vmax, SS_CALL)
 #undef SS_CALL
 }
@@ -715,26 +730,31 @@ int ragged_vmax(int V) {
   return m;
 }
 
-hipError_t launch_isb_down_ragged(const RaggedSpan<IsbDownEntry>& sp, int G, int vt, int vmax, hipStream_t s) {
+hipError_t launch_isb_down_ragged(const RaggedSpan<IsbDownEntry>& sp, int G, int vt, int vmax, bool loop,
+                                  hipStream_t s) {
   if (sp.n < 1 || sp.total < 1) return hipErrorInvalidValue;
-  if (G == 1) return (vt == SS_VOLUME_F32 && vmax == 1) ? down_ragged_t<1, float, 1>(sp, s) : hipErrorInvalidValue;
-#define SS_CALL(T) down_ragged_vt<T>(sp, vmax, s)
+  if (G == 1) {
+    if (vt != SS_VOLUME_F32 || vmax != 1) return hipErrorInvalidValue;
+    return loop ? down_ragged_t<1, float, 1, true>(sp, s) : down_ragged_t<1, float, 1, false>(sp, s);
+  }
+#define SS_CALL(T) down_ragged_vt<T>(sp, vmax, loop, s)
   SS_DISPATCH_T(vt, SS_CALL)
 #undef SS_CALL
 }
 
-hipError_t launch_isb_up_ragged(const RaggedSpan<IsbUpEntry>& sp, int G, int vt, int vmax, bool fuse_argmin,
+hipError_t launch_isb_up_ragged(const RaggedSpan<IsbUpEntry>& sp, int G, int vt, int vmax, bool fuse_argmin, bool loop,
                                 hipStream_t s) {
   if (sp.n < 1 || sp.total < 1) return hipErrorInvalidValue;
-  if (G == 1)
-    return (vt == SS_VOLUME_F32 && vmax == 1 && !fuse_argmin) ? up_ragged_t<1, float, false, 1>(sp, s)
-                                                              : hipErrorInvalidValue;
+  if (G == 1) {
+    if (vt != SS_VOLUME_F32 || vmax != 1 || fuse_argmin) return hipErrorInvalidValue;
+    return loop ? up_ragged_t<1, float, false, 1, true>(sp, s) : up_ragged_t<1, float, false, 1, false>(sp, s);
+  }
   if (fuse_argmin) {
-#define SS_CALL(T) up_ragged_vt<T, true>(sp, vmax, s)
+#define SS_CALL(T) up_ragged_vt<T, true>(sp, vmax, loop, s)
     SS_DISPATCH_T(vt, SS_CALL)
 #undef SS_CALL
   }
-#define SS_CALL(T) up_ragged_vt<T, false>(sp, vmax, s)
+#define SS_CALL(T) up_ragged_vt<T, false>(sp, vmax, loop, s)
   SS_DISPATCH_T(vt, SS_CALL)
 #undef SS_CALL
 }

@@ -13,13 +13,16 @@
 //   reprojection, inpainting (one launch per 16 passes), uint8 casts, merge
 //
 // A pyramid level k holds the level-k pass of every pair that has one (the
-// pyramids of different resolutions have different depths and tail starts).
-// The launches are persistent: as many workgroups as the GPU holds resident,
-// whose wavefronts take items from a work queue (ss_queue.h) until it is
-// empty, so a launch over many small and large pairs is balanced by the queue
-// and not by the hardware dispatcher, and no pair waits for a launch of its
-// own.  The per-item arithmetic is the plain path's (the same device
-// functions), so the results are the plain path's bit for bit.
+// pyramids of different resolutions have different depths and tail starts),
+// one launch per V class of the candidate counts present (the register
+// budget of a class kernel is its own).  Each XCD takes an eighth of every
+// pair, pair after pair (ss_ragged.h), so small and large pairs share
+// launches and no pair waits for a launch of its own.  A launch is
+// persistent (as many workgroups as the GPU holds resident, each worker
+// looping over its items) or dispatched (a worker per item), per pass group
+// as the caller's schedule asks.  The per-item arithmetic is the plain
+// path's (the same device functions), so the results are the plain path's
+// bit for bit.
 //
 // Everything but the image pointers is fixed at creation: the launch
 // sequence, its records (in a device table the caller allocates: the library
@@ -39,10 +42,15 @@
 
 namespace ss {
 
-// Resident workgroups of a kernel: occupancy x CU count, computed once per
-// kernel (the process is assumed to use one kind of GPU); never more
-// workgroups than the items need.
-int persistent_grid(const void* kernel, int block, size_t lds, long long items, int workers_per_block) {
+// Persistent form: the resident workgroups of the kernel (occupancy x CU
+// count, computed once per kernel: the process is assumed to use one kind of
+// GPU), no more than the items need, and at least one per part.
+// Dispatched form: enough workgroups in each part for one worker per item.
+int ragged_grid_of(const void* kernel, int block, size_t lds, const int* xtotal, int workers_per_block, bool loop) {
+  long long most = 0;
+  for (int x = 0; x < kParts; ++x) most = std::max(most, (long long)xtotal[x]);
+  const long long per_part = std::max(1LL, (most + workers_per_block - 1) / workers_per_block);
+  if (!loop) return (int)(per_part * kParts);
   static std::mutex mu;
   static std::vector<std::pair<const void*, int>> cache;
   int cap = 0;
@@ -61,8 +69,7 @@ int persistent_grid(const void* kernel, int block, size_t lds, long long items, 
     std::lock_guard<std::mutex> lk(mu);
     cache.emplace_back(kernel, cap);
   }
-  const long long need = (items + workers_per_block - 1) / workers_per_block;
-  return (int)std::max(1LL, std::min((long long)cap, need));
+  return (int)std::max((long long)kParts, std::min((long long)cap, per_part * kParts));
 }
 
 namespace {
@@ -75,8 +82,8 @@ struct Image {
   char* base = nullptr;
   size_t off = 0;
   template <typename T>
-  size_t take(size_t count) {
-    off = (off + 63) & ~size_t(63);
+  size_t take(size_t count, size_t align = 64) {
+    off = (off + align - 1) & ~(align - 1);
     const size_t o = off;
     off += count * sizeof(T);
     if (h.size() < off) h.resize(off, 0);
@@ -102,21 +109,34 @@ struct Pass {
   }
 };
 
+// Lays out a pass's records, the entries' item ranges and the XCD sequences
+// of their chunks.
 template <typename E>
-RaggedSpan<E> place(Image& im, const Pass<E>& p, WorkQueue* q) {
+RaggedSpan<E> place(Image& im, const Pass<E>& p) {
   RaggedSpan<E> sp{};
   sp.n = (int)p.e.size();
   if (sp.n == 0) return sp;
   const size_t oe = im.take<E>(p.e.size());
   const size_t of = im.take<int>(p.e.size() + 1);
+  const size_t ox = im.take<int>((size_t)kParts * (p.e.size() + 1));
   memcpy(im.host<E>(oe), p.e.data(), sizeof(E) * p.e.size());
   int* first = im.host<int>(of);
   first[0] = 0;
   for (size_t k = 0; k < p.items.size(); ++k) first[k + 1] = first[k] + p.items[k];
   sp.total = first[sp.n];
+  int* xf = im.host<int>(ox);
+  for (int x = 0; x < kParts; ++x) {
+    int* f = xf + x * (sp.n + 1);
+    f[0] = 0;
+    for (int k = 0; k < sp.n; ++k) {
+      const long long n = p.items[k];
+      f[k + 1] = f[k] + (int)((x + 1) * n / kParts - x * n / kParts);
+    }
+    sp.xtotal[x] = f[sp.n];
+  }
   sp.e = im.dev<const E>(oe);
   sp.first = im.dev<const int>(of);
-  sp.q = q;
+  sp.xfirst = im.dev<const int>(ox);
   return sp;
 }
 
@@ -144,6 +164,8 @@ struct Filter {
 
 struct RaggedPlan {
   int F = 0, vt = 0;
+  // per pass group: the persistent form (true) or the dispatched form
+  bool loop_sweep = true, loop_filter = true, loop_stitch = true;
   int n_img = 0, n_stitch = 0;
   std::vector<int> img0, stitch0;  // per frame: first slot in the run tables
   const float** run_img = nullptr;
@@ -164,7 +186,7 @@ namespace {
 
 // One filter (cost: G = 8, volume type vt; distance: G = 1, fp32) over every
 // pair: lv[p] the pair's levels, L[p] its depth, wtab[p] its weight table.
-void plan_filter(Image& im, WorkQueue* q, int G, int vt, const std::vector<const PyrLevel*>& lv,
+void plan_filter(Image& im, int G, int vt, const std::vector<const PyrLevel*>& lv,
                  const std::vector<int>& L, const std::vector<int>& V, const std::vector<float*>& wtab,
                  const std::vector<double>& sigma_s, const std::vector<float*>& dist,
                  const std::vector<ArgminParams>& am, const std::vector<float*>& fill, Filter* out) {
@@ -178,19 +200,26 @@ void plan_filter(Image& im, WorkQueue* q, int G, int vt, const std::vector<const
     max_down = std::max(max_down, last_down[p]);
     max_up = std::max(max_up, first_up[p]);
   }
+  // one launch per level and V class present at that level
+  std::vector<int> classes;
+  for (int p = 0; p < P; ++p)
+    if (std::find(classes.begin(), classes.end(), ragged_vmax(V[p])) == classes.end())
+      classes.push_back(ragged_vmax(V[p]));
+  std::sort(classes.begin(), classes.end());
   for (int k = 1; k <= max_down; ++k) {
-    Pass<IsbDownEntry> pass;
-    int vm = 1;
-    for (int p = 0; p < P; ++p) {
-      if (k > last_down[p]) continue;
-      vm = std::max(vm, V[p]);
-      const PyrLevel* v = lv[p];
-      IsbDownEntry e{};
-      e.a = IsbDownArgs{v[k - 1].g, v[k - 1].c, v[k - 1].h, v[k - 1].w, v[k].g, v[k].c, v[k].h, v[k].w, wtab[p]};
-      e.V = V[p];
-      pass.add(e, isb_items(v[k].h * v[k].w, G));
+    for (int vm : classes) {
+      Pass<IsbDownEntry> pass;
+      for (int p = 0; p < P; ++p) {
+        if (k > last_down[p] || ragged_vmax(V[p]) != vm) continue;
+        const PyrLevel* v = lv[p];
+        IsbDownEntry e{};
+        e.a = IsbDownArgs{v[k - 1].g, v[k - 1].c, v[k - 1].h, v[k - 1].w, v[k].g, v[k].c, v[k].h, v[k].w, wtab[p]};
+        e.V = V[p];
+        pass.add(e, isb_items(v[k].h * v[k].w, G));
+      }
+      if (!pass.e.empty())
+        out->down.push_back(DownLaunch{place(im, pass), G, k == 1 ? vt : SS_VOLUME_F32, vm});
     }
-    out->down.push_back(DownLaunch{place(im, pass, q), G, k == 1 ? vt : SS_VOLUME_F32, ragged_vmax(vm)});
   }
   std::vector<TailEntry> tails;
   unsigned lds = 0;
@@ -212,32 +241,43 @@ void plan_filter(Image& im, WorkQueue* q, int G, int vt, const std::vector<const
     out->tail.lds = lds;
   }
   for (int k = max_up; k >= 0; --k) {
-    Pass<IsbUpEntry> pass;
     const bool fuse = !dist.empty() && k == 0;
-    int vm = 1;
-    for (int p = 0; p < P; ++p) {
-      if (k > first_up[p]) continue;
-      vm = std::max(vm, V[p]);
-      const PyrLevel* v = lv[p];
-      double wu, wd;
-      up_weights(sigma_s[p], k, &wu, &wd);
-      IsbUpEntry e{};
-      e.a = IsbUpArgs{v[k + 1].g, v[k + 1].c, v[k + 1].h, v[k + 1].w, v[k].g, v[k].c, v[k].h, v[k].w,
-                      (float)wu,  (float)wd,  wtab[p],    fuse ? dist[p] : nullptr, fuse ? am[p] : ArgminParams{},
-                      fuse ? fill[p] : nullptr};
-      e.V = V[p];
-      pass.add(e, isb_items(v[k + 1].h * v[k + 1].w, G));
+    for (int vm : classes) {
+      Pass<IsbUpEntry> pass;
+      for (int p = 0; p < P; ++p) {
+        if (k > first_up[p] || ragged_vmax(V[p]) != vm) continue;
+        const PyrLevel* v = lv[p];
+        double wu, wd;
+        up_weights(sigma_s[p], k, &wu, &wd);
+        IsbUpEntry e{};
+        e.a = IsbUpArgs{v[k + 1].g, v[k + 1].c, v[k + 1].h, v[k + 1].w, v[k].g, v[k].c, v[k].h, v[k].w,
+                        (float)wu,  (float)wd,  wtab[p],    fuse ? dist[p] : nullptr, fuse ? am[p] : ArgminParams{},
+                        fuse ? fill[p] : nullptr};
+        e.V = V[p];
+        pass.add(e, isb_items(v[k + 1].h * v[k + 1].w, G));
+      }
+      if (!pass.e.empty())
+        out->up.push_back(UpLaunch{place(im, pass), G, k == 0 ? vt : SS_VOLUME_F32, vm, fuse});
     }
-    out->up.push_back(UpLaunch{place(im, pass, q), G, k == 0 ? vt : SS_VOLUME_F32, ragged_vmax(vm), fuse});
   }
 }
 
 // Lays out the whole plan; with im.base == nullptr it only measures.
-int build(const ss_frame_desc* d, int F, Image& im, RaggedPlan* pl) {
+int build(const ss_frame_desc* d, int F, int schedule, Image& im, RaggedPlan* pl) {
+  // SS_RAGGED_AUTO: the form measured faster per pass group on MI355X (the
+  // dispatched one for every group, on a 16-rig batch of 512^2..1024^2,
+  // 16..64 candidates, 2..4 references)
+  pl->loop_sweep = schedule == SS_RAGGED_PERSISTENT;
+  pl->loop_filter = schedule == SS_RAGGED_PERSISTENT;
+  pl->loop_stitch = schedule == SS_RAGGED_PERSISTENT;
   pl->F = F;
   pl->vt = d[0].volume_type;
-  const size_t oq = im.take<WorkQueue>(1);
-  WorkQueue* q = im.dev<WorkQueue>(oq);
+  const int per = inpaint_steps_per_launch(), tile = inpaint_tile();
+  int max_launches = 0;
+  for (int f = 0; f < F; ++f) {
+    const int K = (d[f].inpaint_iterations + per - 1) / per;
+    max_launches = std::max(max_launches, K + (K & 1));
+  }
   for (int f = 0; f < F; ++f) {
     pl->img0.push_back(pl->n_img);
     pl->stitch0.push_back(pl->n_stitch);
@@ -265,12 +305,8 @@ int build(const ss_frame_desc* d, int F, Image& im, RaggedPlan* pl) {
   Pass<CastEntry> cast;
   Pass<MergeArgs> merge;
   int p = 0;
-  int max_launches = 0;
-  const int per = inpaint_steps_per_launch(), tile = inpaint_tile();
   for (int f = 0; f < F; ++f) {
     const ss_frame_desc& c = d[f];
-    const int K = (c.inpaint_iterations + per - 1) / per;
-    max_launches = std::max(max_launches, K + (K & 1));
     TorchCam cams[SS_MAX_CAMERAS];
     for (int k = 0; k < c.C; ++k) cams[k] = make_torch_cam(c.cams[k]);
     for (int r = 0; r < c.R; ++r, ++p) {
@@ -356,10 +392,10 @@ int build(const ss_frame_desc* d, int F, Image& im, RaggedPlan* pl) {
     m.rgb_out = c.d_rgb_pano;
     merge.add(m, pix_items((long long)c.PH * c.PW));
   }
-  pl->sweep = place(im, sweep, q);
-  plan_filter(im, q, 8, pl->vt, lvp, L, V, wc, ss_cost, dist, am, fill, &pl->cost);
-  plan_filter(im, q, 1, SS_VOLUME_F32, dlp, L, V1, wd, ss_dist, {}, {}, {}, &pl->dist);
-  pl->reproj = place(im, reproj, q);
+  pl->sweep = place(im, sweep);
+  plan_filter(im, 8, pl->vt, lvp, L, V, wc, ss_cost, dist, am, fill, &pl->cost);
+  plan_filter(im, 1, SS_VOLUME_F32, dlp, L, V1, wd, ss_dist, {}, {}, {}, &pl->dist);
+  pl->reproj = place(im, reproj);
   // inpainting: launch j runs passes 16j.. of every map that has them, from
   // the map into the scratch buffer for even j, back for odd j; a map with
   // an odd launch count gets a copy back (steps = 0) in launch K
@@ -385,10 +421,10 @@ int build(const ss_frame_desc* d, int F, Image& im, RaggedPlan* pl) {
         pass.add(e, e.tiles_x * ((c.H + tile - 1) / tile));
       }
     }
-    if (!pass.e.empty()) pl->inpaint.push_back(place(im, pass, q));
+    if (!pass.e.empty()) pl->inpaint.push_back(place(im, pass));
   }
-  pl->cast = place(im, cast, q);
-  pl->merge = place(im, merge, q);
+  pl->cast = place(im, cast);
+  pl->merge = place(im, merge);
   pl->bytes = im.off;
   return SS_OK;
 }
@@ -421,31 +457,33 @@ size_t ragged_table_bytes(const ss_frame_desc* d, int F) {
   if (check_ragged(d, F) != SS_OK) return 0;
   Image im;
   RaggedPlan pl;
-  build(d, F, im, &pl);
-  return pl.bytes;
+  return build(d, F, SS_RAGGED_PERSISTENT, im, &pl) == SS_OK ? pl.bytes : 0;
 }
 
-int ragged_create(const ss_frame_desc* d, int F, void* table, size_t bytes, hipStream_t s, RaggedPlan** out) {
+int ragged_create(const ss_frame_desc* d, int F, int schedule, void* table, size_t bytes, hipStream_t s,
+                  RaggedPlan** out) {
   int rc = check_ragged(d, F);
   if (rc) return rc;
+  if (schedule != SS_RAGGED_PERSISTENT && schedule != SS_RAGGED_DISPATCHED && schedule != SS_RAGGED_AUTO)
+    return report_error(SS_ERR_ARG, "ragged batch: unknown schedule");
   if (!table || (reinterpret_cast<uintptr_t>(table) & 255))
     return report_error(SS_ERR_ARG, "ragged batch: the table must be a 256-byte aligned device buffer");
   RaggedPlan* pl = new (std::nothrow) RaggedPlan();
   if (!pl) return report_error(SS_ERR_ARG, "out of host memory");
   Image im;
   im.base = static_cast<char*>(table);
-  build(d, F, im, pl);
   auto bail = [&](int code) {
     delete pl;
     return code;
   };
+  rc = build(d, F, schedule, im, pl);
+  if (rc) return bail(rc);
   if (bytes < pl->bytes) {
     char m[128];
     snprintf(m, sizeof m, "ragged batch: table of %zu bytes needed, %zu given", pl->bytes, bytes);
     return bail(report_error(SS_ERR_WORKSPACE, m));
   }
-  // the records (and zeroed queue counters and pointer slots), then the
-  // weight tables on top
+  // the records (and zeroed pointer slots), then the weight tables on top
   if (hipMemcpyAsync(table, im.h.data(), pl->bytes, hipMemcpyHostToDevice, s) != hipSuccess)
     return bail(report_error(SS_ERR_HIP, "ragged batch: table upload failed"));
   for (size_t i = 0; i < pl->wt_vi.size(); i += kMaxBatch) {
@@ -464,16 +502,17 @@ int ragged_enqueue(const RaggedPlan* pl, const float* const* match, const float*
                         reinterpret_cast<const void**>(pl->run_img), s));
   RG_HIP(launch_publish(reinterpret_cast<const void* const*>(stitch), pl->n_stitch,
                         reinterpret_cast<const void**>(pl->run_stitch), s));
-  RG_HIP(launch_sweep_ragged(pl->sweep, pl->run_img, pl->vt, s));
+  RG_HIP(launch_sweep_ragged(pl->sweep, pl->run_img, pl->vt, pl->loop_sweep, s));
+  const bool lf = pl->loop_filter, ls = pl->loop_stitch;
   for (const Filter* fl : {&pl->cost, &pl->dist}) {
-    for (const DownLaunch& l : fl->down) RG_HIP(launch_isb_down_ragged(l.sp, l.G, l.vt, l.vmax, s));
+    for (const DownLaunch& l : fl->down) RG_HIP(launch_isb_down_ragged(l.sp, l.G, l.vt, l.vmax, lf, s));
     if (fl->tail.n) RG_HIP(launch_isb_tail_ragged(fl->tail.t, fl->tail.n, fl->tail.G, fl->tail.lds, s));
-    for (const UpLaunch& l : fl->up) RG_HIP(launch_isb_up_ragged(l.sp, l.G, l.vt, l.vmax, l.fuse, s));
+    for (const UpLaunch& l : fl->up) RG_HIP(launch_isb_up_ragged(l.sp, l.G, l.vt, l.vmax, l.fuse, lf, s));
   }
-  RG_HIP(launch_reproject_ragged(pl->reproj, s));
-  for (const auto& sp : pl->inpaint) RG_HIP(launch_inpaint_ragged(sp, s));
-  if (pl->cast.n) RG_HIP(launch_to_u8_ragged(pl->cast, pl->run_stitch, s));
-  RG_HIP(launch_merge_ragged(pl->merge, s));
+  RG_HIP(launch_reproject_ragged(pl->reproj, ls, s));
+  for (const auto& sp : pl->inpaint) RG_HIP(launch_inpaint_ragged(sp, ls, s));
+  if (pl->cast.n) RG_HIP(launch_to_u8_ragged(pl->cast, pl->run_stitch, ls, s));
+  RG_HIP(launch_merge_ragged(pl->merge, ls, s));
   return SS_OK;
 }
 

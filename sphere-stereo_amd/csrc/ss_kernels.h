@@ -160,37 +160,42 @@ struct BlendArgs {
 // ---------------------------------------------------------------------------
 // Ragged persistent launches (ragged.hip): one launch per pass covers the
 // frame-reference pairs of many frames whatever their shapes.  Each launch is
-// sized to what the GPU holds resident; its wavefronts (or workgroups) take
-// work items from a WorkQueue until it is empty.  The records below live in
-// a device table built once per batch; a launch gets a span of them.
+// sized to what the GPU holds resident and its workers (wavefronts, or
+// workgroups for passes with workgroup barriers) loop over the launch's work
+// items.  The records below live in a device table built once per batch; a
+// launch gets a span of them.
 // ---------------------------------------------------------------------------
-// The items of a launch are split into kQueueParts contiguous parts, one per
-// XCD: a worker starts on the part of its XCD (neighbouring items share
-// cache lines of the same images) and then takes from the others.  The
-// counters are zero between launches: the last worker to finish resets them.
-constexpr int kQueueParts = 8;
-struct WorkQueue {
-  unsigned next[kQueueParts];
-  unsigned done;
-  unsigned pad[7];
-};
+// Scheduling: every entry's items are split into kParts contiguous chunks,
+// chunk x = items [x n / kParts, (x + 1) n / kParts) of an entry of n items;
+// the workgroups with blockIdx.x % kParts == x (dispatched round-robin over
+// the 8 XCDs, so all on one XCD) take chunk x of every entry, entry after
+// entry.  All XCDs thus move through the entries together, as in a launch
+// per entry, and the data of the entries in flight is shared in the
+// last-level cache; each XCD gets an eighth of every entry, so the XCDs are
+// balanced without a cost model.  No atomics: measured on MI355X,
+// device-scope atomic adds on shared counters serialise at a few ns each
+// across the chip, which made a dynamic work queue slower than the hardware
+// dispatcher on every pass shorter than the sweep.
+constexpr int kParts = 8;
 
 // Entries e[0..n) of a launch; entry k owns items [first[k], first[k+1]),
-// first[n] = total.
+// first[n] = total.  xfirst[x * (n + 1) + k]: the position of entry k's chunk
+// x in XCD x's sequence (xtotal[x] = xfirst[x * (n + 1) + n]).
 template <typename E>
 struct RaggedSpan {
   const E* e;
   const int* first;
+  const int* xfirst;
   int n, total;
-  WorkQueue* q;
+  int xtotal[kParts];
 };
 
-// Items of the ragged ISB passes: coarse pixels per item for G = 8 (groups of
-// 8 lanes) and G = 1.
-constexpr int kIsbItemPix8 = 64, kIsbItemPix1 = 256;
+// Items of the ragged ISB passes: one wavefront step, 64 / G coarse pixels
+// (G = 8: groups of 8 lanes per pixel; G = 1: a lane per pixel).
+constexpr int kIsbItemPix8 = 8, kIsbItemPix1 = 64;
 
-// One reference of one frame in the ragged sweep: an item is a 64-pixel row
-// segment, swept by one wavefront.
+// One reference of one frame in the ragged sweep: an item is a 64 x 4 pixel
+// tile, swept by one workgroup (a wavefront per row), as in k_sweep.
 struct SweepPair {
   const float4* raylut;
   void* cost;
@@ -339,29 +344,36 @@ int error_stats_max_thresholds();
 hipError_t launch_error_stats(const float* a, const float* b, size_t n, const float* thr, int nthr, int finite_only,
                               double* partial, double* out, hipStream_t s);
 
-// ragged persistent launches (grid: workgroups, sized by the caller with
-// persistent_grid; every workgroup takes items until the queue is empty)
-int persistent_grid(const void* kernel, int block, size_t lds, long long workers_needed, int waves_per_worker);
+// Ragged launches.  loop = true: the persistent form (ragged_grid sizes the
+// grid to the resident workgroups, and every worker loops over its items);
+// loop = false: the dispatched form (a worker per item, at most one each).
+int ragged_grid_of(const void* kernel, int block, size_t lds, const int* xtotal, int workers_per_block, bool loop);
+template <typename E>
+int ragged_grid(const void* kernel, int block, size_t lds, const RaggedSpan<E>& sp, int workers_per_block, bool loop) {
+  return ragged_grid_of(kernel, block, lds, sp.xtotal, workers_per_block, loop);
+}
 hipError_t launch_publish(const void* const* src, int n, const void** dst, hipStream_t s);
-hipError_t launch_sweep_ragged(const RaggedSpan<SweepPair>& sp, const float* const* img, int vt, hipStream_t s);
-// vmax: ragged_vmax of the largest V among the span's entries
+hipError_t launch_sweep_ragged(const RaggedSpan<SweepPair>& sp, const float* const* img, int vt, bool loop,
+                               hipStream_t s);
+// vmax: the V class of the span's entries (ragged_vmax)
 int ragged_vmax(int V);
-hipError_t launch_isb_down_ragged(const RaggedSpan<IsbDownEntry>& sp, int G, int vt, int vmax, hipStream_t s);
-hipError_t launch_isb_up_ragged(const RaggedSpan<IsbUpEntry>& sp, int G, int vt, int vmax, bool fuse_argmin,
+hipError_t launch_isb_down_ragged(const RaggedSpan<IsbDownEntry>& sp, int G, int vt, int vmax, bool loop,
+                                  hipStream_t s);
+hipError_t launch_isb_up_ragged(const RaggedSpan<IsbUpEntry>& sp, int G, int vt, int vmax, bool fuse_argmin, bool loop,
                                 hipStream_t s);
 hipError_t launch_isb_tail_ragged(const TailEntry* t, int n, int G, unsigned lds_bytes, hipStream_t s);
-hipError_t launch_reproject_ragged(const RaggedSpan<ReprojEntry>& sp, hipStream_t s);
-hipError_t launch_inpaint_ragged(const RaggedSpan<InpaintEntry>& sp, hipStream_t s);
-hipError_t launch_to_u8_ragged(const RaggedSpan<CastEntry>& sp, const float* const* src, hipStream_t s);
-hipError_t launch_merge_ragged(const RaggedSpan<MergeArgs>& sp, hipStream_t s);
+hipError_t launch_reproject_ragged(const RaggedSpan<ReprojEntry>& sp, bool loop, hipStream_t s);
+hipError_t launch_inpaint_ragged(const RaggedSpan<InpaintEntry>& sp, bool loop, hipStream_t s);
+hipError_t launch_to_u8_ragged(const RaggedSpan<CastEntry>& sp, const float* const* src, bool loop, hipStream_t s);
+hipError_t launch_merge_ragged(const RaggedSpan<MergeArgs>& sp, bool loop, hipStream_t s);
 // items of one entry of each ragged pass
-inline int sweep_items(int H, int W) { return H * ((W + 63) / 64); }
+inline int sweep_items(int H, int W) { return ((H + 3) / 4) * ((W + 63) / 64); }
 inline int isb_items(int npix, int G) {
   const int per = G == 1 ? kIsbItemPix1 : kIsbItemPix8;
   return (npix + per - 1) / per;
 }
-constexpr int kPixItem = 256;    // reprojection and merge: pixels per item
-constexpr int kCastItem = 1024;  // uint8 cast: values per item
+constexpr int kPixItem = 64;    // reprojection and merge: pixels per item (a lane each)
+constexpr int kCastItem = 256;  // uint8 cast: values per item (4 per lane)
 inline int pix_items(long long n) { return (int)((n + kPixItem - 1) / kPixItem); }
 inline int cast_items(long long n) { return (int)((n + kCastItem - 1) / kCastItem); }
 int inpaint_tile();  // the side of an inpainting item (a tile of the map)
@@ -369,7 +381,8 @@ int inpaint_tile();  // the side of an inpainting item (a tile of the map)
 // ragged.hip: the ragged persistent executor of a batch of frames
 struct RaggedPlan;
 size_t ragged_table_bytes(const ss_frame_desc* d, int F);
-int ragged_create(const ss_frame_desc* d, int F, void* table, size_t bytes, hipStream_t s, RaggedPlan** out);
+int ragged_create(const ss_frame_desc* d, int F, int schedule, void* table, size_t bytes, hipStream_t s,
+                  RaggedPlan** out);
 int ragged_enqueue(const RaggedPlan* pl, const float* const* match, const float* const* stitch, hipStream_t s);
 int ragged_images(const RaggedPlan* pl, int* n_match, int* n_stitch);
 void ragged_free(RaggedPlan* pl);

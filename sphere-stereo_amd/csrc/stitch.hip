@@ -12,7 +12,7 @@
 #include <string.h>
 
 #include "ss_kernels.h"
-#include "ss_queue.h"
+#include "ss_ragged.h"
 
 namespace ss {
 
@@ -73,13 +73,15 @@ __global__ void __launch_bounds__(kBlock) k_reproject(Batch<ReprojectArgs> b, in
   reproject_pixel(b.a[blockIdx.y], H, W, i);
 }
 
-// ragged: a wavefront's item is kPixItem consecutive pixels of one map
+// ragged: a wavefront's item is kPixItem consecutive pixels of one map, a lane each
+template <bool LOOP>
 __global__ void __launch_bounds__(kBlock) k_reproject_ragged(RaggedSpan<ReprojEntry> sp) {
   const int lane = (int)(threadIdx.x & 63);
-  for_each_wave_item(sp, [&](int e, int t) {
+  for_each_wave_item<LOOP>(sp, [&](int e, int t) {
     const ReprojEntry r = load_const(sp.e + e);
     const int n = r.H * r.W;
-    for (int i = t * kPixItem + lane; i < (t + 1) * kPixItem && i < n; i += 64) reproject_pixel(r.a, r.H, r.W, i);
+    const int i = t * kPixItem + lane;
+    if (i < n) reproject_pixel(r.a, r.H, r.W, i);
   });
 }
 
@@ -296,8 +298,9 @@ __global__ void __launch_bounds__(kBlock) k_inpaint_multi(Batch<InpaintArgs> bt,
 
 // ragged: a workgroup's item is one tile of one map, with the map's own
 // step count (0: the tile is copied)
+template <bool LOOP>
 __global__ void __launch_bounds__(kBlock) k_inpaint_ragged(RaggedSpan<InpaintEntry> sp) {
-  for_each_block_item(sp, [&](int e, int t) {
+  for_each_block_item<LOOP>(sp, [&](int e, int t) {
     const InpaintEntry m = load_const(sp.e + e);
     const int by = t / m.tiles_x, bx = t - by * m.tiles_x;
     inpaint_tile(m.a.din, m.a.dout, reinterpret_cast<const uchar2*>(m.a.lut), m.H, m.W, m.steps, m.thr_empty,
@@ -522,19 +525,20 @@ __global__ void __launch_bounds__(kBlock) k_merge(MergeArgs a) {
 }
 
 // ragged: a wavefront's item is kPixItem consecutive panorama pixels of one
-// frame, whose reference count picks the instantiation
+// frame (a lane each), whose reference count picks the instantiation
+template <bool LOOP>
 __global__ void __launch_bounds__(kBlock) k_merge_ragged(RaggedSpan<MergeArgs> sp) {
   const int lane = (int)(threadIdx.x & 63);
-  for_each_wave_item(sp, [&](int e, int t) {
+  for_each_wave_item<LOOP>(sp, [&](int e, int t) {
     const MergeArgs& a = sp.e[e];
-    const int P = a.PH * a.PW;
-    const int end = min((t + 1) * kPixItem, P);
+    const int i = t * kPixItem + lane;
+    if (i >= a.PH * a.PW) return;
     switch (a.R) {
-      case 1: for (int i = t * kPixItem + lane; i < end; i += 64) merge_pixel<1>(a, i); break;
-      case 2: for (int i = t * kPixItem + lane; i < end; i += 64) merge_pixel<2>(a, i); break;
-      case 3: for (int i = t * kPixItem + lane; i < end; i += 64) merge_pixel<3>(a, i); break;
-      case 4: for (int i = t * kPixItem + lane; i < end; i += 64) merge_pixel<4>(a, i); break;
-      default: for (int i = t * kPixItem + lane; i < end; i += 64) merge_pixel<0>(a, i); break;
+      case 1: merge_pixel<1>(a, i); break;
+      case 2: merge_pixel<2>(a, i); break;
+      case 3: merge_pixel<3>(a, i); break;
+      case 4: merge_pixel<4>(a, i); break;
+      default: merge_pixel<0>(a, i); break;
     }
   });
 }
@@ -558,23 +562,23 @@ __global__ void __launch_bounds__(kBlock) k_to_u8(Batch<ToU8Args> b, size_t n) {
 }
 
 // ragged: a wavefront's item is kCastItem consecutive values of one image
-// (4 per lane per step); src is the run's stitch-image table
+// (4 per lane); src is the run's stitch-image table
+template <bool LOOP>
 __global__ void __launch_bounds__(kBlock) k_to_u8_ragged(RaggedSpan<CastEntry> sp, const float* const* src) {
   const int lane = (int)(threadIdx.x & 63);
-  for_each_wave_item(sp, [&](int e, int t) {
+  for_each_wave_item<LOOP>(sp, [&](int e, int t) {
     const CastEntry c = load_const(sp.e + e);
     const float* __restrict__ in = src[c.src];
     uint8_t* __restrict__ out = c.out;
     const bool vec = (reinterpret_cast<uintptr_t>(in) & 15) == 0 && (reinterpret_cast<uintptr_t>(out) & 3) == 0;
-    const long long end = min((long long)(t + 1) * kCastItem, c.n);
-    for (long long i = (long long)t * kCastItem + 4 * lane; i < end; i += 256) {
-      if (vec && i + 4 <= end) {
-        const float4 v = *reinterpret_cast<const float4*>(in + i);
-        *reinterpret_cast<uint32_t*>(out + i) = (uint32_t)(uint8_t)v.x | ((uint32_t)(uint8_t)v.y << 8) |
-                                                ((uint32_t)(uint8_t)v.z << 16) | ((uint32_t)(uint8_t)v.w << 24);
-      } else {
-        for (long long k = i; k < i + 4 && k < end; ++k) out[k] = (uint8_t)in[k];
-      }
+    const long long i = (long long)t * kCastItem + 4 * lane;
+    if (i >= c.n) return;
+    if (vec && i + 4 <= c.n) {
+      const float4 v = *reinterpret_cast<const float4*>(in + i);
+      *reinterpret_cast<uint32_t*>(out + i) = (uint32_t)(uint8_t)v.x | ((uint32_t)(uint8_t)v.y << 8) |
+                                              ((uint32_t)(uint8_t)v.z << 16) | ((uint32_t)(uint8_t)v.w << 24);
+    } else {
+      for (long long k = i; k < i + 4 && k < c.n; ++k) out[k] = (uint8_t)in[k];
     }
   });
 }
@@ -687,36 +691,37 @@ hipError_t launch_publish(const void* const* src, int n, const void** dst, hipSt
   return hipGetLastError();
 }
 
-hipError_t launch_reproject_ragged(const RaggedSpan<ReprojEntry>& sp, hipStream_t s) {
+hipError_t launch_reproject_ragged(const RaggedSpan<ReprojEntry>& sp, bool loop, hipStream_t s) {
   if (sp.n < 1 || sp.total < 1) return hipErrorInvalidValue;
-  const int grid = persistent_grid(reinterpret_cast<const void*>(&k_reproject_ragged), kBlock, 0, sp.total,
-                                   kBlock / 64);
-  hipLaunchKernelGGL(k_reproject_ragged, dim3(grid), dim3(kBlock), 0, s, sp);
+  const auto k = loop ? &k_reproject_ragged<true> : &k_reproject_ragged<false>;
+  const int grid = ragged_grid(reinterpret_cast<const void*>(k), kBlock, 0, sp, kBlock / 64, loop);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, sp);
   return hipGetLastError();
 }
 
-hipError_t launch_inpaint_ragged(const RaggedSpan<InpaintEntry>& sp, hipStream_t s) {
+hipError_t launch_inpaint_ragged(const RaggedSpan<InpaintEntry>& sp, bool loop, hipStream_t s) {
   if (sp.n < 1 || sp.total < 1) return hipErrorInvalidValue;
-  const int grid = persistent_grid(reinterpret_cast<const void*>(&k_inpaint_ragged), kBlock, 0, sp.total, 1);
-  hipLaunchKernelGGL(k_inpaint_ragged, dim3(grid), dim3(kBlock), 0, s, sp);
+  const auto k = loop ? &k_inpaint_ragged<true> : &k_inpaint_ragged<false>;
+  const int grid = ragged_grid(reinterpret_cast<const void*>(k), kBlock, 0, sp, 1, loop);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, sp);
   return hipGetLastError();
 }
 
 int inpaint_tile() { return kTile; }
 
-hipError_t launch_to_u8_ragged(const RaggedSpan<CastEntry>& sp, const float* const* src, hipStream_t s) {
+hipError_t launch_to_u8_ragged(const RaggedSpan<CastEntry>& sp, const float* const* src, bool loop, hipStream_t s) {
   if (sp.n < 1 || sp.total < 1) return hipErrorInvalidValue;
-  const int grid = persistent_grid(reinterpret_cast<const void*>(&k_to_u8_ragged), kBlock, 0, sp.total,
-                                   kBlock / 64);
-  hipLaunchKernelGGL(k_to_u8_ragged, dim3(grid), dim3(kBlock), 0, s, sp, src);
+  const auto k = loop ? &k_to_u8_ragged<true> : &k_to_u8_ragged<false>;
+  const int grid = ragged_grid(reinterpret_cast<const void*>(k), kBlock, 0, sp, kBlock / 64, loop);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, sp, src);
   return hipGetLastError();
 }
 
-hipError_t launch_merge_ragged(const RaggedSpan<MergeArgs>& sp, hipStream_t s) {
+hipError_t launch_merge_ragged(const RaggedSpan<MergeArgs>& sp, bool loop, hipStream_t s) {
   if (sp.n < 1 || sp.total < 1) return hipErrorInvalidValue;
-  const int grid = persistent_grid(reinterpret_cast<const void*>(&k_merge_ragged), kBlock, 0, sp.total,
-                                   kBlock / 64);
-  hipLaunchKernelGGL(k_merge_ragged, dim3(grid), dim3(kBlock), 0, s, sp);
+  const auto k = loop ? &k_merge_ragged<true> : &k_merge_ragged<false>;
+  const int grid = ragged_grid(reinterpret_cast<const void*>(k), kBlock, 0, sp, kBlock / 64, loop);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, sp);
   return hipGetLastError();
 }
 

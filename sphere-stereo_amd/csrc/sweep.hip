@@ -9,7 +9,7 @@
 // every camera and keeps the selected one), which leaves the result
 // unchanged.
 #include "ss_kernels.h"
-#include "ss_queue.h"
+#include "ss_ragged.h"
 
 namespace ss {
 
@@ -327,20 +327,23 @@ struct RowGeom {
   ConstFloatPtr cands;
 };
 
-// The ragged form: items are the row segments of every pair of the span
-// (pair-major, then row, then segment); img is the run's image table.
-template <typename T>
+// The ragged form: items are the 64 x 4 tiles of every pair of the span
+// (pair-major, then tile row, then tile), a wavefront per tile row as in
+// k_sweep; img is the run's image table.
+template <typename T, bool LOOP>
 __global__ void __launch_bounds__(kBlock) k_sweep_ragged(RaggedSpan<SweepPair> sp, const float* const* img) {
   constexpr int kGrp = 32;
   __shared__ float stage[kTileH][kTileW * (kGrp + 1)];
   const int lane = (int)(threadIdx.x & (kTileW - 1)), wv = (int)(threadIdx.x / kTileW);
-  for_each_wave_item(sp, [&](int e, int t) {
+  for_each_block_item<LOOP>(sp, [&](int e, int t) {
     const SweepPair* pp = sp.e + e;
     const ConstPairPtr pc = (ConstPairPtr)pp;
     const RowGeom g{pc->H,     pc->W,    pc->N,     pc->NP,    pc->fastproj, pc->Wf,    pc->Hf,
                     pc->invW,  pc->invH, pc->inv2W, pc->inv2H, pc->halfW,    pc->halfH, pc->cands};
     const int tiles_x = pc->tiles_x;
-    const int y = t / tiles_x, tx = t - y * tiles_x;
+    const int ty = t / tiles_x, tx = t - ty * tiles_x;
+    const int y = ty * kTileH + wv;
+    if (y >= g.H) return;  // this wavefront's row of the tile is past the image
     const float* const* im = img + pc->img0;
     const float* refimg = im[pc->refcam];
     const float4* raylut = pc->raylut;
@@ -401,16 +404,18 @@ void sweep_variant_info(int v, int* chunk, int* group) {
   *group = kInfo[v][1];
 }
 
-template <typename T>
+template <typename T, bool LOOP>
 hipError_t sweep_ragged_t(const RaggedSpan<SweepPair>& sp, const float* const* img, hipStream_t s) {
-  const int grid = persistent_grid(reinterpret_cast<const void*>(&k_sweep_ragged<T>), kBlock, 0, sp.total,
-                                   kBlock / 64);
-  hipLaunchKernelGGL(k_sweep_ragged<T>, dim3(grid), dim3(kBlock), 0, s, sp, img);
+  const auto k = &k_sweep_ragged<T, LOOP>;
+  const int grid = ragged_grid(reinterpret_cast<const void*>(k), kBlock, 0, sp, 1, LOOP);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, sp, img);
   return hipGetLastError();
 }
 
-hipError_t launch_sweep_ragged(const RaggedSpan<SweepPair>& sp, const float* const* img, int vt, hipStream_t s) {
-#define SS_CALL(T) sweep_ragged_t<T>(sp, img, s)
+hipError_t launch_sweep_ragged(const RaggedSpan<SweepPair>& sp, const float* const* img, int vt, bool loop,
+                               hipStream_t s) {
+  if (sp.n < 1 || sp.total < 1) return hipErrorInvalidValue;
+#define SS_CALL(T) (loop ? sweep_ragged_t<T, true>(sp, img, s) : sweep_ragged_t<T, false>(sp, img, s))
   SS_DISPATCH_T(vt, SS_CALL)
 #undef SS_CALL
 }

@@ -13,6 +13,7 @@ SS_MAX_CAMERAS = 8
 SS_MAX_CANDIDATES = 128
 SS_MAX_REFERENCES = 8
 SS_MAX_BATCH_FRAMES = 64
+SS_RAGGED_PERSISTENT, SS_RAGGED_DISPATCHED, SS_RAGGED_AUTO = 0, 1, 2
 SS_INPUT_U8, SS_INPUT_U16, SS_INPUT_F32 = 0, 1, 2
 
 # ss_volume_type: element type of the stored cost volume
@@ -87,7 +88,7 @@ SIGNATURES = {
     "ss_frame_batch_create": (_i, [C.POINTER(ss_frame_desc), _i, C.POINTER(C.c_void_p)]),
     "ss_frame_batch_run": (_i, [_p, _p, _p, _i, _p]),
     "ss_frame_ragged_table_bytes": (_z, [C.POINTER(ss_frame_desc), _i]),
-    "ss_frame_ragged_create": (_i, [C.POINTER(ss_frame_desc), _i, _p, _z, _p, C.POINTER(C.c_void_p)]),
+    "ss_frame_ragged_create": (_i, [C.POINTER(ss_frame_desc), _i, _i, _p, _z, _p, C.POINTER(C.c_void_p)]),
     "ss_frame_destroy": (_i, [_p]),
     "ss_soft_argmin": (_i, [_p, _i, _i, _i, _fl, _fl, _fl, _p, _p]),
     "ss_soft_argmin_backward": (_i, [_p, _p, _i, _i, _i, _fl, _fl, _fl, _p, _p]),

@@ -263,13 +263,16 @@ class FrameBatch:
     images differ, and so may their configurations (matching resolution,
     candidate count, references, sigmas, panorama, inpainting settings).
 
-    ``persistent=True`` (the ragged persistent form): one native executor
-    runs every pass once for all the frames, whatever their configurations,
-    in launches sized to what the GPU holds resident whose wavefronts take
-    work items (row segments, pixel blocks, tiles) from a work queue.  The
-    estimators must share the volume type and have at least 2 candidates and
-    an 8 x 8 matching resolution.  The launch records live in a device table
-    the batch allocates once.
+    ``persistent=True`` (the ragged form): one native executor runs every
+    pass once for all the frames, whatever their configurations (the filter
+    levels once per class of candidate counts present).  ``schedule``:
+    "persistent" launches are sized to what the GPU holds resident and their
+    workers loop over work items (pixel tiles and blocks); "dispatched"
+    launches have a worker per item;
+    "auto" picks per pass the form measured faster.  The estimators must
+    share the volume type and have at least 2 candidates and an 8 x 8
+    matching resolution.  The launch records live in a device table the batch
+    allocates once.
 
     ``persistent=False``: the batch is split into groups of one
     configuration, and each group is one native batch in which every pass
@@ -283,7 +286,7 @@ class FrameBatch:
     ``estimate_RGBD_panorama``.
     """
 
-    def __init__(self, estimators, persistent=False):
+    def __init__(self, estimators, persistent=False, schedule="auto"):
         self.estimators = list(estimators)
         F = len(self.estimators)
         if not 1 <= F <= _lib.SS_MAX_BATCH_FRAMES:
@@ -292,6 +295,8 @@ class FrameBatch:
         if len(devs) != 1:
             raise ValueError("the estimators of a batch must share one device")
         self.persistent = bool(persistent)
+        self.schedule = {"persistent": _lib.SS_RAGGED_PERSISTENT, "dispatched": _lib.SS_RAGGED_DISPATCHED,
+                         "auto": _lib.SS_RAGGED_AUTO}[schedule]
         self._table = None
         self.groups = []
         self._frames = []
@@ -324,7 +329,7 @@ class FrameBatch:
                              "and have >= 2 candidates and an 8 x 8 or larger matching resolution")
         self._table = torch.empty(int(nbytes), dtype=torch.uint8, device=self.estimators[0].device)
         handle = _lib.C.c_void_p()
-        _lib.check(_lib.lib.ss_frame_ragged_create(descs, F, self._table.data_ptr(), int(nbytes),
+        _lib.check(_lib.lib.ss_frame_ragged_create(descs, F, self.schedule, self._table.data_ptr(), int(nbytes),
                                                    self.estimators[0]._stream(), _lib.C.byref(handle)))
         return handle
 

@@ -433,15 +433,17 @@ def test_frame_batch_ragged_matches_single_frames(gpu):
             e.fishey_stitcher.distance_panorama.fill_(-1.0)
 
 
-@pytest.mark.parametrize("vt", ["float32", "float16", "bfloat16"])
-def test_frame_batch_persistent_matches_single_frames(gpu, vt):
+@pytest.mark.parametrize("vt,schedule", [("float32", "persistent"), ("float16", "persistent"), ("bfloat16", "persistent"),
+                                         ("float32", "dispatched"), ("float16", "dispatched"), ("float16", "auto"),
+                                         ("bfloat16", "auto")])
+def test_frame_batch_persistent_matches_single_frames(gpu, vt, schedule):
     """The ragged persistent form: frames of different matching resolutions
     (pyramids of different depths, with and without a coarse tail),
     candidate counts (V = 2, 3, 4, 6, 8), reference and camera sets, sigmas,
     inpainting pass counts (an odd and an even number of 16-pass launches,
     and none), stitch sizes and panoramas in one executor, one launch per
     pass; every frame bit-exact with its own single-frame run, eager and over
-    graph replays."""
+    graph replays, in each scheduling form."""
     from sphere_stereo import _lib
 
     cases = [((80, 72), 32, (0, 2), 12.5, 32, (96, 96), (128, 64)),
@@ -470,7 +472,7 @@ def test_frame_batch_persistent_matches_single_frames(gpu, vt):
     for e in ests:
         e.fishey_stitcher.RGB_panorama.zero_()
         e.fishey_stitcher.distance_panorama.zero_()
-    batch = FrameBatch(ests, persistent=True)
+    batch = FrameBatch(ests, persistent=True, schedule=schedule)
     assert batch.groups == [list(range(len(cases)))]
     for mode in ("eager", "eager", "graph", "graph", "graph"):
         batch.use_graph = mode == "graph"
