@@ -180,10 +180,11 @@ def batching_ragged(args, device):
     """The ragged batching form: 16 rigs, each a seeded scene with its own
     baseline drawn from U(0.15, 0.40) m, matching resolution from {512, 768,
     1024}^2, candidate count from {16, 32, 48, 64} and 2 to 4 reference
-    cameras (1216^2 stitch input, 1024x512 panorama), fp16 volume, one
-    FrameBatch (grouped by configuration, each group one launch per pass and
-    chunk), timed against the same 16 frames run one at a time in the plain
-    fp32 form."""
+    cameras (1216^2 stitch input, 1024x512 panorama), fp16 volume: one
+    FrameBatch grouped by configuration (each group one launch per pass and
+    chunk), and the single-executor form (one launch per pass for all rigs)
+    in its persistent and dispatched schedules, timed against the same 16
+    frames run one at a time in the plain fp32 form."""
     import random
 
     import torch
@@ -247,23 +248,26 @@ def batching_ragged(args, device):
         print("[bench] ragged batch (%d rigs, %d configurations), fp16 %s: %.1f panoramas/s (plain fp32 one at a "
               "time: %.1f)" % (len(sets), len(batch.groups), mode, v, v_plain), file=sys.stderr, flush=True)
     del batch
-    # the ragged persistent form: one executor for all 16 rigs, one launch per pass
-    batch = FrameBatch(ests, persistent=True)
-    for mode in ("eager", "graph"):
-        batch.use_graph = mode == "graph"
-        for _ in range(args.warmup):
-            batch.estimate_RGBD_panoramas(frames)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            batch.estimate_RGBD_panoramas(frames)
-        torch.cuda.synchronize()
-        v = reps * len(frames) / (time.perf_counter() - t0)
-        out["float16_persistent_" + mode] = {"value": round(v, 3), "unit": "panoramas/s",
-                                             "vs_plain_fp32": round(v / v_plain, 4)}
-        print("[bench] ragged batch (%d rigs), fp16 persistent %s: %.1f panoramas/s" % (len(sets), mode, v),
-              file=sys.stderr, flush=True)
-    del batch, ests
+    # the single-executor ragged form: one launch per pass (per V class for the
+    # filter levels) for all 16 rigs, in its two schedules
+    for schedule in ("persistent", "dispatched"):
+        batch = FrameBatch(ests, persistent=True, schedule=schedule)
+        for mode in ("eager", "graph"):
+            batch.use_graph = mode == "graph"
+            for _ in range(args.warmup):
+                batch.estimate_RGBD_panoramas(frames)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                batch.estimate_RGBD_panoramas(frames)
+            torch.cuda.synchronize()
+            v = reps * len(frames) / (time.perf_counter() - t0)
+            out["float16_ragged_%s_%s" % (schedule, mode)] = {"value": round(v, 3), "unit": "panoramas/s",
+                                                               "vs_plain_fp32": round(v / v_plain, 4)}
+            print("[bench] ragged batch (%d rigs), fp16, one executor, %s schedule, %s: %.1f panoramas/s"
+                  % (len(sets), schedule, mode, v), file=sys.stderr, flush=True)
+        del batch
+    del ests
     torch.cuda.empty_cache()
     return out
 
