@@ -5,7 +5,8 @@
 // one launch for the whole batch:
 //
 //   publish (per-run image pointers into the table)
-//   sweep + guides          one launch, items = 64-pixel row segments
+//   sweep + guides          one launch per sweep variant the rigs are tuned
+//                           to (ss_sweep_autotune), items = 64 x 4 tiles
 //   cost filter             one launch per pyramid level (+ one for the
 //                           single-workgroup coarse tails), the level-0 up
 //                           pass fused with the argmin
@@ -170,7 +171,11 @@ struct RaggedPlan {
   std::vector<int> img0, stitch0;  // per frame: first slot in the run tables
   const float** run_img = nullptr;
   const float** run_stitch = nullptr;
-  RaggedSpan<SweepPair> sweep{};
+  struct SweepLaunch {
+    RaggedSpan<SweepPair> sp;
+    int variant;
+  };
+  std::vector<SweepLaunch> sweep;  // one launch per sweep variant the pairs are tuned to
   Filter cost, dist;
   RaggedSpan<ReprojEntry> reproj{};
   std::vector<RaggedSpan<InpaintEntry>> inpaint;
@@ -300,7 +305,7 @@ int build(const ss_frame_desc* d, int F, int schedule, Image& im, RaggedPlan* pl
   std::vector<float*> wc(P), wd(P), dist(P), fill(P);
   std::vector<double> ss_cost(P), ss_dist(P);
   std::vector<ArgminParams> am(P);
-  Pass<SweepPair> sweep;
+  Pass<SweepPair> sweep[4];  // by the configuration's tuned sweep variant (ss_sweep_get_variant)
   Pass<ReprojEntry> reproj;
   Pass<CastEntry> cast;
   Pass<MergeArgs> merge;
@@ -354,7 +359,8 @@ int build(const ss_frame_desc* d, int F, int schedule, Image& im, RaggedPlan* pl
       }
       memcpy(sp.cands, c.cands, sizeof(float) * c.N);
       sp.fastproj = sweep_fast_bounds(cams, c.C, c.rts[r], c.C, c.cands, c.N);
-      sweep.add(sp, sweep_items(c.H, c.W));
+      const int var = ss_sweep_get_variant(c.H, c.W, c.N, c.volume_type);
+      sweep[var >= 0 && var < 4 ? var : 0].add(sp, sweep_items(c.H, c.W));
 
       ReprojEntry re{};
       re.a.din = c.d_dist[r];
@@ -392,7 +398,8 @@ int build(const ss_frame_desc* d, int F, int schedule, Image& im, RaggedPlan* pl
     m.rgb_out = c.d_rgb_pano;
     merge.add(m, pix_items((long long)c.PH * c.PW));
   }
-  pl->sweep = place(im, sweep);
+  for (int v = 0; v < 4; ++v)
+    if (!sweep[v].e.empty()) pl->sweep.push_back(RaggedPlan::SweepLaunch{place(im, sweep[v]), v});
   plan_filter(im, 8, pl->vt, lvp, L, V, wc, ss_cost, dist, am, fill, &pl->cost);
   plan_filter(im, 1, SS_VOLUME_F32, dlp, L, V1, wd, ss_dist, {}, {}, {}, &pl->dist);
   pl->reproj = place(im, reproj);
@@ -502,7 +509,7 @@ int ragged_enqueue(const RaggedPlan* pl, const float* const* match, const float*
                         reinterpret_cast<const void**>(pl->run_img), s));
   RG_HIP(launch_publish(reinterpret_cast<const void* const*>(stitch), pl->n_stitch,
                         reinterpret_cast<const void**>(pl->run_stitch), s));
-  RG_HIP(launch_sweep_ragged(pl->sweep, pl->run_img, pl->vt, pl->loop_sweep, s));
+  for (const auto& l : pl->sweep) RG_HIP(launch_sweep_ragged(l.sp, pl->run_img, pl->vt, l.variant, pl->loop_sweep, s));
   const bool lf = pl->loop_filter, ls = pl->loop_stitch;
   for (const Filter* fl : {&pl->cost, &pl->dist}) {
     for (const DownLaunch& l : fl->down) RG_HIP(launch_isb_down_ragged(l.sp, l.G, l.vt, l.vmax, lf, s));

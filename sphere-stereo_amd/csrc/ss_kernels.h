@@ -353,8 +353,9 @@ int ragged_grid(const void* kernel, int block, size_t lds, const RaggedSpan<E>& 
   return ragged_grid_of(kernel, block, lds, sp.xtotal, workers_per_block, loop);
 }
 hipError_t launch_publish(const void* const* src, int n, const void** dst, hipStream_t s);
-hipError_t launch_sweep_ragged(const RaggedSpan<SweepPair>& sp, const float* const* img, int vt, bool loop,
-                               hipStream_t s);
+// variant: one of the sweep's scheduling variants (sweep_variant_info)
+hipError_t launch_sweep_ragged(const RaggedSpan<SweepPair>& sp, const float* const* img, int vt, int variant,
+                               bool loop, hipStream_t s);
 // vmax: the V class of the span's entries (ragged_vmax)
 int ragged_vmax(int V);
 hipError_t launch_isb_down_ragged(const RaggedSpan<IsbDownEntry>& sp, int G, int vt, int vmax, bool loop,

@@ -330,10 +330,9 @@ struct RowGeom {
 // The ragged form: items are the 64 x 4 tiles of every pair of the span
 // (pair-major, then tile row, then tile), a wavefront per tile row as in
 // k_sweep; img is the run's image table.
-template <typename T, bool LOOP>
+template <typename T, bool LOOP, int CH, int GRP>
 __global__ void __launch_bounds__(kBlock) k_sweep_ragged(RaggedSpan<SweepPair> sp, const float* const* img) {
-  constexpr int kGrp = 32;
-  __shared__ float stage[kTileH][kTileW * (kGrp + 1)];
+  __shared__ float stage[kTileH][kTileW * (GRP + 1)];
   const int lane = (int)(threadIdx.x & (kTileW - 1)), wv = (int)(threadIdx.x / kTileW);
   for_each_block_item<LOOP>(sp, [&](int e, int t) {
     const SweepPair* pp = sp.e + e;
@@ -351,10 +350,10 @@ __global__ void __launch_bounds__(kBlock) k_sweep_ragged(RaggedSpan<SweepPair> s
     uint32_t* guide = pc->guide;
     auto img_of = [&](int k) { return im[k]; };
     if (pc->pow2)
-      sweep_row<T, true, 4, kGrp>(g, raylut, refimg, cost, guide, pp->cam, pp->rt, img_of, y, tx * kTileW, lane,
+      sweep_row<T, true, CH, GRP>(g, raylut, refimg, cost, guide, pp->cam, pp->rt, img_of, y, tx * kTileW, lane,
                                   stage[wv]);
     else
-      sweep_row<T, false, 4, kGrp>(g, raylut, refimg, cost, guide, pp->cam, pp->rt, img_of, y, tx * kTileW, lane,
+      sweep_row<T, false, CH, GRP>(g, raylut, refimg, cost, guide, pp->cam, pp->rt, img_of, y, tx * kTileW, lane,
                                    stage[wv]);
   });
 }
@@ -404,18 +403,29 @@ void sweep_variant_info(int v, int* chunk, int* group) {
   *group = kInfo[v][1];
 }
 
-template <typename T, bool LOOP>
-hipError_t sweep_ragged_t(const RaggedSpan<SweepPair>& sp, const float* const* img, hipStream_t s) {
-  const auto k = &k_sweep_ragged<T, LOOP>;
+template <typename T, bool LOOP, int CH, int GRP>
+hipError_t sweep_ragged_v(const RaggedSpan<SweepPair>& sp, const float* const* img, hipStream_t s) {
+  const auto k = &k_sweep_ragged<T, LOOP, CH, GRP>;
   const int grid = ragged_grid(reinterpret_cast<const void*>(k), kBlock, 0, sp, 1, LOOP);
   hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, sp, img);
   return hipGetLastError();
 }
 
-hipError_t launch_sweep_ragged(const RaggedSpan<SweepPair>& sp, const float* const* img, int vt, bool loop,
-                               hipStream_t s) {
+// the scheduling variants of k_sweep (launch_sweep_t)
+template <typename T, bool LOOP>
+hipError_t sweep_ragged_t(const RaggedSpan<SweepPair>& sp, const float* const* img, int variant, hipStream_t s) {
+  switch (variant) {
+    case 1: return sweep_ragged_v<T, LOOP, 2, 32>(sp, img, s);
+    case 2: return sweep_ragged_v<T, LOOP, 4, 16>(sp, img, s);
+    case 3: return sweep_ragged_v<T, LOOP, 2, 16>(sp, img, s);
+    default: return sweep_ragged_v<T, LOOP, 4, 32>(sp, img, s);
+  }
+}
+
+hipError_t launch_sweep_ragged(const RaggedSpan<SweepPair>& sp, const float* const* img, int vt, int variant,
+                               bool loop, hipStream_t s) {
   if (sp.n < 1 || sp.total < 1) return hipErrorInvalidValue;
-#define SS_CALL(T) (loop ? sweep_ragged_t<T, true>(sp, img, s) : sweep_ragged_t<T, false>(sp, img, s))
+#define SS_CALL(T) (loop ? sweep_ragged_t<T, true>(sp, img, variant, s) : sweep_ragged_t<T, false>(sp, img, variant, s))
   SS_DISPATCH_T(vt, SS_CALL)
 #undef SS_CALL
 }

@@ -337,11 +337,20 @@ class FrameBatch:
         """The sweep autotuner (RGBD_Estimator.autotune) for each configuration
         group of the batch, on its first frame; then drops captured graphs so
         that the next graph replay records the chosen variants.  Returns one
-        result per group.  (The persistent form's sweep has one scheduling;
-        it is not tuned.)"""
-        if self.persistent:
-            return []
+        result per group.  In the single-executor form (``persistent=True``)
+        each configuration is tuned on its first frame and the executor is
+        rebuilt: its sweep runs one launch per variant the configurations
+        picked."""
         frames = list(frames)
+        if self.persistent:
+            out, seen = [], set()
+            for i, e in enumerate(self.estimators):
+                if e._config_key() not in seen:
+                    seen.add(e._config_key())
+                    out.append(e.autotune(frames[i][0], repeats))
+            _lib.lib.ss_frame_destroy(self._frames[0])
+            self._frames[0] = self._create_ragged()
+            return out
         out = []
         for idx in self.groups:
             out.append(self.estimators[idx[0]].autotune(frames[idx[0]][0], repeats))
