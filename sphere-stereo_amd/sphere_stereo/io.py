@@ -1,0 +1,91 @@
+"""Output files of the frame loop: save_rgbd_panorama (utils.py:253-260) writes
+a panorama's colour as an 8-bit PNG and its inverse distance as an OpenEXR
+image, as the reference does through cv2.imwrite.  cv2 is not a dependency
+here: the two encoders below write the formats directly (PNG: RFC 2083
+chunks, zlib-compressed scanlines with filter type 0; OpenEXR 2: one part,
+scanline blocks of one line, no compression, 32-bit float channels).  Host
+code: the files are written from host copies of the outputs.
+"""
+import os
+import struct
+import zlib
+
+import numpy as np
+
+
+def _host(a):
+    if hasattr(a, "detach"):
+        a = a.detach().cpu().numpy()
+    return np.ascontiguousarray(a)
+
+
+def _png_chunk(kind, data):
+    return struct.pack(">I", len(data)) + kind + data + struct.pack(">I", zlib.crc32(kind + data) & 0xFFFFFFFF)
+
+
+def encode_png(img, bgr=True):
+    """PNG bytes of an H x W x 3 (or H x W) uint8 image.  With bgr, the
+    channels are in cv2's BGR order, which cv2.imwrite stores as RGB."""
+    a = _host(img)
+    if a.dtype != np.uint8:
+        raise TypeError("PNG encoder expects uint8, got %s" % a.dtype)
+    if a.ndim == 2:
+        color, rows = 0, a
+    elif a.ndim == 3 and a.shape[2] == 3:
+        color, rows = 2, (a[:, :, ::-1] if bgr else a)
+    else:
+        raise ValueError("PNG encoder expects H x W or H x W x 3, got %s" % (a.shape,))
+    h, w = a.shape[:2]
+    raw = np.concatenate([np.zeros((h, 1), np.uint8), np.ascontiguousarray(rows).reshape(h, -1)], axis=1)
+    ihdr = struct.pack(">IIBBBBB", w, h, 8, color, 0, 0, 0)
+    return (b"\x89PNG\r\n\x1a\n" + _png_chunk(b"IHDR", ihdr) + _png_chunk(b"IDAT", zlib.compress(raw.tobytes(), 6)) +
+            _png_chunk(b"IEND", b""))
+
+
+def _exr_attr(name, kind, value):
+    return name.encode() + b"\0" + kind.encode() + b"\0" + struct.pack("<i", len(value)) + value
+
+
+def encode_exr(img):
+    """OpenEXR bytes of an H x W (channel "Y", as cv2 names a single channel)
+    or H x W x 3 (channels B, G, R) float32 image."""
+    a = _host(img).astype(np.float32, copy=False)
+    if a.ndim == 2:
+        chans = {"Y": a}
+    elif a.ndim == 3 and a.shape[2] == 3:
+        chans = {"B": a[:, :, 0], "G": a[:, :, 1], "R": a[:, :, 2]}
+    else:
+        raise ValueError("EXR encoder expects H x W or H x W x 3, got %s" % (a.shape,))
+    h, w = a.shape[:2]
+    names = sorted(chans)
+    chlist = b"".join(n.encode() + b"\0" + struct.pack("<iB3xii", 2, 0, 1, 1) for n in names) + b"\0"
+    box = struct.pack("<iiii", 0, 0, w - 1, h - 1)
+    header = (_exr_attr("channels", "chlist", chlist) + _exr_attr("compression", "compression", b"\0") +
+              _exr_attr("dataWindow", "box2i", box) + _exr_attr("displayWindow", "box2i", box) +
+              _exr_attr("lineOrder", "lineOrder", b"\0") + _exr_attr("pixelAspectRatio", "float", struct.pack("<f", 1)) +
+              _exr_attr("screenWindowCenter", "v2f", struct.pack("<ff", 0, 0)) +
+              _exr_attr("screenWindowWidth", "float", struct.pack("<f", 1)) + b"\0")
+    head = struct.pack("<ii", 20000630, 2) + header
+    line = 4 * w * len(names)
+    first = len(head) + 8 * h  # after the offset table
+    offsets = struct.pack("<%dQ" % h, *[first + y * (8 + line) for y in range(h)])
+    planes = np.stack([np.ascontiguousarray(chans[n]) for n in names], axis=1).astype("<f4")  # H x C x W
+    blocks = b"".join(struct.pack("<ii", y, line) + planes[y].tobytes() for y in range(h))
+    return head + offsets + blocks
+
+
+def save_rgbd_panorama(rgbd_panoramas, filename, dataset_path):
+    """utils.py:253-260: output/rgb_<name>.png and output/inv_distance_<name>.exr
+    under dataset_path for rgbd_panoramas[filename] = {"rgb": BGR uint8,
+    "inv_distance": float32}; a missing entry is skipped, as there.  The
+    output directory must exist (main.py:128 creates it before the loop)."""
+    try:
+        pano = rgbd_panoramas[filename]
+    except KeyError:
+        return
+    name = os.path.splitext(filename)[0]
+    out = os.path.join(dataset_path, "output")
+    with open(os.path.join(out, "rgb_" + name + ".png"), "wb") as f:
+        f.write(encode_png(pano["rgb"]))
+    with open(os.path.join(out, "inv_distance_" + name + ".exr"), "wb") as f:
+        f.write(encode_exr(pano["inv_distance"]))

@@ -196,3 +196,7 @@ def prepare_inputs(images, matching_resolution, rgb_to_stitch_resolution, refere
     to_match = [resize_linear(img, mc, mr, stream, o) for img, o in zip(images, om)]
     to_stitch = [resize_linear(images[r], sc, sr, stream, o) for r, o in zip(references_indices, os_)]
     return to_match, to_stitch
+
+
+# utils.py:253-260 of the reference lives in io.py (host file writers)
+from .io import save_rgbd_panorama  # noqa: E402,F401

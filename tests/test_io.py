@@ -1,0 +1,98 @@
+"""The output writers (utils.py:253-260): PNG checked by an independent
+decoder (PIL), OpenEXR by a reader written from the OpenEXR 2 file layout
+(no EXR library is in the image: the EXR encoder's parity with cv2.imwrite is
+unpinned beyond that layout)."""
+import io
+import os
+import struct
+
+import numpy as np
+import pytest
+from PIL import Image
+
+from sphere_stereo.io import encode_exr, encode_png, save_rgbd_panorama
+
+
+def test_png_roundtrip_through_pil():
+    rng = np.random.RandomState(0)
+    bgr = rng.randint(0, 256, (37, 53, 3)).astype(np.uint8)
+    got = np.array(Image.open(io.BytesIO(encode_png(bgr))))
+    assert got.shape == (37, 53, 3)
+    np.testing.assert_array_equal(got, bgr[:, :, ::-1])  # cv2 stores BGR arrays as RGB
+    gray = rng.randint(0, 256, (5, 7)).astype(np.uint8)
+    np.testing.assert_array_equal(np.array(Image.open(io.BytesIO(encode_png(gray)))), gray)
+    with pytest.raises(TypeError):
+        encode_png(bgr.astype(np.float32))
+
+
+def read_exr(data):
+    """A reader of single-part scanline OpenEXR files without compression,
+    from the file layout: magic 20000630, version 2, attributes (name, type,
+    size, value) up to an empty name, a table of one uint64 offset per
+    block, blocks of (int32 y, int32 size, channels in header order)."""
+    magic, version = struct.unpack_from("<ii", data, 0)
+    assert magic == 20000630 and version == 2
+    pos, attrs = 8, {}
+    while data[pos] != 0:
+        end = data.index(b"\0", pos)
+        name = data[pos:end].decode()
+        tend = data.index(b"\0", end + 1)
+        kind = data[end + 1:tend].decode()
+        size = struct.unpack_from("<i", data, tend + 1)[0]
+        attrs[name] = (kind, data[tend + 5:tend + 5 + size])
+        pos = tend + 5 + size
+    pos += 1
+    assert attrs["compression"] == ("compression", b"\0")
+    x0, y0, x1, y1 = struct.unpack("<iiii", attrs["dataWindow"][1])
+    w, h = x1 - x0 + 1, y1 - y0 + 1
+    chans, c = [], attrs["channels"][1]
+    while c[0] != 0:
+        end = c.index(b"\0")
+        ptype, _, xs, ys = struct.unpack_from("<iB3xii", c, end + 1)
+        assert ptype == 2 and xs == ys == 1  # FLOAT, full resolution
+        chans.append(c[:end].decode())
+        c = c[end + 17:]
+    offsets = struct.unpack_from("<%dQ" % h, data, pos)
+    out = {n: np.empty((h, w), np.float32) for n in chans}
+    for off in offsets:
+        y, size = struct.unpack_from("<ii", data, off)
+        assert size == 4 * w * len(chans)
+        vals = np.frombuffer(data, "<f4", w * len(chans), off + 8).reshape(len(chans), w)
+        for i, n in enumerate(chans):
+            out[n][y - y0] = vals[i]
+    return out
+
+
+def test_exr_layout_and_roundtrip():
+    rng = np.random.RandomState(1)
+    inv = (rng.rand(19, 31).astype(np.float32) * 2).astype(np.float32)
+    inv[3, 4] = np.inf
+    inv[5, 6] = np.nan
+    inv[0, 0] = -0.0
+    data = encode_exr(inv)
+    # the fixed start of the file: magic, version, then the channel list of "Y"
+    assert data[:8] == bytes([0x76, 0x2F, 0x31, 0x01, 2, 0, 0, 0])
+    assert data[8:8 + 16] == b"channels\0chlist\0"
+    assert struct.unpack_from("<i", data, 24)[0] == 2 + 16 + 1
+    got = read_exr(data)
+    assert list(got) == ["Y"]
+    np.testing.assert_array_equal(got["Y"].view(np.uint32), inv.view(np.uint32))
+    rgb = rng.rand(4, 6, 3).astype(np.float32)
+    got3 = read_exr(encode_exr(rgb))
+    assert sorted(got3) == ["B", "G", "R"]
+    for i, n in enumerate("BGR"):
+        np.testing.assert_array_equal(got3[n], rgb[:, :, i])
+
+
+def test_save_rgbd_panorama(tmp_path):
+    rng = np.random.RandomState(2)
+    pano = {"rgb": rng.randint(0, 256, (8, 16, 3)).astype(np.uint8),
+            "inv_distance": rng.rand(8, 16).astype(np.float32)}
+    os.makedirs(tmp_path / "output")
+    save_rgbd_panorama({"frame_0001.png": pano}, "frame_0001.png", str(tmp_path))
+    rgb = np.array(Image.open(tmp_path / "output" / "rgb_frame_0001.png"))
+    np.testing.assert_array_equal(rgb, pano["rgb"][:, :, ::-1])
+    inv = read_exr((tmp_path / "output" / "inv_distance_frame_0001.exr").read_bytes())["Y"]
+    np.testing.assert_array_equal(inv, pano["inv_distance"])
+    save_rgbd_panorama({}, "missing.png", str(tmp_path))  # a missing frame is skipped
+    assert sorted(os.listdir(tmp_path / "output")) == ["inv_distance_frame_0001.exr", "rgb_frame_0001.png"]
