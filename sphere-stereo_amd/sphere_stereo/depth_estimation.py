@@ -118,6 +118,7 @@ class RGBD_Estimator:
                                               self._rts[r], r, self._cands, self.candidate_count, self.rows,
                                               self.cols, self._vtype, scratch.data_ptr(), int(repeats),
                                               self._stream(), ms, _lib.C.byref(best)))
+        _TUNED[(self.rows, self.cols, self.candidate_count, self._vtype)] = int(best.value)
         return {"variant": int(best.value), "ms": [round(float(v), 4) for v in ms]}
 
     def _config_key(self):
@@ -255,6 +256,45 @@ class RGBD_Estimator:
                 raise ValueError("images to stitch must be float32 [%d, %d, 3] on %s" % (self._stitch_res + (
                     self.device,)))
         return imgs, stitch
+
+
+# sweep variants chosen by RGBD_Estimator.autotune in this process: (rows,
+# cols, candidates, volume type) -> variant
+_TUNED = {}
+
+
+def save_sweep_tuning(path):
+    """Writes the sweep variants tuned in this process (RGBD_Estimator.autotune,
+    FrameBatch.autotune) to a JSON file, with the device they were measured
+    on.  Returns the number of configurations written."""
+    import json
+
+    dev = torch.cuda.get_device_name() if torch.cuda.is_available() else ""
+    entries = [{"rows": k[0], "cols": k[1], "candidates": k[2], "volume_type": k[3], "variant": v}
+               for k, v in sorted(_TUNED.items())]
+    with open(path, "w") as f:
+        json.dump({"device": dev, "sweep_variants": entries}, f, indent=1)
+    return len(entries)
+
+
+def load_sweep_tuning(path):
+    """Applies the variants of a save_sweep_tuning file (ss_sweep_set_variant)
+    when it was written on a device of the same name; every later sweep of
+    those configurations uses them.  Returns the number applied."""
+    import json
+
+    with open(path) as f:
+        data = json.load(f)
+    dev = torch.cuda.get_device_name() if torch.cuda.is_available() else ""
+    if data.get("device", "") != dev:
+        return 0
+    n = 0
+    for e in data.get("sweep_variants", []):
+        key = (int(e["rows"]), int(e["cols"]), int(e["candidates"]), int(e["volume_type"]))
+        _lib.check(_lib.lib.ss_sweep_set_variant(key[0], key[1], key[2], key[3], int(e["variant"])))
+        _TUNED[key] = int(e["variant"])
+        n += 1
+    return n
 
 
 class FrameBatch:

@@ -180,3 +180,35 @@ def test_ragged_table_sizing_and_rejections():
     assert size(_fake_desc(6, 6, 32, 2, F16)) == 0             # a single pyramid level
     assert size(_fake_desc(8, 8, 32, 2, F16)) > 0
     assert lib.ss_frame_ragged_table_bytes(None, 1) == 0
+
+
+def test_sweep_tuning_json_roundtrip(tmp_path):
+    """save_sweep_tuning / load_sweep_tuning: the variants tuned in a process
+    go to JSON and back into the library's per-configuration registry
+    (ss_sweep_set_variant / ss_sweep_get_variant, host state only)."""
+    import json
+
+    from sphere_stereo import depth_estimation as de
+
+    lib = _lib.lib
+    saved = dict(de._TUNED)
+    try:
+        de._TUNED.clear()
+        de._TUNED[(96, 80, 48, _lib.SS_VOLUME_F16)] = 3
+        de._TUNED[(64, 64, 16, _lib.SS_VOLUME_F32)] = 1
+        path = str(tmp_path / "tuning.json")
+        assert de.save_sweep_tuning(path) == 2
+        data = json.load(open(path))
+        assert {e["variant"] for e in data["sweep_variants"]} == {1, 3}
+        assert lib.ss_sweep_get_variant(96, 80, 48, _lib.SS_VOLUME_F16) == 0
+        assert de.load_sweep_tuning(path) == 2
+        assert lib.ss_sweep_get_variant(96, 80, 48, _lib.SS_VOLUME_F16) == 3
+        assert lib.ss_sweep_get_variant(64, 64, 16, _lib.SS_VOLUME_F32) == 1
+        data["device"] = "another GPU"
+        json.dump(data, open(path, "w"))
+        assert de.load_sweep_tuning(path) == 0  # tuned elsewhere: not applied
+    finally:
+        for H, W, N, vt in [(96, 80, 48, _lib.SS_VOLUME_F16), (64, 64, 16, _lib.SS_VOLUME_F32)]:
+            lib.ss_sweep_set_variant(H, W, N, vt, -1)
+        de._TUNED.clear()
+        de._TUNED.update(saved)
