@@ -316,7 +316,8 @@ int ss_frame_batch_run(void *frame, const float *const *d_match, const float *co
  * resolution of at least 8 x 8.  Each pass of the frame (the sweep, each
  * level of the two filters, the coarse tails, the reprojection, each group of
  * 16 inpainting passes, the uint8 casts, the merge) is one launch for all the
- * frames (the filter levels one per class of candidate counts present).  In
+ * frames (the filter levels one per padded candidate count present; the
+ * sweep one per tuned sweep variant).  In
  * the persistent form a launch is sized to the workgroups the GPU holds
  * resident and each worker loops over work items (row segments, pixel
  * blocks, tiles); in the dispatched form each worker takes one item.  Each

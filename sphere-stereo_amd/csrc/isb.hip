@@ -409,36 +409,24 @@ __device__ __forceinline__ void up_item(const IsbUpArgs& a, int item, int lane, 
 }
 
 // Calls f(integral_constant<V>) for the entry's V.  A launch takes the
-// entries of one V class (VMAX = ragged_vmax(V): {1}, {2}, {3, 4}, {6, 8},
-// {12, 16}) and instantiates those V only, so that its register budget, hence
-// its occupancy, is that of its own candidate counts.
+// entries of one V (VMAX = V) and instantiates that V only, so that its
+// register budget, hence its occupancy, is that of its own candidate count.
 template <int VMAX, typename F>
 __device__ __forceinline__ void with_v(int V, F&& f) {
-  switch (V) {
-    case 1: if constexpr (VMAX == 1) f(std::integral_constant<int, 1>()); break;
-    case 2: if constexpr (VMAX == 2) f(std::integral_constant<int, 2>()); break;
-    case 3: if constexpr (VMAX == 4) f(std::integral_constant<int, 3>()); break;
-    case 4: if constexpr (VMAX == 4) f(std::integral_constant<int, 4>()); break;
-    case 6: if constexpr (VMAX == 8) f(std::integral_constant<int, 6>()); break;
-    case 8: if constexpr (VMAX == 8) f(std::integral_constant<int, 8>()); break;
-    case 12: if constexpr (VMAX == 16) f(std::integral_constant<int, 12>()); break;
-    case 16: if constexpr (VMAX == 16) f(std::integral_constant<int, 16>()); break;
-    default: break;
-  }
+  if (V == VMAX) f(std::integral_constant<int, VMAX>());
 }
 
 // Occupancy targets (waves per SIMD, the second __launch_bounds__ argument)
-// of the ragged passes per V class, each the most the class reaches without
-// register spills.  The persistent loop makes the compiler's own choice
-// lower than the plain kernels' for the same V.
-template <int G, int VMAX>
+// of the ragged passes per V, each the most that V reaches without register
+// spills.  The entry lookup makes the compiler's own choice lower than the
+// plain kernels' for the same V.
+template <int G, int V>
 constexpr int down_waves() {
-  return G == 1 ? 7 : (VMAX <= 2 ? 6 : VMAX == 4 ? 5 : 4);
+  return G == 1 ? 7 : (V <= 2 ? 6 : V <= 4 ? 5 : 4);
 }
-template <int G, int VMAX, bool FUSE>
+template <int G, int V, bool FUSE>
 constexpr int up_waves() {
-  return G == 1 ? 4 : (FUSE ? (VMAX == 1 ? 5 : VMAX == 2 ? 4 : VMAX == 4 ? 3 : VMAX == 8 ? 2 : 1)
-                            : (VMAX <= 8 ? 3 : 2));
+  return G == 1 ? 4 : (FUSE ? (V == 1 ? 5 : V == 2 ? 4 : V <= 4 ? 3 : V <= 8 ? 2 : 1) : (V <= 8 ? 3 : 2));
 }
 
 template <int G, typename T, int VMAX, bool LOOP>
@@ -474,8 +462,11 @@ __global__ void __launch_bounds__(kTailBlock) k_isb_tail_ragged(const TailEntry*
   } else {
     with_v<1>(e.V, body);
     with_v<2>(e.V, body);
+    with_v<3>(e.V, body);
     with_v<4>(e.V, body);
+    with_v<6>(e.V, body);
     with_v<8>(e.V, body);
+    with_v<12>(e.V, body);
     with_v<16>(e.V, body);
   }
 }
@@ -704,8 +695,11 @@ hipError_t up_ragged_t(const RaggedSpan<IsbUpEntry>& sp, hipStream_t s) {
   switch (vmax) {                               \
     case 1: return CALL(1);                     \
     case 2: return CALL(2);                     \
+    case 3: return CALL(3);                     \
     case 4: return CALL(4);                     \
+    case 6: return CALL(6);                     \
     case 8: return CALL(8);                     \
+    case 12: return CALL(12);                   \
     case 16: return CALL(16);                   \
     default: return hipErrorInvalidValue;       \
   }
@@ -724,11 +718,7 @@ hipError_t up_ragged_vt(const RaggedSpan<IsbUpEntry>& sp, int vmax, bool loop, h
 #undef SS_CALL
 }
 
-int ragged_vmax(int V) {
-  int m = 1;
-  while (m < V) m *= 2;
-  return m;
-}
+int ragged_vmax(int V) { return V; }
 
 hipError_t launch_isb_down_ragged(const RaggedSpan<IsbDownEntry>& sp, int G, int vt, int vmax, bool loop,
                                   hipStream_t s) {

@@ -15,8 +15,8 @@
 //
 // A pyramid level k holds the level-k pass of every pair that has one (the
 // pyramids of different resolutions have different depths and tail starts),
-// one launch per V class of the candidate counts present (the register
-// budget of a class kernel is its own).  Each XCD takes an eighth of every
+// one launch per candidate-lane count V present (each kernel has the
+// register budget of its own V).  Each XCD takes an eighth of every
 // pair, pair after pair (ss_ragged.h), so small and large pairs share
 // launches and no pair waits for a launch of its own.  A launch is
 // persistent (as many workgroups as the GPU holds resident, each worker
@@ -205,7 +205,7 @@ void plan_filter(Image& im, int G, int vt, const std::vector<const PyrLevel*>& l
     max_down = std::max(max_down, last_down[p]);
     max_up = std::max(max_up, first_up[p]);
   }
-  // one launch per level and V class present at that level
+  // one launch per level and V present at that level
   std::vector<int> classes;
   for (int p = 0; p < P; ++p)
     if (std::find(classes.begin(), classes.end(), ragged_vmax(V[p])) == classes.end())

@@ -356,7 +356,7 @@ hipError_t launch_publish(const void* const* src, int n, const void** dst, hipSt
 // variant: one of the sweep's scheduling variants (sweep_variant_info)
 hipError_t launch_sweep_ragged(const RaggedSpan<SweepPair>& sp, const float* const* img, int vt, int variant,
                                bool loop, hipStream_t s);
-// vmax: the V class of the span's entries (ragged_vmax)
+// vmax: the V of the span's entries (ragged_vmax: one V per launch)
 int ragged_vmax(int V);
 hipError_t launch_isb_down_ragged(const RaggedSpan<IsbDownEntry>& sp, int G, int vt, int vmax, bool loop,
                                   hipStream_t s);

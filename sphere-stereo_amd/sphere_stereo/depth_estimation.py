@@ -305,7 +305,7 @@ class FrameBatch:
 
     ``persistent=True`` (the ragged form): one native executor runs every
     pass once for all the frames, whatever their configurations (the filter
-    levels once per class of candidate counts present).  ``schedule``:
+    levels once per padded candidate count present).  ``schedule``:
     "persistent" launches are sized to what the GPU holds resident and their
     workers loop over work items (pixel tiles and blocks); "dispatched"
     launches have a worker per item;
