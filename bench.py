@@ -248,7 +248,7 @@ def batching_ragged(args, device):
         print("[bench] ragged batch (%d rigs, %d configurations), fp16 %s: %.1f panoramas/s (plain fp32 one at a "
               "time: %.1f)" % (len(sets), len(batch.groups), mode, v, v_plain), file=sys.stderr, flush=True)
     del batch
-    # the single-executor ragged form: one launch per pass (per V class for the
+    # the single-executor ragged form: one launch per pass (per candidate-lane count for the
     # filter levels) for all 16 rigs, in its two schedules
     for schedule in ("persistent", "dispatched"):
         batch = FrameBatch(ests, persistent=True, schedule=schedule)
