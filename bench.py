@@ -251,7 +251,7 @@ def batching_ragged(args, device):
     # the single-executor ragged form: one launch per pass (per candidate-lane count for the
     # filter levels) for all 16 rigs, in its two schedules
     for schedule in ("persistent", "dispatched"):
-        batch = FrameBatch(ests, persistent=True, schedule=schedule)
+        batch = FrameBatch(ests, single_executor=True, schedule=schedule)
         batch.autotune(frames)  # the variants tuned above: rebuilds with one sweep launch per variant
         for mode in ("eager", "graph"):
             batch.use_graph = mode == "graph"

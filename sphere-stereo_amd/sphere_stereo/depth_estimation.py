@@ -303,7 +303,7 @@ class FrameBatch:
     images differ, and so may their configurations (matching resolution,
     candidate count, references, sigmas, panorama, inpainting settings).
 
-    ``persistent=True`` (the ragged form): one native executor runs every
+    ``single_executor=True`` (the ragged form): one native executor runs every
     pass once for all the frames, whatever their configurations (the filter
     levels once per padded candidate count present).  ``schedule``:
     "persistent" launches are sized to what the GPU holds resident and their
@@ -314,7 +314,7 @@ class FrameBatch:
     matching resolution.  The launch records live in a device table the batch
     allocates once.
 
-    ``persistent=False``: the batch is split into groups of one
+    ``single_executor=False``: the batch is split into groups of one
     configuration, and each group is one native batch in which every pass
     after the sweep is one launch for a chunk of frames (up to 16
     frame-reference pairs), so the latency-bound coarse pyramid levels and
@@ -326,7 +326,7 @@ class FrameBatch:
     ``estimate_RGBD_panorama``.
     """
 
-    def __init__(self, estimators, persistent=False, schedule="auto"):
+    def __init__(self, estimators, single_executor=False, schedule="auto"):
         self.estimators = list(estimators)
         F = len(self.estimators)
         if not 1 <= F <= _lib.SS_MAX_BATCH_FRAMES:
@@ -334,13 +334,13 @@ class FrameBatch:
         devs = {e.device for e in self.estimators}
         if len(devs) != 1:
             raise ValueError("the estimators of a batch must share one device")
-        self.persistent = bool(persistent)
+        self.single_executor = bool(single_executor)
         self.schedule = {"persistent": _lib.SS_RAGGED_PERSISTENT, "dispatched": _lib.SS_RAGGED_DISPATCHED,
                          "auto": _lib.SS_RAGGED_AUTO}[schedule]
         self._table = None
         self.groups = []
         self._frames = []
-        if self.persistent:
+        if self.single_executor:
             self.groups.append(list(range(F)))
             self._frames.append(self._create_ragged())
         else:
@@ -365,7 +365,7 @@ class FrameBatch:
         descs = (_lib.ss_frame_desc * F)(*[e._frame_desc() for e in self.estimators])
         nbytes = _lib.lib.ss_frame_ragged_table_bytes(descs, F)
         if nbytes == 0:
-            raise ValueError("these estimators cannot form a persistent batch: they must share the volume type "
+            raise ValueError("these estimators cannot share one executor: they must share the volume type "
                              "and have >= 2 candidates and an 8 x 8 or larger matching resolution")
         self._table = torch.empty(int(nbytes), dtype=torch.uint8, device=self.estimators[0].device)
         handle = _lib.C.c_void_p()
@@ -377,12 +377,12 @@ class FrameBatch:
         """The sweep autotuner (RGBD_Estimator.autotune) for each configuration
         group of the batch, on its first frame; then drops captured graphs so
         that the next graph replay records the chosen variants.  Returns one
-        result per group.  In the single-executor form (``persistent=True``)
+        result per group.  In the single-executor form (``single_executor=True``)
         each configuration is tuned on its first frame and the executor is
         rebuilt: its sweep runs one launch per variant the configurations
         picked."""
         frames = list(frames)
-        if self.persistent:
+        if self.single_executor:
             out, seen = [], set()
             for i, e in enumerate(self.estimators):
                 if e._config_key() not in seen:

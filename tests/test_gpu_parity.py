@@ -472,7 +472,7 @@ def test_frame_batch_persistent_matches_single_frames(gpu, vt, schedule):
     for e in ests:
         e.fishey_stitcher.RGB_panorama.zero_()
         e.fishey_stitcher.distance_panorama.zero_()
-    batch = FrameBatch(ests, persistent=True, schedule=schedule)
+    batch = FrameBatch(ests, single_executor=True, schedule=schedule)
     assert batch.groups == [list(range(len(cases)))]
     for mode in ("eager", "eager", "graph", "graph", "graph"):
         batch.use_graph = mode == "graph"
@@ -496,7 +496,7 @@ def test_frame_batch_persistent_rejects(gpu):
                   rgb_to_stitch_resolution=fr["stitch"], panorama_resolution=(64, 32), sigma_i=10, sigma_s=12.5)
         ests.append(RGBD_Estimator(fr["calibs"], device=gpu, volume_dtype=vt, **kw))
     with pytest.raises(ValueError):
-        FrameBatch(ests, persistent=True)
+        FrameBatch(ests, single_executor=True)
     FrameBatch(ests)  # the grouped form takes them
 
 
