@@ -436,7 +436,7 @@ def test_frame_batch_ragged_matches_single_frames(gpu):
 @pytest.mark.parametrize("vt,schedule", [("float32", "persistent"), ("float16", "persistent"), ("bfloat16", "persistent"),
                                          ("float32", "dispatched"), ("float16", "dispatched"), ("float16", "auto"),
                                          ("bfloat16", "auto")])
-def test_frame_batch_persistent_matches_single_frames(gpu, vt, schedule):
+def test_frame_batch_single_executor_matches_single_frames(gpu, vt, schedule):
     """The ragged persistent form: frames of different matching resolutions
     (pyramids of different depths, with and without a coarse tail),
     candidate counts (V = 2, 3, 4, 6, 8), reference and camera sets, sigmas,
@@ -486,8 +486,8 @@ def test_frame_batch_persistent_matches_single_frames(gpu, vt, schedule):
             e.fishey_stitcher.RGB_panorama.fill_(7)
 
 
-def test_frame_batch_persistent_rejects(gpu):
-    """The persistent form needs one volume type and N >= 2."""
+def test_frame_batch_single_executor_rejects(gpu):
+    """The single executor needs one volume type and N >= 2."""
     ests = []
     for i, (N, vt) in enumerate([(32, "float32"), (16, "float16")]):
         fr = frame(seed=70 + i, resolution=160, matching=(64, 64), refs=(0, 2), device=gpu)
