@@ -37,15 +37,36 @@ def hipcc() -> str:
     return path if os.path.exists(path) else "hipcc"
 
 
-def build(verbose: bool = False, extra_flags=None) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+def build(verbose: bool = False, extra_flags=None, jobs=None) -> str:
+    """Compiles every source to an object in parallel (up to `jobs` hipcc
+    processes, default: the CPU count, at most 16), then links the library."""
+    import concurrent.futures
+    import tempfile
+
+    flags = [f for f in HIPCC_FLAGS if f != "-shared"] + list(extra_flags or [])
+    jobs = jobs or min(16, os.cpu_count() or 1)
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + list(extra_flags or []) + ["-o", tmp] + srcs
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + res.stdout + res.stderr)
+    with tempfile.TemporaryDirectory(prefix="ss_build_") as obj_dir:
+        def compile_one(src):
+            obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
+            cmd = [hipcc()] + flags + ["-c", "-o", obj, os.path.join(CSRC, src)]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            res = subprocess.run(cmd, capture_output=True, text=True)
+            if res.returncode != 0:
+                raise RuntimeError("hipcc failed on %s:\n%s%s" % (src, res.stdout, res.stderr))
+            return obj
+
+        # the slowest sources first, so that they start at once
+        order = sorted(SOURCES, key=lambda f: -os.path.getsize(os.path.join(CSRC, f)))
+        with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
+            objs = dict(zip(order, ex.map(compile_one, order)))
+        cmd = [hipcc()] + HIPCC_FLAGS + list(extra_flags or []) + ["-o", tmp] + [objs[s] for s in SOURCES]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError("hipcc link failed:\n" + res.stdout + res.stderr)
     os.replace(tmp, LIB)
     return LIB
 
