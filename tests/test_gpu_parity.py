@@ -486,6 +486,39 @@ def test_frame_batch_single_executor_matches_single_frames(gpu, vt, schedule):
             e.fishey_stitcher.RGB_panorama.fill_(7)
 
 
+@pytest.mark.parametrize("schedule", ["dispatched", "persistent"])
+def test_frame_batch_single_executor_many_pairs(gpu, schedule):
+    """68 frame-reference pairs (17 frames x 4 references) in one executor:
+    the passes have more than 63 entries, where an item's entry is found by
+    binary search instead of the wavefront ballot; bit-exact against the
+    single-frame runs."""
+    ests, frames, want = [], [], []
+    for i in range(17):
+        fr = frame(seed=80 + i, resolution=160, matching=(32, 32) if i % 2 else (40, 32), stitch=(48, 48),
+                   refs=(0, 1, 2, 3), device=gpu, baseline=0.2 + 0.01 * i)
+        kw = dict(min_dist=0.55, max_dist=100, candidate_count=16 if i % 3 else 24, references_indices=fr["refs"],
+                  reprojection_viewpoint=fr["viewpoint"], masks=fr["masks"], matching_resolution=fr["matching"],
+                  rgb_to_stitch_resolution=fr["stitch"], panorama_resolution=(64, 32), sigma_i=10, sigma_s=12.5)
+        est = RGBD_Estimator(fr["calibs"], device=gpu, volume_dtype="float16", **kw)
+        rgb, dist = est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])
+        want.append((rgb.clone(), dist.clone()))
+        ests.append(est)
+        frames.append(([t.contiguous() for t in fr["to_match"]], [t.contiguous() for t in fr["to_stitch"]]))
+    print("many pairs: 17 frames x 4 references, %s" % schedule, flush=True)
+    for e in ests:
+        e.fishey_stitcher.distance_panorama.fill_(-1.0)
+    batch = FrameBatch(ests, single_executor=True, schedule=schedule)
+    for mode in ("eager", "graph", "graph"):
+        batch.use_graph = mode == "graph"
+        out = batch.estimate_RGBD_panoramas(frames)
+        torch.cuda.synchronize()
+        for i, ((rgb, dist), (wr, wd)) in enumerate(zip(out, want)):
+            assert torch.equal(rgb, wr), "%s rgb panorama of frame %d" % (mode, i)
+            assert torch.equal(dist.view(torch.int32), wd.view(torch.int32)), "%s distance panorama %d" % (mode, i)
+        for e in ests:
+            e.fishey_stitcher.distance_panorama.fill_(-1.0)
+
+
 def test_frame_batch_single_executor_rejects(gpu):
     """The single executor needs one volume type and N >= 2."""
     ests = []
