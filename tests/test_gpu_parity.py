@@ -488,22 +488,27 @@ def test_frame_batch_single_executor_matches_single_frames(gpu, vt, schedule):
 
 @pytest.mark.parametrize("schedule", ["dispatched", "persistent"])
 def test_frame_batch_single_executor_many_pairs(gpu, schedule):
-    """68 frame-reference pairs (17 frames x 4 references) in one executor:
-    the passes have more than 63 entries, where an item's entry is found by
+    """Frame-reference pairs of 17 rigs in one executor (13 rigs of 4
+    cameras x 4 references, 4 rigs of 3 cameras x 3 references: 64 pairs,
+    and each frame's images at its own offset of the run's image table): the
+    passes have more than 63 entries, where an item's entry is found by
     binary search instead of the wavefront ballot; bit-exact against the
     single-frame runs."""
     ests, frames, want = [], [], []
     for i in range(17):
+        cams = 3 if i % 4 == 1 else 4
+        refs = tuple(range(cams))
         fr = frame(seed=80 + i, resolution=160, matching=(32, 32) if i % 2 else (40, 32), stitch=(48, 48),
-                   refs=(0, 1, 2, 3), device=gpu, baseline=0.2 + 0.01 * i)
-        kw = dict(min_dist=0.55, max_dist=100, candidate_count=16 if i % 3 else 24, references_indices=fr["refs"],
-                  reprojection_viewpoint=fr["viewpoint"], masks=fr["masks"], matching_resolution=fr["matching"],
+                   refs=refs, device=gpu, baseline=0.2 + 0.01 * i)
+        calibs, masks, to_match = fr["calibs"][:cams], fr["masks"][:cams], fr["to_match"][:cams]
+        kw = dict(min_dist=0.55, max_dist=100, candidate_count=16 if i % 3 else 24, references_indices=list(refs),
+                  reprojection_viewpoint=fr["viewpoint"], masks=masks, matching_resolution=fr["matching"],
                   rgb_to_stitch_resolution=fr["stitch"], panorama_resolution=(64, 32), sigma_i=10, sigma_s=12.5)
-        est = RGBD_Estimator(fr["calibs"], device=gpu, volume_dtype="float16", **kw)
-        rgb, dist = est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])
+        est = RGBD_Estimator(calibs, device=gpu, volume_dtype="float16", **kw)
+        rgb, dist = est.estimate_RGBD_panorama(to_match, fr["to_stitch"])
         want.append((rgb.clone(), dist.clone()))
         ests.append(est)
-        frames.append(([t.contiguous() for t in fr["to_match"]], [t.contiguous() for t in fr["to_stitch"]]))
+        frames.append(([t.contiguous() for t in to_match], [t.contiguous() for t in fr["to_stitch"]]))
     print("many pairs: 17 frames x 4 references, %s" % schedule, flush=True)
     for e in ests:
         e.fishey_stitcher.distance_panorama.fill_(-1.0)
