@@ -1,0 +1,181 @@
+"""The dataset loop of main.py (main.py:44-140) on this package:
+``python -m sphere_stereo.main --dataset_path DIR [options]`` with the
+reference's arguments.  DIR holds calibration.json (basalt, {"value0": ...}),
+cam<k>/ folders of frames (and an optional cam<k>/mask.png); the panoramas
+go to DIR/output/rgb_<frame>.png and inv_distance_<frame>.exr.
+
+Differences, because cv2, skimage and a display are not available: images
+are decoded with PIL (8-bit and 16-bit greyscale, 8-bit RGB/RGBA as
+cv2.IMREAD_UNCHANGED gives them, RGB turned to cv2's BGR order; other files
+count as unreadable, as cv2.imread's None does there); a mask is resized in
+float32 and rounded to uint8, where cv2 resizes uint8 in fixed point
+(parity unpinned); --visualize is ignored; --evaluate reports the
+inverse-distance MAE / RMSE / bad-pixel ratios (utils.py:233-242, on the GPU)
+against DIR/gt/inv_distance_<frame>.exr when that file is an uncompressed
+OpenEXR this package can read, and no SSIM / PSNR (skimage).
+"""
+import argparse
+import json
+import os
+import struct
+import warnings
+
+import numpy as np
+import torch
+from PIL import Image
+
+from .depth_estimation import RGBD_Estimator
+from .evaluation import error_stats
+from .io import save_rgbd_panorama
+from .utils import parse_json_calib, prepare_inputs, resize_linear
+
+
+def read_image(path):
+    """cv2.imread(path, cv2.IMREAD_UNCHANGED) for the files PIL decodes:
+    HW (greyscale) or HWC in BGR / BGRA order; None when unreadable."""
+    try:
+        with Image.open(path) as im:
+            if im.mode not in ("L", "I;16", "RGB", "RGBA"):
+                return None
+            a = np.array(im)
+    except (OSError, ValueError):
+        return None
+    if a.ndim == 3:
+        a = a[:, :, [2, 1, 0, 3][:a.shape[2]]]
+    return np.ascontiguousarray(a)
+
+
+def read_frame(filename, dataset_path, calibrations):
+    """read_input_images (utils.py:157-204) up to the resize, which runs on
+    the GPU: the C decoded images (uint8 / uint16 as read, float32 clipped
+    to [0, 1] and scaled to [0, 255]) and whether the frame is valid."""
+    images, valid = [], True
+    for k, calib in enumerate(calibrations):
+        path = os.path.join(dataset_path, "cam%d" % k, filename)
+        img = read_image(path)
+        if img is None:
+            warnings.warn("Cannot read image for file " + path)
+            return [], False
+        if img.shape != (calib.original_resolution[1], calib.original_resolution[0], 3):
+            warnings.warn("Invalid image size / channels for file " + path)
+            return [], False
+        if img.dtype == np.float32:
+            if img.max() > 1:
+                img = np.clip(img, 0, 1)
+                warnings.warn("Image has out-of-range float values for file " + path + ". Clipped for processing.")
+            img = img * 255
+        elif img.dtype not in (np.uint8, np.uint16):
+            warnings.warn("Invalide image type for file " + path)
+            return [], False
+        images.append(img)
+    return images, valid
+
+
+def read_exr(data):
+    """Inverse-distance ground truth: the Y channel of a single-part scanline
+    OpenEXR file without compression and with 32-bit float pixels (as
+    io.encode_exr writes); None for any other EXR."""
+    if len(data) < 8 or struct.unpack_from("<ii", data, 0) != (20000630, 2):
+        return None
+    pos, attrs = 8, {}
+    while data[pos] != 0:
+        end = data.index(b"\0", pos)
+        tend = data.index(b"\0", end + 1)
+        size = struct.unpack_from("<i", data, tend + 1)[0]
+        attrs[data[pos:end].decode()] = data[tend + 5:tend + 5 + size]
+        pos = tend + 5 + size
+    pos += 1
+    if attrs.get("compression") != b"\0" or attrs.get("channels", b"")[:2] != b"Y\0":
+        return None
+    if struct.unpack_from("<i", attrs["channels"], 2)[0] != 2 or attrs["channels"][18:19] != b"\0":
+        return None
+    x0, y0, x1, y1 = struct.unpack("<iiii", attrs["dataWindow"])
+    w, h = x1 - x0 + 1, y1 - y0 + 1
+    out = np.empty((h, w), np.float32)
+    for off in struct.unpack_from("<%dQ" % h, data, pos):
+        y = struct.unpack_from("<i", data, off)[0]
+        out[y - y0] = np.frombuffer(data, "<f4", w, off + 8)
+    return out
+
+
+def parse_args(argv=None):
+    p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    p.add_argument("--dataset_path", type=str, default="evaluation_dataset")
+    p.add_argument("--references_indices", nargs="*", type=int, default=[0, 2])
+    p.add_argument("--min_dist", type=float, default=0.55)
+    p.add_argument("--max_dist", type=float, default=100)
+    p.add_argument("--candidate_count", type=int, default=32)
+    p.add_argument("--sigma_i", type=float, default=10)
+    p.add_argument("--sigma_s", type=float, default=25)
+    p.add_argument("--matching_resolution", nargs=2, type=int, default=[1024, 1024])
+    p.add_argument("--rgb_to_stitch_resolution", nargs=2, type=int, default=[1216, 1216])
+    p.add_argument("--panorama_resolution", nargs=2, type=int, default=[2048, 1024])
+    p.add_argument("--device", type=str, default="cuda:0")
+    p.add_argument("--saving", type=lambda s: s.lower() not in ("0", "false", "no"), default=True)
+    p.add_argument("--visualize", type=lambda s: s.lower() not in ("0", "false", "no"), default=False)
+    p.add_argument("--evaluate", type=lambda s: s.lower() not in ("0", "false", "no"), default=False)
+    p.add_argument("--bad_px_ratio_thresholds", nargs="*", type=float, default=[0.1, 0.4])
+    return p.parse_args(argv)
+
+
+def run(args):
+    """The loop of main.py:62-140; returns {filename: {"rgb", "inv_distance"}}
+    (and, with --evaluate, {filename: metrics} as a second value)."""
+    with open(os.path.join(args.dataset_path, "calibration.json")) as f:
+        raw = json.load(f)["value0"]
+    calibrations = parse_json_calib(raw, args.matching_resolution, args.device)
+    viewpoint = torch.zeros([3], device=args.device)
+    for r in args.references_indices:
+        viewpoint += calibrations[r].rt[:3, 3] / len(args.references_indices)
+    mc, mr = args.matching_resolution
+    masks = []
+    for k in range(len(calibrations)):
+        mask = read_image(os.path.join(args.dataset_path, "cam%d" % k, "mask.png"))
+        if mask is not None and mask.ndim == 2:
+            m = resize_linear(torch.from_numpy(mask).to(args.device), mc, mr)
+            masks.append(torch.round(m).clamp(0, 255).unsqueeze(0) / 255)
+        else:
+            masks.append(torch.ones([mr, mc], device=args.device).unsqueeze(0))
+    est = RGBD_Estimator(calibrations, args.min_dist, args.max_dist, args.candidate_count, args.references_indices,
+                         viewpoint, masks, args.matching_resolution, args.rgb_to_stitch_resolution,
+                         args.panorama_resolution, args.sigma_i, args.sigma_s, args.device)
+    filenames = sorted(f for f in os.listdir(os.path.join(args.dataset_path, "cam0")) if f != "mask.png")
+    panoramas = {}
+    for filename in filenames:
+        images, valid = read_frame(filename, args.dataset_path, calibrations)
+        if not valid:
+            continue
+        dev = [torch.from_numpy(i).to(args.device) for i in images]
+        to_match, to_stitch = prepare_inputs(dev, args.matching_resolution, args.rgb_to_stitch_resolution,
+                                             args.references_indices)
+        rgb, distance = est.estimate_RGBD_panorama(to_match, to_stitch)
+        panoramas[filename] = {"rgb": rgb.cpu().numpy(), "inv_distance": 1 / distance.cpu().numpy()}
+    if args.saving:
+        os.makedirs(os.path.join(args.dataset_path, "output"), exist_ok=True)
+        for filename in filenames:
+            save_rgbd_panorama(panoramas, filename, args.dataset_path)
+    if not args.evaluate:
+        return panoramas
+    metrics = {}
+    for filename, pano in panoramas.items():
+        gt_path = os.path.join(args.dataset_path, "gt", "inv_distance_" + os.path.splitext(filename)[0] + ".exr")
+        gt = read_exr(open(gt_path, "rb").read()) if os.path.isfile(gt_path) else None
+        if gt is None or gt.shape != pano["inv_distance"].shape:
+            warnings.warn("Invalid ground truth for file " + filename)
+            continue
+        s = error_stats(torch.from_numpy(pano["inv_distance"]).to(args.device), torch.from_numpy(gt).to(args.device),
+                        args.bad_px_ratio_thresholds, finite_only=False)
+        metrics[filename] = s
+    return panoramas, metrics
+
+
+def main(argv=None):
+    args = parse_args(argv)
+    out = run(args)
+    if args.evaluate:
+        for name, m in sorted(out[1].items()):
+            print(name, m)
+
+
+if __name__ == "__main__":
+    main()
