@@ -74,6 +74,33 @@ def encode_exr(img):
     return head + offsets + blocks
 
 
+def decode_exr(data):
+    """Inverse-distance ground truth: the Y channel of a single-part scanline
+    OpenEXR file without compression and with 32-bit float pixels (as
+    encode_exr writes); None for any other EXR."""
+    if len(data) < 8 or struct.unpack_from("<ii", data, 0) != (20000630, 2):
+        return None
+    pos, attrs = 8, {}
+    while data[pos] != 0:
+        end = data.index(b"\0", pos)
+        tend = data.index(b"\0", end + 1)
+        size = struct.unpack_from("<i", data, tend + 1)[0]
+        attrs[data[pos:end].decode()] = data[tend + 5:tend + 5 + size]
+        pos = tend + 5 + size
+    pos += 1
+    if attrs.get("compression") != b"\0" or attrs.get("channels", b"")[:2] != b"Y\0":
+        return None
+    if struct.unpack_from("<i", attrs["channels"], 2)[0] != 2 or attrs["channels"][18:19] != b"\0":
+        return None
+    x0, y0, x1, y1 = struct.unpack("<iiii", attrs["dataWindow"])
+    w, h = x1 - x0 + 1, y1 - y0 + 1
+    out = np.empty((h, w), np.float32)
+    for off in struct.unpack_from("<%dQ" % h, data, pos):
+        y = struct.unpack_from("<i", data, off)[0]
+        out[y - y0] = np.frombuffer(data, "<f4", w, off + 8)
+    return out
+
+
 def save_rgbd_panorama(rgbd_panoramas, filename, dataset_path):
     """utils.py:253-260: output/rgb_<name>.png and output/inv_distance_<name>.exr
     under dataset_path for rgbd_panoramas[filename] = {"rgb": BGR uint8,

@@ -17,7 +17,6 @@ OpenEXR this package can read, and no SSIM / PSNR (skimage).
 import argparse
 import json
 import os
-import struct
 import warnings
 
 import numpy as np
@@ -26,7 +25,7 @@ from PIL import Image
 
 from .depth_estimation import RGBD_Estimator
 from .evaluation import error_stats
-from .io import save_rgbd_panorama
+from .io import decode_exr, save_rgbd_panorama
 from .utils import parse_json_calib, prepare_inputs, resize_linear
 
 
@@ -49,7 +48,7 @@ def read_frame(filename, dataset_path, calibrations):
     """read_input_images (utils.py:157-204) up to the resize, which runs on
     the GPU: the C decoded images (uint8 / uint16 as read, float32 clipped
     to [0, 1] and scaled to [0, 255]) and whether the frame is valid."""
-    images, valid = [], True
+    images = []
     for k, calib in enumerate(calibrations):
         path = os.path.join(dataset_path, "cam%d" % k, filename)
         img = read_image(path)
@@ -68,34 +67,7 @@ def read_frame(filename, dataset_path, calibrations):
             warnings.warn("Invalide image type for file " + path)
             return [], False
         images.append(img)
-    return images, valid
-
-
-def read_exr(data):
-    """Inverse-distance ground truth: the Y channel of a single-part scanline
-    OpenEXR file without compression and with 32-bit float pixels (as
-    io.encode_exr writes); None for any other EXR."""
-    if len(data) < 8 or struct.unpack_from("<ii", data, 0) != (20000630, 2):
-        return None
-    pos, attrs = 8, {}
-    while data[pos] != 0:
-        end = data.index(b"\0", pos)
-        tend = data.index(b"\0", end + 1)
-        size = struct.unpack_from("<i", data, tend + 1)[0]
-        attrs[data[pos:end].decode()] = data[tend + 5:tend + 5 + size]
-        pos = tend + 5 + size
-    pos += 1
-    if attrs.get("compression") != b"\0" or attrs.get("channels", b"")[:2] != b"Y\0":
-        return None
-    if struct.unpack_from("<i", attrs["channels"], 2)[0] != 2 or attrs["channels"][18:19] != b"\0":
-        return None
-    x0, y0, x1, y1 = struct.unpack("<iiii", attrs["dataWindow"])
-    w, h = x1 - x0 + 1, y1 - y0 + 1
-    out = np.empty((h, w), np.float32)
-    for off in struct.unpack_from("<%dQ" % h, data, pos):
-        y = struct.unpack_from("<i", data, off)[0]
-        out[y - y0] = np.frombuffer(data, "<f4", w, off + 8)
-    return out
+    return images, True
 
 
 def parse_args(argv=None):
@@ -159,7 +131,7 @@ def run(args):
     metrics = {}
     for filename, pano in panoramas.items():
         gt_path = os.path.join(args.dataset_path, "gt", "inv_distance_" + os.path.splitext(filename)[0] + ".exr")
-        gt = read_exr(open(gt_path, "rb").read()) if os.path.isfile(gt_path) else None
+        gt = decode_exr(open(gt_path, "rb").read()) if os.path.isfile(gt_path) else None
         if gt is None or gt.shape != pano["inv_distance"].shape:
             warnings.warn("Invalid ground truth for file " + filename)
             continue

@@ -96,3 +96,14 @@ def test_save_rgbd_panorama(tmp_path):
     np.testing.assert_array_equal(inv, pano["inv_distance"])
     save_rgbd_panorama({}, "missing.png", str(tmp_path))  # a missing frame is skipped
     assert sorted(os.listdir(tmp_path / "output")) == ["inv_distance_frame_0001.exr", "rgb_frame_0001.png"]
+
+
+def test_decode_exr_reads_the_encoder_and_refuses_others():
+    """io.decode_exr (the ground-truth reader of the dataset loop) reads what
+    encode_exr writes, bit for bit, and refuses other layouts."""
+    from sphere_stereo.io import decode_exr
+
+    inv = np.random.RandomState(3).rand(7, 9).astype(np.float32)
+    np.testing.assert_array_equal(decode_exr(encode_exr(inv)), inv)
+    assert decode_exr(encode_exr(np.zeros((2, 3, 3), np.float32))) is None  # B, G, R channels
+    assert decode_exr(b"not an exr") is None
