@@ -107,3 +107,29 @@ def test_decode_exr_reads_the_encoder_and_refuses_others():
     np.testing.assert_array_equal(decode_exr(encode_exr(inv)), inv)
     assert decode_exr(encode_exr(np.zeros((2, 3, 3), np.float32))) is None  # B, G, R channels
     assert decode_exr(b"not an exr") is None
+
+
+def test_dataset_loop_reader_and_arguments(tmp_path):
+    """sphere_stereo.main's host side: images decoded as cv2.IMREAD_UNCHANGED
+    gives them (BGR, greyscale, BGRA kept 4-channel so the frame's shape check
+    refuses it, unreadable files None) and main.py's arguments."""
+    from sphere_stereo import main as ss_main
+
+    rng = np.random.RandomState(4)
+    bgr = rng.randint(0, 256, (6, 5, 3)).astype(np.uint8)
+    (tmp_path / "c.png").write_bytes(encode_png(bgr))
+    np.testing.assert_array_equal(ss_main.read_image(str(tmp_path / "c.png")), bgr)
+    gray = rng.randint(0, 256, (4, 7)).astype(np.uint8)
+    (tmp_path / "g.png").write_bytes(encode_png(gray))
+    np.testing.assert_array_equal(ss_main.read_image(str(tmp_path / "g.png")), gray)
+    rgba = rng.randint(0, 256, (3, 4, 4)).astype(np.uint8)
+    Image.fromarray(rgba, "RGBA").save(tmp_path / "a.png")
+    np.testing.assert_array_equal(ss_main.read_image(str(tmp_path / "a.png")), rgba[:, :, [2, 1, 0, 3]])
+    (tmp_path / "x.png").write_bytes(b"not an image")
+    assert ss_main.read_image(str(tmp_path / "x.png")) is None
+    assert ss_main.read_image(str(tmp_path / "missing.png")) is None
+    a = ss_main.parse_args([])
+    assert (a.references_indices, a.candidate_count, a.matching_resolution, a.panorama_resolution, a.sigma_s) == \
+        ([0, 2], 32, [1024, 1024], [2048, 1024], 25)
+    assert a.saving and not a.evaluate and a.bad_px_ratio_thresholds == [0.1, 0.4]
+    assert not ss_main.parse_args(["--saving", "False"]).saving
