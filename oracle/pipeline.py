@@ -92,11 +92,18 @@ class OraclePipeline:
         self.weights = self.o.normalize_weights(self.o.box_filter(raw, smoothing_radius, 0.0))
         self.thr_empty, self.thr_src = thresholds(max_dist)
 
-    def estimate_fisheye_distance(self, images_to_match, r, keep=None):
-        """depth_estimation.py:137-203 for camera r."""
+    def estimate_fisheye_distance(self, images_to_match, r, keep=None, sel=None):
+        """depth_estimation.py:137-203 for camera r; ``sel`` (H x W camera
+        indices) replaces the selection made at set-up, as the reference's
+        ``selected_camera`` argument does."""
         imgs = np.stack([_np(i) for i in images_to_match])
         guide = self.o.guide(imgs[r])
-        cost = self.o.sweep(imgs, self.sel[r], self.cands, self.cams, self.rts[r], r, self.vtype)
+        if sel is None:
+            sel = self.sel[r]
+        else:  # a value that is no camera index matches no camera (depth_estimation.py:155-170)
+            sel = np.asarray(sel).reshape(self.H, self.W)
+            sel = np.ascontiguousarray(np.where((sel >= 0) & (sel < len(self.calibs)), sel, -1), np.int32)
+        cost = self.o.sweep(imgs, sel, self.cands, self.cams, self.rts[r], r, self.vtype)
         if keep is not None:
             keep["cost_raw"] = cost.copy()
             keep["guide"] = guide.copy()

@@ -39,3 +39,82 @@ def first_mismatch(a, b):
         return None
     i = tuple(bad[0])
     return "%d mismatches, first at %s: %r vs %r" % (len(bad), i, a[i], b[i])
+
+
+# ---------------------------------------------------------------------------
+# the reference's README demo (README.md:27-30): resources/ frames, masks and
+# calibration, and the panorama the reference produced from them
+# ---------------------------------------------------------------------------
+import json  # noqa: E402
+import os  # noqa: E402
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+DEMO = os.path.join(GOLDEN, "demo")
+DEMO_EXPECTED = os.path.join(GOLDEN, "demo_expected_rgb_0.png")
+# python main.py --dataset_path resources --references_indices 2 3, the rest
+# main.py's defaults (main.py:35-51)
+DEMO_ARGS = dict(min_dist=0.55, max_dist=100, candidate_count=32, references_indices=[2, 3],
+                 matching_resolution=(1024, 1024), rgb_to_stitch_resolution=(1216, 1216),
+                 panorama_resolution=(2048, 1024), sigma_i=10, sigma_s=25)
+
+
+def demo_frame(oracle):
+    """The demo frame as main.py:62-81 and read_input_images (utils.py:157-204)
+    hand it to the estimator, on the CPU: frames and masks decoded with PIL
+    (BGR order), the frames resized to the matching resolution with the
+    oracle's INTER_LINEAR (the GPU input stage's bit-exact target), the masks
+    resized the same way, rounded to uint8 and divided by 255 the way torch
+    divides a GPU tensor by a Python number (main.py:79): a multiply by the
+    float32 reciprocal.  Returns
+    (calibrations, estimator keyword arguments, images_to_match,
+    images_to_stitch)."""
+    from sphere_stereo.main import read_image
+    from sphere_stereo.utils import parse_json_calib
+
+    a = DEMO_ARGS
+    mc, mr = a["matching_resolution"]
+    with open(os.path.join(DEMO, "calibration.json")) as f:
+        calibs = parse_json_calib(json.load(f)["value0"], list(a["matching_resolution"]), "cpu")
+    viewpoint = torch.zeros([3])
+    for r in a["references_indices"]:
+        viewpoint += calibs[r].rt[:3, 3] / len(a["references_indices"])
+    masks = []
+    for k in range(len(calibs)):
+        m = read_image(os.path.join(DEMO, "cam%d" % k, "mask.png")).astype(np.float32)
+        m = np.round(oracle.resize_linear(m, mc, mr)).clip(0, 255).astype(np.float32)
+        m = m * (np.float32(1) / np.float32(255))
+        masks.append(torch.from_numpy(m).unsqueeze(0))
+    imgs = [read_image(os.path.join(DEMO, "cam%d" % k, "0.jpg")).astype(np.float32) for k in range(len(calibs))]
+    sc, sr = a["rgb_to_stitch_resolution"]
+    to_match = [oracle.resize_linear(i, mc, mr) for i in imgs]
+    to_stitch = [oracle.resize_linear(imgs[r], sc, sr) for r in a["references_indices"]]
+    kw = dict(a, reprojection_viewpoint=viewpoint, masks=masks)
+    return calibs, kw, to_match, to_stitch
+
+
+def psnr(a, b):
+    d = np.asarray(a, np.float64) - np.asarray(b, np.float64)
+    return 10 * np.log10(255.0 ** 2 / np.mean(d * d))
+
+
+def cpu_args(calibs, kw):
+    """CPU copies of a frame's calibrations and estimator keyword arguments,
+    for the oracle."""
+    cc = [type(c)(c.original_resolution, c.principal.cpu(), c.fl.cpu(), c.xi, c.alpha, c.rt.cpu(),
+                  c.matching_scale.cpu()) for c in calibs]
+    kc = dict(kw)
+    kc["reprojection_viewpoint"] = kw["reprojection_viewpoint"].cpu()
+    kc["masks"] = [m.cpu() for m in kw["masks"]]
+    return cc, kc
+
+
+def oracle_frame(calibs, kw, to_match, to_stitch, **extra):
+    """The oracle's run of one frame: (pipeline, kept intermediates, rgb,
+    distance)."""
+    from pipeline import OraclePipeline
+
+    cc, kc = cpu_args(calibs, kw)
+    ora = OraclePipeline(cc, **kc, **extra)
+    keep = {}
+    rgb, dist = ora.estimate_RGBD_panorama([t.cpu() for t in to_match], [t.cpu() for t in to_stitch], keep=keep)
+    return ora, keep, rgb, dist

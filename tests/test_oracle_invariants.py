@@ -510,3 +510,163 @@ def test_oracle_results_do_not_depend_on_the_thread_count():
         gomp.omp_set_num_threads(before)
     for a, b in zip(outs[0], outs[1]):
         assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------------------------------
+# stitcher: blend weights and merge against hand-derived values
+# ---------------------------------------------------------------------------
+PI_REF = 3.14159265  # the reference's PI literal (stitcher.cu:325)
+
+
+def _ds_project(intr, p):
+    """Double-sphere projection from the model's definition (Usenko et al.,
+    arXiv:1807.08957, eqs. 40-45), float64: d1 = |p|, c = xi d1 + z,
+    d2 = |(x, y, c)|, uv = f xy / (alpha d2 + (1 - alpha) c) + c_xy; and the
+    validity z > -w2 d1 with w1 = alpha / (1 - alpha) for alpha <= 0.5, else
+    (1 - alpha) / alpha, w2 = (w1 + xi) / sqrt(2 w1 xi + xi^2 + 1)."""
+    fx, fy, cx, cy, xi, al = [float(v) for v in intr]
+    x, y, z = p[..., 0], p[..., 1], p[..., 2]
+    d1 = np.sqrt(x * x + y * y + z * z)
+    c = xi * d1 + z
+    d2 = np.sqrt(x * x + y * y + c * c)
+    den = al * d2 + (1 - al) * c
+    w1 = al / (1 - al) if al <= 0.5 else (1 - al) / al
+    w2 = (w1 + xi) / math.sqrt(2 * w1 * xi + xi * xi + 1)
+    return np.stack([fx * x / den + cx, fy * y / den + cy], -1), z > -w2 * d1
+
+
+def _bilinear(img, u, v):
+    """Bilinear read of img (rows x cols [x ch]) at (u, v) from the four
+    surrounding samples."""
+    u0, v0 = np.floor(u).astype(int), np.floor(v).astype(int)
+    fu, fv = u - u0, v - v0
+    if img.ndim == 3:
+        fu, fv = fu[..., None], fv[..., None]
+    return (img[v0, u0] * (1 - fu) * (1 - fv) + img[v0, u0 + 1] * fu * (1 - fv) + img[v0 + 1, u0] * (1 - fu) * fv
+            + img[v0 + 1, u0 + 1] * fu * fv)
+
+
+def _stitch_rig():
+    """Two references of a 64 x 64 matching resolution, yawed +-50 degrees,
+    with different translations from the panorama centre."""
+    W = H = 64
+    intr = np.array([[0.3 * W, 0.31 * W, 32.5, 31.0, -0.18, 0.59], [0.29 * W, 0.3 * W, 31.0, 32.5, -0.2, 0.6]])
+    rot = []
+    for yaw in (math.radians(50), math.radians(-50)):
+        c, s = math.cos(yaw), math.sin(yaw)
+        rot.append(np.array([[c, 0, -s], [0, 1, 0], [s, 0, c]]))
+    trans = np.array([[0.12, -0.02, -0.05], [-0.09, 0.03, 0.07]])
+    return H, W, intr, np.stack(rot), trans
+
+
+def test_blend_lut_weight_closed_form():
+    """createBlendingLutKernel (stitcher.cu:345-398): reference r's raw weight
+    at a panorama pixel is exp(-|p_far - p_near|^2 / (1e-4 * H * W)), H x W
+    the matching resolution (not the panorama's), where p_near / p_far are
+    the pixels of camera r that see the panorama ray at min_dist / max_dist
+    from the panorama centre (ray - translation, each coordinate clamped to
+    [0.1, size - 1.1], the row offset by r * H into the stacked masks); it
+    is 1e-8 where a projection is invalid or the stacked mask read at either
+    pixel is <= 0.99; the raw weights are divided by their sum over the
+    references.  Every expected value here is derived in float64 from the
+    projection's definition, so a wrong normaliser, a flipped translation or
+    a dropped mask test fails."""
+    H, W, intr, rot, trans = _stitch_rig()
+    trans = trans * 0.1  # near-far displacements of a pixel or less: weights far from 0 and 1
+    PH, PW = 96, 192  # PH * PW = 4.5 * H * W
+    masks = np.ones((2, H, W))
+    masks[1, :, 40:] = 0.0  # part of camera 1 unreliable: weight 1e-8 there
+    masks[1, :, 38:40] = 0.5
+    mn, mx = 0.55, 100.0
+    samp, raw = O64.blend_lut(intr, rot.reshape(2, 9), trans, masks, PH, PW, mn, mx)
+    y, x = np.meshgrid(np.arange(PH), np.arange(PW), indexing="ij")
+    phi = (y + 0.5) * PI_REF / PH - PI_REF / 2
+    th = (x + 0.5) * 2 * PI_REF / PW + PI_REF
+    unit = np.stack([np.cos(phi) * np.sin(th), np.sin(phi), np.cos(phi) * np.cos(th)], -1)
+    stacked = masks.reshape(2 * H, W)
+    bw = []
+    for r in range(2):
+        ray = unit @ rot[r].T
+        uv, ok0 = _ds_project(intr[r], ray)
+        want_uv = np.stack([np.clip(uv[..., 0], 0.1, W - 1.1), np.clip(uv[..., 1], 0.1, H - 1.1)], -1)
+        assert np.abs(samp[r] - want_uv).max() < 1e-9
+        pn, ok1 = _ds_project(intr[r], ray * mn - trans[r])
+        pf, ok2 = _ds_project(intr[r], ray * mx - trans[r])
+        pn = np.stack([np.clip(pn[..., 0], 0.1, W - 1.1), r * H + np.clip(pn[..., 1], 0.1, H - 1.1)], -1)
+        pf = np.stack([np.clip(pf[..., 0], 0.1, W - 1.1), r * H + np.clip(pf[..., 1], 0.1, H - 1.1)], -1)
+        ok = ok0 & ok1 & ok2 & (_bilinear(stacked, pn[..., 0], pn[..., 1]) > 0.99) & \
+            (_bilinear(stacked, pf[..., 0], pf[..., 1]) > 0.99)
+        d2 = ((pf - pn) ** 2).sum(-1)
+        bw.append(np.where(ok, np.exp(-d2 / (1e-4 * H * W)), 1e-8))
+    bw = np.stack(bw)
+    want = bw / bw.sum(0)
+    keep = bw.sum(0) > 1e-250
+    assert keep.mean() > 0.9
+    both = keep & (bw[0] > 1e-3) & (bw[1] > 1e-3)
+    assert both.sum() > 200, "pixels where both references carry a real weight"
+    assert np.any(keep & (bw[1] == 1e-8) & (bw[0] > 1e-6)), "pixels where camera 1's mask rules it out"
+    assert np.abs(raw[:, keep] - want[:, keep]).max() < 1e-9
+    # with the panorama's size as the normaliser the weights would differ
+    bw_pano = np.where(bw == 1e-8, 1e-8, bw ** (H * W / (PH * PW)))
+    assert np.abs((bw_pano / bw_pano.sum(0))[:, both] - want[:, both]).max() > 0.05
+    # the fp32 oracle, the bit-exact target of the GPU kernel, agrees to fp32 rounding
+    samp32, raw32 = O32.blend_lut(intr, rot.reshape(2, 9), trans, masks.astype(np.float32), PH, PW, mn, mx)
+    assert np.abs(raw32[:, both] - want[:, both]).max() < 1e-4
+
+
+def test_merge_closed_form():
+    """mergeRGBDPanoramaKernel (stitcher.cu:436-471) on non-constant inputs
+    and non-zero translations.  Each panorama pixel samples a direction d_r
+    of reference r at uv_r = project(d_r); the reprojected and original
+    distance maps are linear in (u, v), so their bilinear reads are known;
+    the colour comes from the point min(reprojected, original) * d_r +
+    t_r, projected into camera r, scaled to the stitch image's size (no
+    half-pixel shift), clamped to [0.1, size - 1.1] and read from a stitch
+    image that is linear in (x, y); distance = 1 / sum_r w_r / reprojected,
+    colour = sum_r w_r * colour_r truncated to uint8.  All of it derived
+    here in float64 from the projection's definition."""
+    H, W, intr, rot, trans = _stitch_rig()
+    trans = trans * 2.5  # large enough that a sign slip moves the colour samples by pixels
+    Hs, Ws = 96, 80
+    PH, PW = 12, 20
+    rng = np.random.RandomState(5)
+    dirs, samp = [], np.zeros((2, PH, PW, 2))
+    for r in range(2):
+        v = rng.randn(PH, PW, 3) * [0.3, 0.3, 0.2] + [0, 0, 1.0]  # well inside the field of view
+        v /= np.linalg.norm(v, axis=-1, keepdims=True)
+        uv, ok = _ds_project(intr[r], v)
+        assert ok.all() and (uv > 2).all() and (uv < W - 3).all()
+        dirs.append(v)
+        samp[r] = uv
+    w0 = rng.uniform(0.1, 0.9, (PH, PW))
+    weights = np.stack([w0, 1 - w0])
+    yy, xx = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    reproj = np.stack([1.0 + 0.02 * xx + 0.01 * yy, 1.2 + 0.01 * xx + 0.015 * yy])
+    dists = np.stack([2.2 - 0.02 * xx + 0.005 * yy, 0.9 + 0.03 * xx - 0.01 * yy])  # crosses reproj: min switches
+    sy, sx = np.meshgrid(np.arange(Hs), np.arange(Ws), indexing="ij")
+    lin = [lambda X, Y: 10 + X + Y, lambda X, Y: 20 + 2 * X, lambda X, Y: 250 - 2 * Y]
+    img = np.stack([np.stack([f(sx, sy) for f in lin], -1)] * 2).astype(np.uint8)
+    dist, rgb = O64.merge(samp, weights, reproj, dists, img, trans, intr)
+    inv, col = 0.0, 0.0
+    switched = 0
+    for r in range(2):
+        u, v = samp[r][..., 0], samp[r][..., 1]
+        rd = _bilinear(reproj[r], u, v)
+        dc = np.minimum(rd, _bilinear(dists[r], u, v))
+        switched += (dc < rd).sum()
+        inv = inv + weights[r] / rd
+        q, _ = _ds_project(intr[r], dirs[r] * dc[..., None] + trans[r])
+        qx = np.clip(q[..., 0] * (Ws / W), 0.1, Ws - 1.1)
+        qy = np.clip(q[..., 1] * (Hs / H), 0.1, Hs - 1.1)
+        col = col + weights[r][..., None] * np.stack([f(qx, qy) for f in lin], -1)
+    assert 0 < switched < 2 * PH * PW
+    assert np.abs(dist - 1 / inv).max() < 1e-12 * np.abs(dist).max()
+    near_int = np.abs(col - np.round(col)) < 1e-6
+    assert np.array_equal(rgb[~near_int], np.floor(col[~near_int]).astype(np.uint8))
+    # a translation of the wrong sign moves every colour sample by pixels
+    _, flipped = O64.merge(samp, weights, reproj, dists, img, -trans, intr)
+    assert np.abs(flipped.astype(int) - rgb.astype(int)).mean() > 3
+    # the fp32 oracle (the GPU kernel's bit-exact target) within its rounding
+    dist32, rgb32 = O32.merge(samp, weights, reproj, dists, img, trans, intr)
+    assert np.abs(dist32 - 1 / inv).max() < 1e-5 * np.abs(dist).max()
+    assert np.abs(rgb32.astype(int) - np.floor(col)).max() <= 1
