@@ -168,6 +168,10 @@ __device__ __forceinline__ void inpaint_tile(const float* __restrict__ din, floa
   const int tid = (int)threadIdx.x;
   const int ox = bx * kTile - kSteps, oy = by * kTile - kSteps;
   const int reach = min(steps, kSteps);
+  // A workgroup of the persistent ragged form runs several tiles in a row:
+  // every thread must be done with the previous tile's read-out (bufa/bufb)
+  // and its last test of `changed` before this tile resets and restages them.
+  __syncthreads();
   if (tid == 0) {
     count = 0;
     changed[0] = 0;
