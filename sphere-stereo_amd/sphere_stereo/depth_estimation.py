@@ -201,27 +201,50 @@ class RGBD_Estimator:
                 return slot, r
         raise ValueError("reference_calibration is not one of the estimator's references")
 
-    def _estimate(self, slot, r, images, guide, out, workspace=None, stream=None):
+    def _estimate(self, slot, r, images, guide, out, workspace=None, stream=None, raylut=None):
         ptrs = _lib.ptr_array(images)
         ws = self._workspace if workspace is None else workspace
         st = self._stream() if stream is None else _lib.C.c_void_p(stream.cuda_stream)
+        lut = self._raylut[slot] if raylut is None else raylut
         _lib.check(_lib.lib.ss_estimate_distance(
-            self._raylut[slot].data_ptr(), ptrs, len(images), self._cams, self._rts[r], r, self._cands,
+            lut.data_ptr(), ptrs, len(images), self._cams, self._rts[r], r, self._cands,
             self.candidate_count, self.rows, self.cols, float(self.sigma_i), float(self.sigma_s), self._vtype,
             guide.data_ptr() if guide is not None else None, out.data_ptr(), ws.data_ptr(), ws.numel(), st))
         return out
 
+    def _selection_raylut(self, r, selected_camera):
+        """Unit-ray table of reference r with a caller-given camera selection
+        ([1, rows, cols] or [rows, cols] integers).  A value outside [0, C)
+        selects no camera: the reference's masked copy (depth_estimation.py:
+        155-170) compares it with every camera index and matches none."""
+        s = torch.as_tensor(selected_camera, device=self.device).reshape(self.rows, self.cols)
+        if s.dtype.is_floating_point or s.dtype == torch.bool:
+            raise ValueError("selected_camera must hold integer camera indices")
+        C = len(self.calibrations)
+        sel = torch.where((s >= 0) & (s < C), s, torch.full_like(s, -1)).to(torch.int8).contiguous()
+        lut = torch.empty([self.rows, self.cols, 4], dtype=torch.float32, device=self.device)
+        _lib.check(_lib.lib.ss_ray_lut(self._cams[r], sel.data_ptr(), self.rows, self.cols, lut.data_ptr(),
+                                       self._stream()))
+        return lut, sel
+
     def estimate_fisheye_distance(self, reference_image, guide, reference_calibration, selected_camera, images):
         """Distance map of one reference fisheye (depth_estimation.py:137-203):
         sweep, cost, cost filter, argmin with parabola refinement, distance
-        filter.  The camera selection of ``reference_calibration`` made at
-        construction is used; ``selected_camera`` is accepted for interface
-        compatibility."""
+        filter.  ``selected_camera`` is the camera each reference pixel is
+        matched against (depth_estimation.py:155-170), as the reference uses
+        the one it is given: the selection built at construction
+        (``selected_cameras[slot]``, or None) runs from its cached ray table;
+        any other is turned into a ray table for this call."""
         slot, r = self._reference_slot(reference_calibration)
         imgs = [_hwc(i) for i in images]
         g = guide.contiguous() if guide is not None else None
         out = torch.empty([self.rows, self.cols], device=self.device)
-        return self._estimate(slot, r, imgs, g, out)
+        lut = None
+        if selected_camera is not None and selected_camera is not self.selected_cameras[slot]:
+            lut, sel = self._selection_raylut(r, selected_camera)
+            self._call_selection = sel  # kept alive until the next call (the kernels may still read it)
+        self._call_raylut = lut
+        return self._estimate(slot, r, imgs, g, out, raylut=lut)
 
     def estimate_RGBD_panorama(self, images_to_match, images_to_stitch):
         """Per-frame entry point (depth_estimation.py:205-243).
