@@ -32,18 +32,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "sphere-stereo_amd"))
 
 HBM_PEAK_GBS = 8000.0
+
+from sphere_stereo.synth import WORKLOADS  # noqa: E402
 METRIC = "RGB-D panoramas/s (4×1024² fisheye, 32 cands) + % HBM peak; FP16 vs FP32"
-
-WORKLOADS = {
-    "c1": dict(matching=(1024, 1024), stitch=(1216, 1216), pano=(2048, 1024), N=32, refs=(0, 2), sigma_i=10.0,
-               sigma_s=25.0, resolution=1216, seed=0),
-    "c0": dict(matching=(512, 512), stitch=(1216, 1216), pano=(1024, 512), N=32, refs=(0, 2), sigma_i=10.0,
-               sigma_s=12.5, resolution=1216, seed=0),
-    # the precision form's configuration: 64 candidates, every camera a reference
-    "c2": dict(matching=(1024, 1024), stitch=(1216, 1216), pano=(2048, 1024), N=64, refs=(0, 1, 2, 3),
-               sigma_i=10.0, sigma_s=25.0, resolution=1216, seed=0),
-}
-
 
 def levels(H, W):
     return int(min(math.log2(W), math.log2(H)) - 1)
@@ -90,27 +81,15 @@ def setup(wl, device, frames_cache=None, build_estimator=True):
     import torch
 
     from sphere_stereo import RGBD_Estimator, synth
-    from sphere_stereo.utils import parse_json_calib
 
-    rig = synth.SyntheticRig(seed=wl["seed"], resolution=wl["resolution"])
+    rendered = None
     if frames_cache and os.path.exists(frames_cache):
         saved = torch.load(frames_cache, weights_only=True)
         rendered = (saved["images"].to(device), saved["masks"].to(device))
-    else:
-        rendered = rig.render(device)
-        if frames_cache:
-            torch.save({"images": rendered[0].cpu(), "masks": rendered[1].cpu()}, frames_cache)
+    calibs, kw, to_match, to_stitch, rendered = synth.workload_frame(wl, device, rendered)
+    if frames_cache and not os.path.exists(frames_cache):
+        torch.save({"images": rendered[0].cpu(), "masks": rendered[1].cpu()}, frames_cache)
     print("[bench] rig rendered", file=sys.stderr, flush=True)
-    calibs = parse_json_calib(rig.basalt_json()["value0"], list(wl["matching"]), device)
-    viewpoint = torch.zeros([3], device=device)
-    for r in wl["refs"]:
-        viewpoint += calibs[r].rt[:3, 3] / len(wl["refs"])
-    to_match, to_stitch, masks = synth.make_frame(rig, wl["matching"], wl["stitch"], wl["refs"], device=device,
-                                                  rendered=rendered)
-    kw = dict(min_dist=0.55, max_dist=100, candidate_count=wl["N"], references_indices=list(wl["refs"]),
-              reprojection_viewpoint=viewpoint, masks=masks, matching_resolution=wl["matching"],
-              rgb_to_stitch_resolution=wl["stitch"], panorama_resolution=wl["pano"], sigma_i=wl["sigma_i"],
-              sigma_s=wl["sigma_s"])
     est = RGBD_Estimator(calibs, device=device, **kw) if build_estimator else None
     return est, to_match, to_stitch, calibs, kw
 
@@ -197,31 +176,11 @@ def batching_ragged(args, device):
     chunk), and the single-executor form (one launch per pass for all rigs)
     in its persistent and dispatched schedules, timed against the same 16
     frames run one at a time in the plain fp32 form."""
-    import random
-
     import torch
 
     from sphere_stereo import FrameBatch, RGBD_Estimator, synth
-    from sphere_stereo.utils import parse_json_calib
 
-    rng = random.Random(3)
-    sets = []
-    for i in range(16):
-        res = rng.choice([512, 768, 1024])
-        N = rng.choice([16, 32, 48, 64])
-        refs = {2: [0, 2], 3: [0, 1, 2], 4: [0, 1, 2, 3]}[rng.choice([2, 3, 4])]
-        srig = synth.SyntheticRig(seed=100 + i, resolution=1216, baseline=rng.uniform(0.15, 0.40))
-        imgs, masks = srig.render(device)
-        calibs = parse_json_calib(srig.basalt_json()["value0"], [res, res], device)
-        viewpoint = torch.zeros([3], device=device)
-        for r in refs:
-            viewpoint += calibs[r].rt[:3, 3] / len(refs)
-        tm, ts, mk = synth.make_frame(srig, (res, res), (1216, 1216), refs, device=device, rendered=(imgs, masks))
-        kw = dict(min_dist=0.55, max_dist=100, candidate_count=N, references_indices=refs,
-                  reprojection_viewpoint=viewpoint, masks=mk, matching_resolution=(res, res),
-                  rgb_to_stitch_resolution=(1216, 1216), panorama_resolution=(1024, 512), sigma_i=10.0,
-                  sigma_s=12.5 * res / 512)
-        sets.append(([t.contiguous() for t in tm], [t.contiguous() for t in ts], calibs, kw, (res, N, len(refs))))
+    sets = [(tm, ts, calibs, kw, shape) for (calibs, kw, tm, ts, shape) in synth.ragged_rigs(device)]
     print("[bench] ragged batch rendered: %s" % [c for (_, _, _, _, c) in sets], file=sys.stderr, flush=True)
     frames = [(tm, ts) for (tm, ts, _, _, _) in sets]
     plain = [RGBD_Estimator(c, device=device, **kw) for (_, _, c, kw, _) in sets]

@@ -203,3 +203,73 @@ def make_frame(rig, matching_resolution, rgb_to_stitch_resolution, references, d
     to_stitch = [resize_bilinear(imgs[k], sc, sr).to(device) for k in references]
     mks = [(resize_bilinear(masks[k], mc, mr) / 255).unsqueeze(0).to(device) for k in range(masks.shape[0])]
     return to_match, to_stitch, mks
+
+
+# ---------------------------------------------------------------------------
+# Named workloads (sizes as the reference passes them: (cols, rows))
+# ---------------------------------------------------------------------------
+WORKLOADS = {
+    # 512^2 matching, 1024 x 512 panorama (the CPU-sized configuration)
+    "c0": dict(matching=(512, 512), stitch=(1216, 1216), pano=(1024, 512), N=32, refs=(0, 2), sigma_i=10.0,
+               sigma_s=12.5, resolution=1216, seed=0),
+    # main.py's defaults: 1024^2 matching, 2048 x 1024 panorama
+    "c1": dict(matching=(1024, 1024), stitch=(1216, 1216), pano=(2048, 1024), N=32, refs=(0, 2), sigma_i=10.0,
+               sigma_s=25.0, resolution=1216, seed=0),
+    # 64 candidates, every camera a reference
+    "c2": dict(matching=(1024, 1024), stitch=(1216, 1216), pano=(2048, 1024), N=64, refs=(0, 1, 2, 3),
+               sigma_i=10.0, sigma_s=25.0, resolution=1216, seed=0),
+}
+
+
+def _frame_of(rig, rendered, matching, stitch, pano, N, refs, sigma_i, sigma_s, device):
+    from .utils import parse_json_calib
+
+    calibs = parse_json_calib(rig.basalt_json()["value0"], list(matching), device)
+    viewpoint = torch.zeros([3], device=device)
+    for r in refs:
+        viewpoint += calibs[r].rt[:3, 3] / len(refs)
+    to_match, to_stitch, masks = make_frame(rig, matching, stitch, list(refs), device=device, rendered=rendered)
+    kw = dict(min_dist=0.55, max_dist=100, candidate_count=N, references_indices=list(refs),
+              reprojection_viewpoint=viewpoint, masks=masks, matching_resolution=tuple(matching),
+              rgb_to_stitch_resolution=tuple(stitch), panorama_resolution=tuple(pano), sigma_i=sigma_i,
+              sigma_s=sigma_s)
+    return calibs, kw, [t.contiguous() for t in to_match], [t.contiguous() for t in to_stitch]
+
+
+def workload_frame(wl, device, rendered=None):
+    """One frame of a workload (a WORKLOADS entry or a dict like one): the
+    seeded rig ray-cast on ``device`` (or ``rendered`` = (images, masks)
+    from an earlier render), its calibrations parsed at the matching
+    resolution, the viewpoint at the references' centre (main.py:67-70).
+    Returns (calibrations, RGBD_Estimator keyword arguments without
+    ``device``, images_to_match, images_to_stitch, rendered)."""
+    rig = SyntheticRig(seed=wl["seed"], resolution=wl["resolution"], baseline=wl.get("baseline", 0.26))
+    if rendered is None:
+        rendered = rig.render(device)
+    calibs, kw, tm, ts = _frame_of(rig, rendered, wl["matching"], wl["stitch"], wl["pano"], wl["N"], wl["refs"],
+                                   wl["sigma_i"], wl["sigma_s"], device)
+    return calibs, kw, tm, ts, rendered
+
+
+def ragged_rigs(device, count=16, seed=3):
+    """The ragged batching workload: ``count`` rigs, each a seeded scene
+    (seeds 100, 101, ...) with its own baseline drawn from U(0.15, 0.40) m,
+    matching resolution from {512, 768, 1024}^2, candidate count from {16,
+    32, 48, 64} and 2 to 4 reference cameras; 1216^2 stitch input, 1024 x 512
+    panorama, sigma_i 10, sigma_s 12.5 x (resolution / 512).  Returns one
+    (calibrations, keyword arguments, images_to_match, images_to_stitch,
+    (resolution, N, references)) per rig."""
+    import random
+
+    rng = random.Random(seed)
+    out = []
+    for i in range(count):
+        res = rng.choice([512, 768, 1024])
+        N = rng.choice([16, 32, 48, 64])
+        refs = {2: [0, 2], 3: [0, 1, 2], 4: [0, 1, 2, 3]}[rng.choice([2, 3, 4])]
+        rig = SyntheticRig(seed=100 + i, resolution=1216, baseline=rng.uniform(0.15, 0.40))
+        rendered = rig.render(device)
+        calibs, kw, tm, ts = _frame_of(rig, rendered, (res, res), (1216, 1216), (1024, 512), N, refs, 10.0,
+                                       12.5 * res / 512, device)
+        out.append((calibs, kw, tm, ts, (res, N, len(refs))))
+    return out

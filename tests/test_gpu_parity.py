@@ -29,6 +29,14 @@ def assert_same(a, b, what):
     assert msg is None, "%s: %s" % (what, msg)
 
 
+def assert_matches_oracle(calibs, kw, to_match, to_stitch, rgb, dist, what, **extra):
+    """A single-frame result against the oracle's run of the same frame (so
+    that a batched result compared with it is compared with the oracle)."""
+    want_rgb, want_dist = OraclePipeline(calibs, **kw, **extra).estimate_RGBD_panorama(to_match, to_stitch)
+    assert_same(to_np(dist), want_dist, "%s: distance panorama" % what)
+    assert_same(to_np(rgb), want_rgb, "%s: rgb panorama" % what)
+
+
 def test_guide_bit_exact(gpu):
     g = torch.Generator().manual_seed(3)
     img = (torch.rand([37, 53, 3], generator=g) * 255).to(gpu)
@@ -277,6 +285,31 @@ def test_estimate_distance_bit_exact(small):
         assert_same(to_np(got), want, "distance map ref %d" % r)
 
 
+def test_estimate_distance_with_given_selection(small, gpu):
+    """estimate_fisheye_distance matches each pixel against the camera the
+    caller's ``selected_camera`` names (depth_estimation.py:155-170), not the
+    selection made at construction: here a selection with every camera index
+    (the reference itself included), -1 and indices that name no camera."""
+    fr, est, ora = small
+    H, W = 72, 80
+    rng = np.random.RandomState(11)
+    for slot, r in enumerate(fr["refs"]):
+        base = ora.sel[r]
+        given = np.where(rng.rand(H, W) < 0.3, rng.randint(-1, 4, size=(H, W)), base).astype(np.int64)
+        given[rng.rand(H, W) < 0.05] = 9  # no camera of the rig
+        given[:5] = -7
+        assert (given != base).mean() > 0.2
+        want = ora.estimate_fisheye_distance(fr["to_match"], r, sel=given)
+        got = est.estimate_fisheye_distance(None, None, fr["calibs"][r],
+                                            torch.from_numpy(given).unsqueeze(0).to(gpu), fr["to_match"])
+        assert_same(to_np(got), want, "distance map ref %d with a given selection" % r)
+        # the selection made at construction takes the cached table, same result as the oracle's own
+        got0 = est.estimate_fisheye_distance(None, None, fr["calibs"][r], est.selected_cameras[slot],
+                                             fr["to_match"])
+        assert_same(to_np(got0), ora.estimate_fisheye_distance(fr["to_match"], r), "distance map ref %d" % r)
+        assert not np.array_equal(to_np(got0), want)
+
+
 def test_stitch_tables_bit_exact(small):
     fr, est, ora = small
     st = est.fishey_stitcher
@@ -369,6 +402,8 @@ def test_frame_batch_matches_single_frames(gpu, volume_dtype):
                   rgb_to_stitch_resolution=fr["stitch"], panorama_resolution=(128, 64), sigma_i=10, sigma_s=12.5)
         est = RGBD_Estimator(fr["calibs"], device=gpu, volume_dtype=volume_dtype, **kw)
         rgb, dist = est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])
+        assert_matches_oracle(fr["calibs"], kw, fr["to_match"], fr["to_stitch"], rgb, dist, "frame %d" % seed,
+                              volume_dtype=volume_dtype)
         want.append((rgb.clone(), dist.clone()))
         kws.append((fr["calibs"], kw))
         ests.append(est)
@@ -414,6 +449,8 @@ def test_frame_batch_ragged_matches_single_frames(gpu):
                   rgb_to_stitch_resolution=fr["stitch"], panorama_resolution=(128, 64), sigma_i=10, sigma_s=12.5)
         est = RGBD_Estimator(fr["calibs"], device=gpu, volume_dtype=vt, **kw)
         rgb, dist = est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])
+        assert_matches_oracle(fr["calibs"], kw, fr["to_match"], fr["to_stitch"], rgb, dist, "frame %d" % i,
+                              volume_dtype=vt)
         want.append((rgb.clone(), dist.clone()))
         ests.append(est)
         frames.append(([t.contiguous() for t in fr["to_match"]], [t.contiguous() for t in fr["to_stitch"]]))
@@ -466,6 +503,8 @@ def test_frame_batch_single_executor_matches_single_frames(gpu, vt, schedule):
             est._frame = est._make_frame()
         print("persistent case %d: %s N=%d refs=%s iters=%d %s" % (i, matching, N, refs, iters, vt), flush=True)
         rgb, dist = est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])
+        assert_matches_oracle(fr["calibs"], kw, fr["to_match"], fr["to_stitch"], rgb, dist, "case %d" % i,
+                              volume_dtype=vt, inpainting_iterations=iters)
         want.append((rgb.clone(), dist.clone()))
         ests.append(est)
         frames.append(([t.contiguous() for t in fr["to_match"]], [t.contiguous() for t in fr["to_stitch"]]))
@@ -506,6 +545,7 @@ def test_frame_batch_single_executor_many_pairs(gpu, schedule):
                   rgb_to_stitch_resolution=fr["stitch"], panorama_resolution=(64, 32), sigma_i=10, sigma_s=12.5)
         est = RGBD_Estimator(calibs, device=gpu, volume_dtype="float16", **kw)
         rgb, dist = est.estimate_RGBD_panorama(to_match, fr["to_stitch"])
+        assert_matches_oracle(calibs, kw, to_match, fr["to_stitch"], rgb, dist, "rig %d" % i, volume_dtype="float16")
         want.append((rgb.clone(), dist.clone()))
         ests.append(est)
         frames.append(([t.contiguous() for t in to_match], [t.contiguous() for t in fr["to_stitch"]]))
