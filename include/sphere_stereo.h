@@ -161,6 +161,20 @@ size_t ss_isb_workspace_bytes(int H, int W, int N, int vtype, int planar);
 int ss_isb_filter(uint32_t *d_guide, void *d_cost, int H, int W, int N, int vtype, double sigma_i, double sigma_s,
                   void *d_workspace, size_t workspace_bytes, ss_stream stream);
 
+/* ss_isb_filter without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * scratch (what ss_isb_workspace_bytes(H, W, N, vtype, 0) reports) comes from
+ * a static device array of SS_ISB_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends (e.g. 1024 x 1024 with 32 candidates needs 44 MiB, 768 x
+ * 768 with 48 needs 37 MiB); larger calls return SS_ERR_WORKSPACE.  The call
+ * allocates nothing.  Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (different streams, other threads) race.  Same
+ * results as ss_isb_filter. */
+#define SS_ISB_NOALLOC_SCRATCH_BYTES (48u << 20)
+int ss_isb_filter_noalloc(uint32_t *d_guide, void *d_cost, int H, int W, int N, int vtype, double sigma_i,
+                          double sigma_s, ss_stream stream);
+
 /* Drop-in ISB_Filter.apply on the reference layout: d_guide H x W x 3 uint8,
  * d_cost N x H x W float32, both filtered in place. */
 int ss_isb_filter_planar(uint8_t *d_guide, float *d_cost, int H, int W, int N, double sigma_i, double sigma_s,

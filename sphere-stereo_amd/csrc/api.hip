@@ -615,12 +615,18 @@ static int isb_filter_impl(uint32_t* d_guide, void* d_cost, int H, int W, int N,
                      SS_STAGE_ISB_UP);
 }
 
-static int check_isb(int H, int W, int N, int vt, void* ws, size_t bytes, int planar) {
+static int check_isb_args(int H, int W, int N, int vt) {
   SS_REQUIRE(H >= 1 && W >= 1, "bad resolution %dx%d", W, H);
   SS_REQUIRE(levels_of(H, W) >= 1, "resolution %dx%d gives no pyramid level (min side must be >= 4)", W, H);
   SS_REQUIRE(N >= 1 && padded(N) > 0, "candidate count %d unsupported (1..%d)", N, SS_MAX_CANDIDATES);
   SS_REQUIRE(valid_vtype(vt), "unknown volume type %d", vt);
   SS_REQUIRE(N > 1 || vt == SS_VOLUME_F32, "a one-candidate volume is filtered in fp32 only");
+  return SS_OK;
+}
+
+static int check_isb(int H, int W, int N, int vt, void* ws, size_t bytes, int planar) {
+  const int rc = check_isb_args(H, W, N, vt);
+  if (rc) return rc;
   const size_t need = ss_isb_workspace_bytes(H, W, N, vt, planar);
   if (!ws || bytes < need)
     return fail(SS_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need, ws ? bytes : (size_t)0);
@@ -633,6 +639,62 @@ int ss_isb_filter(uint32_t* d_guide, void* d_cost, int H, int W, int N, int vtyp
   int rc = check_isb(H, W, N, vtype, d_workspace, workspace_bytes, 0);
   if (rc) return rc;
   Carver cv(d_workspace);
+  rc = isb_filter_impl(d_guide, d_cost, H, W, N, vtype, sigma_i, sigma_s, cv, as_stream(stream));
+  return rc ? rc : ok();
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free ISB filter.  Its scratch (the call's guide-weight table
+// and pyramid levels 1..L-1, laid out as ss_isb_filter's workspace) is a
+// file-scope device array: every GPU the library runs on holds one copy until
+// the process ends, and every call on that GPU shares it, so calls that
+// overlap on one GPU race.  No part of it must start at zero: a call writes
+// every byte it reads (the weight table first, each level by the down pass
+// into it), so the last call's contents are harmless.  On the host the name
+// `scratch` is a stand-in, not a device address: its address on the current
+// device is looked up once per device (hipGetSymbolAddress) and only that
+// address reaches the kernels.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[SS_ISB_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch`, per device
+}  // namespace
+
+// The current device's copy of `scratch`, looked up on the first call there.
+static int scratch_on_current_device(void** out) {
+  int dev = -1;
+  SS_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(SS_ERR_ARG, "device %d outside [0, %d)", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return fail(SS_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return SS_OK;
+}
+
+int ss_isb_filter_noalloc(uint32_t* d_guide, void* d_cost, int H, int W, int N, int vtype, double sigma_i,
+                          double sigma_s, ss_stream stream) {
+  SS_REQUIRE(d_guide && d_cost, "null pointer argument");
+  int rc = check_isb_args(H, W, N, vtype);
+  if (rc) return rc;
+  const size_t need = ss_isb_workspace_bytes(H, W, N, vtype, 0);
+  if (need > (size_t)SS_ISB_NOALLOC_SCRATCH_BYTES)
+    return fail(SS_ERR_WORKSPACE,
+                "ss_isb_filter_noalloc: %dx%d with %d candidates needs %zu bytes of scratch, more than its static "
+                "%zu (use ss_isb_filter with a workspace)",
+                W, H, N, need, (size_t)SS_ISB_NOALLOC_SCRATCH_BYTES);
+  void* ws = nullptr;
+  rc = scratch_on_current_device(&ws);
+  if (rc) return rc;
+  Carver cv(ws);
   rc = isb_filter_impl(d_guide, d_cost, H, W, N, vtype, sigma_i, sigma_s, cv, as_stream(stream));
   return rc ? rc : ok();
 }

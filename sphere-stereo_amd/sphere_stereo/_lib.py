@@ -18,6 +18,7 @@ SS_INPUT_U8, SS_INPUT_U16, SS_INPUT_F32 = 0, 1, 2
 
 # ss_volume_type: element type of the stored cost volume
 SS_VOLUME_F32, SS_VOLUME_F16, SS_VOLUME_BF16 = 0, 1, 2
+SS_ISB_NOALLOC_SCRATCH_BYTES = 48 << 20  # static scratch of ss_isb_filter_noalloc, per GPU
 VOLUME_TYPES = {"float32": SS_VOLUME_F32, "float16": SS_VOLUME_F16, "bfloat16": SS_VOLUME_BF16}
 
 
@@ -72,6 +73,7 @@ SIGNATURES = {
     "ss_isb_workspace_bytes": (_z, [_i, _i, _i, _i, _i]),
     "ss_isb_filter": (_i, [_p, _p, _i, _i, _i, _i, _d, _d, _p, _z, _p]),
     "ss_isb_filter_planar": (_i, [_p, _p, _i, _i, _i, _d, _d, _p, _z, _p]),
+    "ss_isb_filter_noalloc": (_i, [_p, _p, _i, _i, _i, _i, _d, _d, _p]),
     "ss_argmin_refine": (_i, [_p, _i, _i, _i, _i, _fl, _fl, _fl, _p, _p]),
     "ss_estimate_workspace_bytes": (_z, [_i, _i, _i, _i]),
     "ss_estimate_distance": (_i, [_p, _p, _i, _CAM, _p, _i, _p, _i, _i, _i, _d, _d, _i, _p, _p, _p, _z, _p]),

@@ -212,3 +212,21 @@ def test_sweep_tuning_json_roundtrip(tmp_path):
             lib.ss_sweep_set_variant(H, W, N, vt, -1)
         de._TUNED.clear()
         de._TUNED.update(saved)
+
+
+def test_isb_noalloc_limit_and_refusal():
+    """ss_isb_filter_noalloc's static scratch: the header's limit is the one
+    the bindings state, the C1-sized filter fits it, and a call that needs
+    more is refused before any device work."""
+    text = open(HEADER).read()
+    m = re.search(r"#define SS_ISB_NOALLOC_SCRATCH_BYTES \((\d+)u << (\d+)\)", text)
+    assert m and int(m.group(1)) << int(m.group(2)) == _lib.SS_ISB_NOALLOC_SCRATCH_BYTES
+    lib = _lib.lib
+    assert lib.ss_isb_workspace_bytes(1024, 1024, 32, _lib.SS_VOLUME_F32, 0) <= _lib.SS_ISB_NOALLOC_SCRATCH_BYTES
+    need = lib.ss_isb_workspace_bytes(1024, 1024, 64, _lib.SS_VOLUME_F32, 0)
+    assert need > _lib.SS_ISB_NOALLOC_SCRATCH_BYTES
+    one = C.c_void_p(256)  # never dereferenced: the size check comes first
+    rc = lib.ss_isb_filter_noalloc(one, one, 1024, 1024, 64, _lib.SS_VOLUME_F32, 10.0, 25.0, None)
+    assert rc == 4 and b"static" in lib.ss_last_error()
+    rc = lib.ss_isb_filter_noalloc(one, one, 3, 3, 8, _lib.SS_VOLUME_F32, 10.0, 25.0, None)
+    assert rc == 1 and b"pyramid level" in lib.ss_last_error()
