@@ -16,8 +16,8 @@
 // transposed passes run as gathers (each output element owned by one lane,
 // no atomics, deterministic).  The sweep adjoint recomputes every sample
 // (projection, bilinear taps) from the rays, and returns the gradient with
-// respect to the images (scattered with float atomics onto the sampled
-// image), to each camera's double-sphere parameters (fs, cs, xi, alpha) and
+// respect to the images (summed per block in an LDS window of the sampled
+// image, then added with float atomics), to each camera's double-sphere parameters (fs, cs, xi, alpha) and
 // to each (camera <- reference) 3 x 4 transform; the transform gradients
 // chain to 6-DoF extrinsics on the host.  Per-block partial sums are reduced
 // in a fixed order, so the parameter gradients are deterministic too.
@@ -459,11 +459,71 @@ __device__ void unproject_dual(const TorchCam& c, float u, float v, D6 p[3]) {
   p[2] = k * mz - xi;
 }
 
+// Image gradients of a block's samples are first summed in an LDS window of
+// the sampled image: per chunk of kBwdChunk candidates, the bounding box of
+// the taps of the block's lanes that sample its window camera (the smallest
+// camera index among them) is found by a projection-only pass; when it holds
+// at most kWinTexels texels, the chunk's taps are added there (LDS atomics)
+// and the window is then added to the image with one global atomic per
+// touched texel and channel.  The samples of 16 consecutive candidates of a
+// 16 x 16 pixel tile sweep a patch of ~1000 texels (median box ~1700 texels
+// at 768^2, 48 candidates; 95 % of the boxes fit), against ~16000 tap
+// updates: ~16x fewer global atomics.  (Tiles of 256 pixels of one row map
+// to tilted curves whose boxes mostly do not fit.)  Other lanes' taps, and
+// chunks whose box does not fit, use global atomics directly.
+constexpr int kBwdChunk = 16;
+constexpr int kBwdTile = 16;      // kBwdTile^2 == kBlock pixels per workgroup
+constexpr int kWinTexels = 3072;  // 36 KiB of LDS: four workgroups per CU
+static_assert(kBwdTile * kBwdTile == kBlock, "one lane per pixel of a tile");
+
+// The sample of candidate n of a pixel with ray rl in camera (cam, rt):
+// grid_sample coordinates (forward's expression order) and whether its taps
+// can touch the image.
+__device__ __forceinline__ bool bwd_sample_coords(const SweepBwdArgs& a, const TorchCam& cam, const float* rt,
+                                                  const float4& rl, int n, float& Px, float& Py, float& Pz,
+                                                  float& qx, float& qy, float& qz, float& ix, float& iy) {
+  const float d = a.cands[n];
+  Px = d * rl.x;
+  Py = d * rl.y;
+  Pz = d * rl.z;
+  rigid(rt, Px, Py, Pz, qx, qy, qz);
+  float u, v;
+  bool valid;
+  if (!a.fastproj || !project_t_fast(cam, qx, qy, qz, u, v)) project_t(cam, qx, qy, qz, u, v, valid);
+  if (a.pow2) {
+    ix = grid_coord_pow2(u, a.inv2W, a.halfW);
+    iy = grid_coord_pow2(v, a.inv2H, a.halfH);
+  } else {
+    ix = grid_coord2<false>(u, a.Wf, a.invW);
+    iy = grid_coord2<false>(v, a.Hf, a.invH);
+  }
+  const float x0f = floorf(ix), y0f = floorf(iy);
+  return x0f >= -1.0f && x0f <= (float)(a.W - 1) && y0f >= -1.0f && y0f <= (float)(a.H - 1);
+}
+
+__device__ __forceinline__ int block_min(int v, int* slot) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMin(slot, v);
+  return v;
+}
+__device__ __forceinline__ int block_max(int v, int* slot) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(slot, v);
+  return v;
+}
+
 __global__ void __launch_bounds__(kBlock) k_sweep_bwd(SweepBwdArgs a) {
-  const int P = a.H * a.W;
-  const int p0 = blockIdx.x * kBlock + threadIdx.x;
-  const bool active = p0 < P;
-  const int p = active ? p0 : P - 1;
+  __shared__ float win[kWinTexels * 3];
+  __shared__ int wbox[5];  // window camera, min x, max x, min y, max y
+  // a workgroup owns a kBwdTile x kBwdTile tile of reference pixels: its
+  // samples of one candidate fall on a compact patch of the sampled image
+  const int tiles_x = (a.W + kBwdTile - 1) / kBwdTile;
+  const int ty = (int)blockIdx.x / tiles_x, tx = (int)blockIdx.x - ty * tiles_x;
+  const int px = tx * kBwdTile + (int)(threadIdx.x % kBwdTile), py = ty * kBwdTile + (int)(threadIdx.x / kBwdTile);
+  const bool active = px < a.W && py < a.H;
+  const int p = active ? py * a.W + px : 0;
   const float4 rl = a.raylut[p];
   const int k = active && (int)rl.w < a.C ? (int)rl.w : -1;
   const float* rimg = a.img[a.ref] + 3 * (size_t)p;
@@ -472,53 +532,82 @@ __global__ void __launch_bounds__(kBlock) k_sweep_bwd(SweepBwdArgs a) {
 #pragma unroll
   for (int i = 0; i < kRec; ++i) acc[i] = 0.0f;
   float gray[3] = {0.0f, 0.0f, 0.0f}, gref[3] = {0.0f, 0.0f, 0.0f};
-  if (active) {
-    const int kk = k < 0 ? 0 : k;
-    const TorchCam cam = a.cam[kk];
-    float rt[12];
+  const int kk = k < 0 ? 0 : k;
+  const TorchCam cam = a.cam[kk];
+  float rt[12];
 #pragma unroll
-    for (int i = 0; i < 12; ++i) rt[i] = a.rt[kk][i];
-    const float* img = a.img[kk];
-    float* gimg = k >= 0 ? a.gimg[kk] : nullptr;
-    const float* gc = a.gcost + (size_t)p * a.NP;
-    for (int n = 0; n < a.N; ++n) {
+  for (int i = 0; i < 12; ++i) rt[i] = a.rt[kk][i];
+  const float* img = a.img[kk];
+  float* gimg = k >= 0 ? a.gimg[kk] : nullptr;
+  const float* gc = a.gcost + (size_t)p * a.NP;
+
+  // the block's window camera (block-uniform; -1: none)
+  if (threadIdx.x == 0) wbox[0] = 0x7fffffff;
+  __syncthreads();
+  block_min(k >= 0 && gimg ? k : 0x7fffffff, &wbox[0]);
+  __syncthreads();
+  const int kw = wbox[0] == 0x7fffffff ? -1 : wbox[0];
+  const bool inwin = kw >= 0 && k == kw;  // this lane's taps may go to the window
+  float* const gwin = kw >= 0 ? a.gimg[kw] : nullptr;
+
+  for (int n0 = 0; n0 < a.N; n0 += kBwdChunk) {
+    const int n1 = min(n0 + kBwdChunk, a.N);
+    // bounding box of the chunk's in-image taps in the window camera
+    int bx0 = 0x7fffffff, bx1 = -1, by0 = 0x7fffffff, by1 = -1;
+    if (inwin) {
+      for (int n = n0; n < n1; ++n) {
+        float Px, Py, Pz, qx, qy, qz, ix, iy;
+        if (!bwd_sample_coords(a, cam, rt, rl, n, Px, Py, Pz, qx, qy, qz, ix, iy)) continue;
+        const int x0 = (int)floorf(ix), y0 = (int)floorf(iy);
+        bx0 = min(bx0, max(x0, 0));
+        bx1 = max(bx1, min(x0 + 1, a.W - 1));
+        by0 = min(by0, max(y0, 0));
+        by1 = max(by1, min(y0 + 1, a.H - 1));
+      }
+    }
+    if (threadIdx.x == 0) {
+      wbox[1] = 0x7fffffff;
+      wbox[2] = -1;
+      wbox[3] = 0x7fffffff;
+      wbox[4] = -1;
+    }
+    __syncthreads();
+    block_min(bx0, &wbox[1]);
+    block_max(bx1, &wbox[2]);
+    block_min(by0, &wbox[3]);
+    block_max(by1, &wbox[4]);
+    __syncthreads();
+    const int wx0 = wbox[1], wy0 = wbox[3];
+    const int ww = wbox[2] - wx0 + 1, wh = wbox[4] - wy0 + 1;
+    const bool use_win = kw >= 0 && wbox[2] >= 0 && wbox[4] >= 0 && ww * wh <= kWinTexels;  // block-uniform
+    if (use_win) {
+      for (int i = (int)threadIdx.x; i < ww * wh * 3; i += kBlock) win[i] = 0.0f;
+      __syncthreads();
+    }
+
+    for (int n = n0; n < n1 && active; ++n) {
       const float g = gc[n];
       float s[3] = {0.0f, 0.0f, 0.0f}, sx[3] = {0.0f, 0.0f, 0.0f}, sy[3] = {0.0f, 0.0f, 0.0f};
       float Px = 0.0f, Py = 0.0f, Pz = 0.0f, qx = 0.0f, qy = 0.0f, qz = 0.0f;
-      float ix = 0.0f, iy = 0.0f, wx0 = 0.0f, wx1 = 0.0f, wy0 = 0.0f, wy1 = 0.0f;
+      float ix = 0.0f, iy = 0.0f, wx0f = 0.0f, wx1f = 0.0f, wy0f = 0.0f, wy1f = 0.0f;
       int xi0 = 0, yi0 = 0;
       bool tap[4] = {false, false, false, false};
       if (k >= 0) {
-        const float d = a.cands[n];
-        Px = d * rl.x;
-        Py = d * rl.y;
-        Pz = d * rl.z;
-        rigid(rt, Px, Py, Pz, qx, qy, qz);
-        float u, v;
-        bool valid;
-        if (!a.fastproj || !project_t_fast(cam, qx, qy, qz, u, v)) project_t(cam, qx, qy, qz, u, v, valid);
-        if (a.pow2) {
-          ix = grid_coord_pow2(u, a.inv2W, a.halfW);
-          iy = grid_coord_pow2(v, a.inv2H, a.halfH);
-        } else {
-          ix = grid_coord2<false>(u, a.Wf, a.invW);
-          iy = grid_coord2<false>(v, a.Hf, a.invH);
-        }
         // zero-padded bilinear sample (the forward's taps and order) and its
         // derivatives in ix, iy
-        const float x0f = floorf(ix), y0f = floorf(iy);
-        if (x0f >= -1.0f && x0f <= (float)(a.W - 1) && y0f >= -1.0f && y0f <= (float)(a.H - 1)) {
+        if (bwd_sample_coords(a, cam, rt, rl, n, Px, Py, Pz, qx, qy, qz, ix, iy)) {
+          const float x0f = floorf(ix), y0f = floorf(iy);
           xi0 = (int)x0f;
           yi0 = (int)y0f;
-          wx1 = ix - x0f;
-          wx0 = (x0f + 1.0f) - ix;
-          wy1 = iy - y0f;
-          wy0 = (y0f + 1.0f) - iy;
-          const bool bx0 = xi0 >= 0, bx1 = xi0 + 1 < a.W, by0 = yi0 >= 0, by1 = yi0 + 1 < a.H;
-          tap[0] = bx0 && by0;
-          tap[1] = bx1 && by0;
-          tap[2] = bx0 && by1;
-          tap[3] = bx1 && by1;
+          wx1f = ix - x0f;
+          wx0f = (x0f + 1.0f) - ix;
+          wy1f = iy - y0f;
+          wy0f = (y0f + 1.0f) - iy;
+          const bool bx0t = xi0 >= 0, bx1t = xi0 + 1 < a.W, by0t = yi0 >= 0, by1t = yi0 + 1 < a.H;
+          tap[0] = bx0t && by0t;
+          tap[1] = bx1t && by0t;
+          tap[2] = bx0t && by1t;
+          tap[3] = bx1t && by1t;
           float vv[4][3];
 #pragma unroll
           for (int t = 0; t < 4; ++t) {
@@ -526,7 +615,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep_bwd(SweepBwdArgs a) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) vv[t][c] = tap[t] ? img[((size_t)ty * a.W + tx) * 3 + c] : 0.0f;
           }
-          const float wnw = wx0 * wy0, wne = wx1 * wy0, wsw = wx0 * wy1, wse = wx1 * wy1;
+          const float wnw = wx0f * wy0f, wne = wx1f * wy0f, wsw = wx0f * wy1f, wse = wx1f * wy1f;
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
             float tt = vv[0][c] * wnw;
@@ -534,8 +623,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep_bwd(SweepBwdArgs a) {
             tt = tt + vv[2][c] * wsw;
             tt = tt + vv[3][c] * wse;
             s[c] = tt;
-            sx[c] = wy0 * (vv[1][c] - vv[0][c]) + wy1 * (vv[3][c] - vv[2][c]);
-            sy[c] = wx0 * (vv[2][c] - vv[0][c]) + wx1 * (vv[3][c] - vv[1][c]);
+            sx[c] = wy0f * (vv[1][c] - vv[0][c]) + wy1f * (vv[3][c] - vv[2][c]);
+            sy[c] = wx0f * (vv[2][c] - vv[0][c]) + wx1f * (vv[3][c] - vv[1][c]);
           }
         }
       }
@@ -551,13 +640,21 @@ __global__ void __launch_bounds__(kBlock) k_sweep_bwd(SweepBwdArgs a) {
       }
       if (k < 0 || gS == 0.0f) continue;
       if (gimg) {
-        const float wt[4] = {wx0 * wy0, wx1 * wy0, wx0 * wy1, wx1 * wy1};
+        const float wt[4] = {wx0f * wy0f, wx1f * wy0f, wx0f * wy1f, wx1f * wy1f};
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           if (!tap[t]) continue;
-          float* dst = gimg + ((size_t)(yi0 + (t >> 1)) * a.W + xi0 + (t & 1)) * 3;
+          const int tx = xi0 + (t & 1), ty = yi0 + (t >> 1);
+          const int lx = tx - wx0, ly = ty - wy0;
+          if (use_win && inwin && lx >= 0 && lx < ww && ly >= 0 && ly < wh) {
+            float* dst = win + (ly * ww + lx) * 3;
 #pragma unroll
-          for (int c = 0; c < 3; ++c) unsafeAtomicAdd(dst + c, wt[t] * gs[c]);
+            for (int c = 0; c < 3; ++c) atomicAdd(dst + c, wt[t] * gs[c]);
+          } else {
+            float* dst = gimg + ((size_t)ty * a.W + tx) * 3;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) unsafeAtomicAdd(dst + c, wt[t] * gs[c]);
+          }
         }
       }
       // the grid_sample coordinate round trip is the identity in u: dix/du = 1
@@ -604,6 +701,20 @@ __global__ void __launch_bounds__(kBlock) k_sweep_bwd(SweepBwdArgs a) {
 #pragma unroll
       for (int i = 0; i < 3; ++i) gray[i] += d * ((rt[i] * gq[0] + rt[4 + i] * gq[1]) + rt[8 + i] * gq[2]);
     }
+
+    if (use_win) {
+      // the window into the image: one global atomic per touched texel and channel
+      __syncthreads();
+      for (int i = (int)threadIdx.x; i < ww * wh * 3; i += kBlock) {
+        const float v = win[i];
+        if (v != 0.0f) {
+          const int t = i / 3, c = i - 3 * t;
+          const int ly = t / ww, lx = t - ly * ww;
+          unsafeAtomicAdd(gwin + ((size_t)(wy0 + ly) * a.W + wx0 + lx) * 3 + c, v);
+        }
+      }
+    }
+    __syncthreads();  // the window and its box are reused by the next chunk
   }
   if (active && a.gimg[a.ref]) {
     float* dst = a.gimg[a.ref] + 3 * (size_t)p;
@@ -859,6 +970,10 @@ __global__ void __launch_bounds__(64) k_gn_solve(const double* sum, double lambd
 }
 
 inline int blocks_of(long long n) { return (int)((n + kBlock - 1) / kBlock); }
+// k_sweep_bwd's grid: one workgroup per kBwdTile x kBwdTile tile
+inline int sweep_bwd_blocks(int H, int W) {
+  return ((H + kBwdTile - 1) / kBwdTile) * ((W + kBwdTile - 1) / kBwdTile);
+}
 
 int levels_of(int H, int W) {
   if (H < 1 || W < 1) return -1;
@@ -1022,7 +1137,7 @@ int ss_isb_filter_backward(const uint32_t* d_guide, const float* d_grad_out, flo
 
 size_t ss_sweep_backward_workspace_bytes(int H, int W, int C) {
   if (H < 1 || W < 1 || C < 1 || C > SS_MAX_CAMERAS || (long long)H * W >= (1ll << 31)) return 0;
-  return (size_t)blocks_of((long long)H * W) * (C + 1) * kRec * sizeof(float);
+  return (size_t)sweep_bwd_blocks(H, W) * (C + 1) * kRec * sizeof(float);
 }
 
 int ss_sweep_backward(const float* d_raylut, const float* const* d_images, int C, const ss_camera* cams,
@@ -1080,7 +1195,7 @@ int ss_sweep_backward(const float* d_raylut, const float* const* d_images, int C
     if (!in(a.cands[n], 0.0f, big)) fast = false;
   a.fastproj = fast;
   hipStream_t s = as_stream(stream);
-  const int nb = blocks_of((long long)H * W);
+  const int nb = sweep_bwd_blocks(H, W);
   hipLaunchKernelGGL(k_sweep_bwd, dim3(nb), dim3(kBlock), 0, s, a);
   const int width = (C + 1) * kRec;
   hipLaunchKernelGGL(k_sum_records<float>, dim3(width), dim3(kBlock), 0, s, a.partial, nb, width, d_grad_params);
