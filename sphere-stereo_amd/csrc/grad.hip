@@ -475,6 +475,7 @@ constexpr int kBwdChunk = 16;
 constexpr int kBwdTile = 16;      // kBwdTile^2 == kBlock pixels per workgroup
 constexpr int kWinTexels = 3072;  // 36 KiB of LDS: four workgroups per CU
 static_assert(kBwdTile * kBwdTile == kBlock, "one lane per pixel of a tile");
+static_assert(kBwdChunk % 4 == 0, "chunks start at a multiple of 4 candidates");
 
 // The sample of candidate n of a pixel with ray rl in camera (cam, rt):
 // grid_sample coordinates (forward's expression order) and whether its taps
@@ -585,8 +586,12 @@ __global__ void __launch_bounds__(kBlock) k_sweep_bwd(SweepBwdArgs a) {
       __syncthreads();
     }
 
+    float4 g4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     for (int n = n0; n < n1 && active; ++n) {
-      const float g = gc[n];
+      // dL/dcost four candidates at a time (rows are NP >= 8 floats, 32-B aligned)
+      const int nq = n & 3;
+      if (nq == 0) g4 = ld4(gc + n);
+      const float g = nq == 0 ? g4.x : (nq == 1 ? g4.y : (nq == 2 ? g4.z : g4.w));
       float s[3] = {0.0f, 0.0f, 0.0f}, sx[3] = {0.0f, 0.0f, 0.0f}, sy[3] = {0.0f, 0.0f, 0.0f};
       float Px = 0.0f, Py = 0.0f, Pz = 0.0f, qx = 0.0f, qy = 0.0f, qz = 0.0f;
       float ix = 0.0f, iy = 0.0f, wx0f = 0.0f, wx1f = 0.0f, wy0f = 0.0f, wy1f = 0.0f;

@@ -146,3 +146,32 @@ def test_torch_reference_gradient_matches_central_differences():
             fd = (L(p["cams6"], rp) - L(p["cams6"], rm)) / (2 * h)
             ad = float(rel.grad[kk][i])
         assert abs(ad - fd) <= 1e-5 * max(abs(fd), 1.0), (kind, kk, i, ad, fd)
+
+
+def test_resized_principal_point_follows_the_resize():
+    """camera_parameters(pixel_centres="resized"): a point at original pixel
+    coordinate c appears at (c + 0.5) * s - 0.5 after the INTER_LINEAR resize
+    of read_input_images (utils.py:198-201), which the oracle's resize
+    restates: a ramp f(x) = x resized from 1216 to 768 columns reads c there
+    (exact for a linear ramp away from the clamped borders); the reference's
+    c * s is 0.5 * (1 - s) px off."""
+    from sphere_stereo import synth
+    from sphere_stereo.utils import parse_json_calib
+
+    rig = synth.SyntheticRig(seed=0, resolution=1216)
+    calibs = parse_json_calib(rig.basalt_json()["value0"], [768, 768], "cpu")
+    exact = camera_parameters(calibs, "resized")
+    ref = camera_parameters(calibs)
+    s = 768 / 1216
+    assert torch.allclose(exact[:, 2:4], ref[:, 2:4] + 0.5 * s - 0.5, atol=1e-4)
+    assert torch.equal(exact[:, :2], ref[:, :2]) and torch.equal(exact[:, 4:], ref[:, 4:])
+    ramp = np.tile(np.arange(1216, dtype=np.float32), (3, 1))
+    out = O32.resize_linear(ramp, 768, 3)[1].astype(np.float64)
+
+    def at(x):  # the resized ramp read at resized coordinate x
+        x0 = int(np.floor(x))
+        return out[x0] + (x - x0) * (out[x0 + 1] - out[x0])
+
+    for c in (100.0, 608.0, 1000.25):
+        assert abs(at((c + 0.5) * s - 0.5) - c) < 1e-3, c
+        assert abs(at(c * s) - c) > 0.25, c  # 0.5 * (1 / s - 1) = 0.29 original pixels off

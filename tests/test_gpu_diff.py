@@ -226,13 +226,13 @@ def test_full_gradient_matches_torch_and_central_differences(gpu):
 def test_calibration_refinement_reduces_the_error(gpu):
     """Gauss-Newton on the photometric reprojection, at the differentiation
     form's size (4 frames of one rig, 768^2 matching, 48 candidates): camera 1
-    perturbed by 0.5 degrees of rotation, 5 mm of translation and 1 % of fx.
-    Rotation and focal length come back to within a tenth of their
-    perturbation.  A few millimetres of translation are barely observable at
-    this baseline (5 mm moves a point at 3 m by ~0.4 px, less than the depth
-    quantisation moves it): started from the true calibration the same
-    refinement settles ~3 mm away, so translation is only required not to
-    drift beyond its starting error."""
+    perturbed by 0.5 degrees of rotation, 5 mm of translation and 1 % of fx
+    ends within 0.01 degrees, 1.5 mm and 0.1 % of the true calibration, and
+    started from the truth it stays within 1 mm and 0.01 degrees.  The true
+    calibration is the one whose principal point follows the resize of the
+    rendered 1216^2 images to 768^2 (pixel_centres = "resized"); with the
+    reference's principal * scale, 0.18 px off, the refinement settled ~3 mm
+    and 0.02 degrees away from that starting point."""
     from sphere_stereo import synth
     from sphere_stereo.utils import parse_json_calib
 
@@ -263,7 +263,11 @@ def test_calibration_refinement_reduces_the_error(gpu):
     e0, e1 = errors(cams0, rts0), errors(cams, rts)
     print("before", e0, "after", e1, "loss", hist[0]["loss"], hist[-1]["loss"], flush=True)
     assert hist[-1]["loss"] < 0.5 * hist[0]["loss"]
-    assert e1[0] < 0.1 * e0[0] and e1[2] < 0.1 * e0[2] and e1[1] < e0[1], (e0, e1)
+    assert e1[0] < 0.01 and e1[1] < 1.5 and e1[2] < 1e-3, (e0, e1)
+    cams_t, rts_t, _ = rig.refine_calibration(frames, 1, steps=50, cams6=cams_true, rts=rts_true)
+    et = errors(cams_t, rts_t)
+    print("from the truth", et, flush=True)
+    assert et[0] < 0.01 and et[1] < 1.0 and et[2] < 1e-3, et
 
 
 def test_full_size_gradient_against_central_differences(gpu):
@@ -271,12 +275,16 @@ def test_full_size_gradient_against_central_differences(gpu):
     48 candidates, reference 0 of a synthetic frame, soft-argmin readout,
     L = sum_p w_p / d_p with w ~ U(0, 1)) against fp64 central differences
     of the oracle's forward (sweep, ISB filter, soft-argmin) with the same
-    camera selection, on the four camera parameters L depends on most.
-    Tolerances: 10 % of the difference quotient at h = 1e-4 (the step of the
-    method's specification: at this size a step of 1e-4 in alpha or xi
-    already moves a fraction of the 28 M samples across bilinear cells and
-    the kinks of |.|, measured 0.4-6.6 %), 1 % at h = 1e-6, where the
-    quotient converges to the GPU value (measured <= 0.33 %)."""
+    camera selection, on the six camera parameters with the largest
+    gradients.  At h = 1e-6 every one agrees within 2 % (+ 0.2 % of the
+    largest gradient; measured <= 1.3 % for the five largest).  At h = 1e-4,
+    the step of the method's specification, the quotient of this
+    piecewise-smooth loss (bilinear cells, the kinks of |.| and of the clamp
+    at 500) has not converged for every parameter: a step of 1e-4 in the
+    reference camera's alpha moves every one of the 28 M samples, and its
+    quotient reads +5.8e4 against -1.47e5 at h = 1e-6 (GPU: -1.45e5).  The
+    three largest gradients are checked at h = 1e-4 within 10 % (measured
+    <= 0.7 %)."""
     from sphere_stereo import synth
     from sphere_stereo.utils import distance_candidates, parse_json_calib
 
@@ -307,13 +315,19 @@ def test_full_size_gradient_against_central_differences(gpu):
         return float((O64.soft_argmin(Fv, rig.beta, rig.inv0, rig.inv1) * wnp).sum())
 
     names = ["fs0", "fs1", "cs0", "cs1", "xi", "alpha"]
-    for flat in np.argsort(-np.abs(g).ravel())[:4]:
+    rows = []
+    for flat in np.argsort(-np.abs(g).ravel())[:6]:
         k, i = divmod(int(flat), 6)
-        for h, tol in ((1e-4, 0.10), (1e-6, 0.01)):
+        fds = []
+        for h in (1e-4, 1e-6):
             cp, cm = rig.cams6.numpy().copy(), rig.cams6.numpy().copy()
             cp[k, i] += h
             cm[k, i] -= h
-            fd = (L(cp) - L(cm)) / (2 * h)
-            rel_err = abs(g[k, i] - fd) / abs(fd)
-            print("d/d %s[%d] h=%g: gpu %.6g fd %.6g rel %.2e" % (names[i], k, h, g[k, i], fd, rel_err), flush=True)
-            assert rel_err <= tol, (names[i], k, h, g[k, i], fd)
+            fds.append((L(cp) - L(cm)) / (2 * h))
+        rows.append((names[i], k, g[k, i], fds[0], fds[1]))
+        print("d/d %s[%d]: gpu %.6g fd(1e-4) %.6g fd(1e-6) %.6g" % rows[-1], flush=True)
+    gmax = np.abs(g).max()
+    for j, (name, k, gk, fd4, fd6) in enumerate(rows):
+        assert abs(gk - fd6) <= 0.02 * abs(fd6) + 0.002 * gmax, (name, k, gk, fd6)
+        if j < 3:
+            assert abs(gk - fd4) <= 0.10 * abs(fd4), (name, k, gk, fd4)
