@@ -138,8 +138,9 @@ void up_weights(double sigma_s, int k, double* wu, double* wd) {
 
 // The coarse end of a pyramid runs in one workgroup from level s0 on: every
 // level it produces holds at most kTailPixels pixels (one CU then does each
-// pass in about a microsecond) and all of them (guide + NP values per pixel)
-// fit in the LDS budget.  The tail starts at level 1 or deeper (its levels
+// pass in about a microsecond; from 4096 pixels on, a pass is VALU-bound on
+// that CU and costs more than the launches it replaces) and all of them
+// (guide + NP values per pixel, and the weight table) fit in the LDS budget.  The tail starts at level 1 or deeper (its levels
 // are all fp32, and a fused level-0 pass stays a regular launch).  Returns 0
 // when the pyramid has no such tail.
 int tail_start(const PyrLevel* l0, int L, int NP) {
@@ -182,6 +183,8 @@ unsigned tail_levels(const PyrLevel* l0, int s0, int L, int NP, const float* wta
     t->wu[e] = (float)wu;
     t->wd[e] = (float)wd;
   }
+  t->wtab_off = off;
+  off += (unsigned)(kWeightTable * sizeof(float));
   return off;
 }
 

@@ -55,19 +55,25 @@ __device__ __forceinline__ void down_pixel(const IsbDownArgs& a, int pix, bool a
   const int ctr = y2 * a.win + x2;
   const uint32_t gc = a.gin[ctr];
 
-  auto weight = [&](int j) __attribute__((always_inline)) -> float {
+  // neighbour j's guide and weight factor (isb_filter.cu:80-93)
+  auto nguide = [&](int j) __attribute__((always_inline)) -> uint32_t {
     const int row = pick3((j == 0 || j == 6 || j == 7) ? 0 : ((j == 1 || j == 5) ? 1 : 2), ym, y2, yp);
     const int col = pick3((j <= 2) ? 0 : ((j == 3 || j == 7) ? 1 : 2), xm, x2, xp);
+    return a.gin[row * a.win + col];
+  };
+  auto weight = [&](int j, uint32_t gn) __attribute__((always_inline)) -> float {
     const float base = (j & 1) ? 0.5f : 0.25f;
-    return base * (a.wtab[guide_sad_u(gc, a.gin[row * a.win + col])] + 0.01f);
+    return base * (a.wtab[guide_sad_u(gc, gn)] + 0.01f);
   };
 
   float w[8];
+  uint32_t gl = 0;  // G > 1: lane j's neighbour guide (j = lane & 7)
   if constexpr (G == 1) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) w[j] = weight(j);
+    for (int j = 0; j < 8; ++j) w[j] = weight(j, nguide(j));
   } else {
-    const float wl = weight(lane & 7);
+    gl = nguide(lane & 7);
+    const float wl = weight(lane & 7, gl);
 #pragma unroll
     for (int j = 0; j < 8; ++j) w[j] = __shfl(wl, j, G);
   }
@@ -75,19 +81,7 @@ __device__ __forceinline__ void down_pixel(const IsbDownArgs& a, int pix, bool a
 #pragma unroll
   for (int j = 0; j < 8; ++j) ws = ws + w[j];
 
-  if (lane == 0 && active) {
-    float s0 = gch(gc, 0), s1 = gch(gc, 1), s2 = gch(gc, 2);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const uint32_t gn = a.gin[nb[j]];
-      s0 = s0 + gch(gn, 0) * w[j];
-      s1 = s1 + gch(gn, 1) * w[j];
-      s2 = s2 + gch(gn, 2) * w[j];
-    }
-    const float inv = 1.0f / ws;
-    a.gout[pix] = pack_guide(s0 * inv, s1 * inv, s2 * inv);
-  }
-
+  // the cost pass first: its loads do not wait behind the guide store
   const size_t base_c = (size_t)lane * V;
 #pragma unroll
   for (int c0 = 0; c0 < V; c0 += CH) {
@@ -102,6 +96,32 @@ __device__ __forceinline__ void down_pixel(const IsbDownArgs& a, int pix, bool a
 #pragma unroll
     for (int v = 0; v < CH; ++v) acc[v] = acc[v] / ws;
     if (active) store_n<CH>(cout + (size_t)pix * NP + base_c + c0, acc);
+  }
+
+  // guide: each channel's weighted sum in the reference's tap order
+  const float inv = 1.0f / ws;
+  if constexpr (G == 1) {
+    if (active) {
+      float s0 = gch(gc, 0), s1 = gch(gc, 1), s2 = gch(gc, 2);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint32_t gn = a.gin[nb[j]];
+        s0 = s0 + gch(gn, 0) * w[j];
+        s1 = s1 + gch(gn, 1) * w[j];
+        s2 = s2 + gch(gn, 2) * w[j];
+      }
+      a.gout[pix] = pack_guide(s0 * inv, s1 * inv, s2 * inv);
+    }
+  } else {
+    // lane c of the group sums channel c (lanes 3..7 repeat channel 0); the
+    // neighbour guides come from the lanes that loaded them for the weights
+    const int ch = lane < 3 ? lane : 0;
+    float sc = gch(gc, ch);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sc = sc + gch(__shfl(gl, j, G), ch) * w[j];
+    const uint32_t byte = (uint32_t)(uint8_t)(sc * inv);
+    const uint32_t b1 = __shfl(byte, 1, G), b2 = __shfl(byte, 2, G);
+    if (lane == 0 && active) a.gout[pix] = byte | (b1 << 8) | (b2 << 16);
   }
 }
 
@@ -258,28 +278,6 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
     inv[s] = 1.0f / ws;
   }
 
-  if (!FUSE_ARGMIN && lane == 0 && active) {
-    uint32_t gD[9];
-#pragma unroll
-    for (int t = 0; t < 9; ++t) gD[t] = a.gin[tap[t]];
-    uint32_t gout[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      float s0 = gch(gF[s], 0) * a.weight_up, s1 = gch(gF[s], 1) * a.weight_up, s2 = gch(gF[s], 2) * a.weight_up;
-#pragma unroll
-      for (int k = kStencilBegin[s]; k < kStencilBegin[s + 1]; ++k) {
-        const uint32_t g = gD[up_tap_flat(k)];
-        s0 = s0 + gch(g, 0) * w[k];
-        s1 = s1 + gch(g, 1) * w[k];
-        s2 = s2 + gch(g, 2) * w[k];
-      }
-      gout[s] = pack_guide(s0 * inv[s], s1 * inv[s], s2 * inv[s]);
-    }
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-      if (doit[s]) a.gio[fine[s]] = gout[s];
-  }
-
   const size_t base_c = (size_t)lane * V;
   float res[FUSE_ARGMIN ? 4 : 1][FUSE_ARGMIN ? V : 1];
 #pragma unroll
@@ -306,6 +304,56 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
         for (int v = 0; v < CH; ++v) res[s][cb + v] = o[v];
       } else {
         if (active && doit[s]) store_n<CH>(cio + (size_t)fine[s] * NP + base_c + cb, o);
+      }
+    }
+  }
+
+  // guide (not needed after the level-0 pass, whose guide is a copy at the
+  // reference's call sites, depth_estimation.py:177): each fine pixel's
+  // channel sums in the reference's tap order, after the cost pass so that
+  // its loads do not wait behind these stores
+  if constexpr (!FUSE_ARGMIN) {
+    if constexpr (G == 1) {
+      if (active) {
+        uint32_t gD[9];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) gD[t] = a.gin[tap[t]];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          float s0 = gch(gF[s], 0) * a.weight_up, s1 = gch(gF[s], 1) * a.weight_up,
+                s2 = gch(gF[s], 2) * a.weight_up;
+#pragma unroll
+          for (int k = kStencilBegin[s]; k < kStencilBegin[s + 1]; ++k) {
+            const uint32_t g = gD[up_tap_flat(k)];
+            s0 = s0 + gch(g, 0) * w[k];
+            s1 = s1 + gch(g, 1) * w[k];
+            s2 = s2 + gch(g, 2) * w[k];
+          }
+          if (doit[s]) a.gio[fine[s]] = pack_guide(s0 * inv[s], s1 * inv[s], s2 * inv[s]);
+        }
+      }
+    } else {
+      // lane c of the group sums channel c of the four fine pixels (lanes
+      // 3..7 repeat channel 0); byte s of `mine` is fine pixel s's channel
+      const int ch = lane < 3 ? lane : 0;
+      float gD[9];
+#pragma unroll
+      for (int t = 0; t < 9; ++t) gD[t] = gch(a.gin[tap[t]], ch);
+      uint32_t mine = 0;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        float acc = gch(gF[s], ch) * a.weight_up;
+#pragma unroll
+        for (int k = kStencilBegin[s]; k < kStencilBegin[s + 1]; ++k) acc = acc + gD[up_tap_flat(k)] * w[k];
+        mine |= (uint32_t)(uint8_t)(acc * inv[s]) << (8 * s);
+      }
+      const uint32_t m1 = __shfl(mine, 1, G), m2 = __shfl(mine, 2, G);
+      if (lane == 0 && active) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+          if (doit[s])
+            a.gio[fine[s]] = ((mine >> (8 * s)) & 0xffu) | (((m1 >> (8 * s)) & 0xffu) << 8) |
+                             (((m2 >> (8 * s)) & 0xffu) << 16);
       }
     }
   }
@@ -357,8 +405,12 @@ __device__ __forceinline__ void tail_body(const TailLevels& t, uint32_t* const g
   auto cp = [&](int k) __attribute__((always_inline)) -> void* {
     return k == 0 ? c0 : static_cast<void*>(smem + t.coff[k]);
   };
+  // the weight table in LDS: every pass of the tail looks weights up there
+  float* const wtab = reinterpret_cast<float*>(smem + t.wtab_off);
+  for (int i = (int)threadIdx.x; i < kWeightTable; i += kTailBlock) wtab[i] = t.wtab[i];
+  __syncthreads();
   for (int k = 1; k < t.n; ++k) {
-    const IsbDownArgs a{gp(k - 1), cp(k - 1), t.h[k - 1], t.w[k - 1], gp(k), cp(k), t.h[k], t.w[k], t.wtab};
+    const IsbDownArgs a{gp(k - 1), cp(k - 1), t.h[k - 1], t.w[k - 1], gp(k), cp(k), t.h[k], t.w[k], wtab};
     const int npix = t.h[k] * t.w[k];
     for (int base = 0; base < npix; base += slots)
       if (base + slot < npix) down_pixel<G, V, float>(a, base + slot, true, lane);
@@ -366,7 +418,7 @@ __device__ __forceinline__ void tail_body(const TailLevels& t, uint32_t* const g
   }
   for (int k = t.n - 2; k >= 0; --k) {
     const IsbUpArgs a{gp(k + 1), cp(k + 1), t.h[k + 1], t.w[k + 1], gp(k), cp(k), t.h[k], t.w[k],
-                      t.wu[k],   t.wd[k],   t.wtab, nullptr,  ArgminParams{}, nullptr};
+                      t.wu[k],   t.wd[k],   wtab,   nullptr,  ArgminParams{}, nullptr};
     const int npix = t.h[k + 1] * t.w[k + 1];
     for (int base = 0; base < npix; base += slots)
       if (base + slot < npix) up_pixel<G, V, float, false>(a, base + slot, true, lane);

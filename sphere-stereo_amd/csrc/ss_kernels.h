@@ -71,7 +71,9 @@ struct TailLevels {
   int h[kMaxTailLevels], w[kMaxTailLevels];
   float wu[kMaxTailLevels], wd[kMaxTailLevels];  // up pass into entry k
   const float* wtab;
+  unsigned wtab_off;  // the tail's LDS copy of wtab
 };
+static_assert(kTailLdsBytes + kWeightTable * 4 + 16 <= 64 * 1024, "tail levels and weight table in 64 KiB of LDS");
 struct IsbTailArgs {
   TailLevels t;
   uint32_t* g0[kMaxBatch];
