@@ -150,6 +150,19 @@ int ss_sweep_autotune(const float *d_raylut, const float *const *d_images, int C
                       const float *rts, int ref, const float *cands, int N, int H, int W, int vtype, void *d_cost,
                       int repeats, ss_stream stream, float *ms_out, int *best_out);
 
+/* The cost filter's scheduling variants (no counterpart in the reference,
+ * whose filter has one launch shape): lanes per pixel group (8, or 4 for
+ * padded candidate counts up to 64) x tile order (XCD-aware or launch order).
+ * Every variant gives the same bits.  The variant recorded for a
+ * configuration (set, or picked by ss_estimate_filter_autotune; otherwise 0)
+ * is used by ss_isb_filter, ss_isb_filter_planar, ss_isb_filter_noalloc,
+ * ss_estimate_distance and the frame executors of ss_frame_create /
+ * ss_frame_batch_create (not by the single ragged executor). */
+int ss_isb_variant_count(void);
+int ss_isb_variant_info(int variant, int *lanes, int *xcd_order);
+int ss_isb_set_variant(int H, int W, int N, int vtype, int variant); /* -1 forgets the record */
+int ss_isb_get_variant(int H, int W, int N, int vtype);
+
 /* Bytes of workspace for ss_isb_filter (vtype volume) / ss_isb_filter_planar
  * (planar = 1, vtype must be SS_VOLUME_F32): pyramid levels 1..L-1 of guides
  * and volume, the guide-weight table of the call (768 floats), plus the
@@ -200,6 +213,17 @@ int ss_estimate_distance(const float *d_raylut, const float *const *d_images, in
                          const float *rts, int ref, const float *cands, int N, int H, int W, double sigma_i,
                          double sigma_s, int vtype, const uint8_t *d_guide3, float *d_dist, void *d_workspace,
                          size_t workspace_bytes, ss_stream stream);
+
+/* Times the cost filter with its fused argmin (depth_estimation.py:177-198)
+ * in every valid scheduling variant on the level-0 guide and volume that
+ * d_workspace (an ss_estimate_distance workspace) holds from a previous
+ * sweep, `repeats` runs each, and records the fastest for (H, W, N, vtype).
+ * d_dist (H x W) receives the distance map; ms_out[ss_isb_variant_count()]
+ * the per-run ms (-1 for variants that do not exist for N).  Not while the
+ * stream is being captured. */
+int ss_estimate_filter_autotune(void *d_workspace, size_t workspace_bytes, int H, int W, int N, int vtype,
+                                const float *cands, double sigma_i, double sigma_s, float *d_dist, int repeats,
+                                ss_stream stream, float *ms_out, int *best_out);
 
 /* reprojectDistanceKernel (stitcher.cu:181-204) with stitcher.py:184-192's
  * fill: d_out is set to 1e8, then every source pixel takes part in an exact

@@ -222,7 +222,7 @@ namespace {
 // var_inv_i = 1 / (2 sigma_i^2), isb_filter.py:88), already enqueued on s.
 int run_pyramid(Level* const* lv, int nb, int L, int G, int V, int vt, const float* wtab, double sigma_s,
                 bool fuse_last, const ss::ArgminParams& am, float* const* dist, hipStream_t s, int stage_down,
-                int stage_up, float* const* fill = nullptr) {
+                int stage_up, float* const* fill = nullptr, int xcd = 1) {
   if (nb < 1 || nb > ss::kMaxBatch) return fail(SS_ERR_ARG, "batch of %d pyramids outside [1, %d]", nb, ss::kMaxBatch);
   const Level* l0 = lv[0];  // the shapes (identical for every entry)
   const int NP = G * V;
@@ -242,7 +242,7 @@ int run_pyramid(Level* const* lv, int nb, int L, int G, int V, int vt, const flo
       da[b] = ss::IsbDownArgs{v[k - 1].g, v[k - 1].c, v[k - 1].h, v[k - 1].w, v[k].g, v[k].c, v[k].h, v[k].w, wtab};
     }
     StageTimer t(stage_down, s);
-    SS_HIP(ss::launch_isb_down(da, nb, G, V, k == 1 ? vt : SS_VOLUME_F32, s));
+    SS_HIP(ss::launch_isb_down(da, nb, G, V, k == 1 ? vt : SS_VOLUME_F32, xcd, s));
   }
   if (tail) {
     ss::IsbTailArgs ta;
@@ -266,12 +266,13 @@ int run_pyramid(Level* const* lv, int nb, int L, int G, int V, int vt, const flo
                             fuse && fill ? fill[b] : nullptr};
     }
     StageTimer t(fuse ? SS_STAGE_ISB_UP_ARGMIN : stage_up, s);
-    SS_HIP(ss::launch_isb_up(ua, nb, G, V, k == 0 ? vt : SS_VOLUME_F32, fuse, s));
+    SS_HIP(ss::launch_isb_up(ua, nb, G, V, k == 0 ? vt : SS_VOLUME_F32, fuse, xcd, s));
   }
   if (fuse_last && L == 1) {
     for (int b = 0; b < nb; ++b) {
       StageTimer t(SS_STAGE_ISB_UP_ARGMIN, s);
-      SS_HIP(ss::launch_argmin(lv[b][0].c, l0[0].h * l0[0].w, V, vt, am, dist[b], s));
+      // the argmin kernel works in groups of 8 lanes
+      SS_HIP(ss::launch_argmin(lv[b][0].c, l0[0].h * l0[0].w, NP / 8, vt, am, dist[b], s));
     }
   }
   return SS_OK;
@@ -328,6 +329,55 @@ void set_sweep_variant(int H, int W, int N, int vt, int variant) {
       return;
     }
   if (variant >= 0) g_tune.push_back(SweepTune{H, W, N, vt, variant});
+}
+
+// The cost filter's scheduling variants: lanes per pixel group (8: V = NP/8
+// candidates a lane, or 4: twice as many, which halves the per-pixel weight
+// broadcasts, argmin rounds and guide sums per candidate) x tile order
+// (XCD-aware or launch order).  Every variant computes each value with the
+// same operations in the same order.  Groups of 4 exist for NP <= 64.
+constexpr int kIsbVariants = 4;
+struct IsbVariant {
+  int lanes, xcd;
+};
+constexpr IsbVariant kIsbVariant[kIsbVariants] = {{8, 1}, {8, 0}, {4, 1}, {4, 0}};
+
+bool isb_variant_ok(int v, int N) {
+  if (v < 0 || v >= kIsbVariants || N < 2) return v == 0;
+  return kIsbVariant[v].lanes == 8 || padded(N) <= 64;
+}
+
+std::vector<SweepTune> g_isb_tune;  // (H, W, N, vt) -> variant, guarded by g_tune_mu
+
+int isb_variant_for(int H, int W, int N, int vt) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (const SweepTune& t : g_isb_tune)
+    if (t.H == H && t.W == W && t.N == N && t.vt == vt) return t.variant;
+  return 0;
+}
+
+void set_isb_variant(int H, int W, int N, int vt, int variant) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (size_t i = 0; i < g_isb_tune.size(); ++i)
+    if (g_isb_tune[i].H == H && g_isb_tune[i].W == W && g_isb_tune[i].N == N && g_isb_tune[i].vt == vt) {
+      if (variant < 0)
+        g_isb_tune.erase(g_isb_tune.begin() + (long)i);
+      else
+        g_isb_tune[i].variant = variant;
+      return;
+    }
+  if (variant >= 0) g_isb_tune.push_back(SweepTune{H, W, N, vt, variant});
+}
+
+// (G, V, xcd) of the cost filter of a configuration: its tuned variant, or
+// `variant` when >= 0
+void isb_schedule(int H, int W, int N, int vt, int variant, int* G, int* V, int* xcd) {
+  int v = variant >= 0 ? variant : isb_variant_for(H, W, N, vt);
+  if (!isb_variant_ok(v, N)) v = 0;
+  const int NP = padded(N);
+  *G = N == 1 ? 1 : kIsbVariant[v].lanes;
+  *V = N == 1 ? 1 : NP / *G;
+  *xcd = kIsbVariant[v].xcd;
 }
 
 // One sweep launch over references refs[0..R) (R * C <= kSweepMaxPairs):
@@ -523,6 +573,26 @@ int ss_sweep_set_variant(int H, int W, int N, int vtype, int variant) {
 
 int ss_sweep_get_variant(int H, int W, int N, int vtype) { return sweep_variant_for(H, W, N, vtype); }
 
+int ss_isb_variant_count(void) { return kIsbVariants; }
+
+int ss_isb_variant_info(int variant, int* lanes, int* xcd_order) {
+  SS_REQUIRE(lanes && xcd_order, "null output");
+  SS_REQUIRE(variant >= 0 && variant < kIsbVariants, "filter variant %d outside [0, %d)", variant, kIsbVariants);
+  *lanes = kIsbVariant[variant].lanes;
+  *xcd_order = kIsbVariant[variant].xcd;
+  return ok();
+}
+
+int ss_isb_set_variant(int H, int W, int N, int vtype, int variant) {
+  SS_REQUIRE(variant >= -1 && variant < kIsbVariants, "filter variant %d outside [-1, %d)", variant, kIsbVariants);
+  SS_REQUIRE(variant < 0 || isb_variant_ok(variant, N), "filter variant %d does not exist for %d candidates", variant,
+             N);
+  set_isb_variant(H, W, N, vtype, variant);
+  return ok();
+}
+
+int ss_isb_get_variant(int H, int W, int N, int vtype) { return isb_variant_for(H, W, N, vtype); }
+
 int ss_sweep_autotune(const float* d_raylut, const float* const* d_images, int C, const ss_camera* cams,
                       const float* rts, int ref, const float* cands, int N, int H, int W, int vtype, void* d_cost,
                       int repeats, ss_stream stream, float* ms_out, int* best_out) {
@@ -611,11 +681,12 @@ static int isb_filter_impl(uint32_t* d_guide, void* d_cost, int H, int W, int N,
   Level lv[32];
   lv[0] = Level{H, W, d_guide, d_cost};
   carve_levels(cv, lv, L, H, W, NP);
-  const int G = N == 1 ? 1 : 8, V = N == 1 ? 1 : NP / 8;
+  int G, V, xcd;
+  isb_schedule(H, W, N, vt, -1, &G, &V, &xcd);
   ss::ArgminParams am{};
   Level* const one[1] = {lv};
   return run_pyramid(one, 1, L, G, V, vt, wtab, sigma_s, false, am, nullptr, s, SS_STAGE_ISB_DOWN,
-                     SS_STAGE_ISB_UP);
+                     SS_STAGE_ISB_UP, nullptr, xcd);
 }
 
 static int check_isb_args(int H, int W, int N, int vt) {
@@ -785,7 +856,8 @@ int estimate_levels(void* ws, int H, int W, int N, int vt, float* dist, PyrLevel
 }
 
 int estimate_after_sweep(int nb, int H, int W, int N, int vt, const float* cands, double sigma_i, double sigma_s,
-                         float* const* d_dist, void* const* ws, hipStream_t s, float* const* fill) {
+                         float* const* d_dist, void* const* ws, hipStream_t s, float* const* fill, int isb_variant,
+                         bool distance_filter) {
   if (nb < 1 || nb > kMaxBatch) return fail(SS_ERR_ARG, "batch of %d references outside [1, %d]", nb, kMaxBatch);
   const int L = levels_of(H, W);
   Level lv[kMaxBatch][32], dl[kMaxBatch][32];
@@ -819,14 +891,61 @@ int estimate_after_sweep(int nb, int H, int W, int N, int vt, const float* cands
     }
   } else {
     ss::ArgminParams am{N, d0, dN1, (float)(d0 / dN1) - 1.0f};
-    const int rc = run_pyramid(lvp, nb, L, 8, ss::lanes_v(N), vt, wtab, sigma_s, true, am, d_dist, s,
-                               SS_STAGE_ISB_DOWN, SS_STAGE_ISB_UP, L > 1 ? fill : nullptr);
+    int G, V, xcd;
+    isb_schedule(H, W, N, vt, isb_variant, &G, &V, &xcd);
+    const int rc = run_pyramid(lvp, nb, L, G, V, vt, wtab, sigma_s, true, am, d_dist, s, SS_STAGE_ISB_DOWN,
+                               SS_STAGE_ISB_UP, L > 1 ? fill : nullptr, xcd);
     if (rc) return rc;
     if (fill && L == 1)
       for (int b = 0; b < nb; ++b) SS_HIP(ss::launch_fill(fill[b], 1e8f, (size_t)H * W, s));
   }
+  if (!distance_filter) return SS_OK;
   return run_pyramid(dlp, nb, L, 1, 1, SS_VOLUME_F32, wtab + ss::kWeightTable, sigma_s / 2, false, ss::ArgminParams{},
                      nullptr, s, SS_STAGE_DIST_DOWN, SS_STAGE_DIST_UP);
+}
+
+// The cost filter of the estimate path (its fused argmin included) timed per
+// scheduling variant on the level-0 guide and volume in `ws`.
+int time_filter_variants(void* ws, int H, int W, int N, int vt, const float* cands, double sigma_i, double sigma_s,
+                         float* d_dist, int repeats, hipStream_t s, float* ms_out, int* best_out) {
+  hipEvent_t e0, e1;
+  SS_HIP(hipEventCreate(&e0));
+  SS_HIP(hipEventCreate(&e1));
+  void* const wss[1] = {ws};
+  float* const dists[1] = {d_dist};
+  int rc = SS_OK, best = 0;
+  float best_ms = -1.0f;
+  for (int v = 0; v < kIsbVariants && rc == SS_OK; ++v) {
+    if (ms_out) ms_out[v] = -1.0f;
+    if (!isb_variant_ok(v, N)) continue;
+    rc = estimate_after_sweep(1, H, W, N, vt, cands, sigma_i, sigma_s, dists, wss, s, nullptr, v, false);
+    if (rc) break;
+    if (hipEventRecord(e0, s) != hipSuccess) {
+      rc = fail(SS_ERR_HIP, "hipEventRecord failed");
+      break;
+    }
+    for (int i = 0; i < repeats && rc == SS_OK; ++i)
+      rc = estimate_after_sweep(1, H, W, N, vt, cands, sigma_i, sigma_s, dists, wss, s, nullptr, v, false);
+    if (rc) break;
+    float ms = 0.0f;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      rc = fail(SS_ERR_HIP, "timing the filter variants failed");
+      break;
+    }
+    ms /= (float)repeats;
+    if (ms_out) ms_out[v] = ms;
+    if (best_ms < 0.0f || ms < best_ms) {
+      best = v;
+      best_ms = ms;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  set_isb_variant(H, W, N, vt, best);
+  if (best_out) *best_out = best;
+  return SS_OK;
 }
 
 // Sweeps (and guides) of R references into their estimate workspaces, in as
@@ -941,6 +1060,26 @@ int ss_estimate_distance(const float* d_raylut, const float* const* d_images, in
   float* const dists[1] = {d_dist};
   void* const wss[1] = {d_workspace};
   rc = ss::estimate_after_sweep(1, H, W, N, vtype, cands, sigma_i, sigma_s, dists, wss, s, nullptr);
+  return rc ? rc : ok();
+}
+
+int ss_estimate_filter_autotune(void* d_workspace, size_t workspace_bytes, int H, int W, int N, int vtype,
+                                const float* cands, double sigma_i, double sigma_s, float* d_dist, int repeats,
+                                ss_stream stream, float* ms_out, int* best_out) {
+  SS_REQUIRE(d_workspace && cands && d_dist, "null pointer argument");
+  SS_REQUIRE(N >= 2 && N <= SS_MAX_CANDIDATES, "candidate count %d outside [2, %d]", N, SS_MAX_CANDIDATES);
+  SS_REQUIRE(H >= 1 && W >= 1 && levels_of(H, W) >= 1, "resolution %dx%d gives no pyramid level", W, H);
+  SS_REQUIRE(valid_vtype(vtype), "unknown volume type %d", vtype);
+  SS_REQUIRE(repeats >= 1 && repeats <= 1000, "repeats %d outside [1, 1000]", repeats);
+  const size_t need = ss_estimate_workspace_bytes(H, W, N, vtype);
+  if (workspace_bytes < need)
+    return fail(SS_ERR_WORKSPACE, "workspace of %zu bytes needed, %zu given", need, workspace_bytes);
+  hipStream_t s = as_stream(stream);
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  SS_HIP(hipStreamIsCapturing(s, &cs));
+  SS_REQUIRE(cs == hipStreamCaptureStatusNone, "ss_estimate_filter_autotune times launches: not while capturing");
+  const int rc = ss::time_filter_variants(d_workspace, H, W, N, vtype, cands, sigma_i, sigma_s, d_dist, repeats, s,
+                                          ms_out, best_out);
   return rc ? rc : ok();
 }
 

@@ -66,16 +66,24 @@ __device__ __forceinline__ void down_pixel(const IsbDownArgs& a, int pix, bool a
     return base * (a.wtab[guide_sad_u(gc, gn)] + 0.01f);
   };
 
+  // G > 1: lane l evaluates neighbours l, l + G, ... and the group shares
+  // the weights; glv[i] keeps neighbour l + i G's guide for the guide sums
+  constexpr int PER = G == 1 ? 1 : 8 / G;
   float w[8];
-  uint32_t gl = 0;  // G > 1: lane j's neighbour guide (j = lane & 7)
+  uint32_t glv[PER];
   if constexpr (G == 1) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) w[j] = weight(j, nguide(j));
   } else {
-    gl = nguide(lane & 7);
-    const float wl = weight(lane & 7, gl);
+    static_assert(G == 4 || G == 8, "groups of 4 or 8 lanes");
+    float wl[PER];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) w[j] = __shfl(wl, j, G);
+    for (int i = 0; i < PER; ++i) {
+      glv[i] = nguide(lane + i * G);
+      wl[i] = weight(lane + i * G, glv[i]);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) w[j] = __shfl(wl[j / G], j % G, G);
   }
   float ws = 1.0f;
 #pragma unroll
@@ -118,18 +126,19 @@ __device__ __forceinline__ void down_pixel(const IsbDownArgs& a, int pix, bool a
     const int ch = lane < 3 ? lane : 0;
     float sc = gch(gc, ch);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) sc = sc + gch(__shfl(gl, j, G), ch) * w[j];
+    for (int j = 0; j < 8; ++j) sc = sc + gch(__shfl(glv[j / G], j % G, G), ch) * w[j];
     const uint32_t byte = (uint32_t)(uint8_t)(sc * inv);
     const uint32_t b1 = __shfl(byte, 1, G), b2 = __shfl(byte, 2, G);
     if (lane == 0 && active) a.gout[pix] = byte | (b1 << 8) | (b2 << 16);
   }
 }
 
+// xcd: the XCD-aware tile order (xcd_tile), else tiles in launch order
 template <int G, int V, typename T>
-__global__ void __launch_bounds__(kBlock) k_isb_down(Batch<IsbDownArgs> b) {
+__global__ void __launch_bounds__(kBlock) k_isb_down(Batch<IsbDownArgs> b, int xcd) {
   const IsbDownArgs a = b.a[blockIdx.y];
   const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   int pix = (tile * kBlock + (int)threadIdx.x) / G;
   const int npix = a.hout * a.wout;
   const bool active = pix < npix;
@@ -262,10 +271,14 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
 #pragma unroll
     for (int k = 0; k < 25; ++k) w[k] = weight(k);
   } else {
-    static_assert(G == 8, "group of 8 lanes");
-    const float w0 = weight(lane), w1 = weight(lane + 8), w2 = weight(lane + 16);
+    // lane l evaluates taps l, l + G, ... of the first 24 and the group
+    // shares them; every lane evaluates the last
+    static_assert(G == 4 || G == 8, "groups of 4 or 8 lanes");
+    float wv[24 / G];
 #pragma unroll
-    for (int k = 0; k < 24; ++k) w[k] = __shfl(k < 8 ? w0 : (k < 16 ? w1 : w2), k & 7, G);
+    for (int i = 0; i < 24 / G; ++i) wv[i] = weight(lane + i * G);
+#pragma unroll
+    for (int k = 0; k < 24; ++k) w[k] = __shfl(wv[k / G], k % G, G);
     w[24] = weight(24);
   }
 
@@ -375,10 +388,10 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
 }
 
 template <int G, int V, typename T, bool FUSE_ARGMIN>
-__global__ void __launch_bounds__(kBlock) k_isb_up(Batch<IsbUpArgs> b) {
+__global__ void __launch_bounds__(kBlock) k_isb_up(Batch<IsbUpArgs> b, int xcd) {
   const IsbUpArgs a = b.a[blockIdx.y];
   const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   int pix = (tile * kBlock + (int)threadIdx.x) / G;
   const int npix = a.hin * a.win;
   const bool active = pix < npix;
@@ -594,9 +607,9 @@ Batch<A> batch_of(const A* a, int nb) {
 
 // Every entry of a batch has the shape of entry 0.
 template <int G, int V, typename T>
-hipError_t launch_down_gv(const IsbDownArgs* a, int nb, hipStream_t s) {
+hipError_t launch_down_gv(const IsbDownArgs* a, int nb, int xcd, hipStream_t s) {
   const dim3 grid(blocks_for((long long)a[0].hout * a[0].wout * G), nb);
-  hipLaunchKernelGGL((k_isb_down<G, V, T>), grid, dim3(kBlock), 0, s, batch_of(a, nb));
+  hipLaunchKernelGGL((k_isb_down<G, V, T>), grid, dim3(kBlock), 0, s, batch_of(a, nb), xcd);
   return hipGetLastError();
 }
 
@@ -607,12 +620,12 @@ hipError_t launch_tail_gv(const IsbTailArgs& t, int nb, hipStream_t s) {
 }
 
 template <int G, int V, typename T>
-hipError_t launch_up_gv(const IsbUpArgs* a, int nb, bool fuse, hipStream_t s) {
+hipError_t launch_up_gv(const IsbUpArgs* a, int nb, bool fuse, int xcd, hipStream_t s) {
   const dim3 grid(blocks_for((long long)a[0].hin * a[0].win * G), nb);
   if (fuse)
-    hipLaunchKernelGGL((k_isb_up<G, V, T, true>), grid, dim3(kBlock), 0, s, batch_of(a, nb));
+    hipLaunchKernelGGL((k_isb_up<G, V, T, true>), grid, dim3(kBlock), 0, s, batch_of(a, nb), xcd);
   else
-    hipLaunchKernelGGL((k_isb_up<G, V, T, false>), grid, dim3(kBlock), 0, s, batch_of(a, nb));
+    hipLaunchKernelGGL((k_isb_up<G, V, T, false>), grid, dim3(kBlock), 0, s, batch_of(a, nb), xcd);
   return hipGetLastError();
 }
 
@@ -636,16 +649,41 @@ hipError_t launch_argmin_vt(const void* cost, int npix, const ArgminParams& p, f
     default: return hipErrorInvalidValue;            \
   }
 
+// G = 4 holds twice the candidates per lane of G = 8 for the same padded
+// count (NP = 8 V8 = 4 V4, V4 = 2 V8); it is built for NP <= 64.
+#define SS_DISPATCH_V4(V, CALL)                      \
+  switch (V) {                                       \
+    case 2: return CALL(2);                          \
+    case 4: return CALL(4);                          \
+    case 6: return CALL(6);                          \
+    case 8: return CALL(8);                          \
+    case 12: return CALL(12);                        \
+    case 16: return CALL(16);                        \
+    default: return hipErrorInvalidValue;            \
+  }
+
 template <typename T>
-hipError_t down_t(const IsbDownArgs* a, int nb, int V, hipStream_t s) {
-#define SS_CALL(v) launch_down_gv<8, v, T>(a, nb, s)
+hipError_t down_t(const IsbDownArgs* a, int nb, int G, int V, int xcd, hipStream_t s) {
+  if (G == 4) {
+#define SS_CALL(v) launch_down_gv<4, v, T>(a, nb, xcd, s)
+    SS_DISPATCH_V4(V, SS_CALL)
+#undef SS_CALL
+  }
+  if (G != 8) return hipErrorInvalidValue;
+#define SS_CALL(v) launch_down_gv<8, v, T>(a, nb, xcd, s)
   SS_DISPATCH_V(V, SS_CALL)
 #undef SS_CALL
 }
 
 template <typename T>
-hipError_t up_t(const IsbUpArgs* a, int nb, int V, bool fuse, hipStream_t s) {
-#define SS_CALL(v) launch_up_gv<8, v, T>(a, nb, fuse, s)
+hipError_t up_t(const IsbUpArgs* a, int nb, int G, int V, bool fuse, int xcd, hipStream_t s) {
+  if (G == 4) {
+#define SS_CALL(v) launch_up_gv<4, v, T>(a, nb, fuse, xcd, s)
+    SS_DISPATCH_V4(V, SS_CALL)
+#undef SS_CALL
+  }
+  if (G != 8) return hipErrorInvalidValue;
+#define SS_CALL(v) launch_up_gv<8, v, T>(a, nb, fuse, xcd, s)
   SS_DISPATCH_V(V, SS_CALL)
 #undef SS_CALL
 }
@@ -662,10 +700,11 @@ hipError_t argmin_t(const void* cost, int npix, int V, const ArgminParams& p, fl
 // vt: the element type of the level-0 side of the pass (the input of a down
 // pass, the fine level of an up pass); the other side is fp32.  G = 1 (the
 // distance filter) exists for fp32 only.
-hipError_t launch_isb_down(const IsbDownArgs* a, int nb, int G, int V, int vt, hipStream_t s) {
+hipError_t launch_isb_down(const IsbDownArgs* a, int nb, int G, int V, int vt, int xcd, hipStream_t s) {
   if (nb < 1 || nb > kMaxBatch) return hipErrorInvalidValue;
-  if (G == 1) return (V == 1 && vt == SS_VOLUME_F32) ? launch_down_gv<1, 1, float>(a, nb, s) : hipErrorInvalidValue;
-#define SS_CALL(T) down_t<T>(a, nb, V, s)
+  if (G == 1)
+    return (V == 1 && vt == SS_VOLUME_F32) ? launch_down_gv<1, 1, float>(a, nb, xcd, s) : hipErrorInvalidValue;
+#define SS_CALL(T) down_t<T>(a, nb, G, V, xcd, s)
   SS_DISPATCH_T(vt, SS_CALL)
 #undef SS_CALL
 }
@@ -682,20 +721,26 @@ hipError_t launch_weight_tables(const float* vi, float* const* tab, int n, hipSt
   return hipGetLastError();
 }
 
+// The tail's levels are in LDS; it keeps groups of 8 lanes whatever the
+// passes before and after it use (its levels never leave the tail).
 hipError_t launch_isb_tail(const IsbTailArgs& t, int nb, int G, int V, hipStream_t s) {
   if (nb < 1 || nb > kMaxBatch) return hipErrorInvalidValue;
   if (G == 1) return V == 1 ? launch_tail_gv<1, 1>(t, nb, s) : hipErrorInvalidValue;
+  if (G == 4) {
+    if (V % 2) return hipErrorInvalidValue;
+    V /= 2;
+  }
 #define SS_CALL(v) launch_tail_gv<8, v>(t, nb, s)
   SS_DISPATCH_V(V, SS_CALL)
 #undef SS_CALL
 }
 
-hipError_t launch_isb_up(const IsbUpArgs* a, int nb, int G, int V, int vt, bool fuse_argmin, hipStream_t s) {
+hipError_t launch_isb_up(const IsbUpArgs* a, int nb, int G, int V, int vt, bool fuse_argmin, int xcd, hipStream_t s) {
   if (nb < 1 || nb > kMaxBatch) return hipErrorInvalidValue;
   if (G == 1)
-    return (V == 1 && vt == SS_VOLUME_F32 && !fuse_argmin) ? launch_up_gv<1, 1, float>(a, nb, false, s)
+    return (V == 1 && vt == SS_VOLUME_F32 && !fuse_argmin) ? launch_up_gv<1, 1, float>(a, nb, false, xcd, s)
                                                            : hipErrorInvalidValue;
-#define SS_CALL(T) up_t<T>(a, nb, V, fuse_argmin, s)
+#define SS_CALL(T) up_t<T>(a, nb, G, V, fuse_argmin, xcd, s)
   SS_DISPATCH_T(vt, SS_CALL)
 #undef SS_CALL
 }

@@ -273,8 +273,9 @@ bool sweep_fast_bounds(const TorchCam* cams, int C, const float* rt, int npairs,
 // isb.hip (vt: ss_volume_type of the level-0 side of a pass, i.e. the input
 // of a down pass or the fine level of an up pass; pyramid levels >= 1 are
 // fp32, and so is everything with G = 1).  a[0..nb) are the batch entries.
-hipError_t launch_isb_down(const IsbDownArgs* a, int nb, int G, int V, int vt, hipStream_t s);
-hipError_t launch_isb_up(const IsbUpArgs* a, int nb, int G, int V, int vt, bool fuse_argmin, hipStream_t s);
+// xcd: XCD-aware tile order (1) or launch order (0); G: 1, 4 or 8 lanes per pixel
+hipError_t launch_isb_down(const IsbDownArgs* a, int nb, int G, int V, int vt, int xcd, hipStream_t s);
+hipError_t launch_isb_up(const IsbUpArgs* a, int nb, int G, int V, int vt, bool fuse_argmin, int xcd, hipStream_t s);
 hipError_t launch_isb_tail(const IsbTailArgs& t, int nb, int G, int V, hipStream_t s);
 // n weight tables in one launch: tab[i][d] = exp(-d^2 vi[i]) for d < kWeightTable
 hipError_t launch_weight_tables(const float* vi, float* const* tab, int n, hipStream_t s);
@@ -301,8 +302,13 @@ void estimate_level0(void* ws, int H, int W, int N, int vt, uint32_t** g0, void*
 // everything after the sweep, for nb references in the same launches: cost
 // filter + argmin, then the distance filter; fill (nullable, else nb maps of
 // H x W) is set to 1e8 on the way (the reprojection targets)
+// the cost filter (fused argmin included) timed per scheduling variant on the
+// level-0 guide and volume of an estimate workspace; records the fastest
+int time_filter_variants(void* ws, int H, int W, int N, int vt, const float* cands, double sigma_i, double sigma_s,
+                         float* d_dist, int repeats, hipStream_t s, float* ms_out, int* best_out);
 int estimate_after_sweep(int nb, int H, int W, int N, int vt, const float* cands, double sigma_i, double sigma_s,
-                         float* const* d_dist, void* const* ws, hipStream_t s, float* const* fill);
+                         float* const* d_dist, void* const* ws, hipStream_t s, float* const* fill, int isb_variant = -1,
+                         bool distance_filter = true);
 // the z-tested reprojection of nb maps, with or without the 1e8 fill of the
 // targets first
 int reproject_into(int nb, const float* const* d_in, float* const* d_out, int H, int W, const ss_intrinsics* intr,

@@ -74,6 +74,11 @@ SIGNATURES = {
     "ss_isb_filter": (_i, [_p, _p, _i, _i, _i, _i, _d, _d, _p, _z, _p]),
     "ss_isb_filter_planar": (_i, [_p, _p, _i, _i, _i, _d, _d, _p, _z, _p]),
     "ss_isb_filter_noalloc": (_i, [_p, _p, _i, _i, _i, _i, _d, _d, _p]),
+    "ss_isb_variant_count": (_i, []),
+    "ss_isb_variant_info": (_i, [_i, _p, _p]),
+    "ss_isb_set_variant": (_i, [_i, _i, _i, _i, _i]),
+    "ss_isb_get_variant": (_i, [_i, _i, _i, _i]),
+    "ss_estimate_filter_autotune": (_i, [_p, _z, _i, _i, _i, _i, _p, _d, _d, _p, _i, _p, _p, _p]),
     "ss_argmin_refine": (_i, [_p, _i, _i, _i, _i, _fl, _fl, _fl, _p, _p]),
     "ss_estimate_workspace_bytes": (_z, [_i, _i, _i, _i]),
     "ss_estimate_distance": (_i, [_p, _p, _i, _CAM, _p, _i, _p, _i, _i, _i, _d, _d, _i, _p, _p, _p, _z, _p]),

@@ -97,16 +97,20 @@ class RGBD_Estimator:
         return handle
 
     def autotune(self, images_to_match, repeats=5):
-        """Times the sweep's scheduling variants (ss_sweep_autotune: identical
-        results, different candidate chunking and LDS staging) on these
-        images for the first reference and records the fastest for this
-        configuration; every later sweep of the configuration uses it.
-        Returns {"variant": v, "ms": [per-launch ms of each variant]}."""
+        """Times the scheduling variants of the sweep (ss_sweep_autotune:
+        candidate chunking and LDS staging) and of the cost filter
+        (ss_estimate_filter_autotune: lanes per pixel and tile order, timed
+        with the fused argmin on the first reference's swept volume) on these
+        images, and records the fastest of each for this configuration; every
+        later frame of the configuration uses them.  Every variant gives the
+        same bits.  Returns {"variant": v, "ms": [...], "filter_variant": f,
+        "filter_ms": [...]} (ms per launch, or per filter run; -1 for a
+        filter variant that does not exist for this candidate count)."""
         imgs = [_hwc(i) for i in images_to_match]
         if len(imgs) != len(self.calibrations):
             raise ValueError("expected %d images to match" % len(self.calibrations))
         if self.candidate_count < 2:
-            return {"variant": 0, "ms": []}
+            return {"variant": 0, "ms": [], "filter_variant": 0, "filter_ms": []}
         nv = _lib.lib.ss_sweep_variant_count()
         NP = _lib.lib.ss_padded_candidates(self.candidate_count)
         eb = 4 if self.volume_dtype == "float32" else 2
@@ -119,7 +123,20 @@ class RGBD_Estimator:
                                               self.cols, self._vtype, scratch.data_ptr(), int(repeats),
                                               self._stream(), ms, _lib.C.byref(best)))
         _TUNED[(self.rows, self.cols, self.candidate_count, self._vtype)] = int(best.value)
-        return {"variant": int(best.value), "ms": [round(float(v), 4) for v in ms]}
+        # the filter, on the volume and guide a sweep leaves in the workspace
+        dist = torch.empty([self.rows, self.cols], device=self.device)
+        self._estimate(0, r, imgs, None, dist)
+        nf = _lib.lib.ss_isb_variant_count()
+        fms = (_lib.C.c_float * nf)()
+        fbest = _lib.C.c_int(0)
+        ws = self._workspaces[0]
+        _lib.check(_lib.lib.ss_estimate_filter_autotune(ws.data_ptr(), ws.numel(), self.rows, self.cols,
+                                                        self.candidate_count, self._vtype, self._cands,
+                                                        float(self.sigma_i), float(self.sigma_s), dist.data_ptr(),
+                                                        int(repeats), self._stream(), fms, _lib.C.byref(fbest)))
+        _TUNED_FILTER[(self.rows, self.cols, self.candidate_count, self._vtype)] = int(fbest.value)
+        return {"variant": int(best.value), "ms": [round(float(v), 4) for v in ms],
+                "filter_variant": int(fbest.value), "filter_ms": [round(float(v), 4) for v in fms]}
 
     def _config_key(self):
         """What the frames of one native batch share (ss_frame_batch_create
@@ -281,29 +298,35 @@ class RGBD_Estimator:
         return imgs, stitch
 
 
-# sweep variants chosen by RGBD_Estimator.autotune in this process: (rows,
-# cols, candidates, volume type) -> variant
+# variants chosen by RGBD_Estimator.autotune in this process: (rows, cols,
+# candidates, volume type) -> variant, for the sweep and for the cost filter
 _TUNED = {}
+_TUNED_FILTER = {}
 
 
 def save_sweep_tuning(path):
-    """Writes the sweep variants tuned in this process (RGBD_Estimator.autotune,
-    FrameBatch.autotune) to a JSON file, with the device they were measured
-    on.  Returns the number of configurations written."""
+    """Writes the sweep and cost-filter variants tuned in this process
+    (RGBD_Estimator.autotune, FrameBatch.autotune) to a JSON file, with the
+    device they were measured on.  Returns the number of records written."""
     import json
 
     dev = torch.cuda.get_device_name() if torch.cuda.is_available() else ""
-    entries = [{"rows": k[0], "cols": k[1], "candidates": k[2], "volume_type": k[3], "variant": v}
-               for k, v in sorted(_TUNED.items())]
+
+    def entries(tab):
+        return [{"rows": k[0], "cols": k[1], "candidates": k[2], "volume_type": k[3], "variant": v}
+                for k, v in sorted(tab.items())]
+
+    data = {"device": dev, "sweep_variants": entries(_TUNED), "filter_variants": entries(_TUNED_FILTER)}
     with open(path, "w") as f:
-        json.dump({"device": dev, "sweep_variants": entries}, f, indent=1)
-    return len(entries)
+        json.dump(data, f, indent=1)
+    return len(data["sweep_variants"]) + len(data["filter_variants"])
 
 
 def load_sweep_tuning(path):
-    """Applies the variants of a save_sweep_tuning file (ss_sweep_set_variant)
-    when it was written on a device of the same name; every later sweep of
-    those configurations uses them.  Returns the number applied."""
+    """Applies the variants of a save_sweep_tuning file (ss_sweep_set_variant,
+    ss_isb_set_variant) when it was written on a device of the same name;
+    every later frame of those configurations uses them.  Returns the number
+    of records applied."""
     import json
 
     with open(path) as f:
@@ -312,11 +335,13 @@ def load_sweep_tuning(path):
     if data.get("device", "") != dev:
         return 0
     n = 0
-    for e in data.get("sweep_variants", []):
-        key = (int(e["rows"]), int(e["cols"]), int(e["candidates"]), int(e["volume_type"]))
-        _lib.check(_lib.lib.ss_sweep_set_variant(key[0], key[1], key[2], key[3], int(e["variant"])))
-        _TUNED[key] = int(e["variant"])
-        n += 1
+    for name, setter, tab in (("sweep_variants", _lib.lib.ss_sweep_set_variant, _TUNED),
+                              ("filter_variants", _lib.lib.ss_isb_set_variant, _TUNED_FILTER)):
+        for e in data.get(name, []):
+            key = (int(e["rows"]), int(e["cols"]), int(e["candidates"]), int(e["volume_type"]))
+            _lib.check(setter(key[0], key[1], key[2], key[3], int(e["variant"])))
+            tab[key] = int(e["variant"])
+            n += 1
     return n
 
 

@@ -183,35 +183,63 @@ def test_ragged_table_sizing_and_rejections():
 
 
 def test_sweep_tuning_json_roundtrip(tmp_path):
-    """save_sweep_tuning / load_sweep_tuning: the variants tuned in a process
-    go to JSON and back into the library's per-configuration registry
-    (ss_sweep_set_variant / ss_sweep_get_variant, host state only)."""
+    """save_sweep_tuning / load_sweep_tuning: the sweep and cost-filter
+    variants tuned in a process go to JSON and back into the library's
+    per-configuration registries (ss_sweep_set_variant / get_variant,
+    ss_isb_set_variant / get_variant, host state only)."""
     import json
 
     from sphere_stereo import depth_estimation as de
 
     lib = _lib.lib
-    saved = dict(de._TUNED)
+    saved, saved_f = dict(de._TUNED), dict(de._TUNED_FILTER)
     try:
         de._TUNED.clear()
+        de._TUNED_FILTER.clear()
         de._TUNED[(96, 80, 48, _lib.SS_VOLUME_F16)] = 3
         de._TUNED[(64, 64, 16, _lib.SS_VOLUME_F32)] = 1
+        de._TUNED_FILTER[(96, 80, 48, _lib.SS_VOLUME_F16)] = 2
         path = str(tmp_path / "tuning.json")
-        assert de.save_sweep_tuning(path) == 2
+        assert de.save_sweep_tuning(path) == 3
         data = json.load(open(path))
         assert {e["variant"] for e in data["sweep_variants"]} == {1, 3}
+        assert [e["variant"] for e in data["filter_variants"]] == [2]
         assert lib.ss_sweep_get_variant(96, 80, 48, _lib.SS_VOLUME_F16) == 0
-        assert de.load_sweep_tuning(path) == 2
+        assert lib.ss_isb_get_variant(96, 80, 48, _lib.SS_VOLUME_F16) == 0
+        assert de.load_sweep_tuning(path) == 3
         assert lib.ss_sweep_get_variant(96, 80, 48, _lib.SS_VOLUME_F16) == 3
         assert lib.ss_sweep_get_variant(64, 64, 16, _lib.SS_VOLUME_F32) == 1
+        assert lib.ss_isb_get_variant(96, 80, 48, _lib.SS_VOLUME_F16) == 2
         data["device"] = "another GPU"
         json.dump(data, open(path, "w"))
         assert de.load_sweep_tuning(path) == 0  # tuned elsewhere: not applied
     finally:
         for H, W, N, vt in [(96, 80, 48, _lib.SS_VOLUME_F16), (64, 64, 16, _lib.SS_VOLUME_F32)]:
             lib.ss_sweep_set_variant(H, W, N, vt, -1)
+            lib.ss_isb_set_variant(H, W, N, vt, -1)
         de._TUNED.clear()
         de._TUNED.update(saved)
+        de._TUNED_FILTER.clear()
+        de._TUNED_FILTER.update(saved_f)
+
+
+def test_isb_variants():
+    """The cost filter's scheduling variants: (lanes, tile order) per variant,
+    groups of 4 lanes only for padded candidate counts up to 64."""
+    lib = _lib.lib
+    assert lib.ss_isb_variant_count() == 4
+    lanes, xcd = C.c_int(), C.c_int()
+    got = []
+    for v in range(4):
+        assert lib.ss_isb_variant_info(v, C.byref(lanes), C.byref(xcd)) == 0
+        got.append((lanes.value, xcd.value))
+    assert got == [(8, 1), (8, 0), (4, 1), (4, 0)]
+    assert lib.ss_isb_variant_info(4, C.byref(lanes), C.byref(xcd)) == 1
+    assert lib.ss_isb_set_variant(64, 64, 128, 0, 2) == 1 and b"does not exist" in lib.ss_last_error()
+    assert lib.ss_isb_set_variant(64, 64, 64, 0, 3) == 0
+    assert lib.ss_isb_get_variant(64, 64, 64, 0) == 3
+    assert lib.ss_isb_set_variant(64, 64, 64, 0, -1) == 0
+    assert lib.ss_isb_get_variant(64, 64, 64, 0) == 0
 
 
 def test_isb_noalloc_limit_and_refusal():

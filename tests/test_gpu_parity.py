@@ -671,9 +671,76 @@ def test_sweep_variants_identical_and_autotune(gpu, N, vtype):
     try:
         assert 0 <= res["variant"] < nv and len(res["ms"]) == nv and all(m > 0 for m in res["ms"]), res
         assert lib.ss_sweep_get_variant(H, W, N, vt) == res["variant"]
+        nf = lib.ss_isb_variant_count()
+        assert 0 <= res["filter_variant"] < nf and len(res["filter_ms"]) == nf, res
+        assert all(m > 0 for m in res["filter_ms"]) and res["filter_ms"][res["filter_variant"]] == min(res["filter_ms"])
+        assert lib.ss_isb_get_variant(H, W, N, vt) == res["filter_variant"]
         got = est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])
         torch.cuda.synchronize()
         assert torch.equal(got[0], want[0])
         assert torch.equal(got[1].view(torch.int32), want[1].view(torch.int32))
     finally:
         lib.ss_sweep_set_variant(H, W, N, vt, -1)
+        lib.ss_isb_set_variant(H, W, N, vt, -1)
+
+
+@pytest.mark.parametrize("vtype", [0, 1])
+@pytest.mark.parametrize("shape,N", [((33, 17), 8), ((48, 64), 32), ((31, 45), 48), ((40, 24), 64), ((24, 40), 128)])
+def test_isb_filter_variants_identical(gpu, vtype, shape, N):
+    """The cost filter's scheduling variants (4 or 8 lanes per pixel, XCD or
+    launch tile order) give the oracle's bits; a variant that does not exist
+    for the candidate count (4 lanes past 64) is refused."""
+    H, W = shape
+    lib = _lib.lib
+    NP = lib.ss_padded_candidates(N)
+    g = torch.Generator().manual_seed(H * 13 + W + N + vtype)
+    guide = torch.randint(0, 256, [H, W, 3], generator=g, dtype=torch.uint8)
+    guide[: H // 2] = guide[: H // 2] // 8 + 100
+    cost = O32.round_volume((torch.rand([N, H, W], generator=g) * 500).numpy(), vtype)
+    want_c, want_g = O32.isb_filter(guide.numpy(), cost, 10.0, 25.0, vtype=vtype, round_level0=True)
+    vol0 = torch.zeros([H, W, NP], dtype=VOLUME_DTYPES[vtype])
+    vol0[..., :N] = torch.from_numpy(cost).permute(1, 2, 0).to(VOLUME_DTYPES[vtype])
+    g32 = guide.to(torch.int32)
+    packed0 = (g32[..., 0] | (g32[..., 1] << 8) | (g32[..., 2] << 16)).contiguous()
+    nbytes = lib.ss_isb_workspace_bytes(H, W, N, vtype, 0)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=gpu)
+    try:
+        for v in range(lib.ss_isb_variant_count()):
+            rc = lib.ss_isb_set_variant(H, W, N, vtype, v)
+            if NP > 64 and v >= 2:
+                assert rc == 1
+                continue
+            assert rc == 0
+            vol, packed = vol0.clone().to(gpu), packed0.clone().to(gpu)
+            _lib.check(lib.ss_isb_filter(packed.data_ptr(), vol.data_ptr(), H, W, N, vtype, 10.0, 25.0,
+                                         ws.data_ptr(), nbytes, _stream()))
+            got = np.transpose(to_np(vol.float())[..., :N], (2, 0, 1))
+            assert_same(got, want_c, "filtered volume, variant %d" % v)
+            pk = to_np(packed).view(np.uint32)
+            got_g = np.stack([(pk >> (8 * c)) & 255 for c in range(3)], -1).astype(np.uint8)
+            assert_same(got_g, want_g, "guide, variant %d" % v)
+    finally:
+        lib.ss_isb_set_variant(H, W, N, vtype, -1)
+
+
+@pytest.mark.parametrize("volume_dtype", ["float32", "float16"])
+def test_frame_filter_variants_identical(small, gpu, volume_dtype):
+    """Whole frames (cost filter with the fused argmin, the distance filter,
+    the stitch) in every cost-filter variant: the oracle's panorama."""
+    fr, _, _ = small
+    kw = dict(min_dist=0.55, max_dist=100, candidate_count=32, references_indices=fr["refs"],
+              reprojection_viewpoint=fr["viewpoint"], masks=fr["masks"], matching_resolution=fr["matching"],
+              rgb_to_stitch_resolution=fr["stitch"], panorama_resolution=(128, 64), sigma_i=10, sigma_s=12.5)
+    want_rgb, want_dist = OraclePipeline(fr["calibs"], volume_dtype=volume_dtype, **kw).estimate_RGBD_panorama(
+        fr["to_match"], fr["to_stitch"])
+    lib = _lib.lib
+    vt = _lib.VOLUME_TYPES[volume_dtype]
+    try:
+        for v in range(lib.ss_isb_variant_count()):
+            assert lib.ss_isb_set_variant(72, 80, 32, vt, v) == 0
+            est = RGBD_Estimator(fr["calibs"], device=gpu, volume_dtype=volume_dtype, **kw)
+            rgb, dist = est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])
+            assert_same(to_np(dist), want_dist, "distance panorama, filter variant %d" % v)
+            assert_same(to_np(rgb), want_rgb, "rgb panorama, filter variant %d" % v)
+    finally:
+        lib.ss_isb_set_variant(72, 80, 32, vt, -1)
