@@ -369,6 +369,29 @@ void set_isb_variant(int H, int W, int N, int vt, int variant) {
   if (variant >= 0) g_isb_tune.push_back(SweepTune{H, W, N, vt, variant});
 }
 
+// The inpainting tile recorded per map size (H, W); 0 (48 x 48) if none.
+std::vector<SweepTune> g_inpaint_tune;  // N = vt = 0; guarded by g_tune_mu
+
+int inpaint_variant_for(int H, int W) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (const SweepTune& t : g_inpaint_tune)
+    if (t.H == H && t.W == W) return t.variant;
+  return 0;
+}
+
+void set_inpaint_variant(int H, int W, int variant) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (size_t i = 0; i < g_inpaint_tune.size(); ++i)
+    if (g_inpaint_tune[i].H == H && g_inpaint_tune[i].W == W) {
+      if (variant < 0)
+        g_inpaint_tune.erase(g_inpaint_tune.begin() + (long)i);
+      else
+        g_inpaint_tune[i].variant = variant;
+      return;
+    }
+  if (variant >= 0) g_inpaint_tune.push_back(SweepTune{H, W, 0, 0, variant});
+}
+
 // (G, V, xcd) of the cost filter of a configuration: its tuned variant, or
 // `variant` when >= 0
 void isb_schedule(int H, int W, int N, int vt, int variant, int* G, int* V, int* xcd) {
@@ -1000,14 +1023,21 @@ int to_u8_maps(const ToU8Args* a, int nb, size_t n, hipStream_t s) {
 
 int inpaint_maps(int nb, float* const* map, float* const* tmp, const uint8_t* const* lut, int H, int W,
                  int iterations, double thr_empty, double thr_src, hipStream_t s) {
+  return inpaint_maps_v(nb, map, tmp, lut, H, W, iterations, thr_empty, thr_src, -1, s);
+}
+
+int inpaint_maps_v(int nb, float* const* map, float* const* tmp, const uint8_t* const* lut, int H, int W,
+                   int iterations, double thr_empty, double thr_src, int variant, hipStream_t s) {
   if (nb < 1 || nb > kMaxBatch) return fail(SS_ERR_ARG, "batch of %d maps outside [1, %d]", nb, kMaxBatch);
   InpaintArgs a[kMaxBatch];
   for (int b = 0; b < nb; ++b) a[b] = InpaintArgs{map[b], tmp[b], lut[b]};
   const int per = ss::inpaint_steps_per_launch();
+  const int tile = ss::inpaint_tile_of(variant >= 0 ? variant : inpaint_variant_for(H, W));
   bool in_map = true;  // where the current values are
   for (int i = 0; i < iterations; i += per) {
     StageTimer tm(SS_STAGE_INPAINT, s);
-    SS_HIP(ss::launch_inpaint_multi(a, nb, H, W, iterations - i < per ? iterations - i : per, thr_empty, thr_src, s));
+    SS_HIP(ss::launch_inpaint_multi(a, nb, H, W, iterations - i < per ? iterations - i : per, thr_empty, thr_src,
+                                    tile, s));
     for (int b = 0; b < nb; ++b) {
       const float* t = a[b].din;
       a[b].din = a[b].dout;
@@ -1097,6 +1127,79 @@ int ss_inpaint(float* d_map, float* d_tmp, const uint8_t* d_lut, int H, int W, i
   SS_REQUIRE(d_map && d_tmp && d_lut && H >= 1 && W >= 1 && iterations >= 0, "bad argument");
   const int rc = ss::inpaint_maps(1, &d_map, &d_tmp, &d_lut, H, W, iterations, thr_empty, thr_src, as_stream(stream));
   return rc ? rc : ok();
+}
+
+int ss_inpaint_variant_count(void) { return ss::inpaint_tile_count(); }
+
+int ss_inpaint_variant_info(int variant, int* tile) {
+  SS_REQUIRE(tile, "null output");
+  SS_REQUIRE(variant >= 0 && variant < ss::inpaint_tile_count(), "inpainting variant %d outside [0, %d)", variant,
+             ss::inpaint_tile_count());
+  *tile = ss::inpaint_tile_of(variant);
+  return ok();
+}
+
+int ss_inpaint_set_variant(int H, int W, int variant) {
+  SS_REQUIRE(variant >= -1 && variant < ss::inpaint_tile_count(), "inpainting variant %d outside [-1, %d)", variant,
+             ss::inpaint_tile_count());
+  set_inpaint_variant(H, W, variant);
+  return ok();
+}
+
+int ss_inpaint_get_variant(int H, int W) { return inpaint_variant_for(H, W); }
+
+int ss_inpaint_autotune(const float* d_map, float* d_work, float* d_tmp, const uint8_t* d_lut, int H, int W,
+                        int iterations, double thr_empty, double thr_src, int repeats, ss_stream stream,
+                        float* ms_out, int* best_out) {
+  SS_REQUIRE(d_map && d_work && d_tmp && d_lut && H >= 1 && W >= 1 && iterations >= 1, "bad argument");
+  SS_REQUIRE(d_map != d_work && d_work != d_tmp, "the map, its working copy and the scratch must differ");
+  SS_REQUIRE(repeats >= 1 && repeats <= 1000, "repeats %d outside [1, 1000]", repeats);
+  hipStream_t s = as_stream(stream);
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  SS_HIP(hipStreamIsCapturing(s, &cs));
+  SS_REQUIRE(cs == hipStreamCaptureStatusNone, "ss_inpaint_autotune times launches: not while capturing");
+  hipEvent_t e0, e1;
+  SS_HIP(hipEventCreate(&e0));
+  SS_HIP(hipEventCreate(&e1));
+  const size_t bytes = (size_t)H * W * sizeof(float);
+  int rc = SS_OK, best = 0;
+  float best_ms = -1.0f;
+  // each run inpaints a fresh copy of the map: the copy costs every variant the same
+  auto run = [&](int v) -> int {
+    if (hipMemcpyAsync(d_work, d_map, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+      return fail(SS_ERR_HIP, "copying the map failed");
+    float* m = d_work;
+    float* t = d_tmp;
+    return ss::inpaint_maps_v(1, &m, &t, &d_lut, H, W, iterations, thr_empty, thr_src, v, s);
+  };
+  for (int v = 0; v < ss::inpaint_tile_count() && rc == SS_OK; ++v) {
+    rc = run(v);
+    if (rc) break;
+    if (hipEventRecord(e0, s) != hipSuccess) {
+      rc = fail(SS_ERR_HIP, "hipEventRecord failed");
+      break;
+    }
+    for (int i = 0; i < repeats && rc == SS_OK; ++i) rc = run(v);
+    if (rc) break;
+    float ms = 0.0f;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      rc = fail(SS_ERR_HIP, "timing the inpainting variants failed");
+      break;
+    }
+    ms /= (float)repeats;
+    if (ms_out) ms_out[v] = ms;
+    if (best_ms < 0.0f || ms < best_ms) {
+      best = v;
+      best_ms = ms;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  set_inpaint_variant(H, W, best);
+  if (best_out) *best_out = best;
+  return ok();
 }
 
 int ss_merge(const float* d_sampling, const float* d_weights, const float* const* d_reproj,

@@ -328,9 +328,17 @@ hipError_t launch_fill(float* p, float v, size_t n, hipStream_t s);
 hipError_t launch_reproject(const ReprojectArgs* a, int nb, int H, int W, hipStream_t s);
 hipError_t launch_inpaint_step(const float* din, float* dout, const uint8_t* lut, int H, int W, double thr_empty,
                                double thr_src, hipStream_t s);
+// tile: the side of a workgroup's tile, inpaint_tile_of(variant) for the
+// inpaint_tile_count() variants (48, 32, 16); every tile gives the same map
 hipError_t launch_inpaint_multi(const InpaintArgs* a, int nb, int H, int W, int steps, double thr_empty,
-                                double thr_src, hipStream_t s);
+                                double thr_src, int tile, hipStream_t s);
 int inpaint_steps_per_launch();
+int inpaint_tile_count();
+int inpaint_tile_of(int variant);
+// inpaint_maps with the tile of the variant recorded for (H, W), or of
+// `variant` when >= 0
+int inpaint_maps_v(int nb, float* const* map, float* const* tmp, const uint8_t* const* lut, int H, int W,
+                   int iterations, double thr_empty, double thr_src, int variant, hipStream_t s);
 hipError_t launch_inpaint_lut(const ss_intrinsics& c, const float* t, int H, int W, float mn, float mx, uint8_t* lut,
                               hipStream_t s);
 hipError_t launch_box_filter(const float* in, float* out, float* tmp, int B, int H, int W, int r, float pad,

@@ -130,15 +130,21 @@ __global__ void __launch_bounds__(kBlock) k_inpaint_step(const float* __restrict
 // staged region keeps its value for the whole launch (the interior of big
 // holes, the area outside a fisheye's field of view).  The launch also stops
 // after a pass that changes nothing, since every later pass would repeat it.
-// 48 + 2 * 16 = 80: 79 KB of LDS, 2 workgroups per CU.
+// The tile side TILE is a template parameter (the frame path's inpainting
+// autotuner picks it per map size; the ragged executor uses kTile): 48 +
+// 2 * 16 = 80 is 79 KB of LDS, 2 workgroups per CU; 32 (50 KB, 3 per CU) and
+// 16 (19 KB) stage more halo per tile.  The result does not depend on it.
 // blockIdx.z selects the map of the batch.
 constexpr int kTile = 48;
 constexpr int kSteps = 16;
-constexpr int kRegion = kTile + 2 * kSteps;
-constexpr int kRowWords = (kRegion + 31) / 32;  // a row of the region as a bit mask
-static_assert(kRegion * kRegion % kBlock == 0 && kRegion * kRegion <= 65536 && kRegion <= 96 &&
-                  kRegion * kRowWords <= kBlock,
-              "region fits the staging loop, the list and the row masks");
+template <int TILE>
+struct InpaintGeom {
+  static constexpr int kRegion = TILE + 2 * kSteps;
+  static constexpr int kRowWords = (kRegion + 31) / 32;  // a row of the region as a bit mask
+  static_assert(kRegion * kRegion % kBlock == 0 && kRegion * kRegion <= 65536 && kRegion <= 96 &&
+                    kRegion * kRowWords <= kBlock,
+                "region fits the staging loop, the list and the row masks");
+};
 
 // A region row's bit mask (bits 0..95 in three words) shifted by k (0..31)
 // towards higher or lower x.
@@ -155,9 +161,11 @@ __device__ __forceinline__ void row_shr(const uint32_t (&m)[3], int k, uint32_t 
 
 // The passes of one tile (bx, by) of a map: a workgroup's whole body (it
 // returns with every thread past its last barrier).
+template <int TILE>
 __device__ __forceinline__ void inpaint_tile(const float* __restrict__ din, float* __restrict__ dout,
                                              const uchar2* __restrict__ lut, int H, int W, int steps,
                                              double thr_empty, double thr_src, int bx, int by) {
+  constexpr int kRegion = InpaintGeom<TILE>::kRegion, kRowWords = InpaintGeom<TILE>::kRowWords;
   __shared__ float bufa[kRegion * kRegion];
   __shared__ float bufb[kRegion * kRegion];
   __shared__ uchar2 tab[kRegion * kRegion];
@@ -166,7 +174,7 @@ __device__ __forceinline__ void inpaint_tile(const float* __restrict__ din, floa
   __shared__ uint32_t near_[kRegion][kRowWords]; // within `steps` of a source
   __shared__ unsigned count, changed[2];  // changed[s & 1] == s + 1: pass s changed a pixel
   const int tid = (int)threadIdx.x;
-  const int ox = bx * kTile - kSteps, oy = by * kTile - kSteps;
+  const int ox = bx * TILE - kSteps, oy = by * TILE - kSteps;
   const int reach = min(steps, kSteps);
   // A workgroup of the persistent ragged form runs several tiles in a row:
   // every thread must be done with the previous tile's read-out (bufa/bufb)
@@ -286,18 +294,19 @@ __device__ __forceinline__ void inpaint_tile(const float* __restrict__ din, floa
     // pass s is next written in pass s + 2, after every lane has read it.)
     if (changed[s & 1] != (unsigned)(s + 1)) break;
   }
-  for (int i = threadIdx.x; i < kTile * kTile; i += kBlock) {
-    const int lx = kSteps + i % kTile, ly = kSteps + i / kTile;
+  for (int i = threadIdx.x; i < TILE * TILE; i += kBlock) {
+    const int lx = kSteps + i % TILE, ly = kSteps + i / TILE;
     const int gx = ox + lx, gy = oy + ly;
     if (gx < W && gy < H) dout[gy * W + gx] = a[ly * kRegion + lx];
   }
 }
 
+template <int TILE>
 __global__ void __launch_bounds__(kBlock) k_inpaint_multi(Batch<InpaintArgs> bt, int H, int W, int steps,
                                                           double thr_empty, double thr_src) {
   const InpaintArgs& a = bt.a[blockIdx.z];
-  inpaint_tile(a.din, a.dout, reinterpret_cast<const uchar2*>(a.lut), H, W, steps, thr_empty, thr_src, blockIdx.x,
-               blockIdx.y);
+  inpaint_tile<TILE>(a.din, a.dout, reinterpret_cast<const uchar2*>(a.lut), H, W, steps, thr_empty, thr_src,
+                     blockIdx.x, blockIdx.y);
 }
 
 // ragged: a workgroup's item is one tile of one map, with the map's own
@@ -307,8 +316,8 @@ __global__ void __launch_bounds__(kBlock) k_inpaint_ragged(RaggedSpan<InpaintEnt
   for_each_block_item<LOOP>(sp, [&](int e, int t) {
     const InpaintEntry m = load_const(sp.e + e);
     const int by = t / m.tiles_x, bx = t - by * m.tiles_x;
-    inpaint_tile(m.a.din, m.a.dout, reinterpret_cast<const uchar2*>(m.a.lut), m.H, m.W, m.steps, m.thr_empty,
-                 m.thr_src, bx, by);
+    inpaint_tile<kTile>(m.a.din, m.a.dout, reinterpret_cast<const uchar2*>(m.a.lut), m.H, m.W, m.steps, m.thr_empty,
+                        m.thr_src, bx, by);
   });
 }
 
@@ -619,12 +628,31 @@ hipError_t launch_inpaint_step(const float* din, float* dout, const uint8_t* lut
   return hipGetLastError();
 }
 
-hipError_t launch_inpaint_multi(const InpaintArgs* a, int nb, int H, int W, int steps, double thr_empty,
-                                double thr_src, hipStream_t s) {
-  if (nb < 1 || nb > kMaxBatch) return hipErrorInvalidValue;
-  const dim3 grid((W + kTile - 1) / kTile, (H + kTile - 1) / kTile, nb);
-  hipLaunchKernelGGL(k_inpaint_multi, grid, dim3(kBlock), 0, s, batch_of(a, nb), H, W, steps, thr_empty, thr_src);
+template <int TILE>
+hipError_t launch_inpaint_tile(const InpaintArgs* a, int nb, int H, int W, int steps, double thr_empty,
+                               double thr_src, hipStream_t s) {
+  const dim3 grid((W + TILE - 1) / TILE, (H + TILE - 1) / TILE, nb);
+  hipLaunchKernelGGL(k_inpaint_multi<TILE>, grid, dim3(kBlock), 0, s, batch_of(a, nb), H, W, steps, thr_empty,
+                     thr_src);
   return hipGetLastError();
+}
+
+// tile: 48, 32 or 16 (inpaint_tiles)
+hipError_t launch_inpaint_multi(const InpaintArgs* a, int nb, int H, int W, int steps, double thr_empty,
+                                double thr_src, int tile, hipStream_t s) {
+  if (nb < 1 || nb > kMaxBatch) return hipErrorInvalidValue;
+  switch (tile) {
+    case 48: return launch_inpaint_tile<48>(a, nb, H, W, steps, thr_empty, thr_src, s);
+    case 32: return launch_inpaint_tile<32>(a, nb, H, W, steps, thr_empty, thr_src, s);
+    case 16: return launch_inpaint_tile<16>(a, nb, H, W, steps, thr_empty, thr_src, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int inpaint_tile_count() { return 3; }
+int inpaint_tile_of(int variant) {
+  static const int kTiles[3] = {48, 32, 16};
+  return variant >= 0 && variant < 3 ? kTiles[variant] : kTile;
 }
 
 int inpaint_steps_per_launch() { return kSteps; }

@@ -75,6 +75,11 @@ SIGNATURES = {
     "ss_isb_filter_planar": (_i, [_p, _p, _i, _i, _i, _d, _d, _p, _z, _p]),
     "ss_isb_filter_noalloc": (_i, [_p, _p, _i, _i, _i, _i, _d, _d, _p]),
     "ss_isb_variant_count": (_i, []),
+    "ss_inpaint_variant_count": (_i, []),
+    "ss_inpaint_variant_info": (_i, [_i, _p]),
+    "ss_inpaint_set_variant": (_i, [_i, _i, _i]),
+    "ss_inpaint_get_variant": (_i, [_i, _i]),
+    "ss_inpaint_autotune": (_i, [_p, _p, _p, _p, _i, _i, _i, _d, _d, _i, _p, _p, _p]),
     "ss_isb_variant_info": (_i, [_i, _p, _p]),
     "ss_isb_set_variant": (_i, [_i, _i, _i, _i, _i]),
     "ss_isb_get_variant": (_i, [_i, _i, _i, _i]),

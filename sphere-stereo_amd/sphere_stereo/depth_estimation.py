@@ -102,10 +102,13 @@ class RGBD_Estimator:
         (ss_estimate_filter_autotune: lanes per pixel and tile order, timed
         with the fused argmin on the first reference's swept volume) on these
         images, and records the fastest of each for this configuration; every
-        later frame of the configuration uses them.  Every variant gives the
-        same bits.  Returns {"variant": v, "ms": [...], "filter_variant": f,
-        "filter_ms": [...]} (ms per launch, or per filter run; -1 for a
-        filter variant that does not exist for this candidate count)."""
+        later frame of the configuration uses them; then the inpainting tile
+        (ss_inpaint_autotune) on that reference's distance map reprojected as
+        the stitcher reprojects it, recorded for the map size.  Every variant
+        gives the same bits.  Returns {"variant": v, "ms": [...],
+        "filter_variant": f, "filter_ms": [...], "inpaint_variant": i,
+        "inpaint_ms": [...]} (ms per launch or per run; -1 for a filter variant
+        that does not exist for this candidate count)."""
         imgs = [_hwc(i) for i in images_to_match]
         if len(imgs) != len(self.calibrations):
             raise ValueError("expected %d images to match" % len(self.calibrations))
@@ -135,8 +138,25 @@ class RGBD_Estimator:
                                                         float(self.sigma_i), float(self.sigma_s), dist.data_ptr(),
                                                         int(repeats), self._stream(), fms, _lib.C.byref(fbest)))
         _TUNED_FILTER[(self.rows, self.cols, self.candidate_count, self._vtype)] = int(fbest.value)
-        return {"variant": int(best.value), "ms": [round(float(v), 4) for v in ms],
-                "filter_variant": int(fbest.value), "filter_ms": [round(float(v), 4) for v in fms]}
+        out = {"variant": int(best.value), "ms": [round(float(v), 4) for v in ms],
+               "filter_variant": int(fbest.value), "filter_ms": [round(float(v), 4) for v in fms]}
+        # the inpainting tile, on that distance map reprojected as the stitcher does
+        st = self.fishey_stitcher
+        if st.inpainting_iterations > 0:
+            rep = torch.empty([self.rows, self.cols], device=self.device)
+            work, tmp = torch.empty_like(rep), torch.empty_like(rep)
+            _lib.check(_lib.lib.ss_reproject(dist.data_ptr(), rep.data_ptr(), self.rows, self.cols, st.intrinsics[0],
+                                             _lib.float_array(st.translations_list[0]), self._stream()))
+            ni = _lib.lib.ss_inpaint_variant_count()
+            ims = (_lib.C.c_float * ni)()
+            ibest = _lib.C.c_int(0)
+            _lib.check(_lib.lib.ss_inpaint_autotune(rep.data_ptr(), work.data_ptr(), tmp.data_ptr(),
+                                                    st.inpainting_weights_list[0].data_ptr(), self.rows, self.cols,
+                                                    st.inpainting_iterations, st.thr_empty, st.thr_src,
+                                                    int(repeats), self._stream(), ims, _lib.C.byref(ibest)))
+            _TUNED_INPAINT[(self.rows, self.cols)] = int(ibest.value)
+            out.update(inpaint_variant=int(ibest.value), inpaint_ms=[round(float(v), 4) for v in ims])
+        return out
 
     def _config_key(self):
         """What the frames of one native batch share (ss_frame_batch_create
@@ -302,10 +322,11 @@ class RGBD_Estimator:
 # candidates, volume type) -> variant, for the sweep and for the cost filter
 _TUNED = {}
 _TUNED_FILTER = {}
+_TUNED_INPAINT = {}  # (rows, cols) -> inpainting tile variant
 
 
 def save_sweep_tuning(path):
-    """Writes the sweep and cost-filter variants tuned in this process
+    """Writes the sweep, cost-filter and inpainting variants tuned in this process
     (RGBD_Estimator.autotune, FrameBatch.autotune) to a JSON file, with the
     device they were measured on.  Returns the number of records written."""
     import json
@@ -316,15 +337,16 @@ def save_sweep_tuning(path):
         return [{"rows": k[0], "cols": k[1], "candidates": k[2], "volume_type": k[3], "variant": v}
                 for k, v in sorted(tab.items())]
 
-    data = {"device": dev, "sweep_variants": entries(_TUNED), "filter_variants": entries(_TUNED_FILTER)}
+    data = {"device": dev, "sweep_variants": entries(_TUNED), "filter_variants": entries(_TUNED_FILTER),
+            "inpaint_variants": [{"rows": k[0], "cols": k[1], "variant": v} for k, v in sorted(_TUNED_INPAINT.items())]}
     with open(path, "w") as f:
         json.dump(data, f, indent=1)
-    return len(data["sweep_variants"]) + len(data["filter_variants"])
+    return len(data["sweep_variants"]) + len(data["filter_variants"]) + len(data["inpaint_variants"])
 
 
 def load_sweep_tuning(path):
     """Applies the variants of a save_sweep_tuning file (ss_sweep_set_variant,
-    ss_isb_set_variant) when it was written on a device of the same name;
+    ss_isb_set_variant, ss_inpaint_set_variant) when it was written on a device of the same name;
     every later frame of those configurations uses them.  Returns the number
     of records applied."""
     import json
@@ -342,6 +364,11 @@ def load_sweep_tuning(path):
             _lib.check(setter(key[0], key[1], key[2], key[3], int(e["variant"])))
             tab[key] = int(e["variant"])
             n += 1
+    for e in data.get("inpaint_variants", []):
+        key = (int(e["rows"]), int(e["cols"]))
+        _lib.check(_lib.lib.ss_inpaint_set_variant(key[0], key[1], int(e["variant"])))
+        _TUNED_INPAINT[key] = int(e["variant"])
+        n += 1
     return n
 
 

@@ -199,14 +199,19 @@ def test_sweep_tuning_json_roundtrip(tmp_path):
         de._TUNED[(96, 80, 48, _lib.SS_VOLUME_F16)] = 3
         de._TUNED[(64, 64, 16, _lib.SS_VOLUME_F32)] = 1
         de._TUNED_FILTER[(96, 80, 48, _lib.SS_VOLUME_F16)] = 2
+        saved_i = dict(de._TUNED_INPAINT)
+        de._TUNED_INPAINT.clear()
+        de._TUNED_INPAINT[(96, 80)] = 1
         path = str(tmp_path / "tuning.json")
-        assert de.save_sweep_tuning(path) == 3
+        assert de.save_sweep_tuning(path) == 4
         data = json.load(open(path))
         assert {e["variant"] for e in data["sweep_variants"]} == {1, 3}
         assert [e["variant"] for e in data["filter_variants"]] == [2]
         assert lib.ss_sweep_get_variant(96, 80, 48, _lib.SS_VOLUME_F16) == 0
         assert lib.ss_isb_get_variant(96, 80, 48, _lib.SS_VOLUME_F16) == 0
-        assert de.load_sweep_tuning(path) == 3
+        assert lib.ss_inpaint_get_variant(96, 80) == 0
+        assert de.load_sweep_tuning(path) == 4
+        assert lib.ss_inpaint_get_variant(96, 80) == 1
         assert lib.ss_sweep_get_variant(96, 80, 48, _lib.SS_VOLUME_F16) == 3
         assert lib.ss_sweep_get_variant(64, 64, 16, _lib.SS_VOLUME_F32) == 1
         assert lib.ss_isb_get_variant(96, 80, 48, _lib.SS_VOLUME_F16) == 2
@@ -221,6 +226,9 @@ def test_sweep_tuning_json_roundtrip(tmp_path):
         de._TUNED.update(saved)
         de._TUNED_FILTER.clear()
         de._TUNED_FILTER.update(saved_f)
+        lib.ss_inpaint_set_variant(96, 80, -1)
+        de._TUNED_INPAINT.clear()
+        de._TUNED_INPAINT.update(saved_i)
 
 
 def test_isb_variants():
@@ -240,6 +248,10 @@ def test_isb_variants():
     assert lib.ss_isb_get_variant(64, 64, 64, 0) == 3
     assert lib.ss_isb_set_variant(64, 64, 64, 0, -1) == 0
     assert lib.ss_isb_get_variant(64, 64, 64, 0) == 0
+    tile = C.c_int()
+    assert lib.ss_inpaint_variant_count() == 3
+    assert [lib.ss_inpaint_variant_info(v, C.byref(tile)) == 0 and tile.value for v in range(3)] == [48, 32, 16]
+    assert lib.ss_inpaint_set_variant(100, 90, 3) == 1
 
 
 def test_isb_noalloc_limit_and_refusal():

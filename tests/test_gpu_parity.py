@@ -675,6 +675,9 @@ def test_sweep_variants_identical_and_autotune(gpu, N, vtype):
         assert 0 <= res["filter_variant"] < nf and len(res["filter_ms"]) == nf, res
         assert all(m > 0 for m in res["filter_ms"]) and res["filter_ms"][res["filter_variant"]] == min(res["filter_ms"])
         assert lib.ss_isb_get_variant(H, W, N, vt) == res["filter_variant"]
+        ni = lib.ss_inpaint_variant_count()
+        assert 0 <= res["inpaint_variant"] < ni and len(res["inpaint_ms"]) == ni and min(res["inpaint_ms"]) > 0, res
+        assert lib.ss_inpaint_get_variant(H, W) == res["inpaint_variant"]
         got = est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])
         torch.cuda.synchronize()
         assert torch.equal(got[0], want[0])
@@ -682,6 +685,7 @@ def test_sweep_variants_identical_and_autotune(gpu, N, vtype):
     finally:
         lib.ss_sweep_set_variant(H, W, N, vt, -1)
         lib.ss_isb_set_variant(H, W, N, vt, -1)
+        lib.ss_inpaint_set_variant(H, W, -1)
 
 
 @pytest.mark.parametrize("vtype", [0, 1])
@@ -744,3 +748,37 @@ def test_frame_filter_variants_identical(small, gpu, volume_dtype):
             assert_same(to_np(rgb), want_rgb, "rgb panorama, filter variant %d" % v)
     finally:
         lib.ss_isb_set_variant(72, 80, 32, vt, -1)
+
+
+@pytest.mark.parametrize("shape,iters", [((97, 150), 13), ((300, 420), 32)])
+def test_inpaint_tiles_identical(gpu, shape, iters):
+    """Every inpainting tile (48, 32, 16 pixels a side, 16-pixel halo) gives
+    the oracle's map: holes wider than the passes reach, holes at the
+    border, a band of NaNs."""
+    H, W = shape
+    lib = _lib.lib
+    intr = [0.3 * W, 0.3 * W, W / 2, H / 2, -0.18, 0.59]
+    t = [0.05, 0.02, -0.03]
+    rec = _lib.ss_intrinsics(*[float(np.float32(v)) for v in intr])
+    lut = torch.empty([H, W, 2], dtype=torch.uint8, device=gpu)
+    _lib.check(lib.ss_inpaint_lut(rec, _lib.float_array(t), H, W, 0.55, 100, lut.data_ptr(), _stream()))
+    want_lut = to_np(lut)
+    rng = np.random.RandomState(H + iters)
+    d = (1 + rng.rand(H, W) * 50).astype(np.float32)
+    d[H // 8:H // 2, W // 10:W // 2] = 1e8
+    d[H // 2:H // 2 + 4, :] = np.nan
+    for _ in range(20):
+        y, x, h, w = rng.randint(0, H), rng.randint(0, W), rng.randint(1, 40), rng.randint(1, 40)
+        d[y:y + h, x:x + w] = 1e8
+    d[rng.rand(H, W) < 0.03] = 1e8
+    te, ts = 100.1, float(np.float32(100) + np.float32(0.1))
+    want = O32.inpaint(d, want_lut, iters, te, ts)
+    try:
+        for v in range(lib.ss_inpaint_variant_count()):
+            assert lib.ss_inpaint_set_variant(H, W, v) == 0
+            m = torch.from_numpy(d).to(gpu)
+            tmp = torch.empty_like(m)
+            _lib.check(lib.ss_inpaint(m.data_ptr(), tmp.data_ptr(), lut.data_ptr(), H, W, iters, te, ts, _stream()))
+            assert_same(to_np(m), want, "inpainted map, tile variant %d" % v)
+    finally:
+        lib.ss_inpaint_set_variant(H, W, -1)
