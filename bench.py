@@ -137,7 +137,7 @@ def batching_form(args, wl, device, plain_fp32):
         torch.cuda.synchronize()
         gv = args.steps * args.batch / (time.perf_counter() - t0)
         out[dt] = {"value": round(v, 3), "unit": "panoramas/s", "ms_per_batch": round(1e3 * el / args.steps, 4),
-                   "vs_plain_fp32": round(v / plain_fp32, 4), "sweep_autotune": tuned[0],
+                   "vs_plain_fp32": round(v / plain_fp32, 4), "autotune": tuned[0],
                    "graph_replay": {"value": round(gv, 3), "unit": "panoramas/s",
                                     "vs_plain_fp32": round(gv / plain_fp32, 4)}}
         print("[bench] batching form, %d frames, %s volume: %.1f panoramas/s (graph replay %.1f)"
@@ -184,6 +184,8 @@ def batching_ragged(args, device):
     print("[bench] ragged batch rendered: %s" % [c for (_, _, _, _, c) in sets], file=sys.stderr, flush=True)
     frames = [(tm, ts) for (tm, ts, _, _, _) in sets]
     plain = [RGBD_Estimator(c, device=device, **kw) for (_, _, c, kw, _) in sets]
+    for e, (tm, _) in zip(plain, frames):  # tuned like the batched form below
+        e.autotune(tm)
     for _ in range(args.warmup):
         for e, (tm, ts) in zip(plain, frames):
             e.estimate_RGBD_panorama(tm, ts)
@@ -202,6 +204,8 @@ def batching_ragged(args, device):
     tuned = batch.autotune(frames)
     out = {"rigs": len(sets), "configurations": len(batch.groups),
            "sweep_variants": [t["variant"] for t in tuned],
+           "filter_variants": [t.get("filter_variant") for t in tuned],
+           "inpaint_variants": [t.get("inpaint_variant") for t in tuned],
            "shapes": ["%d^2 N%d R%d" % c for (_, _, _, _, c) in sets],
            "plain_fp32_sequential": {"value": round(v_plain, 3), "unit": "panoramas/s"}}
     for mode in ("eager", "graph"):
@@ -556,6 +560,9 @@ def main():
     device = torch.device("cuda:0")
     print("[bench] setting up %s" % args.workload, file=sys.stderr, flush=True)
     est, to_match, to_stitch, calibs, kw = setup(wl, device, args.frames_cache)
+    # the autotuner's variants (sweep, cost filter, inpainting tile: identical
+    # bits) for this configuration, fp32 and the 16-bit volume types
+    tuned_plain = est.autotune(to_match)
     for _ in range(args.warmup):
         est.estimate_RGBD_panorama(to_match, to_stitch)
     torch.cuda.synchronize()
@@ -612,6 +619,7 @@ def main():
     forms = {}
     for dt in ("float16", "bfloat16"):
         est_h = RGBD_Estimator(calibs, device=device, volume_dtype=dt, **kw)
+        tuned_h = est_h.autotune(to_match)
         for _ in range(args.warmup):
             est_h.estimate_RGBD_panorama(to_match, to_stitch)
         torch.cuda.synchronize()
@@ -625,7 +633,7 @@ def main():
         h_el = time.perf_counter() - h0
         forms[dt] = {"value": round(args.steps / h_el, 3), "unit": "panoramas/s",
                      "ms_per_step": round(1e3 * h_el / args.steps, 4),
-                     "vs_fp32": round((args.steps / h_el) / value, 4)}
+                     "vs_fp32": round((args.steps / h_el) / value, 4), "autotune": tuned_h}
         print("[bench] %s volume: %.1f panoramas/s" % (dt, args.steps / h_el), file=sys.stderr, flush=True)
         del est_h
 
@@ -633,11 +641,17 @@ def main():
     ragged = batching_ragged(args, device) if not args.no_ragged else None
     diff_form = differentiation_form(args, device) if not args.no_diff_form else None
 
+    # HBM traffic and VALU counters of the dominant kernel: rocprofv3 counter
+    # runs (--pmc) cannot share a run with this timing, so they are read from
+    # the committed profiles of an earlier counter run of this workload
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get(dominant)
+        if traffic is not None:
+            traffic = dict(traffic, source="profiles/traffic_%s.json (rocprofv3 FETCH_SIZE / WRITE_SIZE runs, not "
+                                           "this run)" % args.workload)
     # the dominant kernel's vector-ALU occupancy (rocprofv3 VALUBusy and
     # VALU instructions per launch, profiles/valu_<workload>.json): the sweep
     # is bound by its fp32 VALU issue, not by HBM
@@ -646,6 +660,9 @@ def main():
     if os.path.exists(vpath):
         with open(vpath) as f:
             valu = json.load(f).get(dominant)
+        if valu is not None:
+            valu = dict(valu, source="profiles/valu_%s.json (rocprofv3 SQ_INSTS_VALU / VALUBusy runs, not this run)"
+                        % args.workload)
 
     out = {
         "metric": METRIC,
@@ -671,6 +688,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "avg_launch_ms": round(avg_launch_ms, 5), "algorithmic_bytes_per_launch": int(bytes_per_launch),
                      "valu": valu},
+        "autotune": tuned_plain,
         "stage_ms_per_frame": {k: round(v, 4) for k, v in per_frame.items() if v > 0},
         "sweep_filter": sweep_filter_summary(wl, per_frame),
         "graph_replay": {"value": round(args.steps / g_elapsed, 3), "unit": "panoramas/s",

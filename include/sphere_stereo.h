@@ -242,16 +242,17 @@ int ss_inpaint(float *d_map, float *d_tmp, const uint8_t *d_lut, int H, int W, i
  * frame executors (ss_frame_create / ss_frame_batch_create) stage tiles of
  * ss_inpaint_variant_info(v) pixels a side (48, 32, 16) with a 16-pixel halo;
  * every tile gives the same map.  The variant recorded for the map size is
- * used, 0 if none.  ss_inpaint_autotune times every variant on copies of
- * d_map (into d_work, with d_tmp as scratch; `repeats` runs each, not while
- * the stream is being captured), records the fastest for (H, W) and returns
- * it; ms_out (nullable) receives the per-run ms of each. */
+ * used, 0 if none.  ss_inpaint_autotune times every variant on nmaps copies
+ * of d_map inpainted in one batch, as a frame's references are (into d_work,
+ * with d_tmp as scratch, nmaps x H x W floats each; `repeats` runs each, not
+ * while the stream is being captured), records the fastest for (H, W) and
+ * returns it; ms_out (nullable) receives the per-run ms of each. */
 int ss_inpaint_variant_count(void);
 int ss_inpaint_variant_info(int variant, int *tile);
 int ss_inpaint_set_variant(int H, int W, int variant); /* -1 forgets the record */
 int ss_inpaint_get_variant(int H, int W);
-int ss_inpaint_autotune(const float *d_map, float *d_work, float *d_tmp, const uint8_t *d_lut, int H, int W,
-                        int iterations, double thr_empty, double thr_src, int repeats, ss_stream stream,
+int ss_inpaint_autotune(const float *d_map, int nmaps, float *d_work, float *d_tmp, const uint8_t *d_lut, int H,
+                        int W, int iterations, double thr_empty, double thr_src, int repeats, ss_stream stream,
                         float *ms_out, int *best_out);
 
 /* mergeRGBDPanoramaKernel (stitcher.cu:423-474).  d_reproj, d_dists, d_imgs:

@@ -303,6 +303,18 @@ bool sweep_fast_bounds(const ss::SweepArgs& a, int C, int npairs) {
   return ss::sweep_fast_bounds(a.cam, C, &a.rt[0][0], npairs, a.cands, a.N);
 }
 
+// The autotuners keep variant 0 (the default) unless another is at least 2 %
+// faster: closer timings are within run-to-run noise.
+constexpr float kTuneMargin = 0.98f;
+
+// The fastest variant of ms[0..n) (ms < 0: not run) against the default.
+int pick_variant(const float* ms, int n) {
+  int best = 0;
+  for (int v = 1; v < n; ++v)
+    if (ms[v] >= 0.0f && ms[v] < ms[best]) best = v;
+  return ms[best] < kTuneMargin * ms[0] ? best : 0;
+}
+
 // The sweep variant chosen per configuration by ss_sweep_autotune (or
 // ss_sweep_set_variant); configurations never tuned use variant 0.
 struct SweepTune {
@@ -637,8 +649,8 @@ int ss_sweep_autotune(const float* d_raylut, const float* const* d_images, int C
   const float* rt[1] = {rts};
   void* cost[1] = {d_cost};
   const int nv = ss::sweep_variant_count();
-  int best = 0;
-  float best_ms = 0.0f;
+  float msv[16];
+  for (int v = 0; v < 16; ++v) msv[v] = -1.0f;
   for (int v = 0; v < nv && rc == SS_OK; ++v) {
     rc = launch_sweep_refs(lut, d_images, C, cams, rt, &ref, 1, cands, N, H, W, vtype, cost, nullptr, s, v);
     if (rc) break;
@@ -657,14 +669,12 @@ int ss_sweep_autotune(const float* d_raylut, const float* const* d_images, int C
     }
     ms /= (float)repeats;
     if (ms_out) ms_out[v] = ms;
-    if (v == 0 || ms < best_ms) {
-      best = v;
-      best_ms = ms;
-    }
+    msv[v] = ms;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
+  const int best = pick_variant(msv, nv);
   set_sweep_variant(H, W, N, vtype, best);
   if (best_out) *best_out = best;
   return ok();
@@ -936,9 +946,10 @@ int time_filter_variants(void* ws, int H, int W, int N, int vt, const float* can
   SS_HIP(hipEventCreate(&e1));
   void* const wss[1] = {ws};
   float* const dists[1] = {d_dist};
-  int rc = SS_OK, best = 0;
-  float best_ms = -1.0f;
+  int rc = SS_OK;
+  float msv[kIsbVariants];
   for (int v = 0; v < kIsbVariants && rc == SS_OK; ++v) {
+    msv[v] = -1.0f;
     if (ms_out) ms_out[v] = -1.0f;
     if (!isb_variant_ok(v, N)) continue;
     rc = estimate_after_sweep(1, H, W, N, vt, cands, sigma_i, sigma_s, dists, wss, s, nullptr, v, false);
@@ -958,14 +969,12 @@ int time_filter_variants(void* ws, int H, int W, int N, int vt, const float* can
     }
     ms /= (float)repeats;
     if (ms_out) ms_out[v] = ms;
-    if (best_ms < 0.0f || ms < best_ms) {
-      best = v;
-      best_ms = ms;
-    }
+    msv[v] = ms;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
+  const int best = pick_variant(msv, kIsbVariants);
   set_isb_variant(H, W, N, vt, best);
   if (best_out) *best_out = best;
   return SS_OK;
@@ -1148,11 +1157,12 @@ int ss_inpaint_set_variant(int H, int W, int variant) {
 
 int ss_inpaint_get_variant(int H, int W) { return inpaint_variant_for(H, W); }
 
-int ss_inpaint_autotune(const float* d_map, float* d_work, float* d_tmp, const uint8_t* d_lut, int H, int W,
-                        int iterations, double thr_empty, double thr_src, int repeats, ss_stream stream,
+int ss_inpaint_autotune(const float* d_map, int nmaps, float* d_work, float* d_tmp, const uint8_t* d_lut, int H,
+                        int W, int iterations, double thr_empty, double thr_src, int repeats, ss_stream stream,
                         float* ms_out, int* best_out) {
   SS_REQUIRE(d_map && d_work && d_tmp && d_lut && H >= 1 && W >= 1 && iterations >= 1, "bad argument");
-  SS_REQUIRE(d_map != d_work && d_work != d_tmp, "the map, its working copy and the scratch must differ");
+  SS_REQUIRE(nmaps >= 1 && nmaps <= ss::kMaxBatch, "map count %d outside [1, %d]", nmaps, ss::kMaxBatch);
+  SS_REQUIRE(d_map != d_work && d_work != d_tmp, "the map, its working copies and the scratch must differ");
   SS_REQUIRE(repeats >= 1 && repeats <= 1000, "repeats %d outside [1, 1000]", repeats);
   hipStream_t s = as_stream(stream);
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -1162,15 +1172,24 @@ int ss_inpaint_autotune(const float* d_map, float* d_work, float* d_tmp, const u
   SS_HIP(hipEventCreate(&e0));
   SS_HIP(hipEventCreate(&e1));
   const size_t bytes = (size_t)H * W * sizeof(float);
-  int rc = SS_OK, best = 0;
-  float best_ms = -1.0f;
-  // each run inpaints a fresh copy of the map: the copy costs every variant the same
+  int rc = SS_OK;
+  float msv[8];
+  for (int v = 0; v < 8; ++v) msv[v] = -1.0f;
+  // each run inpaints fresh copies of the map, nmaps in one batch as the frame
+  // executors run a frame's references: the copies cost every variant the same
+  float* m[ss::kMaxBatch];
+  float* t[ss::kMaxBatch];
+  const uint8_t* l[ss::kMaxBatch];
+  for (int b = 0; b < nmaps; ++b) {
+    m[b] = d_work + (size_t)b * H * W;
+    t[b] = d_tmp + (size_t)b * H * W;
+    l[b] = d_lut;
+  }
   auto run = [&](int v) -> int {
-    if (hipMemcpyAsync(d_work, d_map, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
-      return fail(SS_ERR_HIP, "copying the map failed");
-    float* m = d_work;
-    float* t = d_tmp;
-    return ss::inpaint_maps_v(1, &m, &t, &d_lut, H, W, iterations, thr_empty, thr_src, v, s);
+    for (int b = 0; b < nmaps; ++b)
+      if (hipMemcpyAsync(m[b], d_map, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return fail(SS_ERR_HIP, "copying the map failed");
+    return ss::inpaint_maps_v(nmaps, m, t, l, H, W, iterations, thr_empty, thr_src, v, s);
   };
   for (int v = 0; v < ss::inpaint_tile_count() && rc == SS_OK; ++v) {
     rc = run(v);
@@ -1189,14 +1208,12 @@ int ss_inpaint_autotune(const float* d_map, float* d_work, float* d_tmp, const u
     }
     ms /= (float)repeats;
     if (ms_out) ms_out[v] = ms;
-    if (best_ms < 0.0f || ms < best_ms) {
-      best = v;
-      best_ms = ms;
-    }
+    msv[v] = ms;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
+  const int best = pick_variant(msv, ss::inpaint_tile_count());
   set_inpaint_variant(H, W, best);
   if (best_out) *best_out = best;
   return ok();

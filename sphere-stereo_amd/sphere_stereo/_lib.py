@@ -13,6 +13,7 @@ SS_MAX_CAMERAS = 8
 SS_MAX_CANDIDATES = 128
 SS_MAX_REFERENCES = 8
 SS_MAX_BATCH_FRAMES = 64
+SS_BATCH_PAIRS = 16  # frame-reference pairs per launch of a native batch (chunks of ss_frame_batch_run)
 SS_RAGGED_PERSISTENT, SS_RAGGED_DISPATCHED, SS_RAGGED_AUTO = 0, 1, 2
 SS_INPUT_U8, SS_INPUT_U16, SS_INPUT_F32 = 0, 1, 2
 
@@ -79,7 +80,7 @@ SIGNATURES = {
     "ss_inpaint_variant_info": (_i, [_i, _p]),
     "ss_inpaint_set_variant": (_i, [_i, _i, _i]),
     "ss_inpaint_get_variant": (_i, [_i, _i]),
-    "ss_inpaint_autotune": (_i, [_p, _p, _p, _p, _i, _i, _i, _d, _d, _i, _p, _p, _p]),
+    "ss_inpaint_autotune": (_i, [_p, _i, _p, _p, _p, _i, _i, _i, _d, _d, _i, _p, _p, _p]),
     "ss_isb_variant_info": (_i, [_i, _p, _p]),
     "ss_isb_set_variant": (_i, [_i, _i, _i, _i, _i]),
     "ss_isb_get_variant": (_i, [_i, _i, _i, _i]),

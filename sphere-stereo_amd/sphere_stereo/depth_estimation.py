@@ -96,7 +96,7 @@ class RGBD_Estimator:
         _lib.check(_lib.lib.ss_frame_create(_lib.C.byref(d), _lib.C.byref(handle)))
         return handle
 
-    def autotune(self, images_to_match, repeats=5):
+    def autotune(self, images_to_match, repeats=5, maps=None):
         """Times the scheduling variants of the sweep (ss_sweep_autotune:
         candidate chunking and LDS staging) and of the cost filter
         (ss_estimate_filter_autotune: lanes per pixel and tile order, timed
@@ -104,7 +104,9 @@ class RGBD_Estimator:
         images, and records the fastest of each for this configuration; every
         later frame of the configuration uses them; then the inpainting tile
         (ss_inpaint_autotune) on that reference's distance map reprojected as
-        the stitcher reprojects it, recorded for the map size.  Every variant
+        the stitcher reprojects it, ``maps`` copies inpainted together (default:
+        the number of references, as a frame runs them), recorded for the map
+        size.  Every variant
         gives the same bits.  Returns {"variant": v, "ms": [...],
         "filter_variant": f, "filter_ms": [...], "inpaint_variant": i,
         "inpaint_ms": [...]} (ms per launch or per run; -1 for a filter variant
@@ -143,14 +145,16 @@ class RGBD_Estimator:
         # the inpainting tile, on that distance map reprojected as the stitcher does
         st = self.fishey_stitcher
         if st.inpainting_iterations > 0:
+            nm = len(self.references_indices) if maps is None else int(maps)
             rep = torch.empty([self.rows, self.cols], device=self.device)
-            work, tmp = torch.empty_like(rep), torch.empty_like(rep)
+            work = torch.empty([nm, self.rows, self.cols], device=self.device)
+            tmp = torch.empty_like(work)
             _lib.check(_lib.lib.ss_reproject(dist.data_ptr(), rep.data_ptr(), self.rows, self.cols, st.intrinsics[0],
                                              _lib.float_array(st.translations_list[0]), self._stream()))
             ni = _lib.lib.ss_inpaint_variant_count()
             ims = (_lib.C.c_float * ni)()
             ibest = _lib.C.c_int(0)
-            _lib.check(_lib.lib.ss_inpaint_autotune(rep.data_ptr(), work.data_ptr(), tmp.data_ptr(),
+            _lib.check(_lib.lib.ss_inpaint_autotune(rep.data_ptr(), nm, work.data_ptr(), tmp.data_ptr(),
                                                     st.inpainting_weights_list[0].data_ptr(), self.rows, self.cols,
                                                     st.inpainting_iterations, st.thr_empty, st.thr_src,
                                                     int(repeats), self._stream(), ims, _lib.C.byref(ibest)))
@@ -468,7 +472,12 @@ class FrameBatch:
             return out
         out = []
         for idx in self.groups:
-            out.append(self.estimators[idx[0]].autotune(frames[idx[0]][0], repeats))
+            e = self.estimators[idx[0]]
+            # a native batch runs its frames in chunks of up to SS_BATCH_PAIRS
+            # frame-reference pairs: the maps one inpainting launch takes
+            R = len(e.references_indices)
+            maps = min(len(idx), _lib.SS_BATCH_PAIRS // R) * R
+            out.append(e.autotune(frames[idx[0]][0], repeats, maps=maps))
         for i, h in enumerate(self._frames):
             # recreate the native batches: a captured graph keeps its kernels
             _lib.lib.ss_frame_destroy(h)

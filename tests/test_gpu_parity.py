@@ -673,7 +673,11 @@ def test_sweep_variants_identical_and_autotune(gpu, N, vtype):
         assert lib.ss_sweep_get_variant(H, W, N, vt) == res["variant"]
         nf = lib.ss_isb_variant_count()
         assert 0 <= res["filter_variant"] < nf and len(res["filter_ms"]) == nf, res
-        assert all(m > 0 for m in res["filter_ms"]) and res["filter_ms"][res["filter_variant"]] == min(res["filter_ms"])
+        fm = res["filter_ms"]
+        assert all(m > 0 for m in fm)
+        # the fastest, unless it is within 2 % of the default (variant 0)
+        fv = res["filter_variant"]
+        assert (fv == 0 and min(fm) >= 0.98 * fm[0] - 1e-4) or (fm[fv] == min(fm) and fm[fv] < 0.98 * fm[0] + 1e-4)
         assert lib.ss_isb_get_variant(H, W, N, vt) == res["filter_variant"]
         ni = lib.ss_inpaint_variant_count()
         assert 0 <= res["inpaint_variant"] < ni and len(res["inpaint_ms"]) == ni and min(res["inpaint_ms"]) > 0, res
