@@ -5,9 +5,11 @@ metric part of utils.evaluate_rgbd_panorama (utils.py:233-241): mean absolute
 error, root mean square error and bad pixel ratios (in percent) of an
 evaluated inverse-distance map against a reference one.  The per-pixel error
 is the fp32 ``|a - b|`` of the reference; the sums are fp64 in a fixed order.
+``psnr`` is the panorama colour's PSNR of utils.py:229-232 (on the host).
 """
 import math
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -46,3 +48,17 @@ def error_stats(evaluated, reference, thresholds=(0.1, 0.4), finite_only=True):
     return {"mae": s[0] / count, "rmse": math.sqrt(s[1] / count),
             "bad_px_ratios": [100.0 * s[3 + k] / count for k in range(len(thresholds))],
             "count": count, "skipped": n - count}
+
+
+def psnr(gt_rgb, rgb):
+    """utils.py:229-232's peak_signal_noise_ratio(gt / 255, rgb / 255) of two
+    uint8 images of one shape, as skimage computes it for float32 images in
+    [0, 1]: data range 1, the squared differences in float32, their mean in
+    float64, 10 * log10(1 / mean)."""
+    a = np.asarray(gt_rgb)
+    b = np.asarray(rgb)
+    if a.shape != b.shape or a.dtype != np.uint8 or b.dtype != np.uint8:
+        raise ValueError("psnr expects two uint8 images of one shape")
+    d = a.astype(np.float32) / np.float32(255) - b.astype(np.float32) / np.float32(255)
+    mse = np.mean(d * d, dtype=np.float64)
+    return float(10 * np.log10(1.0 / mse)) if mse > 0 else float("inf")

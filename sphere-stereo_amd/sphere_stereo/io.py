@@ -74,10 +74,41 @@ def encode_exr(img):
     return head + offsets + blocks
 
 
+def png_bit_depth(data):
+    """(bit depth, colour type) from a PNG file's IHDR chunk, or None when the
+    bytes are not a PNG."""
+    if len(data) < 33 or data[:8] != b"\x89PNG\r\n\x1a\n" or data[12:16] != b"IHDR":
+        return None
+    return data[24], data[25]
+
+
+# OpenEXR compression codes: none, ZIPS (one scanline per block), ZIP (16)
+_EXR_LINES_PER_BLOCK = {0: 1, 2: 1, 3: 16}
+
+
+def _exr_unzip(block, size):
+    """OpenEXR ZIP / ZIPS block data: zlib, then the byte-delta predictor
+    (t[i] = t[i-1] + d[i] - 128 mod 256) and the split of even and odd bytes
+    into two halves undone (OpenEXR file layout, "ZIP compression")."""
+    d = np.frombuffer(zlib.decompress(block), np.uint8).astype(np.int64)
+    if d.size != size:
+        raise ValueError("EXR block decompresses to %d bytes, %d expected" % (d.size, size))
+    if d.size:
+        d[1:] -= 128
+        d = np.cumsum(d) & 255
+    t = d.astype(np.uint8)
+    out = np.empty_like(t)
+    half = (t.size + 1) // 2
+    out[0::2] = t[:half]
+    out[1::2] = t[half:]
+    return out.tobytes()
+
+
 def decode_exr(data):
     """Inverse-distance ground truth: the Y channel of a single-part scanline
-    OpenEXR file without compression and with 32-bit float pixels (as
-    encode_exr writes); None for any other EXR."""
+    OpenEXR file with 32-bit float pixels, uncompressed (as encode_exr writes)
+    or ZIP / ZIPS compressed (cv2.imwrite's default for EXR); None for any
+    other EXR."""
     if len(data) < 8 or struct.unpack_from("<ii", data, 0) != (20000630, 2):
         return None
     pos, attrs = 8, {}
@@ -88,16 +119,23 @@ def decode_exr(data):
         attrs[data[pos:end].decode()] = data[tend + 5:tend + 5 + size]
         pos = tend + 5 + size
     pos += 1
-    if attrs.get("compression") != b"\0" or attrs.get("channels", b"")[:2] != b"Y\0":
+    comp = attrs.get("compression", b"\xff")[0]
+    if comp not in _EXR_LINES_PER_BLOCK or attrs.get("channels", b"")[:2] != b"Y\0":
         return None
     if struct.unpack_from("<i", attrs["channels"], 2)[0] != 2 or attrs["channels"][18:19] != b"\0":
         return None
     x0, y0, x1, y1 = struct.unpack("<iiii", attrs["dataWindow"])
     w, h = x1 - x0 + 1, y1 - y0 + 1
+    lines = _EXR_LINES_PER_BLOCK[comp]
+    nblocks = (h + lines - 1) // lines
     out = np.empty((h, w), np.float32)
-    for off in struct.unpack_from("<%dQ" % h, data, pos):
-        y = struct.unpack_from("<i", data, off)[0]
-        out[y - y0] = np.frombuffer(data, "<f4", w, off + 8)
+    for off in struct.unpack_from("<%dQ" % nblocks, data, pos):
+        y, size = struct.unpack_from("<ii", data, off)
+        n = min(lines, y0 + h - y)
+        raw = data[off + 8:off + 8 + size]
+        if comp != 0 and size < 4 * w * n:  # a block stored uncompressed when that is not larger
+            raw = _exr_unzip(raw, 4 * w * n)
+        out[y - y0:y - y0 + n] = np.frombuffer(raw, "<f4", w * n).reshape(n, w)
     return out
 
 

@@ -7,12 +7,19 @@ go to DIR/output/rgb_<frame>.png and inv_distance_<frame>.exr.
 Differences, because cv2, skimage and a display are not available: images
 are decoded with PIL (8-bit and 16-bit greyscale, 8-bit RGB/RGBA as
 cv2.IMREAD_UNCHANGED gives them, RGB turned to cv2's BGR order; other files
-count as unreadable, as cv2.imread's None does there); a mask is resized in
-float32 and rounded to uint8, where cv2 resizes uint8 in fixed point
-(parity unpinned); --visualize is ignored; --evaluate reports the
-inverse-distance MAE / RMSE / bad-pixel ratios (utils.py:233-242, on the GPU)
-against DIR/gt/inv_distance_<frame>.exr when that file is an uncompressed
-OpenEXR this package can read, and no SSIM / PSNR (skimage).
+count as unreadable, as cv2.imread's None does there; 16-bit colour PNGs,
+whose low bytes PIL drops, are refused with a warning); a mask is resized in
+float32 and rounded to uint8, where cv2 resizes uint8 in fixed point (parity
+unpinned; on the README demo frame some mask pixels differ by one level and
+the panorama's PSNR against the reference's output moves by 0.001 dB);
+--visualize is ignored; --evaluate reports the inverse-distance MAE / RMSE /
+bad-pixel ratios (utils.py:233-242, on the GPU) against
+DIR/gt/inv_distance_<frame>.exr (single-channel float32 OpenEXR, uncompressed
+or ZIP / ZIPS; resized to the panorama as utils.py:226-227 does), raises when
+no frame has a readable one, adds the colour PSNR (utils.py:232) when
+DIR/gt/rgb_<frame>.png is readable (resized in float32 and rounded where cv2
+resizes uint8: parity unpinned when the sizes differ) and reports no SSIM
+(skimage's windowed statistics, not restated here).
 """
 import argparse
 import json
@@ -24,14 +31,27 @@ import torch
 from PIL import Image
 
 from .depth_estimation import RGBD_Estimator
-from .evaluation import error_stats
-from .io import decode_exr, save_rgbd_panorama
+from .evaluation import error_stats, psnr
+from .io import decode_exr, png_bit_depth, save_rgbd_panorama
 from .utils import parse_json_calib, prepare_inputs, resize_linear
 
 
 def read_image(path):
     """cv2.imread(path, cv2.IMREAD_UNCHANGED) for the files PIL decodes:
-    HW (greyscale) or HWC in BGR / BGRA order; None when unreadable."""
+    HW (greyscale) or HWC in BGR / BGRA order; None when unreadable.  A PNG
+    of 16-bit colour, which PIL would reduce to its high bytes (cv2 keeps
+    uint16, which read_input_images divides by 255), is refused with a
+    warning instead (decode it with the caller's own reader and hand the
+    uint16 array to prepare_inputs)."""
+    try:
+        with open(path, "rb") as f:
+            head = f.read(64)
+    except OSError:
+        return None
+    depth = png_bit_depth(head)
+    if depth is not None and depth[0] == 16 and depth[1] in (2, 6):
+        warnings.warn("16-bit colour PNG not supported by this reader: " + path)
+        return None
     try:
         with Image.open(path) as im:
             if im.mode not in ("L", "I;16", "RGB", "RGBA"):
@@ -129,15 +149,28 @@ def run(args):
     if not args.evaluate:
         return panoramas
     metrics = {}
+    pc, pr = args.panorama_resolution
     for filename, pano in panoramas.items():
         gt_path = os.path.join(args.dataset_path, "gt", "inv_distance_" + os.path.splitext(filename)[0] + ".exr")
         gt = decode_exr(open(gt_path, "rb").read()) if os.path.isfile(gt_path) else None
-        if gt is None or gt.shape != pano["inv_distance"].shape:
-            warnings.warn("Invalid ground truth for file " + filename)
+        if gt is None:
+            warnings.warn("Invalid ground truth for file " + filename + ". Will be ignored for evaluation")
             continue
-        s = error_stats(torch.from_numpy(pano["inv_distance"]).to(args.device), torch.from_numpy(gt).to(args.device),
-                        args.bad_px_ratio_thresholds, finite_only=False)
+        gt = torch.from_numpy(gt).to(args.device)
+        if tuple(gt.shape) != (pr, pc):
+            gt = resize_linear(gt, pc, pr)  # utils.py:226-227 resizes it to the panorama
+        s = error_stats(torch.from_numpy(pano["inv_distance"]).to(args.device), gt, args.bad_px_ratio_thresholds,
+                        finite_only=False)
+        gt_rgb = read_image(os.path.join(args.dataset_path, "gt", "rgb_" + os.path.splitext(filename)[0] + ".png"))
+        if gt_rgb is not None and gt_rgb.dtype == np.uint8 and gt_rgb.ndim == 3 and gt_rgb.shape[2] == 3:
+            if gt_rgb.shape[:2] != (pr, pc):
+                m = resize_linear(torch.from_numpy(gt_rgb).to(args.device), pc, pr)
+                gt_rgb = torch.round(m).clamp(0, 255).to(torch.uint8).cpu().numpy()
+            s["psnr"] = psnr(gt_rgb, pano["rgb"])
         metrics[filename] = s
+    if panoramas and not metrics:
+        raise RuntimeError("--evaluate: no frame has a readable ground truth under %s/gt (inv_distance_<frame>.exr, "
+                           "single-channel float32, uncompressed or ZIP)" % args.dataset_path)
     return panoramas, metrics
 
 

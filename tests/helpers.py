@@ -118,3 +118,63 @@ def oracle_frame(calibs, kw, to_match, to_stitch, **extra):
     keep = {}
     rgb, dist = ora.estimate_RGBD_panorama([t.cpu() for t in to_match], [t.cpu() for t in to_stitch], keep=keep)
     return ora, keep, rgb, dist
+
+
+def encode_exr_zip(img, lines=16):
+    """A single-channel ("Y", float32) OpenEXR file with ZIP (lines=16) or ZIPS
+    (lines=1) compression, as cv2.imwrite writes EXR by default, written from
+    the OpenEXR 2 file layout with plain loops: per block of scanlines the
+    bytes are split into the even-indexed ones followed by the odd-indexed
+    ones, each byte after the first replaced by (t[i] - t[i-1] + 128) mod 256,
+    and the result zlib-compressed (stored raw when that is not smaller)."""
+    import struct
+    import zlib
+
+    a = np.ascontiguousarray(img, dtype="<f4")
+    h, w = a.shape
+
+    def attr(name, kind, value):
+        return name.encode() + b"\0" + kind.encode() + b"\0" + struct.pack("<i", len(value)) + value
+
+    box = struct.pack("<iiii", 0, 0, w - 1, h - 1)
+    code = {16: 3, 1: 2}[lines]
+    header = (attr("channels", "chlist", b"Y\0" + struct.pack("<iB3xii", 2, 0, 1, 1) + b"\0") +
+              attr("compression", "compression", bytes([code])) + attr("dataWindow", "box2i", box) +
+              attr("displayWindow", "box2i", box) + attr("lineOrder", "lineOrder", b"\0") +
+              attr("pixelAspectRatio", "float", struct.pack("<f", 1)) +
+              attr("screenWindowCenter", "v2f", struct.pack("<ff", 0, 0)) +
+              attr("screenWindowWidth", "float", struct.pack("<f", 1)) + b"\0")
+    head = struct.pack("<ii", 20000630, 2) + header
+    blocks = []
+    for y in range(0, h, lines):
+        raw = a[y:y + lines].tobytes()
+        split = raw[0::2] + raw[1::2]
+        pred = bytearray(split)
+        for i in range(1, len(split)):
+            pred[i] = (split[i] - split[i - 1] + 128) % 256
+        packed = zlib.compress(bytes(pred), 6)
+        if len(packed) >= len(raw):
+            packed = raw
+        blocks.append(struct.pack("<ii", y, len(packed)) + packed)
+    offsets, pos = [], len(head) + 8 * len(blocks)
+    for b in blocks:
+        offsets.append(pos)
+        pos += len(b)
+    return head + struct.pack("<%dQ" % len(blocks), *offsets) + b"".join(blocks)
+
+
+def encode_png16(rgb):
+    """A 16-bit RGB PNG (colour type 2, filter type 0, big-endian samples)
+    from an H x W x 3 uint16 array."""
+    import struct
+    import zlib
+
+    h, w, _ = rgb.shape
+    rows = np.ascontiguousarray(rgb.astype(">u2")).view(np.uint8).reshape(h, -1)
+    raw = np.concatenate([np.zeros((h, 1), np.uint8), rows], axis=1).tobytes()
+
+    def chunk(kind, data):
+        return struct.pack(">I", len(data)) + kind + data + struct.pack(">I", zlib.crc32(kind + data) & 0xFFFFFFFF)
+
+    return (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 16, 2, 0, 0, 0)) +
+            chunk(b"IDAT", zlib.compress(raw)) + chunk(b"IEND", b""))

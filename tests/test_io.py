@@ -109,6 +109,50 @@ def test_decode_exr_reads_the_encoder_and_refuses_others():
     assert decode_exr(b"not an exr") is None
 
 
+def _exr_blocks(data):
+    """(first scanline, stored size) of each block of an EXR file whose
+    header ends with the screenWindowWidth attribute (as the writers here
+    order it)."""
+    key = b"screenWindowWidth\0float\0"
+    table = data.index(key) + len(key) + 4 + 4 + 1  # size, value, end of header
+    out, pos = [], table
+    while True:
+        off = struct.unpack_from("<Q", data, pos)[0]
+        out.append(struct.unpack_from("<ii", data, off))
+        pos += 8
+        if off + 8 + out[-1][1] == len(data):
+            return out
+
+
+def test_decode_exr_zip_and_zips():
+    """ZIP (16 scanlines per block, cv2.imwrite's default) and ZIPS (one)
+    blocks, written by a loop-based writer of the OpenEXR layout
+    (helpers.encode_exr_zip), decode bit for bit: a height that is not a
+    multiple of 16, non-finite values and -0.0, an image of few distinct
+    values (every block compresses, so the predictor and the byte split are
+    exercised), a constant one and one of random bits (blocks that do not
+    compress are stored raw).  No EXR library is in the image, so the match
+    with cv2's own files is unpinned beyond that layout."""
+    from helpers import encode_exr_zip
+    from sphere_stereo.io import decode_exr
+
+    rng = np.random.RandomState(5)
+    steps = (np.floor(np.cumsum(rng.rand(37, 21), axis=1) * 4) / 4).astype(np.float32)
+    steps[2, 3], steps[30, 1], steps[36, 20], steps[0, 0] = np.nan, np.inf, -np.inf, -0.0
+    noise = rng.randint(0, 2 ** 32, (19, 11), dtype=np.uint64).astype(np.uint32).view(np.float32)
+    const = np.full((17, 5), 0.25, np.float32)
+    for img, packed in ((steps, True), (const, True), (noise, False)):
+        for lines in (16, 1):
+            data = encode_exr_zip(img, lines)
+            blocks = _exr_blocks(data)
+            assert [y for y, _ in blocks] == list(range(0, img.shape[0], lines))
+            raw = [4 * img.shape[1] * min(lines, img.shape[0] - y) for y, _ in blocks]
+            assert all((size < r) == packed for (_, size), r in zip(blocks, raw))
+            got = decode_exr(data)
+            assert got is not None and got.shape == img.shape
+            np.testing.assert_array_equal(got.view(np.uint32), img.view(np.uint32))
+
+
 def test_dataset_loop_reader_and_arguments(tmp_path):
     """sphere_stereo.main's host side: images decoded as cv2.IMREAD_UNCHANGED
     gives them (BGR, greyscale, BGRA kept 4-channel so the frame's shape check
@@ -125,6 +169,19 @@ def test_dataset_loop_reader_and_arguments(tmp_path):
     rgba = rng.randint(0, 256, (3, 4, 4)).astype(np.uint8)
     Image.fromarray(rgba, "RGBA").save(tmp_path / "a.png")
     np.testing.assert_array_equal(ss_main.read_image(str(tmp_path / "a.png")), rgba[:, :, [2, 1, 0, 3]])
+    # 16-bit colour: PIL would keep only the high bytes; refused with a warning
+    # (16-bit greyscale decodes as uint16, as cv2.IMREAD_UNCHANGED gives it)
+    from helpers import encode_png16
+    rgb16 = rng.randint(0, 65536, (3, 5, 3)).astype(np.uint16)
+    (tmp_path / "c16.png").write_bytes(encode_png16(rgb16))
+    assert np.array_equal(np.array(Image.open(tmp_path / "c16.png")), (rgb16 >> 8).astype(np.uint8))
+    with pytest.warns(UserWarning, match="16-bit colour PNG"):
+        assert ss_main.read_image(str(tmp_path / "c16.png")) is None
+    g16 = rng.randint(0, 65536, (4, 6)).astype(np.uint16)
+    Image.fromarray(g16).save(tmp_path / "g16.png")
+    got16 = ss_main.read_image(str(tmp_path / "g16.png"))
+    assert got16.dtype == np.uint16
+    np.testing.assert_array_equal(got16, g16)
     (tmp_path / "x.png").write_bytes(b"not an image")
     assert ss_main.read_image(str(tmp_path / "x.png")) is None
     assert ss_main.read_image(str(tmp_path / "missing.png")) is None
@@ -133,3 +190,22 @@ def test_dataset_loop_reader_and_arguments(tmp_path):
         ([0, 2], 32, [1024, 1024], [2048, 1024], 25)
     assert a.saving and not a.evaluate and a.bad_px_ratio_thresholds == [0.1, 0.4]
     assert not ss_main.parse_args(["--saving", "False"]).saving
+
+
+def test_psnr_closed_forms():
+    """evaluation.psnr (utils.py:232): one channel of one pixel off by 255
+    among n values gives 10 log10(n) dB; equal images give infinity; random
+    images agree with the float64 formula."""
+    from sphere_stereo.evaluation import psnr
+
+    a = np.zeros((4, 5, 3), np.uint8)
+    b = a.copy()
+    b[1, 2, 0] = 255
+    assert abs(psnr(a, b) - 10 * np.log10(60)) < 1e-12
+    assert psnr(a, a) == float("inf")
+    rng = np.random.RandomState(6)
+    x, y = (rng.randint(0, 256, (9, 7, 3)).astype(np.uint8) for _ in range(2))
+    ref = 10 * np.log10(1 / np.mean((x / 255.0 - y / 255.0) ** 2))
+    assert abs(psnr(x, y) - ref) < 1e-5
+    with pytest.raises(ValueError):
+        psnr(a, b[:2])
