@@ -97,10 +97,10 @@ def setup(wl, device, frames_cache=None, build_estimator=True):
 def batching_form(args, wl, device, plain_fp32):
     """The batching form: args.batch frames of the workload's configuration,
     each from its own seeded rig (seed, seed + 1, ...), through FrameBatch
-    (every pass after the sweep one launch per chunk of 16 frame-reference
-    pairs, the sweep's variant picked by the autotuner first), timed like the
-    plain form; fp32 and fp16 volumes.  value = panoramas/s over all frames of
-    the batch."""
+    (every pass after the sweep one launch per chunk of frames, the
+    variants picked by the autotuner first), timed like the plain form, on one
+    stream and on two lanes (value); fp32 and fp16 volumes.  value =
+    panoramas/s over all frames of the batch."""
     import torch
 
     from sphere_stereo import FrameBatch, RGBD_Estimator
@@ -111,20 +111,29 @@ def batching_form(args, wl, device, plain_fp32):
         cache = "%s.rig%d" % (args.frames_cache, b) if args.frames_cache and b else args.frames_cache
         _, tm, ts, calibs, kw = setup(wb, device, cache, build_estimator=False)
         sets.append(([t.contiguous() for t in tm], [t.contiguous() for t in ts], calibs, kw))
-    out = {"frames_per_batch": args.batch, "rigs": "seeds %d..%d" % (wl["seed"], wl["seed"] + args.batch - 1)}
+    out = {"frames_per_batch": args.batch, "rigs": "seeds %d..%d" % (wl["seed"], wl["seed"] + args.batch - 1),
+           "lanes": 2}
     for dt in ("float32", "float16"):
         ests = [RGBD_Estimator(c, device=device, volume_dtype=dt, **kw) for (_, _, c, kw) in sets]
         batch = FrameBatch(ests)
         frames = [(tm, ts) for (tm, ts, _, _) in sets]
         tuned = batch.autotune(frames)  # the sweep autotuner, per configuration
-        for _ in range(args.warmup):
-            batch.estimate_RGBD_panoramas(frames)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            batch.estimate_RGBD_panoramas(frames)
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
+
+        def timed(n):
+            for _ in range(args.warmup):
+                batch.estimate_RGBD_panoramas(frames)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(n):
+                batch.estimate_RGBD_panoramas(frames)
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0
+
+        # one stream, then two lanes (chunks of 4 frames: the second chunk's
+        # sweeps beside the first chunk's filters and stitching)
+        v1 = args.steps * args.batch / timed(args.steps)
+        batch.lanes = 2
+        el = timed(args.steps)
         v = args.steps * args.batch / el
         # the same batches replayed from a HIP graph (captured once)
         batch.use_graph = True
@@ -138,10 +147,11 @@ def batching_form(args, wl, device, plain_fp32):
         gv = args.steps * args.batch / (time.perf_counter() - t0)
         out[dt] = {"value": round(v, 3), "unit": "panoramas/s", "ms_per_batch": round(1e3 * el / args.steps, 4),
                    "vs_plain_fp32": round(v / plain_fp32, 4), "autotune": tuned[0],
+                   "one_lane": {"value": round(v1, 3), "unit": "panoramas/s", "vs_plain_fp32": round(v1 / plain_fp32, 4)},
                    "graph_replay": {"value": round(gv, 3), "unit": "panoramas/s",
                                     "vs_plain_fp32": round(gv / plain_fp32, 4)}}
-        print("[bench] batching form, %d frames, %s volume: %.1f panoramas/s (graph replay %.1f)"
-              % (args.batch, dt, v, gv), file=sys.stderr, flush=True)
+        print("[bench] batching form, %d frames, %s volume: %.1f panoramas/s on 2 lanes (one lane %.1f, graph replay "
+              "%.1f)" % (args.batch, dt, v, v1, gv), file=sys.stderr, flush=True)
         del batch, ests
         torch.cuda.empty_cache()
     return out

@@ -365,6 +365,19 @@ int ss_frame_batch_create(const ss_frame_desc *descs, int nframes, void **frame)
  * as ss_frame_run's for each frame; use_graph as ss_frame_run. */
 int ss_frame_batch_run(void *frame, const float *const *d_match, const float *const *d_stitch, int use_graph,
                        ss_stream stream);
+/* No counterpart in the reference (its frames run one after the other on one
+ * stream).  Sets the lanes (1 to 4, default 1) of an executor of
+ * ss_frame_batch_create: with L > 1 a run cuts its frames into L or more
+ * chunks of whole frames (at most one chunk per frame) that alternate over L
+ * streams of the executor (created here, default priority): chunk k's sweeps
+ * start when chunk k - 1's are over, so each chunk's filters, reprojection,
+ * inpainting and merges run beside the next chunk's sweeps.  The stream
+ * given to the run still waits for all of it, and every result is the same,
+ * bit for bit.  An executor of one frame runs on one stream whatever its
+ * lanes.  Drops a captured graph.  The ragged executor has one lane. */
+int ss_frame_set_lanes(void *frame, int lanes);
+/* The executor's lanes (-1 for a null handle). */
+int ss_frame_get_lanes(void *frame);
 /* No counterpart in the reference (the batching form).  The ragged persistent
  * form of ss_frame_batch_create: the nframes frames may differ in every size
  * and setting (C, R, N, H, W, Hs, Ws, PH, PW, candidates, sigmas, inpainting

@@ -15,7 +15,13 @@
 // served the other, so the overlap it got was incidental, and timing a
 // kernel with events then counted the waits on the other queue.)  The frame
 // can be captured once into a HIP graph and replayed, which removes the
-// per-launch host cost of its ~30 kernels.
+// per-launch host cost of its ~30 kernels.  A batch can spread its chunks of
+// frames over up to 4 lanes (streams): the sweeps, bound by vector-ALU
+// issue, run one chunk after the other, and each chunk's HBM- and
+// latency-bound passes run beside the next chunk's sweeps (8 frames of
+// 1024^2 in two chunks of 4: about 5 % more panoramas/s than one stream).
+// Chunks hold whole frames: cutting one frame's references over two lanes
+// ran 4 % slower than one stream, so a one-frame executor has one lane.
 #include <stdio.h>
 #include <string.h>
 
@@ -46,11 +52,24 @@ hipError_t pool_init(StreamPool& p) {
   return p.err;
 }
 
+// Lane streams of one executor (ss_frame_set_lanes): at most as many as the
+// hardware queues HIP gives a process by default.
+constexpr int kMaxLanes = 4;
+
 // A batch of F frames (F = 1 for ss_frame_create) with one configuration:
 // the rigs, tables, buffers and images of the frames differ.
 struct Frame {
   std::vector<ss_frame_desc> d;
   StreamPool* pool = nullptr;
+  int dev = 0;
+  int lanes = 1;
+  hipStream_t lane[kMaxLanes] = {};
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> swept, done;  // per chunk
+  // graph capture: one stream per chunk (chunk k on lane k mod L would make
+  // the lanes wait on each other in a cycle once a lane takes two chunks,
+  // which stream capture cannot record)
+  std::vector<hipStream_t> cap;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<const float*> cap_match, cap_stitch;
@@ -74,19 +93,86 @@ struct Frame {
     if (rc_ != SS_OK) return rc_; \
   } while (0)
 
-// The batch's work, in order on `s`.  Frames go in chunks of as many as fit
-// kMaxBatch (frame, reference) pairs: each frame's sweep (one launch for its
-// references), then every later pass once for the whole chunk, then each
-// frame's merge.  A chunk's volumes are produced and consumed together, so
-// its passes meet them in the last-level cache where they fit.
-int enqueue(Frame* f, const float* const* match, const float* const* stitch, hipStream_t s) {
+// Frames per chunk.  With one lane, as many as fit kMaxBatch frame-reference
+// pairs.  With L lanes the batch is cut into at least L chunks (or one per
+// frame when there are fewer than L frames).  A chunk always holds whole
+// frames: a frame's references stay in one chunk, so its merge waits for
+// nothing but its own chunk.
+int chunk_frames(int F, int R, int L) {
+  const int per = ss::kMaxBatch / R;  // frames per launch (R <= kMaxBatch)
+  return std::min(per, L <= 1 ? F : (F + L - 1) / L);
+}
+
+int ensure_capture_streams(Frame* f, size_t n) {
+  while (f->cap.size() < n) {
+    hipStream_t st = nullptr;
+    FR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    f->cap.push_back(st);
+  }
+  return SS_OK;
+}
+
+int ensure_events(Frame* f, size_t n) {
+  while (f->swept.size() < n) {
+    hipEvent_t a = nullptr, b = nullptr;
+    FR_HIP(hipEventCreateWithFlags(&a, hipEventDisableTiming));
+    const hipError_t e = hipEventCreateWithFlags(&b, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      (void)hipEventDestroy(a);
+      FR_HIP(e);
+    }
+    f->swept.push_back(a);
+    f->done.push_back(b);
+  }
+  return SS_OK;
+}
+
+// The events (and, for a capture, the streams) the chunks of a run use,
+// created before the run (not while a stream captures).
+int prepare_lanes(Frame* f, bool capturing) {
+  if (f->ragged || f->lanes <= 1) return SS_OK;
+  const int F = (int)f->d.size();
+  const int cf = chunk_frames(F, f->d[0].R, f->lanes);
+  const size_t K = (size_t)((F + cf - 1) / cf);
+  FR_CALL(ensure_events(f, K));
+  if (capturing) FR_CALL(ensure_capture_streams(f, K));
+  return SS_OK;
+}
+
+// The batch's work, enqueued after the work already on `s`, which waits for
+// all of it.  Each chunk of frames runs its frames' sweeps (one launch per
+// frame for its references), then every later pass once for the whole
+// chunk, then each frame's merge.  A chunk's volumes are produced and
+// consumed together, so its passes meet them in the last-level cache where
+// they fit.  With one lane every chunk runs on `s`.  With L lanes chunk k
+// runs on lane k mod L and starts its sweeps when chunk k - 1's sweeps are
+// over: the sweeps (bound by vector-ALU issue) run one after the other, and
+// each chunk's filters, reprojection, inpainting and merges (bound by HBM
+// and by launch latency) run beside the next chunk's sweeps.  Under capture
+// (`capturing`) chunk k goes to a stream of its own that also waits for
+// chunk k - L, which records the same order.
+int enqueue(Frame* f, const float* const* match, const float* const* stitch, hipStream_t s, bool capturing) {
   if (f->ragged) return ss::ragged_enqueue(f->ragged, match, stitch, s);
   const int F = (int)f->d.size();
   const ss_frame_desc& c = f->d[0];  // the shared configuration
   const int R = c.R, C = c.C;
-  const int per = ss::kMaxBatch / R;  // frames per chunk (R <= kMaxBatch)
-  for (int f0 = 0; f0 < F; f0 += per) {
-    const int nf = F - f0 < per ? F - f0 : per, n = nf * R;
+  const int cf = chunk_frames(F, R, f->lanes);
+  const int K = (F + cf - 1) / cf;
+  const int used = std::min(f->lanes, K);  // lanes that get a chunk
+  const bool multi = used > 1;
+  if (multi) {
+    if (f->swept.size() < (size_t)K || (capturing && f->cap.size() < (size_t)K))
+      return ss::report_error(SS_ERR_ARG, "frame executor: lane events not prepared");
+    FR_HIP(hipEventRecord(f->fork, s));
+  }
+  for (int k = 0; k < K; ++k) {
+    const int f0 = k * cf, nf = std::min(cf, F - f0), n = nf * R;
+    hipStream_t ls = !multi ? s : capturing ? f->cap[k] : f->lane[k % used];
+    if (multi) {
+      if (k < used || capturing) FR_HIP(hipStreamWaitEvent(ls, f->fork, 0));
+      if (capturing && k >= used) FR_HIP(hipStreamWaitEvent(ls, f->done[k - used], 0));
+      if (k > 0) FR_HIP(hipStreamWaitEvent(ls, f->swept[k - 1], 0));
+    }
     float* dist[ss::kMaxBatch];
     const float* cdist[ss::kMaxBatch];
     void* ws[ss::kMaxBatch];
@@ -113,14 +199,15 @@ int enqueue(Frame* f, const float* const* match, const float* const* stitch, hip
         cast[j] = ss::ToU8Args{stitch[(size_t)fi * R + r], d.d_stitch_u8[r]};
       }
       FR_CALL(ss::sweep_references(d.d_raylut, match + (size_t)fi * C, C, d.cams, rts, d.refs, R, d.cands, d.N, d.H,
-                                   d.W, d.volume_type, d.d_workspace, s));
+                                   d.W, d.volume_type, d.d_workspace, ls));
     }
+    if (multi) FR_HIP(hipEventRecord(f->swept[k], ls));
     // the fused argmin pass also fills the reprojection targets with 1e8
-    FR_CALL(ss::estimate_after_sweep(n, c.H, c.W, c.N, c.volume_type, c.cands, c.sigma_i, c.sigma_s, dist, ws, s,
+    FR_CALL(ss::estimate_after_sweep(n, c.H, c.W, c.N, c.volume_type, c.cands, c.sigma_i, c.sigma_s, dist, ws, ls,
                                      reproj));
-    FR_CALL(ss::reproject_into(n, cdist, reproj, c.H, c.W, intr, trans, false, s));
-    FR_CALL(ss::inpaint_maps(n, reproj, scratch, lut, c.H, c.W, c.inpaint_iterations, c.thr_empty, c.thr_src, s));
-    FR_CALL(ss::to_u8_maps(cast, n, (size_t)c.Hs * c.Ws * 3, s));
+    FR_CALL(ss::reproject_into(n, cdist, reproj, c.H, c.W, intr, trans, false, ls));
+    FR_CALL(ss::inpaint_maps(n, reproj, scratch, lut, c.H, c.W, c.inpaint_iterations, c.thr_empty, c.thr_src, ls));
+    FR_CALL(ss::to_u8_maps(cast, n, (size_t)c.Hs * c.Ws * 3, ls));
     for (int fi = f0; fi < f0 + nf; ++fi) {
       const ss_frame_desc& d = f->d[fi];
       const float* rp[SS_MAX_REFERENCES];
@@ -132,9 +219,13 @@ int enqueue(Frame* f, const float* const* match, const float* const* stitch, hip
         imgs[r] = d.d_stitch_u8[r];
       }
       FR_CALL(ss_merge(d.d_sampling, d.d_weights, rp, ds, imgs, d.Hs, d.Ws, &d.trans[0][0], d.intr, R, d.H, d.W, d.PH,
-                       d.PW, d.d_dist_pano, d.d_rgb_pano, s));
+                       d.PW, d.d_dist_pano, d.d_rgb_pano, ls));
     }
+    if (multi) FR_HIP(hipEventRecord(f->done[k], ls));
   }
+  // `s` waits for the last chunk of every lane (under capture: every chunk's
+  // stream, each joined back explicitly)
+  for (int k = capturing ? 0 : K - used; multi && k < K; ++k) FR_HIP(hipStreamWaitEvent(s, f->done[k], 0));
   return SS_OK;
 }
 
@@ -195,6 +286,7 @@ int ss_frame_batch_create(const ss_frame_desc* descs, int nframes, void** frame)
   if (!f) return ss::report_error(SS_ERR_ARG, "out of host memory");
   f->d.assign(descs, descs + nframes);
   f->pool = &g_pool[dev];
+  f->dev = dev;
   f->n_match = (size_t)nframes * descs[0].C;
   f->n_stitch = (size_t)nframes * descs[0].R;
   *frame = f;
@@ -231,6 +323,7 @@ int ss_frame_ragged_create(const ss_frame_desc* descs, int nframes, int schedule
   }
   f->d.assign(descs, descs + nframes);
   f->pool = &g_pool[dev];
+  f->dev = dev;
   f->ragged = plan;
   int nm = 0, ns = 0;
   ss::ragged_images(plan, &nm, &ns);
@@ -253,7 +346,8 @@ int ss_frame_batch_run(void* frame, const float* const* d_match, const float* co
     if (!d_stitch[r]) return ss::report_error(SS_ERR_ARG, "null stitch image");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   std::lock_guard<std::mutex> lk(f->pool->mu);
-  if (!use_graph) return enqueue(f, d_match, d_stitch, s);
+  FR_CALL(prepare_lanes(f, use_graph != 0));
+  if (!use_graph) return enqueue(f, d_match, d_stitch, s, false);
   if (ss::profiler_active()) return ss::report_error(SS_ERR_UNSUPPORTED, "graph replay while kernel timing is on");
 
   // captured on the device's capture stream (the caller's may be the null
@@ -266,7 +360,7 @@ int ss_frame_batch_run(void* frame, const float* const* d_match, const float* co
   if (!same) {
     FR_CALL(drop_graph(f));
     FR_HIP(hipStreamBeginCapture(main, hipStreamCaptureModeRelaxed));
-    const int rc = enqueue(f, d_match, d_stitch, main);
+    const int rc = enqueue(f, d_match, d_stitch, main, true);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(main, &g);
     if (rc != SS_OK) {
@@ -288,11 +382,40 @@ int ss_frame_run(void* frame, const float* const* d_match, const float* const* d
   return ss_frame_batch_run(frame, d_match, d_stitch, use_graph, stream);
 }
 
+int ss_frame_set_lanes(void* frame, int lanes) {
+  Frame* f = static_cast<Frame*>(frame);
+  if (!f) return ss::report_error(SS_ERR_ARG, "null frame");
+  if (lanes < 1 || lanes > kMaxLanes) return ss::report_error(SS_ERR_ARG, "lanes outside [1, 4]");
+  if (f->ragged && lanes != 1) return ss::report_error(SS_ERR_UNSUPPORTED, "the ragged executor runs on one lane");
+  int dev = 0;
+  FR_HIP(hipGetDevice(&dev));
+  if (dev != f->dev) return ss::report_error(SS_ERR_ARG, "the frame executor belongs to another device");
+  std::lock_guard<std::mutex> lk(f->pool->mu);
+  if (lanes == f->lanes) return SS_OK;
+  if (lanes > 1 && !f->fork) FR_HIP(hipEventCreateWithFlags(&f->fork, hipEventDisableTiming));
+  for (int l = 0; l < lanes; ++l)
+    if (!f->lane[l]) FR_HIP(hipStreamCreateWithFlags(&f->lane[l], hipStreamNonBlocking));
+  FR_CALL(drop_graph(f));  // a captured graph keeps the old schedule
+  f->lanes = lanes;
+  return SS_OK;
+}
+
+int ss_frame_get_lanes(void* frame) {
+  const Frame* f = static_cast<const Frame*>(frame);
+  return f ? f->lanes : -1;
+}
+
 int ss_frame_destroy(void* frame) {
   Frame* f = static_cast<Frame*>(frame);
   if (!f) return SS_OK;
   drop_graph(f);
   ss::ragged_free(f->ragged);
+  for (hipStream_t l : f->lane)
+    if (l) (void)hipStreamDestroy(l);
+  for (hipStream_t l : f->cap) (void)hipStreamDestroy(l);
+  for (hipEvent_t e : f->swept) (void)hipEventDestroy(e);
+  for (hipEvent_t e : f->done) (void)hipEventDestroy(e);
+  if (f->fork) (void)hipEventDestroy(f->fork);
   delete f;
   return SS_OK;
 }

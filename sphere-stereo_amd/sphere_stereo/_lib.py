@@ -103,6 +103,8 @@ SIGNATURES = {
     "ss_frame_ragged_table_bytes": (_z, [C.POINTER(ss_frame_desc), _i]),
     "ss_frame_ragged_create": (_i, [C.POINTER(ss_frame_desc), _i, _i, _p, _z, _p, C.POINTER(C.c_void_p)]),
     "ss_frame_destroy": (_i, [_p]),
+    "ss_frame_set_lanes": (_i, [_p, _i]),
+    "ss_frame_get_lanes": (_i, [_p]),
     "ss_soft_argmin": (_i, [_p, _i, _i, _i, _fl, _fl, _fl, _p, _p]),
     "ss_soft_argmin_backward": (_i, [_p, _p, _i, _i, _i, _fl, _fl, _fl, _p, _p]),
     "ss_isb_backward_workspace_bytes": (_z, [_i, _i, _i]),

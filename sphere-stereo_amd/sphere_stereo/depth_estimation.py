@@ -2,12 +2,11 @@
 
 The estimator keeps the reference's constructor, ``select_camera``,
 ``estimate_fisheye_distance`` and ``estimate_RGBD_panorama``.  A frame is one
-ss_frame_run call into the native executor: one sweep launch covers every
-reference (and writes their guides), then each reference camera's chain (cost
-filter with the final up-pass fused with the argmin, distance filter, uint8
-cast of its stitch image, reprojection, inpainting) runs on a stream of its
-own, and the merge runs on torch's current stream, which waits for the frame.
-With ``use_graph = True`` the frame is captured once into a HIP graph and
+ss_frame_run call into the native executor, on torch's current stream: one
+sweep launch covers every reference (and writes their guides), then every
+later pass (each cost filter level, the final up pass fused with the argmin,
+the distance filter, the uint8 cast of the stitch images, the reprojection,
+the inpainting) is one launch for all references, then the merge.  With ``use_graph = True`` the frame is captured once into a HIP graph and
 replayed on later calls with the same input tensors.  Init-time work (camera
 selection, unit-ray tables, stitching tables) also runs on the GPU.
 
@@ -405,7 +404,7 @@ class FrameBatch:
     ``estimate_RGBD_panorama``.
     """
 
-    def __init__(self, estimators, single_executor=False, schedule="auto"):
+    def __init__(self, estimators, single_executor=False, schedule="auto", lanes=1):
         self.estimators = list(estimators)
         F = len(self.estimators)
         if not 1 <= F <= _lib.SS_MAX_BATCH_FRAMES:
@@ -432,12 +431,35 @@ class FrameBatch:
                 self.groups.append(idx)
         self.use_graph = False
         self._inputs = None
+        if lanes != 1:
+            self.lanes = lanes
 
     def _create_group(self, idx):
         descs = (_lib.ss_frame_desc * len(idx))(*[self.estimators[i]._frame_desc() for i in idx])
         handle = _lib.C.c_void_p()
         _lib.check(_lib.lib.ss_frame_batch_create(descs, len(idx), _lib.C.byref(handle)))
+        lanes = getattr(self, "_lanes", 1)
+        if lanes != 1:
+            _lib.check(_lib.lib.ss_frame_set_lanes(handle, lanes))
         return handle
+
+    @property
+    def lanes(self):
+        """Streams each configuration group's frames alternate over
+        (ss_frame_set_lanes): with L > 1 the group is cut into L or more
+        chunks of whole frames and each chunk's filters, reprojection,
+        inpainting and merges run beside the next chunk's sweeps.  The
+        results are the same.  The single executor has one lane."""
+        return getattr(self, "_lanes", 1)
+
+    @lanes.setter
+    def lanes(self, n):
+        n = int(n)
+        if self.single_executor and n != 1:
+            raise ValueError("the single executor runs on one lane")
+        for h in self._frames:
+            _lib.check(_lib.lib.ss_frame_set_lanes(h, n))
+        self._lanes = n
 
     def _create_ragged(self):
         F = len(self.estimators)

@@ -78,6 +78,8 @@ def test_argument_errors_before_device_work():
     assert lib.ss_isb_filter_planar(C.c_void_p(8), C.c_void_p(16), 16, 16, 4, 10.0, 25.0, None, 0, None) == 4
     assert lib.ss_merge(None, None, None, None, None, 8, 8, None, None, 1, 8, 8, 4, 4, None, None, None) == 1
     assert lib.ss_profile_collect(None, None) == 1
+    assert lib.ss_frame_set_lanes(None, 2) == 1 and b"null frame" in lib.ss_last_error()
+    assert lib.ss_frame_get_lanes(None) == -1
 
 
 def test_inpaint_thresholds_follow_constant_substitution():

@@ -432,6 +432,74 @@ def test_frame_batch_matches_single_frames(gpu, volume_dtype):
     assert b"differ" in _lib.lib.ss_last_error()
 
 
+def _lane_frames(gpu, count, refs, seed0):
+    ests, frames = [], []
+    for i in range(count):
+        fr = frame(seed=seed0 + i, resolution=160, matching=(64, 48), stitch=(64, 64), refs=refs, device=gpu,
+                   baseline=0.12 + 0.03 * i)
+        kw = dict(min_dist=0.55, max_dist=100, candidate_count=16, references_indices=fr["refs"],
+                  reprojection_viewpoint=fr["viewpoint"], masks=fr["masks"], matching_resolution=fr["matching"],
+                  rgb_to_stitch_resolution=fr["stitch"], panorama_resolution=(128, 64), sigma_i=10, sigma_s=12.5)
+        ests.append((RGBD_Estimator(fr["calibs"], device=gpu, **kw), fr, kw))
+        frames.append(([t.contiguous() for t in fr["to_match"]], [t.contiguous() for t in fr["to_stitch"]]))
+    return ests, frames
+
+
+def test_frame_lanes_identical(gpu):
+    """ss_frame_set_lanes: the chunks of a run spread over 2 to 4 streams
+    (sweeps one after the other, each chunk's later passes beside the next
+    chunk's sweeps) give the bits of the one-stream run, eager and replayed
+    from graphs: nine frames of four references (three chunks of up to four
+    frames, so a lane takes two chunks) and two frames of three references
+    over four lanes (one chunk per frame).  The one-stream results are
+    checked against the oracle for two frames."""
+    ests, frames = _lane_frames(gpu, 9, (0, 1, 2, 3), 70)
+    want = []
+    for i, ((est, fr, kw), (tm, ts)) in enumerate(zip(ests, frames)):
+        rgb, dist = est.estimate_RGBD_panorama(tm, ts)
+        if i in (0, 8):
+            assert_matches_oracle(fr["calibs"], kw, fr["to_match"], fr["to_stitch"], rgb, dist, "frame %d" % i)
+        want.append((rgb.clone(), dist.clone()))
+
+    def check(out, wants, what):
+        torch.cuda.synchronize()
+        for i, ((rgb, dist), (wr, wd)) in enumerate(zip(out, wants)):
+            assert torch.equal(rgb, wr), "%s: rgb panorama of frame %d" % (what, i)
+            assert torch.equal(dist.view(torch.int32), wd.view(torch.int32)), "%s: distance panorama %d" % (what, i)
+
+    def clear(estimators):
+        for e in estimators:
+            e.fishey_stitcher.RGB_panorama.zero_()
+            e.fishey_stitcher.distance_panorama.fill_(-1.0)
+
+    batch = FrameBatch([e for e, _, _ in ests])
+    for lanes in (2, 3, 4, 1):
+        batch.lanes = lanes
+        assert batch.lanes == lanes and all(_lib.lib.ss_frame_get_lanes(h) == lanes for h in batch._frames)
+        for mode in ("eager", "graph", "graph"):
+            clear(e for e, _, _ in ests)
+            batch.use_graph = mode == "graph"
+            check(batch.estimate_RGBD_panoramas(frames), want, "9 frames, %d lanes, %s" % (lanes, mode))
+            junk = torch.randn(1 << 20, device=gpu).sum().item()  # eager work between replays
+            assert junk == junk
+    batch.use_graph = False
+    del batch
+
+    ests3, frames3 = _lane_frames(gpu, 2, (0, 1, 3), 90)
+    want3 = []
+    for (est, _, _), (tm, ts) in zip(ests3, frames3):
+        rgb, dist = est.estimate_RGBD_panorama(tm, ts)
+        want3.append((rgb.clone(), dist.clone()))
+    batch3 = FrameBatch([e for e, _, _ in ests3], lanes=4)
+    for mode in ("eager", "graph", "graph"):
+        clear(e for e, _, _ in ests3)
+        batch3.use_graph = mode == "graph"
+        check(batch3.estimate_RGBD_panoramas(frames3), want3, "2 frames of 3 references, 4 lanes, %s" % mode)
+    with pytest.raises(_lib.SphereStereoError, match="lanes outside"):
+        batch3.lanes = 5
+    batch3.use_graph = False
+
+
 def test_frame_batch_ragged_matches_single_frames(gpu):
     """A ragged batch: rigs of different matching resolutions, candidate
     counts, reference sets and volume types (FrameBatch groups them by
