@@ -286,7 +286,8 @@ def diff_setup(device):
 def differentiation_form(args, device):
     """The differentiation form: soft-argmin inverse distance of both
     references of 4 frames, forward and forward + backward (gradients to the
-    images, all intrinsics and all poses) timed per batch; then the
+    images, all intrinsics and all poses; and to intrinsics and poses only)
+    timed per batch; then the
     Gauss-Newton refinement of camera 1 from a perturbation of 0.5 degrees,
     5 mm and 1 % of fx over 50 steps."""
     import numpy as np
@@ -299,11 +300,11 @@ def differentiation_form(args, device):
     wmap = torch.rand((rig.rows, rig.cols), generator=gen).to(device)
     req = [[t.clone().requires_grad_(True) for t in f] for f in frames]
 
-    def fwd(grad):
+    def fwd(grad, images=True):
         cams6 = rig.cams6.clone().requires_grad_(grad)
         rts = rig.rts.clone().requires_grad_(grad)
         loss = 0.0
-        for imgs in (req if grad else frames):
+        for imgs in (req if grad and images else frames):
             for inv in rig(imgs, cams6, rts):
                 loss = loss + (inv * wmap).sum()
         if grad:
@@ -312,19 +313,25 @@ def differentiation_form(args, device):
 
     out = {"frames_per_batch": DIFF_FORM["frames"], "matching": list(DIFF_FORM["matching"]), "candidates": DIFF_FORM["N"],
            "references": list(DIFF_FORM["refs"]), "temperature": DIFF_FORM["temperature"], "dtype": "f32"}
-    for name, grad in (("forward", False), ("forward_backward", True)):
+    # forward; forward + backward to images, intrinsics and poses; forward +
+    # backward to intrinsics and poses only (what the calibration refinement
+    # needs: no image-gradient scatter)
+    for name, grad, images in (("forward", False, False), ("forward_backward", True, True),
+                               ("forward_backward_calibration", True, False)):
         ctx = torch.no_grad() if not grad else torch.enable_grad()
         with ctx:
             for _ in range(2):
-                fwd(grad)
+                fwd(grad, images)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             reps = 5
             for _ in range(reps):
-                fwd(grad)
+                fwd(grad, images)
             torch.cuda.synchronize()
             el = (time.perf_counter() - t0) / reps
         out[name] = {"ms_per_batch": round(1e3 * el, 3), "frames_per_s": round(DIFF_FORM["frames"] / el, 2)}
+        if grad:
+            out[name]["vs_forward"] = round(el / (out["forward"]["ms_per_batch"] * 1e-3), 3)
         print("[bench] differentiation form %s: %.2f ms per batch of %d frames" % (name, 1e3 * el,
               DIFF_FORM["frames"]), file=sys.stderr, flush=True)
     ct, rt = rig.cams6.clone(), rig.rts.clone()

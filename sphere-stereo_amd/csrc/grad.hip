@@ -460,36 +460,77 @@ __device__ void unproject_dual(const TorchCam& c, float u, float v, D6 p[3]) {
 }
 
 // Image gradients of a block's samples are first summed in an LDS window of
-// the sampled image: per chunk of kBwdChunk candidates, the bounding box of
-// the taps of the block's lanes that sample its window camera (the smallest
-// camera index among them) is found by a projection-only pass; when it holds
-// at most kWinTexels texels, the chunk's taps are added there (LDS atomics)
-// and the window is then added to the image with one global atomic per
-// touched texel and channel.  The samples of 16 consecutive candidates of a
-// 16 x 16 pixel tile sweep a patch of ~1000 texels (median box ~1700 texels
-// at 768^2, 48 candidates; 95 % of the boxes fit), against ~16000 tap
-// updates: ~16x fewer global atomics.  (Tiles of 256 pixels of one row map
-// to tilted curves whose boxes mostly do not fit.)  Other lanes' taps, and
-// chunks whose box does not fit, use global atomics directly.
+// the sampled image: per chunk of kBwdChunk candidates, a box of the
+// sampled image around the taps of the block's lanes that sample its window
+// camera (the smallest camera index among them) is found from three of the
+// chunk's candidates (first, middle, last; the samples of one pixel move
+// along a smooth epipolar curve), widened by kBwdMargin texels.  When it
+// holds at most kWinTexels texels, taps inside it are added there (LDS
+// atomics) and the window is then added to the image with one global atomic
+// per touched texel and channel.  The samples of 16 consecutive candidates
+// of a 16 x 16 pixel tile sweep a patch of ~1000 texels (median box ~1700
+// texels at 768^2, 48 candidates; 95 % of the boxes fit), against ~16000
+// tap updates: ~16x fewer global atomics.  (Tiles of 256 pixels of one row
+// map to tilted curves whose boxes mostly do not fit.)  Other lanes' taps,
+// taps outside the box and chunks whose box does not fit use global atomics
+// directly, so the box only decides speed, never the sums.
 constexpr int kBwdChunk = 16;
 constexpr int kBwdTile = 16;      // kBwdTile^2 == kBlock pixels per workgroup
 constexpr int kWinTexels = 3072;  // 36 KiB of LDS: four workgroups per CU
+constexpr int kBwdMargin = 2;
 static_assert(kBwdTile * kBwdTile == kBlock, "one lane per pixel of a tile");
 static_assert(kBwdChunk % 4 == 0, "chunks start at a multiple of 4 candidates");
 
-// The sample of candidate n of a pixel with ray rl in camera (cam, rt):
-// grid_sample coordinates (forward's expression order) and whether its taps
-// can touch the image.
-__device__ __forceinline__ bool bwd_sample_coords(const SweepBwdArgs& a, const TorchCam& cam, const float* rt,
-                                                  const float4& rl, int n, float& Px, float& Py, float& Pz,
-                                                  float& qx, float& qy, float& qz, float& ix, float& iy) {
-  const float d = a.cands[n];
-  Px = d * rl.x;
-  Py = d * rl.y;
-  Pz = d * rl.z;
-  rigid(rt, Px, Py, Pz, qx, qy, qz);
+// The sample of candidate n of a pixel with ray rl in camera (cam, rt): the
+// point in the camera, project_t's intermediates, the grid_sample
+// coordinates (the forward's bits), which of its four taps lie in the image
+// (bits 0-3: nw, ne, sw, se; bit 4: any tap can touch the image) and the raw
+// texels of its taps, loaded from clamped addresses so that no load waits
+// behind a branch (the caller zeroes the taps outside the image).
+struct BwdSample {
+  float qx, qy, qz, d1, cc, d2, nrm, ix, iy;
+  int x0, y0;
+  unsigned taps;
+  float vv[4][3];
+};
+
+__device__ __forceinline__ void bwd_fetch(const SweepBwdArgs& a, const TorchCam& cam, const float* rt,
+                                          const float4& rl, const float* img, float d, BwdSample& s) {
+  rigid(rt, d * rl.x, d * rl.y, d * rl.z, s.qx, s.qy, s.qz);
   float u, v;
+  project_t_parts(cam, s.qx, s.qy, s.qz, a.fastproj != 0, u, v, s.d1, s.cc, s.d2, s.nrm);
+  if (a.pow2) {
+    s.ix = grid_coord_pow2(u, a.inv2W, a.halfW);
+    s.iy = grid_coord_pow2(v, a.inv2H, a.halfH);
+  } else {
+    s.ix = grid_coord2<false>(u, a.Wf, a.invW);
+    s.iy = grid_coord2<false>(v, a.Hf, a.invH);
+  }
+  const float x0f = floorf(s.ix), y0f = floorf(s.iy);
+  const bool touch = x0f >= -1.0f && x0f <= (float)(a.W - 1) && y0f >= -1.0f && y0f <= (float)(a.H - 1);
+  s.x0 = (int)(touch ? x0f : 0.0f);
+  s.y0 = (int)(touch ? y0f : 0.0f);
+  const bool bx0 = s.x0 >= 0, bx1 = s.x0 + 1 < a.W, by0 = s.y0 >= 0, by1 = s.y0 + 1 < a.H;
+  s.taps = touch ? ((bx0 && by0 ? 1u : 0u) | (bx1 && by0 ? 2u : 0u) | (bx0 && by1 ? 4u : 0u) |
+                    (bx1 && by1 ? 8u : 0u) | 16u)
+                 : 0u;
+  const int xa = bx0 ? s.x0 : 0, xb = bx1 ? s.x0 + 1 : a.W - 1;
+  const int ya = by0 ? s.y0 : 0, yb = by1 ? s.y0 + 1 : a.H - 1;
+  const float* const p[4] = {img + ((size_t)ya * a.W + xa) * 3, img + ((size_t)ya * a.W + xb) * 3,
+                             img + ((size_t)yb * a.W + xa) * 3, img + ((size_t)yb * a.W + xb) * 3};
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s.vv[t][c] = p[t][c];
+}
+
+// The grid_sample coordinates of candidate n (the box pass).
+__device__ __forceinline__ bool bwd_sample_coords(const SweepBwdArgs& a, const TorchCam& cam, const float* rt,
+                                                  const float4& rl, int n, float& ix, float& iy) {
+  const float d = a.cands[n];
+  float qx, qy, qz, u, v;
   bool valid;
+  rigid(rt, d * rl.x, d * rl.y, d * rl.z, qx, qy, qz);
   if (!a.fastproj || !project_t_fast(cam, qx, qy, qz, u, v)) project_t(cam, qx, qy, qz, u, v, valid);
   if (a.pow2) {
     ix = grid_coord_pow2(u, a.inv2W, a.halfW);
@@ -553,17 +594,19 @@ __global__ void __launch_bounds__(kBlock) k_sweep_bwd(SweepBwdArgs a) {
 
   for (int n0 = 0; n0 < a.N; n0 += kBwdChunk) {
     const int n1 = min(n0 + kBwdChunk, a.N);
-    // bounding box of the chunk's in-image taps in the window camera
+    // the window box from the chunk's first, middle and last candidates
     int bx0 = 0x7fffffff, bx1 = -1, by0 = 0x7fffffff, by1 = -1;
     if (inwin) {
-      for (int n = n0; n < n1; ++n) {
-        float Px, Py, Pz, qx, qy, qz, ix, iy;
-        if (!bwd_sample_coords(a, cam, rt, rl, n, Px, Py, Pz, qx, qy, qz, ix, iy)) continue;
+      const int probe[3] = {n0, (n0 + n1 - 1) / 2, n1 - 1};
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        float ix, iy;
+        if (!bwd_sample_coords(a, cam, rt, rl, probe[i], ix, iy)) continue;
         const int x0 = (int)floorf(ix), y0 = (int)floorf(iy);
-        bx0 = min(bx0, max(x0, 0));
-        bx1 = max(bx1, min(x0 + 1, a.W - 1));
-        by0 = min(by0, max(y0, 0));
-        by1 = max(by1, min(y0 + 1, a.H - 1));
+        bx0 = min(bx0, max(x0 - kBwdMargin, 0));
+        bx1 = max(bx1, min(x0 + 1 + kBwdMargin, a.W - 1));
+        by0 = min(by0, max(y0 - kBwdMargin, 0));
+        by1 = max(by1, min(y0 + 1 + kBwdMargin, a.H - 1));
       }
     }
     if (threadIdx.x == 0) {
@@ -585,41 +628,54 @@ __global__ void __launch_bounds__(kBlock) k_sweep_bwd(SweepBwdArgs a) {
       for (int i = (int)threadIdx.x; i < ww * wh * 3; i += kBlock) win[i] = 0.0f;
       __syncthreads();
     }
+    // the chunk's candidates in a register (lane i: candidate n0 + i mod 16),
+    // read back with readlane: a scalar load in the loop would wait for the
+    // LDS atomics before it (they share one counter).  The empty asm keeps
+    // the load here, where every lane runs it: sunk into the branch below,
+    // lanes that skip the branch would leave their copy unset.
+    float cand_v = a.cands[min(n0 + (int)(threadIdx.x & (kBwdChunk - 1)), a.N - 1)];
+    asm volatile("" : "+v"(cand_v));
+    auto cand = [&](int n) __attribute__((always_inline)) {
+      return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cand_v), n - n0));
+    };
 
-    float4 g4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    for (int n = n0; n < n1 && active; ++n) {
-      // dL/dcost four candidates at a time (rows are NP >= 8 floats, 32-B aligned)
-      const int nq = n & 3;
-      if (nq == 0) g4 = ld4(gc + n);
-      const float g = nq == 0 ? g4.x : (nq == 1 ? g4.y : (nq == 2 ? g4.z : g4.w));
-      float s[3] = {0.0f, 0.0f, 0.0f}, sx[3] = {0.0f, 0.0f, 0.0f}, sy[3] = {0.0f, 0.0f, 0.0f};
-      float Px = 0.0f, Py = 0.0f, Pz = 0.0f, qx = 0.0f, qy = 0.0f, qz = 0.0f;
-      float ix = 0.0f, iy = 0.0f, wx0f = 0.0f, wx1f = 0.0f, wy0f = 0.0f, wy1f = 0.0f;
-      int xi0 = 0, yi0 = 0;
-      bool tap[4] = {false, false, false, false};
-      if (k >= 0) {
+    if (active && k < 0) {
+      // no camera sampled: the cost is |0 - r| summed (depth_estimation.py:150-152)
+      for (int n = n0; n < n1; ++n) {
+        const float g = gc[n];
+        const float S = (fabsf(-r[0]) + fabsf(-r[1])) + fabsf(-r[2]);
+        const float gS = S <= 500.0f ? g : 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gref[c] -= (-r[c]) > 0.0f ? gS : ((-r[c]) < 0.0f ? -gS : 0.0f);
+      }
+    } else if (active) {
+      // candidate n + 1's projection and texel loads are issued before
+      // candidate n is used, so their latency overlaps n's arithmetic
+      BwdSample cur, nxt;
+      bwd_fetch(a, cam, rt, rl, img, cand(n0), cur);
+      float4 g4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      for (int n = n0; n < n1; ++n) {
+        if (n + 1 < n1) bwd_fetch(a, cam, rt, rl, img, cand(n + 1), nxt);
+        // dL/dcost four candidates at a time (rows are NP >= 8 floats, 32-B aligned)
+        const int nq = n & 3;
+        if (nq == 0) g4 = ld4(gc + n);
+        const float g = nq == 0 ? g4.x : (nq == 1 ? g4.y : (nq == 2 ? g4.z : g4.w));
         // zero-padded bilinear sample (the forward's taps and order) and its
         // derivatives in ix, iy
-        if (bwd_sample_coords(a, cam, rt, rl, n, Px, Py, Pz, qx, qy, qz, ix, iy)) {
-          const float x0f = floorf(ix), y0f = floorf(iy);
-          xi0 = (int)x0f;
-          yi0 = (int)y0f;
-          wx1f = ix - x0f;
-          wx0f = (x0f + 1.0f) - ix;
-          wy1f = iy - y0f;
-          wy0f = (y0f + 1.0f) - iy;
-          const bool bx0t = xi0 >= 0, bx1t = xi0 + 1 < a.W, by0t = yi0 >= 0, by1t = yi0 + 1 < a.H;
-          tap[0] = bx0t && by0t;
-          tap[1] = bx1t && by0t;
-          tap[2] = bx0t && by1t;
-          tap[3] = bx1t && by1t;
+        float sv[3] = {0.0f, 0.0f, 0.0f}, sx[3] = {0.0f, 0.0f, 0.0f}, sy[3] = {0.0f, 0.0f, 0.0f};
+        float wx0f = 0.0f, wx1f = 0.0f, wy0f = 0.0f, wy1f = 0.0f;
+        const bool touch = (cur.taps & 16u) != 0u;
+        if (touch) {
+          const float x0f = (float)cur.x0, y0f = (float)cur.y0;
+          wx1f = cur.ix - x0f;
+          wx0f = (x0f + 1.0f) - cur.ix;
+          wy1f = cur.iy - y0f;
+          wy0f = (y0f + 1.0f) - cur.iy;
           float vv[4][3];
 #pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            const int tx = xi0 + (t & 1), ty = yi0 + (t >> 1);
+          for (int t = 0; t < 4; ++t)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) vv[t][c] = tap[t] ? img[((size_t)ty * a.W + tx) * 3 + c] : 0.0f;
-          }
+            for (int c = 0; c < 3; ++c) vv[t][c] = (cur.taps >> t) & 1u ? cur.vv[t][c] : 0.0f;
           const float wnw = wx0f * wy0f, wne = wx1f * wy0f, wsw = wx0f * wy1f, wse = wx1f * wy1f;
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
@@ -627,84 +683,87 @@ __global__ void __launch_bounds__(kBlock) k_sweep_bwd(SweepBwdArgs a) {
             tt = tt + vv[1][c] * wne;
             tt = tt + vv[2][c] * wsw;
             tt = tt + vv[3][c] * wse;
-            s[c] = tt;
+            sv[c] = tt;
             sx[c] = wy0f * (vv[1][c] - vv[0][c]) + wy1f * (vv[3][c] - vv[2][c]);
             sy[c] = wx0f * (vv[2][c] - vv[0][c]) + wx1f * (vv[3][c] - vv[1][c]);
           }
         }
-      }
-      const float S = (fabsf(s[0] - r[0]) + fabsf(s[1] - r[1])) + fabsf(s[2] - r[2]);
-      // torch.clamp(max=500) passes the gradient where S <= 500
-      const float gS = S <= 500.0f ? g : 0.0f;
-      float gs[3];
+        const float S = (fabsf(sv[0] - r[0]) + fabsf(sv[1] - r[1])) + fabsf(sv[2] - r[2]);
+        // torch.clamp(max=500) passes the gradient where S <= 500
+        const float gS = S <= 500.0f ? g : 0.0f;
+        float gs[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const float dlt = s[c] - r[c];
-        gs[c] = dlt > 0.0f ? gS : (dlt < 0.0f ? -gS : 0.0f);
-        gref[c] -= gs[c];
-      }
-      if (k < 0 || gS == 0.0f) continue;
-      if (gimg) {
-        const float wt[4] = {wx0f * wy0f, wx1f * wy0f, wx0f * wy1f, wx1f * wy1f};
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          if (!tap[t]) continue;
-          const int tx = xi0 + (t & 1), ty = yi0 + (t >> 1);
-          const int lx = tx - wx0, ly = ty - wy0;
-          if (use_win && inwin && lx >= 0 && lx < ww && ly >= 0 && ly < wh) {
-            float* dst = win + (ly * ww + lx) * 3;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) atomicAdd(dst + c, wt[t] * gs[c]);
-          } else {
-            float* dst = gimg + ((size_t)ty * a.W + tx) * 3;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) unsafeAtomicAdd(dst + c, wt[t] * gs[c]);
-          }
+        for (int c = 0; c < 3; ++c) {
+          const float dlt = sv[c] - r[c];
+          gs[c] = dlt > 0.0f ? gS : (dlt < 0.0f ? -gS : 0.0f);
+          gref[c] -= gs[c];
         }
-      }
-      // the grid_sample coordinate round trip is the identity in u: dix/du = 1
-      const float gu = (gs[0] * sx[0] + gs[1] * sx[1]) + gs[2] * sx[2];
-      const float gv = (gs[0] * sy[0] + gs[1] * sy[1]) + gs[2] * sy[2];
-      // utils.py:75-94 backwards
-      const float xy2 = qx * qx + qy * qy;
-      const float d1 = sqrtf(xy2 + qz * qz);
-      const float cc = cam.xi * d1 + qz;
-      const float d2 = sqrtf(xy2 + cc * cc);
-      const float nrm = cam.a * d2 + cam.one_m_a * cc;
-      const float inv = 1.0f / nrm;
-      acc[0] += gu * qx * inv;
-      acc[1] += gv * qy * inv;
-      acc[2] += gu;
-      acc[3] += gv;
-      float gqx = gu * cam.fs0 * inv, gqy = gv * cam.fs1 * inv, gqz = 0.0f;
-      const float gnrm = -(gu * cam.fs0 * qx + gv * cam.fs1 * qy) * inv * inv;
-      acc[5] += gnrm * (d2 - cc);
-      const float gd2 = gnrm * cam.a;
-      float gcc = gnrm * cam.one_m_a;
-      float gxy2 = 0.0f;
-      if (d2 > 0.0f) {
-        gxy2 += gd2 * 0.5f / d2;
-        gcc += gd2 * cc / d2;
-      }
-      acc[4] += gcc * d1;
-      const float gd1 = gcc * cam.xi;
-      gqz += gcc;
-      if (d1 > 0.0f) {
-        gxy2 += gd1 * 0.5f / d1;
-        gqz += gd1 * qz / d1;
-      }
-      gqx += 2.0f * qx * gxy2;
-      gqy += 2.0f * qy * gxy2;
-      const float gq[3] = {gqx, gqy, gqz}, Pv[3] = {Px, Py, Pz};
+        // a sample with no tap in the image has no derivative in the
+        // sampled image nor in its coordinates
+        if (touch && gS != 0.0f) {
+          if (gimg) {
+            const float wt[4] = {wx0f * wy0f, wx1f * wy0f, wx0f * wy1f, wx1f * wy1f};
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
+            for (int t = 0; t < 4; ++t) {
+              if (!((cur.taps >> t) & 1u)) continue;
+              const int tx = cur.x0 + (t & 1), ty = cur.y0 + (t >> 1);
+              const int lx = tx - wx0, ly = ty - wy0;
+              if (use_win && inwin && lx >= 0 && lx < ww && ly >= 0 && ly < wh) {
+                float* dst = win + (ly * ww + lx) * 3;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) acc[6 + 4 * j + i] += gq[j] * Pv[i];
-        acc[6 + 4 * j + 3] += gq[j];
+                for (int c = 0; c < 3; ++c) atomicAdd(dst + c, wt[t] * gs[c]);
+              } else {
+                float* dst = gimg + ((size_t)ty * a.W + tx) * 3;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) unsafeAtomicAdd(dst + c, wt[t] * gs[c]);
+              }
+            }
+          }
+          // the grid_sample coordinate round trip is the identity in u: dix/du = 1
+          const float gu = (gs[0] * sx[0] + gs[1] * sx[1]) + gs[2] * sx[2];
+          const float gv = (gs[0] * sy[0] + gs[1] * sy[1]) + gs[2] * sy[2];
+          // utils.py:75-94 backwards, on the forward's intermediates; the
+          // reciprocals of the derivative need no correct rounding
+          const float qx = cur.qx, qy = cur.qy, qz = cur.qz, d1 = cur.d1, cc = cur.cc, d2 = cur.d2;
+          const float inv = __builtin_amdgcn_rcpf(cur.nrm);
+          acc[0] += gu * qx * inv;
+          acc[1] += gv * qy * inv;
+          acc[2] += gu;
+          acc[3] += gv;
+          float gqx = gu * cam.fs0 * inv, gqy = gv * cam.fs1 * inv, gqz = 0.0f;
+          const float gnrm = -(gu * cam.fs0 * qx + gv * cam.fs1 * qy) * inv * inv;
+          acc[5] += gnrm * (d2 - cc);
+          const float gd2 = gnrm * cam.a;
+          float gcc = gnrm * cam.one_m_a;
+          float gxy2 = 0.0f;
+          if (d2 > 0.0f) {
+            const float r2 = __builtin_amdgcn_rcpf(d2);
+            gxy2 += gd2 * 0.5f * r2;
+            gcc += gd2 * cc * r2;
+          }
+          acc[4] += gcc * d1;
+          const float gd1 = gcc * cam.xi;
+          gqz += gcc;
+          if (d1 > 0.0f) {
+            const float r1 = __builtin_amdgcn_rcpf(d1);
+            gxy2 += gd1 * 0.5f * r1;
+            gqz += gd1 * qz * r1;
+          }
+          gqx += 2.0f * qx * gxy2;
+          gqy += 2.0f * qy * gxy2;
+          const float d = cand(n);
+          const float gq[3] = {gqx, gqy, gqz}, Pv[3] = {d * rl.x, d * rl.y, d * rl.z};
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) acc[6 + 4 * j + i] += gq[j] * Pv[i];
+            acc[6 + 4 * j + 3] += gq[j];
+          }
+#pragma unroll
+          for (int i = 0; i < 3; ++i) gray[i] += d * ((rt[i] * gq[0] + rt[4 + i] * gq[1]) + rt[8 + i] * gq[2]);
+        }
+        cur = nxt;
       }
-      const float d = a.cands[n];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) gray[i] += d * ((rt[i] * gq[0] + rt[4 + i] * gq[1]) + rt[8 + i] * gq[2]);
     }
 
     if (use_win) {

@@ -140,6 +140,33 @@ __device__ __forceinline__ bool project_t_fast(const TorchCam& c, float qx, floa
   return fabsf(nrm) >= 0x1p-30f;
 }
 
+// project_t with its intermediate values (d1, cc, d2, nrm) as well, for a
+// chain rule through it: project_t_fast's operations where `fast` allows
+// them and they are certainly exact, else project_t's.  u, v and the
+// intermediates are project_t's bits either way.
+__device__ __forceinline__ void project_t_parts(const TorchCam& c, float qx, float qy, float qz, bool fast, float& u,
+                                                float& v, float& d1, float& cc, float& d2, float& nrm) {
+  const float xy2 = qx * qx + qy * qy;
+  if (fast) {
+    d1 = sqrt_rn_normal(xy2 + qz * qz);
+    cc = c.xi * d1 + qz;
+    d2 = sqrt_rn_normal(xy2 + cc * cc);
+    nrm = c.a * d2 + c.one_m_a * cc;
+    if (fabsf(nrm) >= 0x1p-30f) {
+      const RcpRN q = rcp_refined(nrm);
+      u = div_rn_normal(c.fs0 * qx, q) + c.cs0;
+      v = div_rn_normal(c.fs1 * qy, q) + c.cs1;
+      return;
+    }
+  }
+  d1 = sqrtf(xy2 + qz * qz);
+  cc = c.xi * d1 + qz;
+  d2 = sqrtf(xy2 + cc * cc);
+  nrm = c.a * d2 + c.one_m_a * cc;
+  u = (c.fs0 * qx) / nrm + c.cs0;
+  v = (c.fs1 * qy) / nrm + c.cs1;
+}
+
 // torch.matmul(cat([p, 1]), rt.T) for one 3x4 row-major transform, as the
 // k-ordered multiply-add chain of a GEMM micro-kernel.
 __device__ __forceinline__ void rigid(const float* rt, float px, float py, float pz, float& qx, float& qy,
