@@ -222,7 +222,7 @@ namespace {
 // var_inv_i = 1 / (2 sigma_i^2), isb_filter.py:88), already enqueued on s.
 int run_pyramid(Level* const* lv, int nb, int L, int G, int V, int vt, const float* wtab, double sigma_s,
                 bool fuse_last, const ss::ArgminParams& am, float* const* dist, hipStream_t s, int stage_down,
-                int stage_up, float* const* fill = nullptr, int xcd = 1) {
+                int stage_up, float* const* fill = nullptr, int xcd = 1, bool guide0_unused = false) {
   if (nb < 1 || nb > ss::kMaxBatch) return fail(SS_ERR_ARG, "batch of %d pyramids outside [1, %d]", nb, ss::kMaxBatch);
   const Level* l0 = lv[0];  // the shapes (identical for every entry)
   const int NP = G * V;
@@ -264,6 +264,7 @@ int run_pyramid(Level* const* lv, int nb, int L, int G, int V, int vt, const flo
       ua[b] = ss::IsbUpArgs{v[k + 1].g, v[k + 1].c, v[k + 1].h, v[k + 1].w, v[k].g, v[k].c, v[k].h, v[k].w,
                             (float)wu,  (float)wd,  wtab,       fuse ? dist[b] : nullptr, am,
                             fuse && fill ? fill[b] : nullptr};
+      ua[b].skip_guide = k == 0 && guide0_unused ? 1 : 0;
     }
     StageTimer t(fuse ? SS_STAGE_ISB_UP_ARGMIN : stage_up, s);
     SS_HIP(ss::launch_isb_up(ua, nb, G, V, k == 0 ? vt : SS_VOLUME_F32, fuse, xcd, s));
@@ -933,8 +934,10 @@ int estimate_after_sweep(int nb, int H, int W, int N, int vt, const float* cands
       for (int b = 0; b < nb; ++b) SS_HIP(ss::launch_fill(fill[b], 1e8f, (size_t)H * W, s));
   }
   if (!distance_filter) return SS_OK;
+  // the level-0 guide is not read after the distance filter (the reference
+  // filters a copy of it, depth_estimation.py:201): its last up pass skips it
   return run_pyramid(dlp, nb, L, 1, 1, SS_VOLUME_F32, wtab + ss::kWeightTable, sigma_s / 2, false, ss::ArgminParams{},
-                     nullptr, s, SS_STAGE_DIST_DOWN, SS_STAGE_DIST_UP);
+                     nullptr, s, SS_STAGE_DIST_DOWN, SS_STAGE_DIST_UP, nullptr, 1, true);
 }
 
 // The cost filter of the estimate path (its fused argmin included) timed per

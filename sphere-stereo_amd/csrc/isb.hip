@@ -326,7 +326,9 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
   // channel sums in the reference's tap order, after the cost pass so that
   // its loads do not wait behind these stores
   if constexpr (!FUSE_ARGMIN) {
-    if constexpr (G == 1) {
+    if (a.skip_guide) {
+      // launch-uniform: nothing reads this fine guide afterwards
+    } else if constexpr (G == 1) {
       if (active) {
         uint32_t gD[9];
 #pragma unroll

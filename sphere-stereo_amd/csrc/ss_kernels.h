@@ -44,6 +44,7 @@ struct IsbUpArgs {
   float* dist;  // fused argmin output (level-0 distance map)
   ArgminParams am;
   float* fill;  // nullable: with the fused argmin, set to 1e8 at every level-0 pixel
+  int skip_guide = 0;  // 1: the fine level's guide is not needed afterwards (not written)
 };
 
 // One launch runs the same pass for up to kMaxBatch references of equal
