@@ -543,6 +543,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--frames-cache", default=None, help="reuse rendered fisheyes saved at this path")
+    ap.add_argument("--tuning", default=None, help="autotuner results of the plain and precision forms: loaded from "
+                    "this JSON file when it exists (no tuning run), else tuned and saved there (counter runs, which "
+                    "serialise kernels, then time the variants of a normal run)")
     ap.add_argument("--batch", type=int, default=8,
                     help="frames per batch of the batching form (0: skip it); each from its own seeded rig")
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged batching form (16 rigs of mixed "
@@ -579,7 +582,13 @@ def main():
     est, to_match, to_stitch, calibs, kw = setup(wl, device, args.frames_cache)
     # the autotuner's variants (sweep, cost filter, inpainting tile: identical
     # bits) for this configuration, fp32 and the 16-bit volume types
-    tuned_plain = est.autotune(to_match)
+    from sphere_stereo.depth_estimation import load_sweep_tuning, save_sweep_tuning
+
+    tuning_loaded = bool(args.tuning) and os.path.exists(args.tuning)
+    if tuning_loaded:
+        tuned_plain = {"loaded": args.tuning, "records": load_sweep_tuning(args.tuning)}
+    else:
+        tuned_plain = est.autotune(to_match)
     for _ in range(args.warmup):
         est.estimate_RGBD_panorama(to_match, to_stitch)
     torch.cuda.synchronize()
@@ -636,7 +645,7 @@ def main():
     forms = {}
     for dt in ("float16", "bfloat16"):
         est_h = RGBD_Estimator(calibs, device=device, volume_dtype=dt, **kw)
-        tuned_h = est_h.autotune(to_match)
+        tuned_h = {"loaded": args.tuning} if tuning_loaded else est_h.autotune(to_match)
         for _ in range(args.warmup):
             est_h.estimate_RGBD_panorama(to_match, to_stitch)
         torch.cuda.synchronize()
@@ -654,6 +663,8 @@ def main():
         print("[bench] %s volume: %.1f panoramas/s" % (dt, args.steps / h_el), file=sys.stderr, flush=True)
         del est_h
 
+    if args.tuning and not tuning_loaded:
+        save_sweep_tuning(args.tuning)
     batching = batching_form(args, wl, device, value) if args.batch > 0 else None
     ragged = batching_ragged(args, device) if not args.no_ragged else None
     diff_form = differentiation_form(args, device) if not args.no_diff_form else None

@@ -105,11 +105,16 @@ class RGBD_Estimator:
         (ss_inpaint_autotune) on that reference's distance map reprojected as
         the stitcher reprojects it, ``maps`` copies inpainted together (default:
         the number of references, as a frame runs them), recorded for the map
-        size.  Every variant
-        gives the same bits.  Returns {"variant": v, "ms": [...],
-        "filter_variant": f, "filter_ms": [...], "inpaint_variant": i,
-        "inpaint_ms": [...]} (ms per launch or per run; -1 for a filter variant
-        that does not exist for this candidate count)."""
+        size; with ``maps`` left at its default the tile is then picked by
+        timing this estimator's whole frame with each tile (zero stitch
+        images).  A variant other than the default is taken only when it is at
+        least 2 % faster.  Every variant gives the same bits.  Returns
+        {"variant": v, "ms": [...], "filter_variant": f, "filter_ms": [...],
+        "inpaint_variant": i, "inpaint_ms": [...], "inpaint_frame_ms": [...]}
+        (ms per launch, per run or per frame; -1 for a filter variant that
+        does not exist for this candidate count).  A captured frame graph is
+        dropped (the next graph call captures the tuned kernels), and the
+        frames it times overwrite the estimator's output buffers."""
         imgs = [_hwc(i) for i in images_to_match]
         if len(imgs) != len(self.calibrations):
             raise ValueError("expected %d images to match" % len(self.calibrations))
@@ -159,6 +164,48 @@ class RGBD_Estimator:
                                                     int(repeats), self._stream(), ims, _lib.C.byref(ibest)))
             _TUNED_INPAINT[(self.rows, self.cols)] = int(ibest.value)
             out.update(inpaint_variant=int(ibest.value), inpaint_ms=[round(float(v), 4) for v in ims])
+            if maps is None:
+                # for this estimator's own frame, whole frames decide: the
+                # inpainting launch meets its maps right after the
+                # reprojection wrote them there, which the copies above do
+                # not reproduce (a tile faster there by 7 % ran 10 % slower
+                # inside the frame)
+                fms_f = self._time_frames_per_inpaint_tile(imgs, ni, int(repeats))
+                pick = 0
+                for v in range(1, ni):
+                    if fms_f[v] < 0.98 * fms_f[0] and fms_f[v] < fms_f[pick]:
+                        pick = v
+                _lib.check(_lib.lib.ss_inpaint_set_variant(self.rows, self.cols, pick))
+                _TUNED_INPAINT[(self.rows, self.cols)] = pick
+                out.update(inpaint_variant=pick, inpaint_frame_ms=[round(v, 4) for v in fms_f])
+        # a captured frame graph keeps the kernels it recorded
+        if self._frame:
+            _lib.lib.ss_frame_destroy(self._frame)
+            self._frame = self._make_frame()
+        return out
+
+    def _time_frames_per_inpaint_tile(self, imgs, ni, repeats):
+        """ms per frame (eager, zero stitch images) with each inpainting tile
+        variant of this map size."""
+        st = self.fishey_stitcher
+        stitch = [torch.zeros([st.srows, st.scols, 3], device=self.device)
+                  for _ in self.references_indices]
+        stream = torch.cuda.current_stream(self.device)
+        graph, self.use_graph = self.use_graph, False
+        out = []
+        try:
+            for v in range(ni):
+                _lib.check(_lib.lib.ss_inpaint_set_variant(self.rows, self.cols, v))
+                self.estimate_RGBD_panorama(imgs, stitch)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(max(1, repeats)):
+                    self.estimate_RGBD_panorama(imgs, stitch)
+                e1.record(stream)
+                e1.synchronize()
+                out.append(e0.elapsed_time(e1) / max(1, repeats))
+        finally:
+            self.use_graph = graph
         return out
 
     def _config_key(self):

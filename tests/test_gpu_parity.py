@@ -750,6 +750,11 @@ def test_sweep_variants_identical_and_autotune(gpu, N, vtype):
         ni = lib.ss_inpaint_variant_count()
         assert 0 <= res["inpaint_variant"] < ni and len(res["inpaint_ms"]) == ni and min(res["inpaint_ms"]) > 0, res
         assert lib.ss_inpaint_get_variant(H, W) == res["inpaint_variant"]
+        # the estimator's own frame decides the inpainting tile, with the same margin
+        ff = res["inpaint_frame_ms"]
+        iv = res["inpaint_variant"]
+        assert len(ff) == ni and min(ff) > 0
+        assert (iv == 0 and min(ff) >= 0.98 * ff[0] - 1e-4) or (ff[iv] == min(ff) and ff[iv] < 0.98 * ff[0] + 1e-4)
         got = est.estimate_RGBD_panorama(fr["to_match"], fr["to_stitch"])
         torch.cuda.synchronize()
         assert torch.equal(got[0], want[0])
