@@ -225,6 +225,27 @@ __device__ __forceinline__ float argmin_refine_group(const float* vals, int lane
 }
 
 // ---------------------------------------------------------------------------
+// The neighbour lane's value (0 <-> 1, 2 <-> 3 of each quad of four lanes):
+// a DPP move, one VALU instruction and no LDS round trip.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int quad_swap(int v) {
+  constexpr int kCtrl = 1 | (0 << 2) | (3 << 4) | (2 << 6);
+  return __builtin_amdgcn_update_dpp(0, v, kCtrl, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float quad_swap(float v) { return __int_as_float(quad_swap(__float_as_int(v))); }
+
+// The fused argmin's form: through LDS (fewer VALU instructions) for 16-bit
+// level-0 volumes, with groups of 4 or 8 lanes and at most 8 candidates per
+// lane (16 or 32 KB of LDS per workgroup); in registers otherwise.  The
+// fp32 pass moves twice the volume bytes, and its register form keeps the
+// schedule that issues every volume load before the weights are computed
+// (measured faster at 1024^2, 32 candidates: 97-101 against 109-128 us).
+template <int G, int V, typename T>
+constexpr bool lds_argmin() {
+  return G >= 4 && V <= 8 && sizeof(T) == 2;
+}
+
+// ---------------------------------------------------------------------------
 // Up-sampling: guideUpsample2xKernel (isb_filter.cu:132-330).  FUSE_ARGMIN
 // replaces the level-0 write-back with the argmin of argmin_refine_group.
 // ---------------------------------------------------------------------------
@@ -374,23 +395,96 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
   }
 
   if constexpr (FUSE_ARGMIN) {
-    // the group reductions need every lane; the refinement of fine pixel s
-    // (three divisions) then runs once, in lane s, instead of in all 8
-    ArgminGroup mine = {};
+    // (fine[] are the fine pixels' indices in the level-0 distance map too)
+    if constexpr (lds_argmin<G, V, T>()) {
+      // The group's filtered values go to LDS (fine pixel major, candidate
+      // contiguous); G / 4 lanes then scan each fine pixel's candidates, a
+      // contiguous part each, merge their partial results with a DPP move,
+      // and read the refinement's neighbours back from LDS.  First minimum,
+      // maximum and neighbours are those of argmin_reduce_group.
+      constexpr int LPP = G / 4, SPAN = NP / LPP;
+      __shared__ __align__(16) float abuf[kBlock / G][4][NP];
+      float* const buf = &abuf[threadIdx.x / G][0][0];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const ArgminGroup g = argmin_reduce_group<G, V>(res[s], lane, a.am);
-      if (lane == s) mine = g;
-    }
-    if (active && lane < 4 && doit[lane]) {
-      a.dist[fine[lane]] = argmin_finish(mine, a.am);
-      if (a.fill) a.fill[fine[lane]] = 1e8f;
+      for (int s = 0; s < 4; ++s) store_n<V>(buf + s * NP + lane * V, res[s]);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int s = lane / LPP, n0 = (lane % LPP) * SPAN, N = a.am.N;
+      const float* const pv = buf + s * NP;
+      float vals[SPAN];
+      load_n<SPAN>(pv + n0, vals);
+      float m = __builtin_huge_valf(), M = -__builtin_huge_valf();
+      int im = 0x7fffffff;
+      if (N == NP) {
+#pragma unroll
+        for (int v = 0; v < SPAN; ++v) {
+          if (vals[v] < m) {
+            m = vals[v];
+            im = n0 + v;
+          }
+          M = fmaxf(M, vals[v]);
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < SPAN; ++v) {
+          if (n0 + v < N) {
+            if (vals[v] < m) {
+              m = vals[v];
+              im = n0 + v;
+            }
+            M = fmaxf(M, vals[v]);
+          }
+        }
+      }
+      if constexpr (LPP == 2) {
+        const float om = quad_swap(m), oM = quad_swap(M);
+        const int oi = quad_swap(im);
+        if (om < m || (om == m && oi < im)) {
+          m = om;
+          im = oi;
+        }
+        M = fmaxf(M, oM);
+      }
+      const ArgminGroup g{m, M, pv[max(im - 1, 0)], pv[min(im + 1, N - 1)], im};
+      const float d = argmin_finish(g, a.am);
+      // the next call of this wavefront rewrites the buffer
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const bool mine_s = s == 0 ? doit[0] : (s == 1 ? doit[1] : (s == 2 ? doit[2] : doit[3]));
+      const int at = s == 0 ? fine[0] : (s == 1 ? fine[1] : (s == 2 ? fine[2] : fine[3]));
+      if (active && lane % LPP == 0 && mine_s) {
+        a.dist[at] = d;
+        if (a.fill) a.fill[at] = 1e8f;
+      }
+    } else {
+      // the group reductions need every lane; the refinement of fine pixel s
+      // (three divisions) then runs once, in lane s, instead of in all 8
+      ArgminGroup mine = {};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const ArgminGroup g = argmin_reduce_group<G, V>(res[s], lane, a.am);
+        if (lane == s) mine = g;
+      }
+      if (active && lane < 4 && doit[lane]) {
+        a.dist[fine[lane]] = argmin_finish(mine, a.am);
+        if (a.fill) a.fill[fine[lane]] = 1e8f;
+      }
     }
   }
 }
 
+// The fused level-0 pass over a 16-bit volume (the LDS argmin of up_pixel) is
+// held to 96 VGPRs, 5 waves per SIMD, with up to 4 candidates per lane (its
+// register count is otherwise just above that).
+template <int V, typename T, bool FUSE_ARGMIN>
+constexpr int up_plain_waves() {
+  return FUSE_ARGMIN && V <= 4 && lds_argmin<8, V, T>() ? 5 : 1;
+}
+
 template <int G, int V, typename T, bool FUSE_ARGMIN>
-__global__ void __launch_bounds__(kBlock) k_isb_up(Batch<IsbUpArgs> b, int xcd) {
+__global__ void __launch_bounds__(kBlock, (up_plain_waves<V, T, FUSE_ARGMIN>())) k_isb_up(Batch<IsbUpArgs> b, int xcd) {
   const IsbUpArgs a = b.a[blockIdx.y];
   const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
   const int tile = xcd ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x;
