@@ -234,15 +234,16 @@ __device__ __forceinline__ int quad_swap(int v) {
 }
 __device__ __forceinline__ float quad_swap(float v) { return __int_as_float(quad_swap(__float_as_int(v))); }
 
-// The fused argmin's form: through LDS (fewer VALU instructions) for 16-bit
-// level-0 volumes, with groups of 4 or 8 lanes and at most 8 candidates per
-// lane (16 or 32 KB of LDS per workgroup); in registers otherwise.  The
-// fp32 pass moves twice the volume bytes, and its register form keeps the
-// schedule that issues every volume load before the weights are computed
-// (measured faster at 1024^2, 32 candidates: 97-101 against 109-128 us).
+// The fused argmin's form: through LDS with groups of 4 or 8 lanes and at
+// most 8 candidates per lane (16 or 32 KB of LDS per workgroup), in
+// registers otherwise.  Each fine pixel's filtered values go to LDS as soon
+// as they are computed, so the pass holds fewer registers than the register
+// form (78 against 90 VGPRs at 8 lanes x 4 candidates, fp32) and still
+// issues every volume load before the weights are computed; it also takes
+// fewer VALU instructions (807 against 937 per wave).
 template <int G, int V, typename T>
 constexpr bool lds_argmin() {
-  return G >= 4 && V <= 8 && sizeof(T) == 2;
+  return G >= 4 && V <= 8;
 }
 
 // ---------------------------------------------------------------------------
@@ -313,7 +314,13 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
   }
 
   const size_t base_c = (size_t)lane * V;
-  float res[FUSE_ARGMIN ? 4 : 1][FUSE_ARGMIN ? V : 1];
+  constexpr bool kLdsArgmin = FUSE_ARGMIN && lds_argmin<G, V, T>();
+  // the fused argmin's inputs: fine pixel s's candidates of this lane, in
+  // registers, or written to the group's LDS rows as soon as they are done
+  float res[FUSE_ARGMIN && !kLdsArgmin ? 4 : 1][FUSE_ARGMIN && !kLdsArgmin ? V : 1];
+  constexpr int kAbufGroups = kLdsArgmin ? kBlock / G : 1, kAbufNP = kLdsArgmin ? NP : 1;
+  __shared__ __align__(16) float abuf[kAbufGroups][4][kAbufNP];
+  float* const buf = &abuf[kLdsArgmin ? threadIdx.x / G : 0][0][0];
 #pragma unroll
   for (int cb = 0; cb < V; cb += CH) {
     float cD[9][CH];
@@ -333,7 +340,9 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
         for (int k = kStencilBegin[s]; k < kStencilBegin[s + 1]; ++k) acc = acc + w[k] * cD[up_tap_flat(k)][v];
         o[v] = acc * inv[s];
       }
-      if constexpr (FUSE_ARGMIN) {
+      if constexpr (kLdsArgmin) {
+        store_n<CH>(buf + s * NP + lane * V + cb, o);
+      } else if constexpr (FUSE_ARGMIN) {
 #pragma unroll
         for (int v = 0; v < CH; ++v) res[s][cb + v] = o[v];
       } else {
@@ -403,10 +412,6 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
       // and read the refinement's neighbours back from LDS.  First minimum,
       // maximum and neighbours are those of argmin_reduce_group.
       constexpr int LPP = G / 4, SPAN = NP / LPP;
-      __shared__ __align__(16) float abuf[kBlock / G][4][NP];
-      float* const buf = &abuf[threadIdx.x / G][0][0];
-#pragma unroll
-      for (int s = 0; s < 4; ++s) store_n<V>(buf + s * NP + lane * V, res[s]);
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -475,9 +480,8 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
   }
 }
 
-// The fused level-0 pass over a 16-bit volume (the LDS argmin of up_pixel) is
-// held to 96 VGPRs, 5 waves per SIMD, with up to 4 candidates per lane (its
-// register count is otherwise just above that).
+// The fused level-0 pass with the LDS argmin of up_pixel and up to 4
+// candidates per lane is held to 96 VGPRs (at least 5 waves per SIMD).
 template <int V, typename T, bool FUSE_ARGMIN>
 constexpr int up_plain_waves() {
   return FUSE_ARGMIN && V <= 4 && lds_argmin<8, V, T>() ? 5 : 1;
