@@ -474,6 +474,14 @@ __device__ void unproject_dual(const TorchCam& c, float u, float v, D6 p[3]) {
 // map to tilted curves whose boxes mostly do not fit.)  Other lanes' taps,
 // taps outside the box and chunks whose box does not fit use global atomics
 // directly, so the box only decides speed, never the sums.
+//
+// The window sums in fixed point: per chunk, a power-of-two scale 2^(18-e)
+// from the block's largest |dL/dcost| over the chunk, gmax < 2^e, bounds
+// every contribution by 2^18 and a texel's sum of at most 4096 of them
+// (kBlock pixels x kBwdChunk candidates) by 2^30, so the sums are exact
+// int32 LDS atomics (order-independent, and measured ~30 % faster than
+// float LDS atomics); each contribution is rounded to 2^(e-18), a relative
+// 2^-19 of gmax.
 constexpr int kBwdChunk = 16;
 constexpr int kBwdTile = 16;      // kBwdTile^2 == kBlock pixels per workgroup
 constexpr int kWinTexels = 3072;  // 36 KiB of LDS: four workgroups per CU
@@ -558,7 +566,7 @@ __device__ __forceinline__ int block_max(int v, int* slot) {
 
 __global__ void __launch_bounds__(kBlock) k_sweep_bwd(SweepBwdArgs a) {
   __shared__ float win[kWinTexels * 3];
-  __shared__ int wbox[5];  // window camera, min x, max x, min y, max y
+  __shared__ int wbox[6];  // window camera, min x, max x, min y, max y, gmax bits
   // a workgroup owns a kBwdTile x kBwdTile tile of reference pixels: its
   // samples of one candidate fall on a compact patch of the sampled image
   const int tiles_x = (a.W + kBwdTile - 1) / kBwdTile;
@@ -609,18 +617,35 @@ __global__ void __launch_bounds__(kBlock) k_sweep_bwd(SweepBwdArgs a) {
         by1 = max(by1, min(y0 + 1 + kBwdMargin, a.H - 1));
       }
     }
+    // the largest |dL/dcost| of the chunk over the lanes that add to the window
+    float gl = 0.0f;
+    if (inwin && active) {
+      for (int n = n0; n < n1; n += 4) {
+        const float4 g4 = ld4(gc + n);
+        const float v[4] = {g4.x, g4.y, g4.z, g4.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (n + i < n1) gl = fmaxf(gl, fabsf(v[i]));
+      }
+    }
     if (threadIdx.x == 0) {
       wbox[1] = 0x7fffffff;
       wbox[2] = -1;
       wbox[3] = 0x7fffffff;
       wbox[4] = -1;
+      wbox[5] = 0;
     }
     __syncthreads();
     block_min(bx0, &wbox[1]);
     block_max(bx1, &wbox[2]);
     block_min(by0, &wbox[3]);
     block_max(by1, &wbox[4]);
+    block_max(__float_as_int(gl), &wbox[5]);  // non-negative floats order as their bits
     __syncthreads();
+    int gexp = 0;
+    (void)frexpf(__int_as_float(wbox[5]), &gexp);
+    gexp = max(gexp, -100);
+    const float qscale = ldexpf(1.0f, 18 - gexp), qinv = ldexpf(1.0f, gexp - 18);
     const int wx0 = wbox[1], wy0 = wbox[3];
     const int ww = wbox[2] - wx0 + 1, wh = wbox[4] - wy0 + 1;
     const bool use_win = kw >= 0 && wbox[2] >= 0 && wbox[4] >= 0 && ww * wh <= kWinTexels;  // block-uniform
@@ -709,9 +734,9 @@ __global__ void __launch_bounds__(kBlock) k_sweep_bwd(SweepBwdArgs a) {
               const int tx = cur.x0 + (t & 1), ty = cur.y0 + (t >> 1);
               const int lx = tx - wx0, ly = ty - wy0;
               if (use_win && inwin && lx >= 0 && lx < ww && ly >= 0 && ly < wh) {
-                float* dst = win + (ly * ww + lx) * 3;
+                int* dst = reinterpret_cast<int*>(win) + (ly * ww + lx) * 3;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) atomicAdd(dst + c, wt[t] * gs[c]);
+                for (int c = 0; c < 3; ++c) atomicAdd(dst + c, __float2int_rn(wt[t] * gs[c] * qscale));
               } else {
                 float* dst = gimg + ((size_t)ty * a.W + tx) * 3;
 #pragma unroll
@@ -770,11 +795,11 @@ __global__ void __launch_bounds__(kBlock) k_sweep_bwd(SweepBwdArgs a) {
       // the window into the image: one global atomic per touched texel and channel
       __syncthreads();
       for (int i = (int)threadIdx.x; i < ww * wh * 3; i += kBlock) {
-        const float v = win[i];
-        if (v != 0.0f) {
+        const int q = reinterpret_cast<const int*>(win)[i];
+        if (q != 0) {
           const int t = i / 3, c = i - 3 * t;
           const int ly = t / ww, lx = t - ly * ww;
-          unsafeAtomicAdd(gwin + ((size_t)(wy0 + ly) * a.W + wx0 + lx) * 3 + c, v);
+          unsafeAtomicAdd(gwin + ((size_t)(wy0 + ly) * a.W + wx0 + lx) * 3 + c, (float)q * qinv);
         }
       }
     }
