@@ -137,14 +137,18 @@ void up_weights(double sigma_s, int k, double* wu, double* wd) {
 }
 
 // The coarse end of a pyramid runs in one workgroup from level s0 on: every
-// level it produces holds at most kTailPixels pixels (one CU then does each
-// pass in about a microsecond; from 4096 pixels on, a pass is VALU-bound on
-// that CU and costs more than the launches it replaces) and all of them
-// (guide + NP values per pixel, and the weight table) fit in the LDS budget.  The tail starts at level 1 or deeper (its levels
-// are all fp32, and a fused level-0 pass stays a regular launch).  Returns 0
-// when the pyramid has no such tail.
-int tail_start(const PyrLevel* l0, int L, int NP) {
-  const long long kTailPixels = 256;
+// level it produces needs at most kTailLanes lanes (pixels x G, the lanes per
+// pixel), i.e. at most two pixels per thread of the 1024-thread workgroup, so
+// that one CU does each pass in one or two wavefronts' issue time (from four
+// pixels per thread on, a pass is VALU-bound on that CU and costs more than
+// the launch it replaces), and all of its levels (guide + NP values per
+// pixel, and the weight table) fit in the LDS budget.  With 8 lanes per pixel
+// the tail starts where the next level holds 256 pixels; the one-lane distance
+// filter reaches one level further (2048 pixels).  The tail starts at level
+// 1 or deeper (its levels are all fp32, and a fused level-0 pass stays a
+// regular launch).  Returns 0 when the pyramid has no such tail.
+int tail_start(const PyrLevel* l0, int L, int G, int NP) {
+  const long long kTailLanes = 2048;
   auto lds_need = [&](int from) {
     size_t b = 0;
     for (int k = from + 1; k < L; ++k) {
@@ -155,7 +159,7 @@ int tail_start(const PyrLevel* l0, int L, int NP) {
   };
   int s0 = L - 1;
   for (int k = 1; k < L - 1; ++k)
-    if ((long long)l0[k + 1].h * l0[k + 1].w <= kTailPixels && lds_need(k) <= (size_t)kTailLdsBytes) {
+    if ((long long)l0[k + 1].h * l0[k + 1].w * G <= kTailLanes && lds_need(k) <= (size_t)kTailLdsBytes) {
       s0 = k;
       break;
     }
@@ -226,7 +230,7 @@ int run_pyramid(Level* const* lv, int nb, int L, int G, int V, int vt, const flo
   if (nb < 1 || nb > ss::kMaxBatch) return fail(SS_ERR_ARG, "batch of %d pyramids outside [1, %d]", nb, ss::kMaxBatch);
   const Level* l0 = lv[0];  // the shapes (identical for every entry)
   const int NP = G * V;
-  const int s0 = ss::tail_start(l0, L, NP);
+  const int s0 = ss::tail_start(l0, L, G, NP);
   const bool tail = s0 > 0;
   const int last_down = tail ? s0 : L - 1;
   ss::IsbDownArgs da[ss::kMaxBatch];

@@ -199,7 +199,7 @@ void plan_filter(Image& im, int G, int vt, const std::vector<const PyrLevel*>& l
   std::vector<int> s0(P), last_down(P), first_up(P);
   int max_down = 0, max_up = -1;
   for (int p = 0; p < P; ++p) {
-    s0[p] = tail_start(lv[p], L[p], G * V[p]);
+    s0[p] = tail_start(lv[p], L[p], G, G * V[p]);
     last_down[p] = s0[p] ? s0[p] : L[p] - 1;
     first_up[p] = last_down[p] - 1;
     max_down = std::max(max_down, last_down[p]);

@@ -264,7 +264,7 @@ struct PyrLevel {
 };
 int isb_levels(int H, int W);
 void up_weights(double sigma_s, int k, double* wu, double* wd);
-int tail_start(const PyrLevel* l0, int L, int NP);
+int tail_start(const PyrLevel* l0, int L, int G, int NP);
 unsigned tail_levels(const PyrLevel* l0, int s0, int L, int NP, const float* wtab, double sigma_s, TailLevels* t);
 // an estimate workspace's cost pyramid lv[0..L) (level 0 = guide + volume)
 // and distance pyramid dl[0..L) (level 0 = guide + dist); returns L
