@@ -469,11 +469,15 @@ __device__ void unproject_dual(const TorchCam& c, float u, float v, D6 p[3]) {
 // atomics) and the window is then added to the image with one global atomic
 // per touched texel and channel.  The samples of 16 consecutive candidates
 // of a 16 x 16 pixel tile sweep a patch of ~1000 texels (median box ~1700
-// texels at 768^2, 48 candidates; 95 % of the boxes fit), against ~16000
-// tap updates: ~16x fewer global atomics.  (Tiles of 256 pixels of one row
-// map to tilted curves whose boxes mostly do not fit.)  Other lanes' taps,
-// taps outside the box and chunks whose box does not fit use global atomics
-// directly, so the box only decides speed, never the sums.
+// texels at 768^2, 48 candidates), against ~16000 tap updates: ~16x fewer
+// global atomics.  (Tiles of 256 pixels of one row map to tilted curves
+// whose boxes mostly do not fit.)  A chunk whose box exceeds the window is
+// halved (16 -> 8 -> 4 candidates) until it fits: at 768^2 and 48
+// candidates, 10 % of the 16-candidate chunks did not fit, and their 13.6 %
+// of the taps, sent to global atomics, took ~60 % of the kernel's time.
+// Other lanes' taps, taps outside the box and chunks whose box does not fit
+// even at 4 candidates use global atomics directly, so the box only decides
+// speed, never the sums.
 //
 // The window sums in fixed point: per chunk, a power-of-two scale 2^(18-e)
 // from the block's largest |dL/dcost| over the chunk, gmax < 2^e, bounds
@@ -483,11 +487,14 @@ __device__ void unproject_dual(const TorchCam& c, float u, float v, D6 p[3]) {
 // float LDS atomics); each contribution is rounded to 2^(e-18), a relative
 // 2^-19 of gmax.
 constexpr int kBwdChunk = 16;
+constexpr int kBwdMinChunk = 4;  // chunks stay multiples of 4 candidates (the 16-byte dL/dcost loads)
 constexpr int kBwdTile = 16;      // kBwdTile^2 == kBlock pixels per workgroup
 constexpr int kWinTexels = 3072;  // 36 KiB of LDS: four workgroups per CU
 constexpr int kBwdMargin = 2;
 static_assert(kBwdTile * kBwdTile == kBlock, "one lane per pixel of a tile");
-static_assert(kBwdChunk % 4 == 0, "chunks start at a multiple of 4 candidates");
+static_assert(kBwdChunk % 4 == 0 && kBwdMinChunk % 4 == 0 && kBwdChunk % kBwdMinChunk == 0 &&
+                  (kBwdChunk / kBwdMinChunk & (kBwdChunk / kBwdMinChunk - 1)) == 0,
+              "chunks start at a multiple of 4 candidates");
 
 // The sample of candidate n of a pixel with ray rl in camera (cam, rt): the
 // point in the camera, project_t's intermediates, the grid_sample
@@ -600,55 +607,70 @@ __global__ void __launch_bounds__(kBlock) k_sweep_bwd(SweepBwdArgs a) {
   const bool inwin = kw >= 0 && k == kw;  // this lane's taps may go to the window
   float* const gwin = kw >= 0 ? a.gimg[kw] : nullptr;
 
-  for (int n0 = 0; n0 < a.N; n0 += kBwdChunk) {
-    const int n1 = min(n0 + kBwdChunk, a.N);
-    // the window box from the chunk's first, middle and last candidates
-    int bx0 = 0x7fffffff, bx1 = -1, by0 = 0x7fffffff, by1 = -1;
-    if (inwin) {
-      const int probe[3] = {n0, (n0 + n1 - 1) / 2, n1 - 1};
+  for (int n0 = 0, len = kBwdChunk; n0 < a.N;) {
+    // the chunk [n0, n1): len candidates, halved down to kBwdMinChunk while
+    // the window box of its samples exceeds kWinTexels (the next chunk starts
+    // at twice the length that fitted)
+    int n1, wx0, wy0, ww, wh;
+    float qscale, qinv;
+    bool use_win;
+    for (;;) {
+      n1 = min(n0 + len, a.N);
+      // the window box from the chunk's first, middle and last candidates
+      int bx0 = 0x7fffffff, bx1 = -1, by0 = 0x7fffffff, by1 = -1;
+      if (inwin) {
+        const int probe[3] = {n0, (n0 + n1 - 1) / 2, n1 - 1};
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        float ix, iy;
-        if (!bwd_sample_coords(a, cam, rt, rl, probe[i], ix, iy)) continue;
-        const int x0 = (int)floorf(ix), y0 = (int)floorf(iy);
-        bx0 = min(bx0, max(x0 - kBwdMargin, 0));
-        bx1 = max(bx1, min(x0 + 1 + kBwdMargin, a.W - 1));
-        by0 = min(by0, max(y0 - kBwdMargin, 0));
-        by1 = max(by1, min(y0 + 1 + kBwdMargin, a.H - 1));
+        for (int i = 0; i < 3; ++i) {
+          float ix, iy;
+          if (!bwd_sample_coords(a, cam, rt, rl, probe[i], ix, iy)) continue;
+          const int x0 = (int)floorf(ix), y0 = (int)floorf(iy);
+          bx0 = min(bx0, max(x0 - kBwdMargin, 0));
+          bx1 = max(bx1, min(x0 + 1 + kBwdMargin, a.W - 1));
+          by0 = min(by0, max(y0 - kBwdMargin, 0));
+          by1 = max(by1, min(y0 + 1 + kBwdMargin, a.H - 1));
+        }
       }
-    }
-    // the largest |dL/dcost| of the chunk over the lanes that add to the window
-    float gl = 0.0f;
-    if (inwin && active) {
-      for (int n = n0; n < n1; n += 4) {
-        const float4 g4 = ld4(gc + n);
-        const float v[4] = {g4.x, g4.y, g4.z, g4.w};
+      // the largest |dL/dcost| of the chunk over the lanes that add to the window
+      float gl = 0.0f;
+      if (inwin && active) {
+        for (int n = n0; n < n1; n += 4) {
+          const float4 g4 = ld4(gc + n);
+          const float v[4] = {g4.x, g4.y, g4.z, g4.w};
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-          if (n + i < n1) gl = fmaxf(gl, fabsf(v[i]));
+          for (int i = 0; i < 4; ++i)
+            if (n + i < n1) gl = fmaxf(gl, fabsf(v[i]));
+        }
       }
+      if (threadIdx.x == 0) {
+        wbox[1] = 0x7fffffff;
+        wbox[2] = -1;
+        wbox[3] = 0x7fffffff;
+        wbox[4] = -1;
+        wbox[5] = 0;
+      }
+      __syncthreads();
+      block_min(bx0, &wbox[1]);
+      block_max(bx1, &wbox[2]);
+      block_min(by0, &wbox[3]);
+      block_max(by1, &wbox[4]);
+      block_max(__float_as_int(gl), &wbox[5]);  // non-negative floats order as their bits
+      __syncthreads();
+      int gexp = 0;
+      (void)frexpf(__int_as_float(wbox[5]), &gexp);
+      gexp = max(gexp, -100);
+      qscale = ldexpf(1.0f, 18 - gexp);
+      qinv = ldexpf(1.0f, gexp - 18);
+      wx0 = wbox[1];
+      wy0 = wbox[3];
+      ww = wbox[2] - wx0 + 1;
+      wh = wbox[4] - wy0 + 1;
+      const bool boxed = kw >= 0 && wbox[2] >= 0 && wbox[4] >= 0;
+      use_win = boxed && ww * wh <= kWinTexels;  // block-uniform, as is everything below
+      if (use_win || !boxed || len <= kBwdMinChunk) break;
+      len /= 2;
+      __syncthreads();  // every lane has read the box before it is reset
     }
-    if (threadIdx.x == 0) {
-      wbox[1] = 0x7fffffff;
-      wbox[2] = -1;
-      wbox[3] = 0x7fffffff;
-      wbox[4] = -1;
-      wbox[5] = 0;
-    }
-    __syncthreads();
-    block_min(bx0, &wbox[1]);
-    block_max(bx1, &wbox[2]);
-    block_min(by0, &wbox[3]);
-    block_max(by1, &wbox[4]);
-    block_max(__float_as_int(gl), &wbox[5]);  // non-negative floats order as their bits
-    __syncthreads();
-    int gexp = 0;
-    (void)frexpf(__int_as_float(wbox[5]), &gexp);
-    gexp = max(gexp, -100);
-    const float qscale = ldexpf(1.0f, 18 - gexp), qinv = ldexpf(1.0f, gexp - 18);
-    const int wx0 = wbox[1], wy0 = wbox[3];
-    const int ww = wbox[2] - wx0 + 1, wh = wbox[4] - wy0 + 1;
-    const bool use_win = kw >= 0 && wbox[2] >= 0 && wbox[4] >= 0 && ww * wh <= kWinTexels;  // block-uniform
     if (use_win) {
       for (int i = (int)threadIdx.x; i < ww * wh * 3; i += kBlock) win[i] = 0.0f;
       __syncthreads();
@@ -804,6 +826,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep_bwd(SweepBwdArgs a) {
       }
     }
     __syncthreads();  // the window and its box are reused by the next chunk
+    n0 = n1;
+    len = min(2 * len, kBwdChunk);
   }
   if (active && a.gimg[a.ref]) {
     float* dst = a.gimg[a.ref] + 3 * (size_t)p;
