@@ -16,7 +16,8 @@ against the same frames run one at a time ("batching_form_ragged").  Last, the
 differentiation form (4 frames of one rig at 768^2, 48 candidates): forward
 and forward + backward time per batch and a 50-step Gauss-Newton calibration
 refinement ("differentiation_form"; --no-diff-form skips it).
---diff-report checks its gradient against fp64 central differences instead.
+--diff-report checks its gradient against fp64 central differences instead;
+--issue-rates prints the measured per-instruction-class issue costs instead.
 
 Prints one JSON line (see the roofline / cpu_baseline fields) and a per-stage
 split on stderr.
@@ -534,6 +535,21 @@ def precision_report(args):
     print(json.dumps(line), flush=True)
 
 
+def issue_rates():
+    """Runs the issue-rate probe (built by __graft_entry__.build() from
+    sphere-stereo_amd/csrc/probes/issue_rates.hip) as a child process and
+    passes its markdown tables through."""
+    import subprocess
+    exe = os.path.join(ROOT, "sphere-stereo_amd", "sphere_stereo", "issue_rates")
+    if not os.path.exists(exe):
+        sys.exit("the issue-rate probe is not built (%s); run __graft_entry__.build()" % exe)
+    res = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    sys.stdout.write(res.stdout)
+    if res.returncode != 0:
+        sys.stderr.write(res.stderr)
+        sys.exit(res.returncode)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -559,7 +575,13 @@ def main():
                     help="instead of the throughput line: the differentiation form's GPU gradient against fp64 "
                          "central differences (h = 1e-4) of the oracle at full size")
     ap.add_argument("--diff-params", type=int, default=6, help="parameters checked by --diff-report")
+    ap.add_argument("--issue-rates", action="store_true",
+                    help="instead of the throughput line: the issue-rate probe's table (SIMD cycles per wave64 "
+                         "instruction by class, LDS atomic throughput) that the kernels' rooflines are priced with")
     args = ap.parse_args()
+    if args.issue_rates:
+        issue_rates()
+        return
     if args.diff_report:
         diff_report(args)
         return
@@ -680,9 +702,10 @@ def main():
         if traffic is not None:
             traffic = dict(traffic, source="profiles/traffic_%s.json (rocprofv3 FETCH_SIZE / WRITE_SIZE runs, not "
                                            "this run)" % args.workload)
-    # the dominant kernel's vector-ALU occupancy (rocprofv3 VALUBusy and
-    # VALU instructions per launch, profiles/valu_<workload>.json): the sweep
-    # is bound by its fp32 VALU issue, not by HBM
+    # the dominant kernel's VALU counters (rocprofv3 SQ_INSTS_VALU by class,
+    # VALUBusy, profiles/valu_<workload>.json) and its VALU issue time priced
+    # at measured per-class rates: the sweep is bound neither by HBM nor by
+    # VALU issue (~2/3 of its time), but by its bilinear gathers
     valu = None
     vpath = os.path.join(ROOT, "profiles", "valu_%s.json" % args.workload)
     if os.path.exists(vpath):

@@ -71,5 +71,25 @@ def build(verbose: bool = False, extra_flags=None, jobs=None) -> str:
     return LIB
 
 
+PROBE_SRC = os.path.join(CSRC, "probes", "issue_rates.hip")
+PROBE_BIN = os.path.join(HERE, "issue_rates")
+
+
+def build_probes(verbose: bool = False) -> str:
+    """Compiles the issue-rate probe (csrc/probes/issue_rates.hip), a
+    stand-alone gfx950 program that `bench.py --issue-rates` runs."""
+    flags = [f for f in HIPCC_FLAGS if f not in ("-shared", "-fPIC")]
+    tmp = PROBE_BIN + ".tmp"
+    cmd = [hipcc()] + flags + ["-o", tmp, PROBE_SRC]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("hipcc failed on %s:\n%s%s" % (PROBE_SRC, res.stdout, res.stderr))
+    os.replace(tmp, PROBE_BIN)
+    return PROBE_BIN
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_probes(verbose="-v" in sys.argv))
