@@ -125,9 +125,15 @@ def _sweep_inputs(gpu, p):
     return imgs, arr, raylut
 
 
-@pytest.mark.parametrize("ref", [0, 2])
-def test_sweep_backward_matches_torch(gpu, ref):
-    p = problem(matching=(40, 36), N=8, ref=ref)
+@pytest.mark.parametrize("matching,N,ref", [((40, 36), 8, 0), ((40, 36), 8, 2), ((256, 256), 16, 0),
+                                             ((256, 256), 16, 2)])
+def test_sweep_backward_matches_torch(gpu, matching, N, ref):
+    """The sweep adjoint against autograd of the float64 restatement.  At
+    256^2 with 16 candidates a 16-candidate chunk spans the whole disparity
+    range and about one chunk in ten has a box of samples larger than the
+    LDS window, so the chunk-halving path (and its per-chunk fixed-point
+    scale) is exercised; at 40 x 36 every box fits."""
+    p = problem(matching=matching, N=N, ref=ref)
     H, W = p["imgs32"].shape[1:3]
     N, Cn = p["N"], len(p["imgs32"])
     NP = _lib.lib.ss_padded_candidates(N)
@@ -158,10 +164,24 @@ def test_sweep_backward_matches_torch(gpu, ref):
         assert float((g_cams[k] - cams6.grad[k]).abs().max()) <= tol, (k, g_cams[k], cams6.grad[k])
         tol = 2e-3 * float(rel.grad[k].abs().max()) + 1e-3
         assert float((gp[k, 6:].view(3, 4) - rel.grad[k]).abs().max()) <= tol, (k, gp[k, 6:], rel.grad[k])
+    # Image gradients: a sample whose |sample - reference| lies within
+    # rounding of 0 takes the other sign of |.|'s derivative in fp32 than in
+    # fp64 (a kink), which moves the entries it touches by up to 2 |dL/dcost|;
+    # at 40 x 36 no sample does, at 256^2 a handful.  Every other entry agrees
+    # to 2e-3 of the largest.
+    gmax = float(gC[..., :N].abs().max())
     for k in range(Cn):
         got = g_imgs[k].cpu().double()
         want = images[k].grad if images[k].grad is not None else torch.zeros_like(got)  # never sampled
-        assert float((got - want).abs().max()) <= 2e-3 * float(want.abs().max()) + 1e-6, k
+        err = (got - want).abs()
+        tol = 2e-3 * float(want.abs().max()) + 1e-6
+        bad = int((err > tol).sum())
+        rel = float(err.norm()) / max(float(want.norm()), 1e-30)
+        print("image %d: max |err| %.3g (tol %.3g), %d of %d entries over, relative L2 %.3g" %
+              (k, float(err.max()), tol, bad, err.numel(), rel), flush=True)
+        assert bad <= err.numel() // 5000, (k, bad)
+        assert float(err.max()) <= 4 * gmax + tol, (k, float(err.max()))  # at most two flips on one entry
+        assert rel <= 2e-3, (k, rel)
 
 
 def test_full_gradient_matches_torch_and_central_differences(gpu):
