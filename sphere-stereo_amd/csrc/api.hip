@@ -895,7 +895,7 @@ int estimate_levels(void* ws, int H, int W, int N, int vt, float* dist, PyrLevel
 
 int estimate_after_sweep(int nb, int H, int W, int N, int vt, const float* cands, double sigma_i, double sigma_s,
                          float* const* d_dist, void* const* ws, hipStream_t s, float* const* fill, int isb_variant,
-                         bool distance_filter) {
+                         bool distance_filter, bool tables_ready) {
   if (nb < 1 || nb > kMaxBatch) return fail(SS_ERR_ARG, "batch of %d references outside [1, %d]", nb, kMaxBatch);
   const int L = levels_of(H, W);
   Level lv[kMaxBatch][32], dl[kMaxBatch][32];
@@ -913,7 +913,7 @@ int estimate_after_sweep(int nb, int H, int W, int N, int vt, const float* cands
     dlp[b] = dl[b];
   }
   const float d0 = cands[0], dN1 = cands[N - 1];
-  {
+  if (!tables_ready) {
     // the cost filter's table (sigma_i) and the distance filter's (sigma_i / 2,
     // depth_estimation.py:201-202), one launch
     const float vi[2] = {(float)(1 / (2 * sigma_i * sigma_i)), (float)(1 / (2 * (sigma_i / 2) * (sigma_i / 2)))};

@@ -307,9 +307,11 @@ void estimate_level0(void* ws, int H, int W, int N, int vt, uint32_t** g0, void*
 // level-0 guide and volume of an estimate workspace; records the fastest
 int time_filter_variants(void* ws, int H, int W, int N, int vt, const float* cands, double sigma_i, double sigma_s,
                          float* d_dist, int repeats, hipStream_t s, float* ms_out, int* best_out);
+// tables_ready: entry 0's workspace already holds the guide weight tables of
+// sigma_i (the frame executor computes them on its first eager run)
 int estimate_after_sweep(int nb, int H, int W, int N, int vt, const float* cands, double sigma_i, double sigma_s,
                          float* const* d_dist, void* const* ws, hipStream_t s, float* const* fill, int isb_variant = -1,
-                         bool distance_filter = true);
+                         bool distance_filter = true, bool tables_ready = false);
 // the z-tested reprojection of nb maps, with or without the 1e8 fill of the
 // targets first
 int reproject_into(int nb, const float* const* d_in, float* const* d_out, int H, int W, const ss_intrinsics* intr,
