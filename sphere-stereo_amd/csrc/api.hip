@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <vector>
@@ -426,9 +427,11 @@ void isb_schedule(int H, int W, int N, int vt, int variant, int* G, int* V, int*
 // variant < 0: the configuration's tuned variant
 int launch_sweep_refs(const float* const* d_raylut, const float* const* d_images, int C, const ss_camera* cams,
                       const float* const* rts, const int* refs, int R, const float* cands, int N, int H, int W,
-                      int vt, void* const* costs, uint32_t* const* guides, hipStream_t s, int variant = -1) {
+                      int vt, void* const* costs, uint32_t* const* guides, hipStream_t s, int variant = -1,
+                      const ss::SweepCast* cast = nullptr) {
   ss::SweepArgs a;
   memset(&a, 0, sizeof a);
+  if (cast) a.cast = *cast;
   for (int k = 0; k < C; ++k) {
     a.img[k] = d_images[k];
     a.cam[k] = ss::make_torch_cam(cams[k]);
@@ -992,7 +995,27 @@ int time_filter_variants(void* ws, int H, int W, int N, int vt, const float* can
 // to run and only the guides are computed.
 int sweep_references(const float* const* d_raylut, const float* const* d_images, int C, const ss_camera* cams,
                      const float* const* rts, const int* refs, int R, const float* cands, int N, int H, int W,
-                     int vt, void* const* ws, hipStream_t s) {
+                     int vt, void* const* ws, hipStream_t s, const ToU8Args* cast, int ncast, size_t cast_n,
+                     bool* cast_done) {
+  // the uint8 casts ride on the (last) sweep launch when they fit SweepCast
+  SweepCast sc;
+  memset(&sc, 0, sizeof sc);
+  bool fuse = cast && ncast > 0 && ncast <= SS_MAX_REFERENCES && cast_n % 4 == 0 && cast_n / 4 > 0 &&
+              cast_n / 4 * (size_t)ncast < ((size_t)1 << 31) && N > 1;
+  for (int i = 0; fuse && i < ncast; ++i)
+    fuse = (reinterpret_cast<uintptr_t>(cast[i].in) & 15) == 0 && (reinterpret_cast<uintptr_t>(cast[i].out) & 3) == 0;
+  if (fuse) {
+    for (int i = 0; i < ncast; ++i) {
+      sc.in[i] = cast[i].in;
+      sc.out[i] = cast[i].out;
+    }
+    sc.n = ncast;
+    sc.quads = (unsigned)(cast_n / 4);
+    // about 8 quads per lane, at most 1024 workgroups
+    const size_t total = (size_t)sc.quads * ncast;
+    sc.blocks = (int)std::min<size_t>(1024, (total + 256 * 8 - 1) / (256 * 8));
+  }
+  if (cast_done) *cast_done = fuse;
   void* costs[SS_MAX_REFERENCES];
   uint32_t* guides[SS_MAX_REFERENCES];
   for (int r = 0; r < R; ++r) estimate_level0(ws[r], H, W, N, vt, &guides[r], &costs[r]);
@@ -1007,7 +1030,7 @@ int sweep_references(const float* const* d_raylut, const float* const* d_images,
   for (int r0 = 0; r0 < R; r0 += per) {
     const int n = R - r0 < per ? R - r0 : per;
     const int rc = launch_sweep_refs(d_raylut + r0, d_images, C, cams, rts + r0, refs + r0, n, cands, N, H, W, vt,
-                                     costs + r0, guides + r0, s);
+                                     costs + r0, guides + r0, s, -1, fuse && r0 + n >= R ? &sc : nullptr);
     if (rc) return rc;
   }
   return SS_OK;

@@ -187,6 +187,7 @@ int enqueue(Frame* f, const float* const* match, const float* const* stitch, hip
     ss_intrinsics intr[ss::kMaxBatch];
     const float* trans[ss::kMaxBatch];
     ss::ToU8Args cast[ss::kMaxBatch];
+    bool cast_all = true;
     for (int fi = f0; fi < f0 + nf; ++fi) {
       const ss_frame_desc& d = f->d[fi];
       const float* rts[SS_MAX_REFERENCES];
@@ -203,8 +204,12 @@ int enqueue(Frame* f, const float* const* match, const float* const* stitch, hip
         trans[j] = d.trans[r];
         cast[j] = ss::ToU8Args{stitch[(size_t)fi * R + r], d.d_stitch_u8[r]};
       }
+      // the frame's uint8 stitch-image casts ride on its sweep launch
+      bool cast_f = false;
       FR_CALL(ss::sweep_references(d.d_raylut, match + (size_t)fi * C, C, d.cams, rts, d.refs, R, d.cands, d.N, d.H,
-                                   d.W, d.volume_type, d.d_workspace, ls));
+                                   d.W, d.volume_type, d.d_workspace, ls, cast + (fi - f0) * R, R,
+                                   (size_t)c.Hs * c.Ws * 3, &cast_f));
+      cast_all = cast_all && cast_f;
     }
     if (multi) FR_HIP(hipEventRecord(f->swept[k], ls));
     // the fused argmin pass also fills the reprojection targets with 1e8
@@ -215,7 +220,7 @@ int enqueue(Frame* f, const float* const* match, const float* const* stitch, hip
     if (!capturing && !ready) f->tables_done.push_back(ws[0]);
     FR_CALL(ss::reproject_into(n, cdist, reproj, c.H, c.W, intr, trans, false, ls));
     FR_CALL(ss::inpaint_maps(n, reproj, scratch, lut, c.H, c.W, c.inpaint_iterations, c.thr_empty, c.thr_src, ls));
-    FR_CALL(ss::to_u8_maps(cast, n, (size_t)c.Hs * c.Ws * 3, ls));
+    if (!cast_all) FR_CALL(ss::to_u8_maps(cast, n, (size_t)c.Hs * c.Ws * 3, ls));
     for (int fi = f0; fi < f0 + nf; ++fi) {
       const ss_frame_desc& d = f->d[fi];
       const float* rp[SS_MAX_REFERENCES];

@@ -93,6 +93,16 @@ struct SweepRef {
   uint32_t* guide;   // nullable: packed guide written from the same pass
   int rt0;
 };
+// Images converted to uint8 (the stitcher's cast, ToU8Args) by `blocks`
+// extra workgroups at the end of a sweep launch: they run in the slots the
+// sweep's last round of tiles leaves free.  Each image has `quads` groups of 4
+// values (16-byte aligned input, 4-byte aligned output); blocks == 0: none.
+struct SweepCast {
+  const float* in[SS_MAX_REFERENCES];
+  uint8_t* out[SS_MAX_REFERENCES];
+  int n, blocks;
+  unsigned quads;
+};
 struct SweepArgs {
   SweepRef ref[SS_MAX_REFERENCES];
   const float* img[SS_MAX_CAMERAS];
@@ -105,6 +115,7 @@ struct SweepArgs {
   float Wf, Hf, invW, invH, inv2W, inv2H, halfW, halfH;
   int pow2;      // both sizes are powers of two
   int fastproj;  // the parameters are within sweep_fast_bounds: project_t_fast applies
+  SweepCast cast;
 };
 
 struct SelectArgs {
@@ -322,9 +333,13 @@ int to_u8_maps(const ToU8Args* a, int nb, size_t n, hipStream_t s);
 int inpaint_maps(int nb, float* const* map, float* const* tmp, const uint8_t* const* lut, int H, int W,
                  int iterations, double thr_empty, double thr_src, hipStream_t s);
 // one sweep launch over R references (each with its own workspace)
+// cast (optional): ncast images of cast_n values converted to uint8 by the
+// last sweep launch's trailing workgroups when sizes and alignment allow
+// (*cast_done tells whether they were; otherwise the caller casts them)
 int sweep_references(const float* const* d_raylut, const float* const* d_images, int C, const ss_camera* cams,
                      const float* const* rts, const int* refs, int R, const float* cands, int N, int H, int W,
-                     int vt, void* const* ws, hipStream_t s);
+                     int vt, void* const* ws, hipStream_t s, const ToU8Args* cast = nullptr, int ncast = 0,
+                     size_t cast_n = 0, bool* cast_done = nullptr);
 
 // stitch.hip
 hipError_t launch_fill(float* p, float v, size_t n, hipStream_t s);

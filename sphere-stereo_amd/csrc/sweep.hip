@@ -296,10 +296,31 @@ __device__ __forceinline__ void sweep_row(const P& a, const float4* __restrict__
   }
 }
 
+// The trailing cast workgroups of a sweep launch: block b of `blocks` takes
+// every blocks-th group of 256 quads, each lane one quad (a 16-byte load and
+// a 4-byte store; the truncating conversion of the stitcher's k_to_u8).
+__device__ __forceinline__ void sweep_cast(const SweepCast& c, int b) {
+  const unsigned total = c.quads * (unsigned)c.n;
+  for (unsigned q = (unsigned)b * kBlock + threadIdx.x; q < total; q += (unsigned)c.blocks * kBlock) {
+    const int img = (int)(q / c.quads);
+    const size_t i = (size_t)(q - (unsigned)img * c.quads) * 4;
+    const float4 v = *reinterpret_cast<const float4*>(c.in[img] + i);
+    const uint32_t w = (uint32_t)(uint8_t)v.x | ((uint32_t)(uint8_t)v.y << 8) | ((uint32_t)(uint8_t)v.z << 16) |
+                       ((uint32_t)(uint8_t)v.w << 24);
+    *reinterpret_cast<uint32_t*>(c.out[img] + i) = w;
+  }
+}
+
 template <typename T, bool POW2, int CH, int GRP>
 __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
   __shared__ float stage[kTileH][kTileW * (GRP + 1)];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  // the sweep's tiles first (XCD-aware order over them), then the cast blocks
+  const int nsweep = (int)gridDim.x - a.cast.blocks;
+  if ((int)blockIdx.x >= nsweep) {
+    sweep_cast(a.cast, (int)blockIdx.x - nsweep);
+    return;
+  }
+  const int tile = xcd_tile(blockIdx.x, nsweep);
   const int r = tile / a.blocks_per_ref;
   const int t = tile - r * a.blocks_per_ref;
   const int ty = t / a.tiles_x, tx = t - ty * a.tiles_x;
@@ -360,7 +381,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep_ragged(RaggedSpan<SweepPair> s
 
 template <typename T, int CH, int GRP>
 void launch_sweep_v(const SweepArgs& a, hipStream_t s) {
-  const dim3 grid(a.blocks_per_ref * a.R);
+  const dim3 grid(a.blocks_per_ref * a.R + a.cast.blocks);
   if (a.pow2)
     hipLaunchKernelGGL((k_sweep<T, true, CH, GRP>), grid, dim3(kBlock), 0, s, a);
   else
