@@ -225,9 +225,13 @@ namespace {
 // 0; levels >= 1 are fp32.
 // wtab: the filter's guide weight table (launch_weight_tables with
 // var_inv_i = 1 / (2 sigma_i^2), isb_filter.py:88), already enqueued on s.
+// rp_out / rp_intr / rp_t (optional, G = V = 1): the level-0 pass also
+// reprojects each pixel's final value into rp_out[b] (reproject_splat).
 int run_pyramid(Level* const* lv, int nb, int L, int G, int V, int vt, const float* wtab, double sigma_s,
                 bool fuse_last, const ss::ArgminParams& am, float* const* dist, hipStream_t s, int stage_down,
-                int stage_up, float* const* fill = nullptr, int xcd = 1, bool guide0_unused = false) {
+                int stage_up, float* const* fill = nullptr, int xcd = 1, bool guide0_unused = false,
+                float* const* rp_out = nullptr, const ss_intrinsics* rp_intr = nullptr,
+                const float* const* rp_t = nullptr) {
   if (nb < 1 || nb > ss::kMaxBatch) return fail(SS_ERR_ARG, "batch of %d pyramids outside [1, %d]", nb, ss::kMaxBatch);
   const Level* l0 = lv[0];  // the shapes (identical for every entry)
   const int NP = G * V;
@@ -270,9 +274,14 @@ int run_pyramid(Level* const* lv, int nb, int L, int G, int V, int vt, const flo
                             (float)wu,  (float)wd,  wtab,       fuse ? dist[b] : nullptr, am,
                             fuse && fill ? fill[b] : nullptr};
       ua[b].skip_guide = k == 0 && guide0_unused ? 1 : 0;
+      if (k == 0 && rp_out) {
+        ua[b].rp_out = rp_out[b];
+        ua[b].rp_c = rp_intr[b];
+        memcpy(ua[b].rp_t, rp_t[b], sizeof ua[b].rp_t);
+      }
     }
     StageTimer t(fuse ? SS_STAGE_ISB_UP_ARGMIN : stage_up, s);
-    SS_HIP(ss::launch_isb_up(ua, nb, G, V, k == 0 ? vt : SS_VOLUME_F32, fuse, xcd, s));
+    SS_HIP(ss::launch_isb_up(ua, nb, G, V, k == 0 ? vt : SS_VOLUME_F32, fuse, xcd, s, k == 0 && rp_out != nullptr));
   }
   if (fuse_last && L == 1) {
     for (int b = 0; b < nb; ++b) {
@@ -898,7 +907,9 @@ int estimate_levels(void* ws, int H, int W, int N, int vt, float* dist, PyrLevel
 
 int estimate_after_sweep(int nb, int H, int W, int N, int vt, const float* cands, double sigma_i, double sigma_s,
                          float* const* d_dist, void* const* ws, hipStream_t s, float* const* fill, int isb_variant,
-                         bool distance_filter, bool tables_ready) {
+                         bool distance_filter, bool tables_ready, const ss_intrinsics* rp_intr,
+                         const float* const* rp_t, bool* reprojected) {
+  if (reprojected) *reprojected = false;
   if (nb < 1 || nb > kMaxBatch) return fail(SS_ERR_ARG, "batch of %d references outside [1, %d]", nb, kMaxBatch);
   const int L = levels_of(H, W);
   Level lv[kMaxBatch][32], dl[kMaxBatch][32];
@@ -942,9 +953,14 @@ int estimate_after_sweep(int nb, int H, int W, int N, int vt, const float* cands
   }
   if (!distance_filter) return SS_OK;
   // the level-0 guide is not read after the distance filter (the reference
-  // filters a copy of it, depth_estimation.py:201): its last up pass skips it
-  return run_pyramid(dlp, nb, L, 1, 1, SS_VOLUME_F32, wtab + ss::kWeightTable, sigma_s / 2, false, ss::ArgminParams{},
-                     nullptr, s, SS_STAGE_DIST_DOWN, SS_STAGE_DIST_UP, nullptr, 1, true);
+  // filters a copy of it, depth_estimation.py:201): its last up pass skips it.
+  // With rp_intr, that pass also reprojects the final distances into fill.
+  const bool rp = rp_intr && rp_t && fill && L > 1;
+  const int rc = run_pyramid(dlp, nb, L, 1, 1, SS_VOLUME_F32, wtab + ss::kWeightTable, sigma_s / 2, false,
+                             ss::ArgminParams{}, nullptr, s, SS_STAGE_DIST_DOWN, SS_STAGE_DIST_UP, nullptr, 1, true,
+                             rp ? fill : nullptr, rp ? rp_intr : nullptr, rp ? rp_t : nullptr);
+  if (rc == SS_OK && reprojected) *reprojected = rp;
+  return rc;
 }
 
 // The cost filter of the estimate path (its fused argmin included) timed per

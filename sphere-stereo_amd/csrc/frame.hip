@@ -215,10 +215,12 @@ int enqueue(Frame* f, const float* const* match, const float* const* stitch, hip
     // the fused argmin pass also fills the reprojection targets with 1e8
     const bool ready =
         !capturing && std::find(f->tables_done.begin(), f->tables_done.end(), ws[0]) != f->tables_done.end();
+    // ... and the distance filter's last pass reprojects the final distances
+    bool reprojected = false;
     FR_CALL(ss::estimate_after_sweep(n, c.H, c.W, c.N, c.volume_type, c.cands, c.sigma_i, c.sigma_s, dist, ws, ls,
-                                     reproj, -1, true, ready));
+                                     reproj, -1, true, ready, intr, trans, &reprojected));
     if (!capturing && !ready) f->tables_done.push_back(ws[0]);
-    FR_CALL(ss::reproject_into(n, cdist, reproj, c.H, c.W, intr, trans, false, ls));
+    if (!reprojected) FR_CALL(ss::reproject_into(n, cdist, reproj, c.H, c.W, intr, trans, false, ls));
     FR_CALL(ss::inpaint_maps(n, reproj, scratch, lut, c.H, c.W, c.inpaint_iterations, c.thr_empty, c.thr_src, ls));
     if (!cast_all) FR_CALL(ss::to_u8_maps(cast, n, (size_t)c.Hs * c.Ws * 3, ls));
     for (int fi = f0; fi < f0 + nf; ++fi) {

@@ -252,8 +252,9 @@ constexpr bool lds_argmin() {
 // ---------------------------------------------------------------------------
 // One coarse pixel of the up pass (its four fine pixels), handled by the G
 // lanes of a group; inactive groups as in down_pixel.
-template <int G, int V, typename T, bool FUSE_ARGMIN>
+template <int G, int V, typename T, bool FUSE_ARGMIN, bool REPROJ = false>
 __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool active, int lane) {
+  static_assert(!REPROJ || (G == 1 && V == 1 && !FUSE_ARGMIN), "reprojection: the one-lane distance filter");
   constexpr int NP = G * V;
   constexpr int CH = chunk_of<V, T>();
   const float* cin = static_cast<const float*>(a.cin);  // coarse level: fp32
@@ -346,7 +347,10 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
 #pragma unroll
         for (int v = 0; v < CH; ++v) res[s][cb + v] = o[v];
       } else {
-        if (active && doit[s]) store_n<CH>(cio + (size_t)fine[s] * NP + base_c + cb, o);
+        if (active && doit[s]) {
+          store_n<CH>(cio + (size_t)fine[s] * NP + base_c + cb, o);
+          if constexpr (REPROJ) reproject_splat(a.rp_c, a.rp_t, a.rp_out, a.hio, a.wio, fine[s], o[0]);
+        }
       }
     }
   }
@@ -487,7 +491,7 @@ constexpr int up_plain_waves() {
   return FUSE_ARGMIN && V <= 4 && lds_argmin<8, V, T>() ? 5 : 1;
 }
 
-template <int G, int V, typename T, bool FUSE_ARGMIN>
+template <int G, int V, typename T, bool FUSE_ARGMIN, bool REPROJ = false>
 __global__ void __launch_bounds__(kBlock, (up_plain_waves<V, T, FUSE_ARGMIN>())) k_isb_up(Batch<IsbUpArgs> b, int xcd) {
   const IsbUpArgs a = b.a[blockIdx.y];
   const int lane = (G == 1) ? 0 : (int)(threadIdx.x & (G - 1));
@@ -497,7 +501,7 @@ __global__ void __launch_bounds__(kBlock, (up_plain_waves<V, T, FUSE_ARGMIN>()))
   const bool active = pix < npix;
   if (G == 1 && !active) return;
   if (!active) pix = npix - 1;
-  up_pixel<G, V, T, FUSE_ARGMIN>(a, pix, active, lane);
+  up_pixel<G, V, T, FUSE_ARGMIN, REPROJ>(a, pix, active, lane);
 }
 
 // The coarse end of the pyramid in one workgroup per batch entry: the down passes into
@@ -720,8 +724,15 @@ hipError_t launch_tail_gv(const IsbTailArgs& t, int nb, hipStream_t s) {
 }
 
 template <int G, int V, typename T>
-hipError_t launch_up_gv(const IsbUpArgs* a, int nb, bool fuse, int xcd, hipStream_t s) {
+hipError_t launch_up_gv(const IsbUpArgs* a, int nb, bool fuse, int xcd, hipStream_t s, bool reproj = false) {
   const dim3 grid(blocks_for((long long)a[0].hin * a[0].win * G), nb);
+  if constexpr (G == 1 && V == 1 && std::is_same<T, float>::value) {
+    if (reproj) {
+      hipLaunchKernelGGL((k_isb_up<1, 1, float, false, true>), grid, dim3(kBlock), 0, s, batch_of(a, nb), xcd);
+      return hipGetLastError();
+    }
+  }
+  if (reproj) return hipErrorInvalidValue;
   if (fuse)
     hipLaunchKernelGGL((k_isb_up<G, V, T, true>), grid, dim3(kBlock), 0, s, batch_of(a, nb), xcd);
   else
@@ -835,11 +846,13 @@ hipError_t launch_isb_tail(const IsbTailArgs& t, int nb, int G, int V, hipStream
 #undef SS_CALL
 }
 
-hipError_t launch_isb_up(const IsbUpArgs* a, int nb, int G, int V, int vt, bool fuse_argmin, int xcd, hipStream_t s) {
+hipError_t launch_isb_up(const IsbUpArgs* a, int nb, int G, int V, int vt, bool fuse_argmin, int xcd, hipStream_t s,
+                         bool reproj) {
   if (nb < 1 || nb > kMaxBatch) return hipErrorInvalidValue;
   if (G == 1)
-    return (V == 1 && vt == SS_VOLUME_F32 && !fuse_argmin) ? launch_up_gv<1, 1, float>(a, nb, false, xcd, s)
+    return (V == 1 && vt == SS_VOLUME_F32 && !fuse_argmin) ? launch_up_gv<1, 1, float>(a, nb, false, xcd, s, reproj)
                                                            : hipErrorInvalidValue;
+  if (reproj) return hipErrorInvalidValue;
 #define SS_CALL(T) up_t<T>(a, nb, G, V, fuse_argmin, xcd, s)
   SS_DISPATCH_T(vt, SS_CALL)
 #undef SS_CALL

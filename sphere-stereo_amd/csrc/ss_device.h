@@ -243,6 +243,34 @@ __device__ __forceinline__ void project_c(const ss_intrinsics& c, float px, floa
   v = (c.fy * py) * inv + c.cy;
 }
 
+// __float2int_rn: round half to even, NaN -> 0, saturating
+__device__ __forceinline__ int cvt_rni(float v) {
+  if (v != v) return 0;
+  const float r = rintf(v);
+  if (r >= 2147483647.0f) return 2147483647;
+  if (r <= -2147483648.0f) return -2147483647 - 1;
+  return (int)r;
+}
+
+// reprojectDistanceKernel (stitcher.cu:181-204) for pixel i of an H x W map
+// holding distance d, as an exact atomic z-minimum into dout (H x W, filled
+// with 1e8 beforehand): distances are >= 0 (or NaN, which never wins), so the
+// uint order of the bit patterns is the float order.
+__device__ __forceinline__ void reproject_splat(const ss_intrinsics& c, const float* t, float* dout, int H, int W,
+                                                int i, float d) {
+  float px, py, pz, u, v;
+  bool valid = true;
+  unproject_c(c, (float)(i % W), (float)(i / W), px, py, pz);
+  px = px * d - t[0];
+  py = py * d - t[1];
+  pz = pz * d - t[2];
+  project_c<false>(c, px, py, pz, u, v, valid);
+  const float dist = sqrtf((px * px + py * py) + pz * pz);
+  const long long o = (long long)cvt_rni(v) * W + cvt_rni(u);
+  if (o >= 0 && o < (long long)H * W && dist < 1e8f)
+    atomicMin(reinterpret_cast<unsigned int*>(dout) + o, __float_as_uint(dist));
+}
+
 // ---------------------------------------------------------------------------
 // Launch helpers
 // ---------------------------------------------------------------------------

@@ -45,6 +45,11 @@ struct IsbUpArgs {
   ArgminParams am;
   float* fill;  // nullable: with the fused argmin, set to 1e8 at every level-0 pixel
   int skip_guide = 0;  // 1: the fine level's guide is not needed afterwards (not written)
+  // the one-lane level-0 pass of a distance filter may also reproject every
+  // fine pixel's final distance (reproject_splat) into rp_out (1e8-filled)
+  float* rp_out = nullptr;
+  ss_intrinsics rp_c = {};
+  float rp_t[3] = {0.0f, 0.0f, 0.0f};
 };
 
 // One launch runs the same pass for up to kMaxBatch references of equal
@@ -287,7 +292,9 @@ bool sweep_fast_bounds(const TorchCam* cams, int C, const float* rt, int npairs,
 // fp32, and so is everything with G = 1).  a[0..nb) are the batch entries.
 // xcd: XCD-aware tile order (1) or launch order (0); G: 1, 4 or 8 lanes per pixel
 hipError_t launch_isb_down(const IsbDownArgs* a, int nb, int G, int V, int vt, int xcd, hipStream_t s);
-hipError_t launch_isb_up(const IsbUpArgs* a, int nb, int G, int V, int vt, bool fuse_argmin, int xcd, hipStream_t s);
+// reproj: G = V = 1 only, the entries' rp_* fields set
+hipError_t launch_isb_up(const IsbUpArgs* a, int nb, int G, int V, int vt, bool fuse_argmin, int xcd, hipStream_t s,
+                         bool reproj = false);
 hipError_t launch_isb_tail(const IsbTailArgs& t, int nb, int G, int V, hipStream_t s);
 // n weight tables in one launch: tab[i][d] = exp(-d^2 vi[i]) for d < kWeightTable
 hipError_t launch_weight_tables(const float* vi, float* const* tab, int n, hipStream_t s);
@@ -320,9 +327,14 @@ int time_filter_variants(void* ws, int H, int W, int N, int vt, const float* can
                          float* d_dist, int repeats, hipStream_t s, float* ms_out, int* best_out);
 // tables_ready: entry 0's workspace already holds the guide weight tables of
 // sigma_i (the frame executor computes them on its first eager run)
+// rp_intr / rp_t (optional, with fill): the distance filter's last pass also
+// reprojects the final distances into fill[b] (reproject_into's work);
+// *reprojected tells whether it did (it cannot without an up pass, L == 1)
 int estimate_after_sweep(int nb, int H, int W, int N, int vt, const float* cands, double sigma_i, double sigma_s,
                          float* const* d_dist, void* const* ws, hipStream_t s, float* const* fill, int isb_variant = -1,
-                         bool distance_filter = true, bool tables_ready = false);
+                         bool distance_filter = true, bool tables_ready = false,
+                         const ss_intrinsics* rp_intr = nullptr, const float* const* rp_t = nullptr,
+                         bool* reprojected = nullptr);
 // the z-tested reprojection of nb maps, with or without the 1e8 fill of the
 // targets first
 int reproject_into(int nb, const float* const* d_in, float* const* d_out, int H, int W, const ss_intrinsics* intr,

@@ -21,15 +21,6 @@ namespace {
 constexpr int kBlock = 256;
 inline int blocks_for(long long threads) { return (int)((threads + kBlock - 1) / kBlock); }
 
-// __float2int_rn: round half to even, NaN -> 0, saturating
-__device__ __forceinline__ int cvt_rni(float v) {
-  if (v != v) return 0;
-  const float r = rintf(v);
-  if (r >= 2147483647.0f) return 2147483647;
-  if (r <= -2147483648.0f) return -2147483647 - 1;
-  return (int)r;
-}
-
 // bilinear read with floor and no bounds check (stitcher.cu:89-116)
 __device__ __forceinline__ float interp_f(const float* __restrict__ m, int cols, float u, float v) {
   const int u1 = (int)floorf(u), v1 = (int)floorf(v), u2 = u1 + 1, v2 = v1 + 1;
@@ -46,25 +37,10 @@ __global__ void __launch_bounds__(kBlock) k_fill(float* __restrict__ p, float v,
   if (i < n) p[i] = v;
 }
 
-// reprojectDistanceKernel (stitcher.cu:181-204) as an exact atomic z-minimum;
-// map blockIdx.y of the batch
+// reprojectDistanceKernel (stitcher.cu:181-204) as an exact atomic z-minimum
+// (reproject_splat); map blockIdx.y of the batch
 __device__ __forceinline__ void reproject_pixel(const ReprojectArgs& a, int H, int W, int i) {
-  const ss_intrinsics c = a.c;
-  float* const dout = a.dout;
-  float px, py, pz, u, v;
-  bool valid = true;
-  unproject_c(c, (float)(i % W), (float)(i / W), px, py, pz);
-  const float d = a.din[i];
-  px = px * d - a.t[0];
-  py = py * d - a.t[1];
-  pz = pz * d - a.t[2];
-  project_c<false>(c, px, py, pz, u, v, valid);
-  const float dist = sqrtf((px * px + py * py) + pz * pz);
-  const long long o = (long long)cvt_rni(v) * W + cvt_rni(u);
-  // distances are >= 0 (or NaN, which never wins), so the uint order of the
-  // bit patterns is the float order; 1e8 is the fill value
-  if (o >= 0 && o < (long long)H * W && dist < 1e8f)
-    atomicMin(reinterpret_cast<unsigned int*>(dout) + o, __float_as_uint(dist));
+  reproject_splat(a.c, a.t, a.dout, H, W, i, a.din[i]);
 }
 
 __global__ void __launch_bounds__(kBlock) k_reproject(Batch<ReprojectArgs> b, int H, int W) {
