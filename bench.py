@@ -51,7 +51,10 @@ def algorithmic_bytes(stage, wl, C=4):
     L = levels(H, W)
     P = [math.ceil(H / 2 ** s) * math.ceil(W / 2 ** s) for s in range(L)]
     if stage == "sweep":
-        return R * (N * P[0] * e + P[0] * (12 * C + 1))
+        # the volume, images and guide, plus the stitch images' uint8 cast,
+        # which runs as trailing workgroups of the frame's sweep launch
+        sc, sr = wl["stitch"]
+        return R * (N * P[0] * e + P[0] * (12 * C + 1)) + R * sc * sr * 3 * 5
     if stage == "isb_down":
         return R * sum((P[s - 1] + P[s]) * (g + N * e) for s in range(1, L))
     if stage == "isb_up":
@@ -61,7 +64,8 @@ def algorithmic_bytes(stage, wl, C=4):
     if stage == "dist_down":
         return R * sum((P[s - 1] + P[s]) * (g + 4) for s in range(1, L))
     if stage == "dist_up":
-        return R * sum((P[s + 1] + 2 * P[s]) * (g + 4) for s in range(0, L - 1))
+        # its level-0 pass also scatters the reprojection (4 B per pixel)
+        return R * sum((P[s + 1] + 2 * P[s]) * (g + 4) for s in range(0, L - 1)) + R * 4 * P[0]
     if stage == "reproject":
         return R * (12 * P[0])  # read the map, scatter, plus the fill
     if stage == "inpaint":
