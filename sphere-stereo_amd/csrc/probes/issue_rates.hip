@@ -520,6 +520,26 @@ __global__ void __launch_bounds__(kBlock) k_lds(float* out, int seed) {
   out[blockIdx.x * kBlock + threadIdx.x] = (float)s[threadIdx.x];
 }
 
+// HBM streams: read, write and copy of n float4 per buffer, grid-stride
+// with 16-byte accesses (the read kernel folds what it reads into one value
+// per thread, so the loads stay).
+__global__ void __launch_bounds__(kBlock) k_stream_read(const float4* __restrict__ in, size_t n, float* out) {
+  float acc = 0.0f;
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+    const float4 v = in[i];
+    acc += (v.x + v.y) + (v.z + v.w);
+  }
+  out[(size_t)blockIdx.x * kBlock + threadIdx.x] = acc;
+}
+__global__ void __launch_bounds__(kBlock) k_stream_write(float4* __restrict__ o, size_t n, float v) {
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+    o[i] = make_float4(v, v, v, v);
+}
+__global__ void __launch_bounds__(kBlock) k_stream_copy(const float4* __restrict__ in, float4* __restrict__ o,
+                                                        size_t n) {
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) o[i] = in[i];
+}
+
 typedef void (*ValuFn)(float*, float);
 
 int time_kernel(ValuFn k, int blocks, float* out, float* best_ms) {
@@ -688,6 +708,51 @@ int main() {
     const double cu_cycles = cus * kClockHz * ms * 1e-3;
     printf("| %s | %.2f | %.2f |\n", r.name, cu_cycles / (lane_ops / 64), lane_ops / cu_cycles);
     fflush(stdout);
+  }
+  {
+    // 2 GiB per buffer: far beyond the 256 MB Infinity Cache
+    const size_t n = ((size_t)2 << 30) / sizeof(float4);
+    float4 *a = nullptr, *b = nullptr;
+    float* acc = nullptr;
+    const int sblocks = cus * 16;
+    CHECK(hipMalloc(&a, n * sizeof(float4)));
+    CHECK(hipMalloc(&b, n * sizeof(float4)));
+    CHECK(hipMalloc(&acc, (size_t)sblocks * kBlock * sizeof(float)));
+    hipLaunchKernelGGL(k_stream_write, dim3(sblocks), dim3(kBlock), 0, 0, a, n, 1.0f);
+    CHECK(hipGetLastError());
+    CHECK(hipDeviceSynchronize());
+    hipEvent_t e0, e1;
+    CHECK(hipEventCreate(&e0));
+    CHECK(hipEventCreate(&e1));
+    printf("\n| HBM stream (2 GiB buffers, 16-byte accesses) | GB/s | fraction of 8 TB/s |\n|---|---|---|\n");
+    for (int which = 0; which < 3; ++which) {
+      float best = 1e30f;
+      for (int rep = 0; rep < 5; ++rep) {
+        CHECK(hipEventRecord(e0, 0));
+        if (which == 0)
+          hipLaunchKernelGGL(k_stream_read, dim3(sblocks), dim3(kBlock), 0, 0, a, n, acc);
+        else if (which == 1)
+          hipLaunchKernelGGL(k_stream_write, dim3(sblocks), dim3(kBlock), 0, 0, b, n, (float)rep);
+        else
+          hipLaunchKernelGGL(k_stream_copy, dim3(sblocks), dim3(kBlock), 0, 0, a, b, n);
+        CHECK(hipGetLastError());
+        CHECK(hipEventRecord(e1, 0));
+        CHECK(hipEventSynchronize(e1));
+        float ms;
+        CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) best = ms;
+      }
+      const double bytes = (double)n * sizeof(float4) * (which == 2 ? 2 : 1);
+      const double gbs = bytes / (best * 1e-3) * 1e-9;
+      printf("| %s | %.0f | %.2f |\n", which == 0 ? "read" : (which == 1 ? "write" : "copy (read + write)"), gbs,
+             gbs / 8000.0);
+      fflush(stdout);
+    }
+    CHECK(hipEventDestroy(e0));
+    CHECK(hipEventDestroy(e1));
+    CHECK(hipFree(a));
+    CHECK(hipFree(b));
+    CHECK(hipFree(acc));
   }
   CHECK(hipFree(out));
   printf("status ok\n");
