@@ -107,7 +107,8 @@ __device__ __forceinline__ void down_pixel(const IsbDownArgs& a, int pix, bool a
   }
 
   // guide: each channel's weighted sum in the reference's tap order
-  const float inv = 1.0f / ws;
+  // 1 <= ws <= 1 + 8 * 0.5 * 1.01: rcp_rn is exactly 1.0f / ws
+  const float inv = rcp_rn(ws);
   if constexpr (G == 1) {
     if (active) {
       float s0 = gch(gc, 0), s1 = gch(gc, 1), s2 = gch(gc, 2);
@@ -311,7 +312,10 @@ __device__ __forceinline__ void up_pixel(const IsbUpArgs& a, int pix, bool activ
     float ws = a.weight_up;
 #pragma unroll
     for (int k = kStencilBegin[s]; k < kStencilBegin[s + 1]; ++k) ws = ws + w[k];
-    inv[s] = 1.0f / ws;
+    // ws = weight_up + weight_down * (sum of base * (table + 0.01)) with
+    // weight_up + weight_down = 1 and at least 4 base weights >= 1, so
+    // 0.04 <= ws <= 1 + 16 * 1.01: rcp_rn is exactly 1.0f / ws
+    inv[s] = rcp_rn(ws);
   }
 
   const size_t base_c = (size_t)lane * V;

@@ -5,10 +5,13 @@
 // mixes with.  ALU-only inline asm (no memory operands in any asm block);
 // every kernel writes one value per thread into a buffer sized for its grid.
 // Built by _build.py into sphere_stereo/issue_rates; run by
-// `python bench.py --issue-rates`.
+// `python bench.py --issue-rates`.  With `--check` it only runs the
+// exhaustive equality checks of the exact fast forms that the kernels use
+// (exit status 1 on any mismatch).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <string.h>
 
 #define CHECK(x)                                                             \
   do {                                                                       \
@@ -478,6 +481,28 @@ __global__ void __launch_bounds__(kBlock) k_sqrt(float* out, float seed) {
   for (int i = 0; i < 8; ++i) s += x[i];
   out[blockIdx.x * kBlock + threadIdx.x] = s;
 }
+// The refined reciprocal fma(fma(-d, r0, 1), r0, r0), r0 = v_rcp_f32(d),
+// against the correctly rounded 1.0f / d (this file is compiled with
+// correctly rounded division) on every float in [lo, hi) and its negation;
+// counts the inputs where they differ.
+__global__ void __launch_bounds__(kBlock) k_rcp_compare(unsigned long long* bad, uint32_t lo, uint32_t hi,
+                                                        uint32_t per) {
+  const uint64_t b0 = (uint64_t)lo + ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * per;
+  unsigned long long n = 0;
+  for (uint32_t i = 0; i < per; ++i) {
+    const uint64_t b = b0 + i;
+    if (b >= hi) break;
+#pragma unroll
+    for (int sgn = 0; sgn < 2; ++sgn) {
+      const float d = __uint_as_float((uint32_t)b | (sgn ? 0x80000000u : 0u));
+      const float r0 = __builtin_amdgcn_rcpf(d);
+      const float r = __builtin_fmaf(__builtin_fmaf(-d, r0, 1.0f), r0, r0);
+      n += __float_as_uint(r) != __float_as_uint(1.0f / d);
+    }
+  }
+  if (n) atomicAdd(bad, n);
+}
+
 // Both forms on every float in [lo, hi): each thread a run of `per` bit
 // patterns; counts the inputs where they differ.
 __global__ void __launch_bounds__(kBlock) k_sqrt_compare(unsigned long long* bad, uint32_t lo, uint32_t hi,
@@ -591,7 +616,41 @@ int time_lds(int blocks, float* out, float* best_ms) {
 
 }  // namespace
 
-int main() {
+// Exhaustive equality checks of the exact fast forms the kernels rely on;
+// returns the number of mismatching inputs, or -1 on a HIP error.
+long long run_checks() {
+  unsigned long long* bad = nullptr;
+  if (hipMalloc(&bad, sizeof *bad) != hipSuccess) return -1;
+  long long total = 0;
+  const uint32_t per = 4096;
+  struct C { const char* what; uint32_t lo, hi; int rcp; };
+  const C cs[] = {{"refined reciprocal vs 1.0f / d, normal |d| < 2^126, both signs", 0x00800000u, 0x7E800000u, 1},
+                  {"square root, select form vs integer form, [2^-96, inf)", 0x0F800000u, 0x7F800000u, 0}};
+  for (const C& c : cs) {
+    if (hipMemset(bad, 0, sizeof *bad) != hipSuccess) return -1;
+    const uint64_t threads = ((uint64_t)(c.hi - c.lo) + per - 1) / per;
+    const dim3 grid((unsigned)((threads + kBlock - 1) / kBlock));
+    if (c.rcp)
+      hipLaunchKernelGGL(k_rcp_compare, grid, dim3(kBlock), 0, 0, bad, c.lo, c.hi, per);
+    else
+      hipLaunchKernelGGL(k_sqrt_compare, grid, dim3(kBlock), 0, 0, bad, c.lo, c.hi, per);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return -1;
+    unsigned long long nbad = 0;
+    if (hipMemcpy(&nbad, bad, sizeof nbad, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    printf("%s: %llu of %llu inputs differ\n", c.what, nbad,
+           (unsigned long long)(c.hi - c.lo) * (c.rcp ? 2ull : 1ull));
+    total += (long long)nbad;
+  }
+  hipFree(bad);
+  return total;
+}
+
+int main(int argc, char** argv) {
+  if (argc > 1 && strcmp(argv[1], "--check") == 0) {
+    const long long n = run_checks();
+    printf("status %s\n", n == 0 ? "ok" : "mismatch");
+    return n == 0 ? 0 : 1;
+  }
   hipDeviceProp_t p;
   CHECK(hipGetDeviceProperties(&p, 0));
   const int cus = p.multiProcessorCount;
