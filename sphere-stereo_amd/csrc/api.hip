@@ -218,29 +218,6 @@ bool sweep_fast_bounds(const TorchCam* cams, int C, const float* rt, int npairs,
 
 namespace {
 
-// The one-lane down span's plan for down passes 1 .. last_down: the span's
-// input level (the first one from which levels up to last_down, at least
-// two of them and at most kSpanMaxLevels, fit the span's LDS budget with
-// 4 x 4 tiles at the last level), the passes before it being regular
-// launches; -1 when no span applies.
-int down_span_plan(const Level* l0, int last_down, int* tile) {
-  const int T = 4;
-  for (int sp = 0; last_down - sp >= 2; ++sp) {
-    const int n = last_down - sp;
-    if (n > ss::kSpanMaxLevels) continue;
-    int h[ss::kSpanMaxLevels + 1], w[ss::kSpanMaxLevels + 1];
-    for (int i = 0; i <= n; ++i) {
-      h[i] = l0[sp + i].h;
-      w[i] = l0[sp + i].w;
-    }
-    if (ss::down_span_lds(h, w, n, T) <= ss::kSpanLdsBytes) {
-      *tile = T;
-      return sp;
-    }
-  }
-  return -1;
-}
-
 // ISB_Filter.apply's two launch loops (isb_filter.py:87-134) for nb pyramids
 // of the same shape in the same launches (lv[b] are the levels of entry b).
 // With fuse_last, the level-0 up-pass computes the argmin into dist[b]
@@ -268,35 +245,13 @@ int run_pyramid(Level* const* lv, int nb, int L, int G, int V, int vt, const flo
   // of Infinity Cache, less than two fp32 1024^2 x 32 volumes) are read
   // before they are evicted; the fused level-0 up pass then takes them first
   // to last, again most recently read first.
-  // the one-lane filter's down passes from level sp on run as one span launch
-  int span_tile = 0;
-  const int sp = G == 1 && V == 1 && vt == SS_VOLUME_F32 ? down_span_plan(l0, last_down, &span_tile) : -1;
-  for (int k = 1; k <= (sp >= 0 ? sp : last_down); ++k) {
+  for (int k = 1; k <= last_down; ++k) {
     for (int b = 0; b < nb; ++b) {
       const Level* v = lv[k == 1 ? nb - 1 - b : b];
       da[b] = ss::IsbDownArgs{v[k - 1].g, v[k - 1].c, v[k - 1].h, v[k - 1].w, v[k].g, v[k].c, v[k].h, v[k].w, wtab};
     }
     StageTimer t(stage_down, s);
     SS_HIP(ss::launch_isb_down(da, nb, G, V, k == 1 ? vt : SS_VOLUME_F32, xcd, s));
-  }
-  if (sp >= 0) {
-    ss::DownSpanArgs sa;
-    memset(&sa, 0, sizeof sa);
-    sa.n = last_down - sp;
-    sa.tile = span_tile;
-    for (int i = 0; i <= sa.n; ++i) {
-      sa.h[i] = l0[sp + i].h;
-      sa.w[i] = l0[sp + i].w;
-    }
-    sa.tiles_x = (sa.w[sa.n] + span_tile - 1) / span_tile;
-    sa.wtab = wtab;
-    for (int b = 0; b < nb; ++b)
-      for (int i = 0; i <= sa.n; ++i) {
-        sa.e[b].g[i] = lv[b][sp + i].g;
-        sa.e[b].c[i] = static_cast<float*>(lv[b][sp + i].c);
-      }
-    StageTimer t(stage_down, s);
-    SS_HIP(ss::launch_isb_down_span(sa, nb, s));
   }
   if (tail) {
     ss::IsbTailArgs ta;

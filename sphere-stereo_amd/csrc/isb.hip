@@ -38,33 +38,28 @@ constexpr int kBlock = 256;
 // that the shuffles see every lane, and store nothing.  T is the storage
 // type of the input level (level 0 may be 16-bit); the output level is fp32,
 // as are all sums and weights.
-// Coarse pixel (x, y) of the output level.  The input level's pixel (col,
-// row) is element (row - iy0) * istride + (col - ix0) of a.gin / a.cin (the
-// whole level: origin (0, 0), stride a.win; or a window of it in LDS), and
-// the results go to element o of a.gout / a.cout.  The clamps at the level's
-// edges use its full size (a.hin, a.win) either way.
 template <int G, int V, typename T>
-__device__ __forceinline__ void down_pixel_at(const IsbDownArgs& a, int x, int y, int ix0, int iy0, int istride, int o,
-                                              bool active, int lane) {
+__device__ __forceinline__ void down_pixel(const IsbDownArgs& a, int pix, bool active, int lane) {
   constexpr int NP = G * V;
   constexpr int CH = chunk_of<V, T>();
   const T* cin = static_cast<const T*>(a.cin);
   float* cout = static_cast<float*>(a.cout);
 
+  const int x = pix % a.wout, y = pix / a.wout;
   const int x2 = 2 * x, y2 = 2 * y;
   const int xp = min(x2 + 1, a.win - 1), yp = min(y2 + 1, a.hin - 1);
   const int xm = max(x2 - 1, 0), ym = max(y2 - 1, 0);
   // neighbour order of isb_filter.cu:68-77
-  auto at = [&](int row, int col) __attribute__((always_inline)) { return (row - iy0) * istride + (col - ix0); };
-  const int nb[8] = {at(ym, xm), at(y2, xm), at(yp, xm), at(yp, x2), at(yp, xp), at(y2, xp), at(ym, xp), at(ym, x2)};
-  const int ctr = at(y2, x2);
+  const int nb[8] = {ym * a.win + xm, y2 * a.win + xm, yp * a.win + xm, yp * a.win + x2,
+                     yp * a.win + xp, y2 * a.win + xp, ym * a.win + xp, ym * a.win + x2};
+  const int ctr = y2 * a.win + x2;
   const uint32_t gc = a.gin[ctr];
 
   // neighbour j's guide and weight factor (isb_filter.cu:80-93)
   auto nguide = [&](int j) __attribute__((always_inline)) -> uint32_t {
     const int row = pick3((j == 0 || j == 6 || j == 7) ? 0 : ((j == 1 || j == 5) ? 1 : 2), ym, y2, yp);
     const int col = pick3((j <= 2) ? 0 : ((j == 3 || j == 7) ? 1 : 2), xm, x2, xp);
-    return a.gin[at(row, col)];
+    return a.gin[row * a.win + col];
   };
   auto weight = [&](int j, uint32_t gn) __attribute__((always_inline)) -> float {
     const float base = (j & 1) ? 0.5f : 0.25f;
@@ -108,7 +103,7 @@ __device__ __forceinline__ void down_pixel_at(const IsbDownArgs& a, int x, int y
     }
 #pragma unroll
     for (int v = 0; v < CH; ++v) acc[v] = acc[v] / ws;
-    if (active) store_n<CH>(cout + (size_t)o * NP + base_c + c0, acc);
+    if (active) store_n<CH>(cout + (size_t)pix * NP + base_c + c0, acc);
   }
 
   // guide: each channel's weighted sum in the reference's tap order
@@ -124,7 +119,7 @@ __device__ __forceinline__ void down_pixel_at(const IsbDownArgs& a, int x, int y
         s1 = s1 + gch(gn, 1) * w[j];
         s2 = s2 + gch(gn, 2) * w[j];
       }
-      a.gout[o] = pack_guide(s0 * inv, s1 * inv, s2 * inv);
+      a.gout[pix] = pack_guide(s0 * inv, s1 * inv, s2 * inv);
     }
   } else {
     // lane c of the group sums channel c (lanes 3..7 repeat channel 0); the
@@ -135,13 +130,8 @@ __device__ __forceinline__ void down_pixel_at(const IsbDownArgs& a, int x, int y
     for (int j = 0; j < 8; ++j) sc = sc + gch(__shfl(glv[j / G], j % G, G), ch) * w[j];
     const uint32_t byte = (uint32_t)(uint8_t)(sc * inv);
     const uint32_t b1 = __shfl(byte, 1, G), b2 = __shfl(byte, 2, G);
-    if (lane == 0 && active) a.gout[o] = byte | (b1 << 8) | (b2 << 16);
+    if (lane == 0 && active) a.gout[pix] = byte | (b1 << 8) | (b2 << 16);
   }
-}
-
-template <int G, int V, typename T>
-__device__ __forceinline__ void down_pixel(const IsbDownArgs& a, int pix, bool active, int lane) {
-  down_pixel_at<G, V, T>(a, pix % a.wout, pix / a.wout, 0, 0, a.win, pix, active, lane);
 }
 
 // xcd: the XCD-aware tile order (xcd_tile), else tiles in launch order
@@ -563,71 +553,6 @@ __global__ void __launch_bounds__(kTailBlock) k_isb_tail(IsbTailArgs t) {
   tail_body<G, V>(t.t, t.g0[blockIdx.x], t.c0[blockIdx.x], smem);
 }
 
-// The one-lane down span (DownSpanArgs): workgroup (t, b) computes, for
-// levels k = 1 .. n after the input level, the window of level k that tile t
-// of level n depends on: level 1 from the input level in global memory, each
-// later one from the previous window in LDS, the last one (the tile itself)
-// straight into global memory.  While level k + 1 is computed, the pixels of
-// level k's window that the tile owns (its footprint, 2^(n - k) tile sides)
-// go to global memory too; the footprints of the tiles partition each level,
-// so every pixel is written once, with the bits every window holds for it.
-__global__ void __launch_bounds__(kSpanBlock) k_isb_down_span(DownSpanArgs a) {
-  extern __shared__ __align__(16) char smem[];
-  const DownSpanEntry& e = a.e[blockIdx.y];
-  const int n = a.n, tile = a.tile;
-  unsigned goff[kSpanMaxLevels + 1], coff[kSpanMaxLevels + 1];
-  int mw[kSpanMaxLevels + 1], mh[kSpanMaxLevels + 1];
-  down_span_layout(a.h, a.w, n, tile, goff, coff, mw, mh);
-  const int tx = (int)blockIdx.x % a.tiles_x, ty = (int)blockIdx.x / a.tiles_x;
-  int x0[kSpanMaxLevels + 1], x1[kSpanMaxLevels + 1], y0[kSpanMaxLevels + 1], y1[kSpanMaxLevels + 1];
-  x0[n] = tx * tile;
-  x1[n] = min(x0[n] + tile, a.w[n]) - 1;
-  y0[n] = ty * tile;
-  y1[n] = min(y0[n] + tile, a.h[n]) - 1;
-  for (int k = n - 1; k >= 1; --k) {
-    x0[k] = max(2 * x0[k + 1] - 1, 0);
-    x1[k] = min(2 * x1[k + 1] + 1, a.w[k] - 1);
-    y0[k] = max(2 * y0[k + 1] - 1, 0);
-    y1[k] = min(2 * y1[k + 1] + 1, a.h[k] - 1);
-  }
-  for (int k = 1; k <= n; ++k) {
-    const bool first = k == 1, last = k == n;
-    IsbDownArgs d;
-    d.gin = first ? e.g[0] : reinterpret_cast<const uint32_t*>(smem + goff[k - 1]);
-    d.cin = first ? static_cast<const void*>(e.c[0]) : static_cast<const void*>(smem + coff[k - 1]);
-    d.hin = a.h[k - 1];
-    d.win = a.w[k - 1];
-    d.gout = last ? e.g[n] : reinterpret_cast<uint32_t*>(smem + goff[k]);
-    d.cout = last ? static_cast<void*>(e.c[n]) : static_cast<void*>(smem + coff[k]);
-    d.hout = a.h[k];
-    d.wout = a.w[k];
-    d.wtab = a.wtab;
-    const int ix0 = first ? 0 : x0[k - 1], iy0 = first ? 0 : y0[k - 1];
-    const int istride = first ? a.w[0] : x1[k - 1] - x0[k - 1] + 1;
-    const int ww = x1[k] - x0[k] + 1, wh = y1[k] - y0[k] + 1;
-    for (int i = (int)threadIdx.x; i < ww * wh; i += kSpanBlock) {
-      const int x = x0[k] + i % ww, y = y0[k] + i / ww;
-      down_pixel_at<1, 1, float>(d, x, y, ix0, iy0, istride, last ? y * a.w[n] + x : i, true, 0);
-    }
-    if (!first) {
-      // the tile's footprint in level k - 1's window, to global memory
-      const int sc = tile << (n - k + 1);
-      const int cx0 = tx * sc, cx1 = min(cx0 + sc, a.w[k - 1]) - 1;
-      const int cy0 = ty * sc, cy1 = min(cy0 + sc, a.h[k - 1]) - 1;
-      const int cw = cx1 - cx0 + 1, ch = cy1 - cy0 + 1;
-      const uint32_t* wg = reinterpret_cast<const uint32_t*>(smem + goff[k - 1]);
-      const float* wc = reinterpret_cast<const float*>(smem + coff[k - 1]);
-      for (int i = (int)threadIdx.x; i < cw * ch; i += kSpanBlock) {
-        const int x = cx0 + i % cw, y = cy0 + i / cw;
-        const int wi = (y - iy0) * istride + (x - ix0);
-        e.g[k - 1][y * a.w[k - 1] + x] = wg[wi];
-        e.c[k - 1][y * a.w[k - 1] + x] = wc[wi];
-      }
-    }
-    __syncthreads();  // level k's window is complete before level k + 1 reads it
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Ragged forms: a wavefront's item is kIsbItemPix8 (G = 8) or kIsbItemPix1
 // (G = 1) consecutive coarse pixels of one entry, one step; the entry's V
@@ -923,21 +848,6 @@ hipError_t launch_isb_tail(const IsbTailArgs& t, int nb, int G, int V, hipStream
 #define SS_CALL(v) launch_tail_gv<8, v>(t, nb, s)
   SS_DISPATCH_V(V, SS_CALL)
 #undef SS_CALL
-}
-
-unsigned down_span_lds(const int* h, const int* w, int n, int tile) {
-  unsigned goff[kSpanMaxLevels + 1], coff[kSpanMaxLevels + 1];
-  int mw[kSpanMaxLevels + 1], mh[kSpanMaxLevels + 1];
-  return down_span_layout(h, w, n, tile, goff, coff, mw, mh);
-}
-
-hipError_t launch_isb_down_span(const DownSpanArgs& a, int nb, hipStream_t s) {
-  if (nb < 1 || nb > kMaxBatch || a.n < 2 || a.n > kSpanMaxLevels || a.tile < 1) return hipErrorInvalidValue;
-  const unsigned lds = down_span_lds(a.h, a.w, a.n, a.tile);
-  if (lds > kSpanLdsBytes) return hipErrorInvalidValue;
-  const int tiles_y = (a.h[a.n] + a.tile - 1) / a.tile;
-  hipLaunchKernelGGL(k_isb_down_span, dim3(a.tiles_x * tiles_y, nb), dim3(kSpanBlock), lds, s, a);
-  return hipGetLastError();
 }
 
 hipError_t launch_isb_up(const IsbUpArgs* a, int nb, int G, int V, int vt, bool fuse_argmin, int xcd, hipStream_t s,

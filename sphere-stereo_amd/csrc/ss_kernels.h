@@ -66,50 +66,6 @@ struct Batch {
   A a[kMaxBatch];
 };
 
-// The one-lane down span (distance filter): levels s + 1 .. s + n of up to
-// kMaxBatch pyramids of equal shape from level s in one launch.  Workgroup
-// (t, b) owns tile t (tile x tile pixels) of level s + n of entry b; it
-// computes, level by level, the window of each coarser-than-s level that its
-// tile depends on (halo pixels are recomputed by neighbouring workgroups with
-// the same bits), keeps levels s + 1 .. s + n - 1 in LDS and writes the
-// pixels it owns (its tile's footprint) at every level to global memory.
-constexpr int kSpanMaxLevels = 6;
-constexpr int kSpanBlock = 256;
-constexpr unsigned kSpanLdsBytes = 48 * 1024;
-struct DownSpanEntry {
-  uint32_t* g[kSpanMaxLevels + 1];  // guides of levels s .. s + n
-  float* c[kSpanMaxLevels + 1];     // values (one per pixel)
-};
-struct DownSpanArgs {
-  DownSpanEntry e[kMaxBatch];
-  int n, tile, tiles_x;
-  int h[kSpanMaxLevels + 1], w[kSpanMaxLevels + 1];
-  const float* wtab;
-};
-// The LDS layout of a span's windows: level k (1 .. n - 1, relative to s)
-// gets the largest window a tile needs there, mw[k] x mh[k] pixels (2 x the
-// next level's window + 1, at most the level), guides at goff[k] and values
-// at coff[k].  Returns the bytes; host and kernel lay it out the same way.
-__host__ __device__ inline unsigned down_span_layout(const int* h, const int* w, int n, int tile, unsigned* goff,
-                                                     unsigned* coff, int* mw, int* mh) {
-  mw[n] = tile < w[n] ? tile : w[n];
-  mh[n] = tile < h[n] ? tile : h[n];
-  unsigned off = 0;
-  for (int k = n - 1; k >= 1; --k) {
-    mw[k] = 2 * mw[k + 1] + 1 < w[k] ? 2 * mw[k + 1] + 1 : w[k];
-    mh[k] = 2 * mh[k + 1] + 1 < h[k] ? 2 * mh[k + 1] + 1 : h[k];
-    const unsigned px = (unsigned)(mw[k] * mh[k]);
-    goff[k] = off;
-    off += (px * 4u + 15u) & ~15u;
-    coff[k] = off;
-    off += (px * 4u + 15u) & ~15u;
-  }
-  return off;
-}
-// the bytes of down_span_layout (host side)
-unsigned down_span_lds(const int* h, const int* w, int n, int tile);
-
-
 // Levels s0..s0+n-1 of a pyramid for the single-workgroup coarse tail:
 // entry 0 in global memory, entries 1..n-1 at byte offsets in LDS.  Workgroup
 // b runs the tail of batch entry b (its own level-s0 buffers g0[b], c0[b]).
@@ -336,7 +292,6 @@ bool sweep_fast_bounds(const TorchCam* cams, int C, const float* rt, int npairs,
 // fp32, and so is everything with G = 1).  a[0..nb) are the batch entries.
 // xcd: XCD-aware tile order (1) or launch order (0); G: 1, 4 or 8 lanes per pixel
 hipError_t launch_isb_down(const IsbDownArgs* a, int nb, int G, int V, int vt, int xcd, hipStream_t s);
-hipError_t launch_isb_down_span(const DownSpanArgs& a, int nb, hipStream_t s);
 // reproj: G = V = 1 only, the entries' rp_* fields set
 hipError_t launch_isb_up(const IsbUpArgs* a, int nb, int G, int V, int vt, bool fuse_argmin, int xcd, hipStream_t s,
                          bool reproj = false);
