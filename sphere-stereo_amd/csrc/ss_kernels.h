@@ -420,7 +420,8 @@ inline int isb_items(int npix, int G) {
   return (npix + per - 1) / per;
 }
 constexpr int kPixItem = 64;    // reprojection and merge: pixels per item (a lane each)
-constexpr int kCastItem = 256;  // uint8 cast: values per item (4 per lane)
+constexpr int kCastRounds = 8;                // uint8 cast: rounds of 4 values per lane in an item
+constexpr int kCastItem = 256 * kCastRounds;  // uint8 cast: values per item (a wavefront)
 inline int pix_items(long long n) { return (int)((n + kPixItem - 1) / kPixItem); }
 inline int cast_items(long long n) { return (int)((n + kCastItem - 1) / kCastItem); }
 int inpaint_tile();  // the side of an inpainting item (a tile of the map)

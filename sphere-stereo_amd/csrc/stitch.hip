@@ -550,8 +550,15 @@ __global__ void __launch_bounds__(kBlock) k_to_u8(Batch<ToU8Args> b, size_t n) {
   for (size_t k = i; k < i + 4 && k < n; ++k) out[k] = (uint8_t)in[k];
 }
 
-// ragged: a wavefront's item is kCastItem consecutive values of one image
-// (4 per lane); src is the run's stitch-image table
+// ragged: a wavefront's item is kCastItem consecutive values of one image,
+// kCastRounds rounds of 4 per lane (each round a contiguous 1 KB of the
+// image; in a whole item every round's load is issued before the stores,
+// so a wavefront keeps kCastRounds loads in flight); src is the run's
+// stitch-image table
+__device__ __forceinline__ uint32_t pack_u8x4(const float4& v) {
+  return (uint32_t)(uint8_t)v.x | ((uint32_t)(uint8_t)v.y << 8) | ((uint32_t)(uint8_t)v.z << 16) |
+         ((uint32_t)(uint8_t)v.w << 24);
+}
 template <bool LOOP>
 __global__ void __launch_bounds__(kBlock) k_to_u8_ragged(RaggedSpan<CastEntry> sp, const float* const* src) {
   const int lane = (int)(threadIdx.x & 63);
@@ -560,14 +567,23 @@ __global__ void __launch_bounds__(kBlock) k_to_u8_ragged(RaggedSpan<CastEntry> s
     const float* __restrict__ in = src[c.src];
     uint8_t* __restrict__ out = c.out;
     const bool vec = (reinterpret_cast<uintptr_t>(in) & 15) == 0 && (reinterpret_cast<uintptr_t>(out) & 3) == 0;
-    const long long i = (long long)t * kCastItem + 4 * lane;
-    if (i >= c.n) return;
-    if (vec && i + 4 <= c.n) {
-      const float4 v = *reinterpret_cast<const float4*>(in + i);
-      *reinterpret_cast<uint32_t*>(out + i) = (uint32_t)(uint8_t)v.x | ((uint32_t)(uint8_t)v.y << 8) |
-                                              ((uint32_t)(uint8_t)v.z << 16) | ((uint32_t)(uint8_t)v.w << 24);
-    } else {
-      for (long long k = i; k < i + 4 && k < c.n; ++k) out[k] = (uint8_t)in[k];
+    const long long base = (long long)t * kCastItem, i0 = base + 4 * lane;
+    if (vec && base + kCastItem <= c.n) {
+      float4 v[kCastRounds];
+#pragma unroll
+      for (int r = 0; r < kCastRounds; ++r) v[r] = *reinterpret_cast<const float4*>(in + i0 + 256 * r);
+#pragma unroll
+      for (int r = 0; r < kCastRounds; ++r) *reinterpret_cast<uint32_t*>(out + i0 + 256 * r) = pack_u8x4(v[r]);
+      return;
+    }
+    // the image's last item, or unaligned buffers
+    for (int r = 0; r < kCastRounds; ++r) {
+      const long long i = i0 + 256 * r;
+      if (i >= c.n) break;
+      if (vec && i + 4 <= c.n)
+        *reinterpret_cast<uint32_t*>(out + i) = pack_u8x4(*reinterpret_cast<const float4*>(in + i));
+      else
+        for (long long k = i; k < i + 4 && k < c.n; ++k) out[k] = (uint8_t)in[k];
     }
   });
 }
