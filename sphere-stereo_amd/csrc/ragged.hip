@@ -180,7 +180,13 @@ struct RaggedPlan {
   RaggedSpan<ReprojEntry> reproj{};
   std::vector<RaggedSpan<InpaintEntry>> inpaint;
   RaggedSpan<CastEntry> cast{};
-  RaggedSpan<MergeArgs> merge{};
+  // the merge: one launch per reference count present (rn 1-4; 0: more),
+  // each kernel with the register budget of its own count
+  struct MergeLaunch {
+    RaggedSpan<MergeArgs> sp;
+    int rn;
+  };
+  std::vector<MergeLaunch> merge;
   // creation only: the weight tables to build (vi, destination)
   std::vector<float> wt_vi;
   std::vector<float*> wt_dst;
@@ -308,7 +314,7 @@ int build(const ss_frame_desc* d, int F, int schedule, Image& im, RaggedPlan* pl
   Pass<SweepPair> sweep[4];  // by the configuration's tuned sweep variant (ss_sweep_get_variant)
   Pass<ReprojEntry> reproj;
   Pass<CastEntry> cast;
-  Pass<MergeArgs> merge;
+  Pass<MergeArgs> merge[5];  // by reference count: 1-4, and 0 for more
   int p = 0;
   for (int f = 0; f < F; ++f) {
     const ss_frame_desc& c = d[f];
@@ -396,7 +402,7 @@ int build(const ss_frame_desc* d, int F, int schedule, Image& im, RaggedPlan* pl
     m.PW = c.PW;
     m.dist_out = c.d_dist_pano;
     m.rgb_out = c.d_rgb_pano;
-    merge.add(m, pix_items((long long)c.PH * c.PW));
+    merge[c.R >= 1 && c.R <= 4 ? c.R : 0].add(m, pix_items((long long)c.PH * c.PW));
   }
   for (int v = 0; v < 4; ++v)
     if (!sweep[v].e.empty()) pl->sweep.push_back(RaggedPlan::SweepLaunch{place(im, sweep[v]), v});
@@ -431,7 +437,8 @@ int build(const ss_frame_desc* d, int F, int schedule, Image& im, RaggedPlan* pl
     if (!pass.e.empty()) pl->inpaint.push_back(place(im, pass));
   }
   pl->cast = place(im, cast);
-  pl->merge = place(im, merge);
+  for (int rn : {1, 2, 3, 4, 0})
+    if (!merge[rn].e.empty()) pl->merge.push_back(RaggedPlan::MergeLaunch{place(im, merge[rn]), rn});
   pl->bytes = im.off;
   return SS_OK;
 }
@@ -519,7 +526,7 @@ int ragged_enqueue(const RaggedPlan* pl, const float* const* match, const float*
   RG_HIP(launch_reproject_ragged(pl->reproj, ls, s));
   for (const auto& sp : pl->inpaint) RG_HIP(launch_inpaint_ragged(sp, ls, s));
   if (pl->cast.n) RG_HIP(launch_to_u8_ragged(pl->cast, pl->run_stitch, ls, s));
-  RG_HIP(launch_merge_ragged(pl->merge, ls, s));
+  for (const auto& l : pl->merge) RG_HIP(launch_merge_ragged(l.sp, l.rn, ls, s));
   return SS_OK;
 }
 

@@ -412,7 +412,8 @@ hipError_t launch_isb_tail_ragged(const TailEntry* t, int n, int G, unsigned lds
 hipError_t launch_reproject_ragged(const RaggedSpan<ReprojEntry>& sp, bool loop, hipStream_t s);
 hipError_t launch_inpaint_ragged(const RaggedSpan<InpaintEntry>& sp, bool loop, hipStream_t s);
 hipError_t launch_to_u8_ragged(const RaggedSpan<CastEntry>& sp, const float* const* src, bool loop, hipStream_t s);
-hipError_t launch_merge_ragged(const RaggedSpan<MergeArgs>& sp, bool loop, hipStream_t s);
+// rn: the reference count of every entry of sp (1-4), or 0 (any count, a loop)
+hipError_t launch_merge_ragged(const RaggedSpan<MergeArgs>& sp, int rn, bool loop, hipStream_t s);
 // items of one entry of each ragged pass
 inline int sweep_items(int H, int W) { return ((H + 3) / 4) * ((W + 63) / 64); }
 inline int isb_items(int npix, int G) {

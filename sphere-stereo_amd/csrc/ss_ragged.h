@@ -95,7 +95,9 @@ __device__ __forceinline__ void ragged_run(const RaggedSpan<E>& sp, int x, int j
 template <bool LOOP, typename E, typename F>
 __device__ __forceinline__ void for_each_wave_item(const RaggedSpan<E>& sp, F&& f) {
   const int wpb = (int)(blockDim.x / 64), x = (int)(blockIdx.x % kParts);
-  const int j = (int)(blockIdx.x / kParts) * wpb + (int)(threadIdx.x / 64);
+  // the wavefront's index is the same in all its lanes; saying so lets the
+  // compiler keep the entry and its record in SGPRs (scalar loads)
+  const int j = __builtin_amdgcn_readfirstlane((int)(blockIdx.x / kParts) * wpb + (int)(threadIdx.x / 64));
   if constexpr (LOOP) {
     const int blocks = ((int)gridDim.x - x + kParts - 1) / kParts;
     ragged_run(sp, x, j, blocks * wpb, f);

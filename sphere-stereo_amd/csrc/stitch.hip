@@ -514,20 +514,21 @@ __global__ void __launch_bounds__(kBlock) k_merge(MergeArgs a) {
 }
 
 // ragged: a wavefront's item is kPixItem consecutive panorama pixels of one
-// frame (a lane each), whose reference count picks the instantiation
-template <bool LOOP>
+// frame (a lane each); every frame of the launch has RN references (RN = 0:
+// any count, the generic loop), as k_merge<RN>
+template <bool LOOP, int RN>
 __global__ void __launch_bounds__(kBlock) k_merge_ragged(RaggedSpan<MergeArgs> sp) {
   const int lane = (int)(threadIdx.x & 63);
   for_each_wave_item<LOOP>(sp, [&](int e, int t) {
-    const MergeArgs& a = sp.e[e];
     const int i = t * kPixItem + lane;
-    if (i >= a.PH * a.PW) return;
-    switch (a.R) {
-      case 1: merge_pixel<1>(a, i); break;
-      case 2: merge_pixel<2>(a, i); break;
-      case 3: merge_pixel<3>(a, i); break;
-      case 4: merge_pixel<4>(a, i); break;
-      default: merge_pixel<0>(a, i); break;
+    if constexpr (RN > 0) {
+      // every index into the record is fixed: read it through the constant
+      // address space (scalar loads into SGPRs)
+      const MergeArgs a = load_const(sp.e + e);
+      if (i < a.PH * a.PW) merge_pixel<RN>(a, i);
+    } else {
+      const MergeArgs& a = sp.e[e];
+      if (i < a.PH * a.PW) merge_pixel<RN>(a, i);
     }
   });
 }
@@ -741,12 +742,24 @@ hipError_t launch_to_u8_ragged(const RaggedSpan<CastEntry>& sp, const float* con
   return hipGetLastError();
 }
 
-hipError_t launch_merge_ragged(const RaggedSpan<MergeArgs>& sp, bool loop, hipStream_t s) {
-  if (sp.n < 1 || sp.total < 1) return hipErrorInvalidValue;
-  const auto k = loop ? &k_merge_ragged<true> : &k_merge_ragged<false>;
+template <int RN>
+hipError_t merge_ragged_rn(const RaggedSpan<MergeArgs>& sp, bool loop, hipStream_t s) {
+  const auto k = loop ? &k_merge_ragged<true, RN> : &k_merge_ragged<false, RN>;
   const int grid = ragged_grid(reinterpret_cast<const void*>(k), kBlock, 0, sp, kBlock / 64, loop);
   hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, sp);
   return hipGetLastError();
+}
+
+hipError_t launch_merge_ragged(const RaggedSpan<MergeArgs>& sp, int rn, bool loop, hipStream_t s) {
+  if (sp.n < 1 || sp.total < 1) return hipErrorInvalidValue;
+  switch (rn) {
+    case 1: return merge_ragged_rn<1>(sp, loop, s);
+    case 2: return merge_ragged_rn<2>(sp, loop, s);
+    case 3: return merge_ragged_rn<3>(sp, loop, s);
+    case 4: return merge_ragged_rn<4>(sp, loop, s);
+    case 0: return merge_ragged_rn<0>(sp, loop, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace ss
